@@ -1,0 +1,1087 @@
+// gapx host code: model upload, device workspaces, the evaluation pipeline
+// and the C ABI declared in include/gapx.h.
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/gapx.h"
+#include "gapx_internal.h"
+
+namespace gapx {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+static int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// ---------------------------------------------------------------------------
+// model
+// ---------------------------------------------------------------------------
+
+static int build_tables(Model &m) {
+  SFTables &t = m.sf;
+  t = SFTables{};
+  std::vector<double> cuts;
+  for (int s = 0; s < m.nsf; ++s) {
+    int ty = m.sf_type[s];
+    if (ty < 1 || ty > 4) continue;
+    bool seen = false;
+    for (double c : cuts) seen = seen || (c == m.sf_cut[s]);
+    if (!seen) cuts.push_back(m.sf_cut[s]);
+  }
+  std::sort(cuts.begin(), cuts.end(), [](double a, double b) { return a > b; });
+  if ((int)cuts.size() > 16) return fail(GAPX_ERR_MODEL, "more than 16 distinct symmetry-function cutoffs");
+  t.group_rc = cuts;
+  t.n_groups = (int)cuts.size();
+  auto group_of = [&](double c) {
+    for (int g = 0; g < (int)cuts.size(); ++g)
+      if (cuts[g] == c) return g;
+    return 0;
+  };
+  struct Ang {
+    int g, feat, zi;
+    double eta, lam, zr, pref;
+  };
+  std::vector<Ang> ang;
+  for (int s = 0; s < m.nsf; ++s) {
+    const int ty = m.sf_type[s];
+    const double al = m.sf_alpha[s], rc = m.sf_cut[s];
+    if (rc <= 0) return fail(GAPX_ERR_MODEL, "symmetry-function cutoff must be positive");
+    if (ty == 1 || ty == 3) {
+      t.rad_a.push_back(ty == 1 ? al : 4.0);   // type 3 has alpha fixed at 4 (wacsf.f90:439)
+      t.rad_rs.push_back(ty == 1 ? 0.0 : al);  // and the file's alpha column is Rs (:438)
+      t.rad_group.push_back(group_of(rc));
+      t.rad_feat.push_back(s);
+      t.rc_max_rad = std::max(t.rc_max_rad, rc);
+    } else if (ty == 2 || ty == 4) {
+      const double z = m.sf_zeta.empty() ? 1.0 : m.sf_zeta[s];
+      Ang a;
+      a.g = group_of(rc);
+      a.feat = s;
+      a.eta = al;
+      a.lam = (ty == 2) ? 1.0 : -1.0;
+      a.zr = z;
+      a.zi = (z == std::floor(z) && z >= 1.0 && z <= 64.0) ? (int)z : -1;
+      a.pref = std::pow(2.0, 1.0 - z);
+      ang.push_back(a);
+      t.rc_max_ang = std::max(t.rc_max_ang, rc);
+    } else {
+      std::fprintf(stderr, "gapx: unknown symmetry-function type %d on line %d; its features are 0\n", ty, s + 1);
+    }
+  }
+  std::stable_sort(ang.begin(), ang.end(), [](const Ang &x, const Ang &y) {
+    if (x.g != y.g) return x.g < y.g;
+    if (x.eta != y.eta) return x.eta < y.eta;
+    if (x.lam != y.lam) return x.lam > y.lam;
+    return x.zr < y.zr;
+  });
+  for (const Ang &a : ang) {
+    t.ang_eta.push_back(a.eta);
+    t.ang_lam.push_back(a.lam);
+    t.ang_pref.push_back(a.pref);
+    t.ang_zeta.push_back(a.zi);
+    t.ang_zeta_real.push_back(a.zr);
+    t.ang_group.push_back(a.g);
+    t.ang_feat.push_back(a.feat);
+  }
+  t.n_rad = (int)t.rad_a.size();
+  t.n_ang = (int)t.ang_eta.size();
+  return GAPX_OK;
+}
+
+static uint16_t f16_bits(double v) { return __builtin_bit_cast(uint16_t, (_Float16)v); }
+static double f16_val(uint16_t b) { return (double)__builtin_bit_cast(_Float16, b); }
+static uint16_t bf16_bits(double v) {
+  float f = (float)v;
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  uint32_t r = 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)((u + r) >> 16);
+}
+static double bf16_val(uint16_t b) { return (double)__builtin_bit_cast(float, (uint32_t)b << 16); }
+
+struct Arena {
+  std::vector<std::pair<void **, size_t>> items;
+  template <typename T>
+  void add(T **p, size_t count) { items.push_back({(void **)p, ((count * sizeof(T) + 255) / 256) * 256}); }
+  size_t total() const {
+    size_t s = 0;
+    for (auto &it : items) s += it.second;
+    return s;
+  }
+};
+
+static int upload_model(Model &m) {
+  const int D = m.D, M = m.M, Dp = m.Dp, Mp = m.Mp;
+  const SFTables &t = m.sf;
+  // host-side derived arrays
+  std::vector<double> wz(kMaxSpeciesZ, 0.0);
+  std::vector<int> hasz(kMaxSpeciesZ, 0);
+  for (int k = 0; k < m.nspecies; ++k) {
+    if (m.z[k] < 0 || m.z[k] >= kMaxSpeciesZ) return fail(GAPX_ERR_MODEL, "atomic number out of range in model header");
+    wz[m.z[k]] = m.w[k];  // later lines win, as in gap_calc.f90:77-82
+    hasz[m.z[k]] = 1;
+  }
+  std::vector<double> tinv(Dp, 0.0), mu(Dp, 0.0), mt((size_t)Mp * Dp, 0.0), mn(Mp, 0.0), cf(Mp, 0.0);
+  for (int d = 0; d < D; ++d) {
+    if (!(m.theta[d] != 0.0)) return fail(GAPX_ERR_MODEL, "theta must be non-zero");
+    tinv[d] = 1.0 / m.theta[d];
+  }
+  for (int d = 0; d < D; ++d) {
+    double s = 0.0;
+    for (int k = 0; k < M; ++k) s += m.mm[(size_t)k * D + d] * tinv[d];
+    mu[d] = s / M;
+  }
+  for (int k = 0; k < M; ++k) {
+    double n = 0.0;
+    for (int d = 0; d < D; ++d) {
+      double v = m.mm[(size_t)k * D + d] * tinv[d] - mu[d];
+      mt[(size_t)k * Dp + d] = v;
+      n += v * v;
+    }
+    mn[k] = n;
+    cf[k] = m.coeff[k];
+  }
+  std::vector<uint16_t> hi((size_t)Mp * Dp), lo((size_t)Mp * Dp), bhi((size_t)Mp * Dp), blo((size_t)Mp * Dp);
+  std::vector<uint16_t> thi((size_t)Mp * Dp), tlo((size_t)Mp * Dp), tbhi((size_t)Mp * Dp), tblo((size_t)Mp * Dp);
+  for (int k = 0; k < Mp; ++k)
+    for (int d = 0; d < Dp; ++d) {
+      const size_t a = (size_t)k * Dp + d, b = (size_t)d * Mp + k;
+      const double v = mt[a];
+      hi[a] = thi[b] = f16_bits(v);
+      lo[a] = tlo[b] = f16_bits(v - f16_val(hi[a]));
+      bhi[a] = tbhi[b] = bf16_bits(v);
+      blo[a] = tblo[b] = bf16_bits(v - bf16_val(bhi[a]));
+    }
+  // device layout
+  Arena ar;
+  double *rad_a, *rad_rs, *ang_eta, *ang_lam, *ang_pref, *ang_zr, *group_rc;
+  int *rad_group, *rad_feat, *ang_zeta, *ang_group, *ang_feat;
+  const int nr = std::max(1, t.n_rad), na = std::max(1, t.n_ang), ng = std::max(1, t.n_groups);
+  ar.add(&rad_a, nr); ar.add(&rad_rs, nr); ar.add(&rad_group, nr); ar.add(&rad_feat, nr);
+  ar.add(&ang_eta, na); ar.add(&ang_lam, na); ar.add(&ang_pref, na); ar.add(&ang_zr, na);
+  ar.add(&ang_zeta, na); ar.add(&ang_group, na); ar.add(&ang_feat, na); ar.add(&group_rc, ng);
+  ar.add(&m.d_wz, kMaxSpeciesZ); ar.add(&m.d_hasz, kMaxSpeciesZ);
+  ar.add(&m.d_theta_inv, Dp); ar.add(&m.d_mu, Dp); ar.add(&m.d_mt, (size_t)Mp * Dp); ar.add(&m.d_mn, Mp);
+  ar.add(&m.d_coeff, Mp);
+  ar.add(&m.d_mt_hi, (size_t)Mp * Dp); ar.add(&m.d_mt_lo, (size_t)Mp * Dp);
+  ar.add(&m.d_mt_bhi, (size_t)Mp * Dp); ar.add(&m.d_mt_blo, (size_t)Mp * Dp);
+  ar.add(&m.d_mtT_hi, (size_t)Mp * Dp); ar.add(&m.d_mtT_lo, (size_t)Mp * Dp);
+  ar.add(&m.d_mtT_bhi, (size_t)Mp * Dp); ar.add(&m.d_mtT_blo, (size_t)Mp * Dp);
+  const size_t bytes = ar.total();
+  char *blk = nullptr;
+  GAPX_HIP(hipGetDevice(&m.device));
+  if (hipMalloc((void **)&blk, bytes) != hipSuccess) return fail(GAPX_ERR_NOMEM, "hipMalloc of the model failed");
+  m.dev_block = blk;
+  m.dev_bytes = bytes;
+  size_t off = 0;
+  for (auto &it : ar.items) {
+    *it.first = blk + off;
+    off += it.second;
+  }
+  auto put = [&](void *dst, const void *src, size_t n) { return hipMemcpy(dst, src, n, hipMemcpyHostToDevice); };
+  std::vector<double> zero(1, 0.0);
+  std::vector<int> izero(1, 0);
+#define PUT(dst, vec, T)                                                                     \
+  GAPX_HIP(put(dst, (vec).empty() ? (const void *)zero.data() : (const void *)(vec).data(), \
+               std::max<size_t>(1, (vec).size()) * sizeof(T)))
+  PUT(rad_a, t.rad_a, double); PUT(rad_rs, t.rad_rs, double);
+  if (!t.rad_group.empty()) { GAPX_HIP(put(rad_group, t.rad_group.data(), t.rad_group.size() * 4)); GAPX_HIP(put(rad_feat, t.rad_feat.data(), t.rad_feat.size() * 4)); }
+  PUT(ang_eta, t.ang_eta, double); PUT(ang_lam, t.ang_lam, double); PUT(ang_pref, t.ang_pref, double);
+  PUT(ang_zr, t.ang_zeta_real, double);
+  if (!t.ang_group.empty()) {
+    GAPX_HIP(put(ang_zeta, t.ang_zeta.data(), t.ang_zeta.size() * 4));
+    GAPX_HIP(put(ang_group, t.ang_group.data(), t.ang_group.size() * 4));
+    GAPX_HIP(put(ang_feat, t.ang_feat.data(), t.ang_feat.size() * 4));
+  }
+  PUT(group_rc, t.group_rc, double);
+#undef PUT
+  GAPX_HIP(put(m.d_wz, wz.data(), wz.size() * 8));
+  GAPX_HIP(put(m.d_hasz, hasz.data(), hasz.size() * 4));
+  GAPX_HIP(put(m.d_theta_inv, tinv.data(), Dp * 8));
+  GAPX_HIP(put(m.d_mu, mu.data(), Dp * 8));
+  GAPX_HIP(put(m.d_mt, mt.data(), mt.size() * 8));
+  GAPX_HIP(put(m.d_mn, mn.data(), Mp * 8));
+  GAPX_HIP(put(m.d_coeff, cf.data(), Mp * 8));
+  GAPX_HIP(put(m.d_mt_hi, hi.data(), hi.size() * 2));
+  GAPX_HIP(put(m.d_mt_lo, lo.data(), lo.size() * 2));
+  GAPX_HIP(put(m.d_mt_bhi, bhi.data(), bhi.size() * 2));
+  GAPX_HIP(put(m.d_mt_blo, blo.data(), blo.size() * 2));
+  GAPX_HIP(put(m.d_mtT_hi, thi.data(), thi.size() * 2));
+  GAPX_HIP(put(m.d_mtT_lo, tlo.data(), tlo.size() * 2));
+  GAPX_HIP(put(m.d_mtT_bhi, tbhi.data(), tbhi.size() * 2));
+  GAPX_HIP(put(m.d_mtT_blo, tblo.data(), tblo.size() * 2));
+  SFDev &sd = m.sfd;
+  sd.n_rad = t.n_rad;
+  sd.n_ang = t.n_ang;
+  sd.n_groups = t.n_groups;
+  sd.nsf = m.nsf;
+  sd.rc_max_rad = t.rc_max_rad;
+  sd.rc_max_ang = t.rc_max_ang;
+  sd.rad_a = rad_a; sd.rad_rs = rad_rs; sd.rad_group = rad_group; sd.rad_feat = rad_feat;
+  sd.ang_eta = ang_eta; sd.ang_lam = ang_lam; sd.ang_pref = ang_pref; sd.ang_zeta_real = ang_zr;
+  sd.ang_zeta = ang_zeta; sd.ang_group = ang_group; sd.ang_feat = ang_feat; sd.group_rc = group_rc;
+  return GAPX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context (device workspace for a batch of structures)
+// ---------------------------------------------------------------------------
+
+struct DevMem {
+  void *p = nullptr;
+  size_t cap = 0;
+};
+
+enum Stage { kNeighbors, kDescFwd, kContraction, kFinish, kDescBwd, kForces, kNumStages };
+static const char *kStageNames[kNumStages] = {"neighbor_list", "descriptors_fwd", "gpr_contraction",
+                                              "gpr_finish", "descriptors_bwd", "forces_stress"};
+
+struct Ctx {
+  Model *m = nullptr;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int nstruct = 0, N = 0, Np = 0, nbins = 0;
+  double rcut = 6.0;
+  std::vector<int> natoms, offsets;
+  std::vector<double> lat;
+  std::vector<StructParams> sp;
+  std::map<std::string, DevMem> mem;
+  size_t bytes = 0;
+  int nb_cap = 0, KP = 8;
+  int last_flags = -1;
+  bool ran = false;
+  int chunk = 16;
+  // timing
+  bool timing = false;
+  hipEvent_t ev[2 * kNumStages] = {};
+  double stage_ms[kNumStages] = {};
+  int stage_launches[kNumStages] = {};
+  bool stage_pending[kNumStages] = {};
+  // graph replay
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  int graph_flags = -1;
+  int graph_nb_cap = -1, graph_KP = -1, graph_N = -1;
+
+  template <typename T>
+  int get(const char *name, size_t count, T **out) {
+    DevMem &d = mem[name];
+    size_t need = std::max<size_t>(count, 1) * sizeof(T);
+    if (d.cap < need) {
+      if (d.p) {
+        hipFree(d.p);
+        bytes -= d.cap;
+      }
+      size_t alloc = need + need / 8;
+      if (hipMalloc(&d.p, alloc) != hipSuccess) {
+        d.p = nullptr;
+        d.cap = 0;
+        return fail(GAPX_ERR_NOMEM, std::string("hipMalloc failed for ") + name);
+      }
+      d.cap = alloc;
+      bytes += alloc;
+      invalidate_graph();
+    }
+    *out = (T *)d.p;
+    return GAPX_OK;
+  }
+  template <typename T>
+  T *ptr(const char *name) {
+    auto it = mem.find(name);
+    return it == mem.end() ? nullptr : (T *)it->second.p;
+  }
+  void invalidate_graph() {
+    if (gexec) {
+      hipGraphExecDestroy(gexec);
+      gexec = nullptr;
+    }
+    graph_flags = -1;
+  }
+};
+
+static void inv3(const double *A, double *B) {
+  double det = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) + A[2] * (A[3] * A[7] - A[4] * A[6]);
+  B[0] = (A[4] * A[8] - A[5] * A[7]) / det;
+  B[1] = (A[2] * A[7] - A[1] * A[8]) / det;
+  B[2] = (A[1] * A[5] - A[2] * A[4]) / det;
+  B[3] = (A[5] * A[6] - A[3] * A[8]) / det;
+  B[4] = (A[0] * A[8] - A[2] * A[6]) / det;
+  B[5] = (A[2] * A[3] - A[0] * A[5]) / det;
+  B[6] = (A[3] * A[7] - A[4] * A[6]) / det;
+  B[7] = (A[1] * A[6] - A[0] * A[7]) / det;
+  B[8] = (A[0] * A[4] - A[1] * A[3]) / det;
+}
+
+static int struct_params(Ctx &c) {
+  c.sp.resize(c.nstruct);
+  int bin_base = 0;
+  for (int s = 0; s < c.nstruct; ++s) {
+    StructParams &P = c.sp[s];
+    std::memcpy(P.L, &c.lat[9 * s], 9 * sizeof(double));
+    const double *L = P.L;
+    // Det() of gap_calc.f90:290-299
+    const double det = L[0] * (L[4] * L[8] - L[5] * L[7]) - L[1] * (L[3] * L[8] - L[5] * L[6]) +
+                       L[2] * (L[3] * L[7] - L[6] * L[4]);
+    if (!(std::fabs(det) > 1e-12)) return fail(GAPX_ERR_ARG, "singular lattice");
+    P.vol = std::fabs(det);
+    inv3(P.L, P.Linv);
+    for (int k = 0; k < 3; ++k) {
+      double g = std::sqrt(P.Linv[k] * P.Linv[k] + P.Linv[3 + k] * P.Linv[3 + k] + P.Linv[6 + k] * P.Linv[6 + k]);
+      double dk = 1.0 / g;
+      int nbk = (int)std::floor(dk / c.rcut);
+      nbk = std::max(1, std::min(nbk, 128));
+      P.nb[k] = nbk;
+      P.S[k] = (int)std::ceil(c.rcut * nbk / dk);
+    }
+    P.bin_base = bin_base;
+    bin_base += P.nb[0] * P.nb[1] * P.nb[2];
+    P.atom_base = c.offsets[s];
+    P.natoms = c.natoms[s];
+    P.pad = 0;
+  }
+  c.nbins = bin_base;
+  return GAPX_OK;
+}
+
+static int ctx_load(Ctx &c, int nstruct, const int *natoms, const int *species, const double *lat,
+                    const double *pos, double rcut, bool fortran) {
+  if (nstruct <= 0 || !natoms || !species || !lat || !pos) return fail(GAPX_ERR_ARG, "empty batch");
+  if (!(rcut > 0)) return fail(GAPX_ERR_ARG, "rcut must be positive");
+  GAPX_HIP(hipSetDevice(c.device));
+  c.nstruct = nstruct;
+  c.rcut = rcut;
+  c.natoms.assign(natoms, natoms + nstruct);
+  c.offsets.assign(nstruct + 1, 0);
+  for (int s = 0; s < nstruct; ++s) {
+    if (natoms[s] <= 0) return fail(GAPX_ERR_ARG, "structure with no atoms");
+    c.offsets[s + 1] = c.offsets[s] + natoms[s];
+  }
+  c.N = c.offsets[nstruct];
+  c.Np = round_up(c.N, 64);
+  // species must be listed in the model header (the reference leaves their
+  // weight undefined, gap_calc.f90:73-83)
+  std::vector<int> listed(kMaxSpeciesZ, 0);
+  for (int k = 0; k < c.m->nspecies; ++k) listed[c.m->z[k]] = 1;
+  for (int i = 0; i < c.N; ++i)
+    if (species[i] < 0 || species[i] >= kMaxSpeciesZ || !listed[species[i]]) {
+      char buf[128];
+      std::snprintf(buf, sizeof buf, "atom %d has atomic number %d, which the model does not list", i + 1, species[i]);
+      return fail(GAPX_ERR_SPECIES, buf);
+    }
+  c.lat.resize(9 * nstruct);
+  std::vector<double> hpos(3 * (size_t)c.N);
+  if (fortran) {  // single structure: LAT(3,3) and POS(NA,3) column-major
+    for (int k = 0; k < 3; ++k)
+      for (int x = 0; x < 3; ++x) c.lat[3 * k + x] = lat[k + 3 * x];
+    for (int i = 0; i < c.N; ++i)
+      for (int x = 0; x < 3; ++x) hpos[3 * (size_t)i + x] = pos[i + (size_t)c.N * x];
+  } else {
+    std::memcpy(c.lat.data(), lat, 9 * sizeof(double) * nstruct);
+    std::memcpy(hpos.data(), pos, 3 * sizeof(double) * c.N);
+  }
+  int rc = struct_params(c);
+  if (rc) return rc;
+  std::vector<int> astruct(c.N);
+  for (int s = 0; s < nstruct; ++s)
+    for (int i = c.offsets[s]; i < c.offsets[s + 1]; ++i) astruct[i] = s;
+  int *d_species, *d_astruct;
+  double *d_pos;
+  StructParams *d_sp;
+  if ((rc = c.get("species", c.N, &d_species)) || (rc = c.get("astruct", c.N, &d_astruct)) ||
+      (rc = c.get("pos", 3 * (size_t)c.N, &d_pos)) || (rc = c.get("sp", nstruct, &d_sp)))
+    return rc;
+  GAPX_HIP(hipMemcpyAsync(d_species, species, sizeof(int) * c.N, hipMemcpyHostToDevice, c.stream));
+  GAPX_HIP(hipMemcpyAsync(d_astruct, astruct.data(), sizeof(int) * c.N, hipMemcpyHostToDevice, c.stream));
+  GAPX_HIP(hipMemcpyAsync(d_pos, hpos.data(), sizeof(double) * 3 * c.N, hipMemcpyHostToDevice, c.stream));
+  GAPX_HIP(hipMemcpyAsync(d_sp, c.sp.data(), sizeof(StructParams) * nstruct, hipMemcpyHostToDevice, c.stream));
+  GAPX_HIP(hipStreamSynchronize(c.stream));
+  c.ran = false;
+  c.invalidate_graph();
+  return GAPX_OK;
+}
+
+struct Bufs {
+  NeighborBufs nb{};
+  DescOut out{};
+  GprArgs g{};
+  double *slot_f = nullptr, *atom_vir = nullptr, *force = nullptr, *ene = nullptr, *stress = nullptr;
+  int *bin_cursor = nullptr;
+};
+
+static void stage_begin(Ctx &c, int st) {
+  if (c.timing) hipEventRecord(c.ev[2 * st], c.stream);
+}
+static void stage_end(Ctx &c, int st) {
+  if (c.timing) {
+    hipEventRecord(c.ev[2 * st + 1], c.stream);
+    c.stage_pending[st] = true;
+  }
+}
+
+static void collect_timing(Ctx &c) {
+  if (!c.timing) return;
+  for (int st = 0; st < kNumStages; ++st) {
+    if (!c.stage_pending[st]) continue;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c.ev[2 * st], c.ev[2 * st + 1]) == hipSuccess) {
+      c.stage_ms[st] += ms;
+      c.stage_launches[st] += 1;
+    }
+    c.stage_pending[st] = false;
+  }
+}
+
+// Neighbour list up to the counts; returns total and max count (synchronises).
+static int run_neighbors(Ctx &c, int flags, Bufs &b) {
+  NeighborBufs &nb = b.nb;
+  int rc;
+  int *stats;
+  if ((rc = c.get("bin_count", c.nbins, &nb.bin_count)) || (rc = c.get("bin_start", c.nbins + 1, &nb.bin_start)) ||
+      (rc = c.get("bin_cursor", c.nbins, &nb.bin_cursor)) || (rc = c.get("bin_atoms", c.N, &nb.bin_atoms)) ||
+      (rc = c.get("atom_bin", c.N, &nb.atom_bin)) || (rc = c.get("wpos", 3 * (size_t)c.N, &nb.wpos)) ||
+      (rc = c.get("shift", 3 * (size_t)c.N, &nb.shift)) || (rc = c.get("nb_count", c.N, &nb.nb_count)) ||
+      (rc = c.get("nb_start", c.N + 1, &nb.nb_start)) || (rc = c.get("stats", 4, &stats)))
+    return rc;
+  nb.total = stats;
+  nb.overflow = stats + 2;
+  const int *d_species = c.ptr<int>("species");
+  const int *d_astruct = c.ptr<int>("astruct");
+  const double *d_pos = c.ptr<double>("pos");
+  const StructParams *d_sp = c.ptr<StructParams>("sp");
+  (void)d_species;
+  const int cap = (flags & GAPX_REF_NEIGHBOR_CAP)
+                      ? kRefNeighborCap
+                      : std::min(1024, max_neighbors_for_lds(!(flags & GAPX_MIXED), std::max(1, c.m->sf.n_groups), c.m->Dp));
+  GAPX_HIP(launch_fill_int(c.stream, nb.bin_count, c.nbins, 0));
+  GAPX_HIP(launch_fill_int(c.stream, nb.bin_cursor, c.nbins, 0));
+  GAPX_HIP(launch_fill_int(c.stream, stats, 4, 0));
+  GAPX_HIP(launch_wrap_bin(c.stream, c.N, d_pos, d_astruct, d_sp, nb));
+  GAPX_HIP(launch_scan(c.stream, nb.bin_count, nb.bin_start, c.nbins, nullptr));
+  GAPX_HIP(launch_scatter_bins(c.stream, c.N, c.nbins, nb));
+  GAPX_HIP(launch_count_neighbors(c.stream, c.N, d_astruct, d_sp, c.rcut, cap, nb));
+  GAPX_HIP(launch_scan(c.stream, nb.nb_count, nb.nb_start, c.N, stats));
+  int h[4];
+  GAPX_HIP(hipMemcpyAsync(h, stats, sizeof h, hipMemcpyDeviceToHost, c.stream));
+  GAPX_HIP(hipStreamSynchronize(c.stream));
+  if (h[2] > 0) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "%d atoms have more than %d neighbours (largest %d)", h[2], cap, h[1]);
+    return fail(GAPX_ERR_NEIGHBORS, buf);
+  }
+  const int total = h[0];
+  c.KP = std::max(8, round_up(h[1], 8));
+  if ((rc = c.get("nb_j", total, &nb.nb_j)) || (rc = c.get("nb_img", 3 * (size_t)total, &nb.nb_img)) ||
+      (rc = c.get("nb_d", 4 * (size_t)total, &nb.nb_d)) || (rc = c.get("nb_rev", total, &nb.nb_rev)))
+    return rc;
+  c.nb_cap = total;
+  return GAPX_OK;
+}
+
+static int gpr_layout(Ctx &c, GprArgs &g) {
+  const Model &m = *c.m;
+  g.N = c.N;
+  g.Np = c.Np;
+  g.Dp = m.Dp;
+  g.Mp = m.Mp;
+  const int tiles = c.Np / 64;
+  int splits = std::max(1, (1024 + tiles - 1) / tiles);
+  splits = std::min(splits, m.Mp / 32);
+  int per = round_up((m.Mp + splits - 1) / splits, 32);
+  splits = (m.Mp + per - 1) / per;
+  g.splits = splits;
+  g.pts_per_split = per;
+  return GAPX_OK;
+}
+
+static int ctx_run(Ctx &c, int flags) {
+  if (c.N <= 0) return fail(GAPX_ERR_ARG, "no structures loaded");
+  GAPX_HIP(hipSetDevice(c.device));
+  Model &m = *c.m;
+  const bool fp64 = !(flags & GAPX_MIXED);
+  const bool grad = (flags & GAPX_GRAD) != 0;
+  const int Dp = m.Dp;
+  Bufs b;
+  int rc;
+  stage_begin(c, kNeighbors);
+  if ((rc = run_neighbors(c, flags, b))) return rc;
+  NeighborBufs &nb = b.nb;
+  const int *d_species = c.ptr<int>("species");
+  const int *d_astruct = c.ptr<int>("astruct");
+  const StructParams *d_sp = c.ptr<StructParams>("sp");
+  GAPX_HIP(launch_fill_neighbors(c.stream, c.N, d_astruct, d_sp, c.rcut, nb));
+  if (grad) GAPX_HIP(launch_reverse_slots(c.stream, c.N, nb));
+  stage_end(c, kNeighbors);
+
+  // descriptors
+  DescOut &o = b.out;
+  o.N = c.N;
+  o.Dp = Dp;
+  const size_t ND = (size_t)c.Np * Dp;
+  if ((rc = c.get("xn", c.Np, &o.xn))) return rc;
+  if (fp64) {
+    if ((rc = c.get("x64", ND, &o.x64)) || (rc = c.get("xt64", ND, &o.xt64))) return rc;
+  } else {
+    if ((rc = c.get("x32", ND, &o.x32)) || (rc = c.get("xt32", ND, &o.xt32)) || (rc = c.get("xt_hi", ND, &o.xt_hi)) ||
+        (rc = c.get("xt_lo", ND, &o.xt_lo)))
+      return rc;
+  }
+  // padding rows [N, Np) of the contraction operands stay zero
+  if (c.Np > c.N) {
+    if (fp64) GAPX_HIP(launch_fill_double(c.stream, o.xt64 + (size_t)c.N * Dp, (size_t)(c.Np - c.N) * Dp, 0.0));
+    GAPX_HIP(launch_fill_double(c.stream, o.xn + c.N, c.Np - c.N, 0.0));
+    if (!fp64) {
+      GAPX_HIP(launch_fill_int(c.stream, (int *)(o.xt_hi + (size_t)c.N * Dp), (c.Np - c.N) * Dp / 2, 0));
+      GAPX_HIP(launch_fill_int(c.stream, (int *)(o.xt_lo + (size_t)c.N * Dp), (c.Np - c.N) * Dp / 2, 0));
+    }
+  }
+  const int split_mode = fp64 ? 0 : ((flags & GAPX_BF16) ? 2 : 1);
+  stage_begin(c, kDescFwd);
+  GAPX_HIP(launch_descriptors_fwd(c.stream, fp64, c.N, d_species, m.d_wz, nb, m.sfd, m.d_theta_inv, m.d_mu,
+                                  split_mode, o, c.chunk, c.KP));
+  stage_end(c, kDescFwd);
+
+  // contraction
+  GprArgs &g = b.g;
+  gpr_layout(c, g);
+  g.xt64 = o.xt64;
+  g.xn = o.xn;
+  g.xt_hi = o.xt_hi;
+  g.xt_lo = o.xt_lo;
+  g.xt32 = o.xt32;
+  g.mt = m.d_mt;
+  g.mn = m.d_mn;
+  g.coeff = m.d_coeff;
+  const bool bf = (flags & GAPX_BF16) != 0;
+  g.mt_hi = bf ? m.d_mt_bhi : m.d_mt_hi;
+  g.mt_lo = bf ? m.d_mt_blo : m.d_mt_lo;
+  g.mtT_hi = bf ? m.d_mtT_bhi : m.d_mtT_hi;
+  g.mtT_lo = bf ? m.d_mtT_blo : m.d_mtT_lo;
+  g.theta_inv = m.d_theta_inv;
+  g.grad = grad ? 1 : 0;
+  g.bf16 = bf ? 1 : 0;
+  if ((rc = c.get("Epart", (size_t)g.splits * c.Np, &g.Epart)) || (rc = c.get("eatom", c.Np, &g.eatom))) return rc;
+  if (grad) {
+    if ((rc = c.get("Gpart", (size_t)g.splits * ND, &g.Gpart))) return rc;
+    if (fp64) {
+      if ((rc = c.get("g64", ND, &g.g64))) return rc;
+    } else {
+      if ((rc = c.get("g32", ND, &g.g32))) return rc;
+    }
+  }
+  stage_begin(c, kContraction);
+  if (fp64)
+    GAPX_HIP(launch_gpr_fp64(c.stream, g));
+  else
+    GAPX_HIP(launch_gpr_mixed(c.stream, g));
+  stage_end(c, kContraction);
+  stage_begin(c, kFinish);
+  GAPX_HIP(launch_gpr_finish(c.stream, g, fp64));
+  stage_end(c, kFinish);
+
+  if ((rc = c.get("force", 3 * (size_t)c.N, &b.force)) || (rc = c.get("ene", c.nstruct, &b.ene)) ||
+      (rc = c.get("stress", 6 * (size_t)c.nstruct, &b.stress)) || (rc = c.get("atom_vir", 6 * (size_t)c.N, &b.atom_vir)))
+    return rc;
+  if (grad) {
+    if ((rc = c.get("slot_f", 3 * (size_t)std::max(1, c.nb_cap), &b.slot_f))) return rc;
+    stage_begin(c, kDescBwd);
+    GAPX_HIP(launch_descriptors_bwd(c.stream, fp64, c.N, d_species, m.d_wz, nb, m.sfd, Dp, g.g64, g.g32, b.slot_f,
+                                    b.atom_vir, c.KP));
+    stage_end(c, kDescBwd);
+  }
+  stage_begin(c, kForces);
+  if (grad)
+    GAPX_HIP(launch_gather_forces(c.stream, c.N, nb, b.slot_f, b.force));
+  else
+    GAPX_HIP(launch_fill_double(c.stream, b.force, 3 * (size_t)c.N, 0.0));
+  GAPX_HIP(launch_struct_reduce(c.stream, c.nstruct, d_sp, g.eatom, b.atom_vir, grad ? 1 : 0, b.ene, b.stress));
+  stage_end(c, kForces);
+  c.last_flags = flags;
+  c.ran = true;
+  return GAPX_OK;
+}
+
+static int ctx_results(Ctx &c, double *ene, double *force, double *stress6, double *eatom, bool fortran) {
+  if (!c.ran) return fail(GAPX_ERR_ARG, "no evaluation has run on this context");
+  GAPX_HIP(hipStreamSynchronize(c.stream));
+  collect_timing(c);
+  if (ene) GAPX_HIP(hipMemcpy(ene, c.ptr<double>("ene"), sizeof(double) * c.nstruct, hipMemcpyDeviceToHost));
+  if (stress6) GAPX_HIP(hipMemcpy(stress6, c.ptr<double>("stress"), sizeof(double) * 6 * c.nstruct, hipMemcpyDeviceToHost));
+  if (eatom) GAPX_HIP(hipMemcpy(eatom, c.ptr<double>("eatom"), sizeof(double) * c.N, hipMemcpyDeviceToHost));
+  if (force) {
+    if (!fortran) {
+      GAPX_HIP(hipMemcpy(force, c.ptr<double>("force"), sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost));
+    } else {
+      std::vector<double> f(3 * (size_t)c.N);
+      GAPX_HIP(hipMemcpy(f.data(), c.ptr<double>("force"), sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost));
+      for (int i = 0; i < c.N; ++i)
+        for (int x = 0; x < 3; ++x) force[i + (size_t)c.N * x] = f[3 * (size_t)i + x];
+    }
+  }
+  return GAPX_OK;
+}
+
+static int ctx_create(Model *m, Ctx **out) {
+  Ctx *c = new Ctx();
+  c->m = m;
+  c->device = m->device;
+  if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(GAPX_ERR_HIP, "could not create a HIP stream");
+  }
+  for (auto &e : c->ev) hipEventCreate(&e);
+  const char *ch = std::getenv("GAPX_DESC_CHUNK");
+  if (ch) c->chunk = std::atoi(ch);
+  *out = c;
+  return GAPX_OK;
+}
+
+static void ctx_free(Ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  c->invalidate_graph();
+  for (auto &kv : c->mem)
+    if (kv.second.p) hipFree(kv.second.p);
+  for (auto &e : c->ev)
+    if (e) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+// ---------------------------------------------------------------------------
+// model file (gap_parameters) reader, Fortran list-directed rules
+// ---------------------------------------------------------------------------
+
+struct ListReader {
+  std::vector<std::string> lines;
+  size_t pos = 0;
+  bool read(int n, std::vector<double> &out) {
+    out.clear();
+    while ((int)out.size() < n) {
+      if (pos >= lines.size()) return false;
+      std::string s = lines[pos++];
+      for (char &ch : s)
+        if (ch == ',' || ch == '\t' || ch == '\r') ch = ' ';
+      std::istringstream is(s);
+      std::string tok;
+      while (is >> tok) {
+        for (char &ch : tok)
+          if (ch == 'd' || ch == 'D') ch = 'e';
+        char *end = nullptr;
+        double v = std::strtod(tok.c_str(), &end);
+        if (end == tok.c_str()) return false;
+        out.push_back(v);
+      }
+    }
+    return true;
+  }
+  void skip() { ++pos; }
+};
+
+static int read_model_file(const char *path, Model &m) {
+  std::ifstream f(path);
+  if (!f) return fail(GAPX_ERR_FILE, std::string(path) + " does not exist");
+  ListReader r;
+  std::string line;
+  while (std::getline(f, line)) r.lines.push_back(line);
+  std::vector<double> v;
+  auto need = [&](int n, const char *what) {
+    if (!r.read(n, v)) {
+      set_error(std::string("model file ") + path + ": cannot read " + what);
+      return false;
+    }
+    return true;
+  };
+  if (!need(1, "nspecies")) return GAPX_ERR_FILE;
+  const int ns = (int)v[0];
+  if (ns <= 0 || ns > kMaxSpeciesZ) return fail(GAPX_ERR_FILE, "bad species count");
+  for (int k = 0; k < ns; ++k) {
+    if (!need(2, "species weight")) return GAPX_ERR_FILE;
+    m.z.push_back((int)v[0]);
+    m.w.push_back(v[1]);
+  }
+  m.nspecies = ns;
+  if (!need(1, "nsf")) return GAPX_ERR_FILE;
+  const int nsf = (int)v[0];
+  if (nsf <= 0) return fail(GAPX_ERR_FILE, "bad symmetry-function count");
+  for (int s = 0; s < nsf; ++s) {
+    const size_t first = r.pos;
+    if (!need(3, "symmetry function line")) return GAPX_ERR_FILE;
+    m.sf_type.push_back((int)v[0]);
+    m.sf_alpha.push_back(v[1]);
+    m.sf_cut.push_back(v[2]);
+    double z = 1.0;
+    if (r.pos == first + 1) {  // optional 4th column on the same record: zeta
+      std::istringstream is(r.lines[first]);
+      std::string t[4];
+      if (is >> t[0] >> t[1] >> t[2] >> t[3]) {
+        for (char &ch : t[3])
+          if (ch == 'd' || ch == 'D') ch = 'e';
+        char *end = nullptr;
+        double zz = std::strtod(t[3].c_str(), &end);
+        if (end != t[3].c_str()) z = zz;
+      }
+    }
+    m.sf_zeta.push_back(z);
+  }
+  m.nsf = nsf;
+  if (!need(2, "nsparseX des_len")) return GAPX_ERR_FILE;
+  m.M = (int)v[0];
+  m.D = (int)v[1];
+  if (m.M <= 0 || m.D <= 0) return fail(GAPX_ERR_FILE, "bad nsparseX / des_len");
+  r.skip();
+  r.skip();
+  r.skip();
+  if (!need(m.D, "theta")) return GAPX_ERR_FILE;
+  m.theta = v;
+  m.mm.resize((size_t)m.M * m.D);
+  for (int k = 0; k < m.M; ++k) {
+    if (!need(m.D, "sparse point")) return GAPX_ERR_FILE;
+    std::copy(v.begin(), v.end(), m.mm.begin() + (size_t)k * m.D);
+  }
+  if (!need(m.M, "coefficients")) return GAPX_ERR_FILE;
+  m.coeff = v;
+  return GAPX_OK;
+}
+
+static int finish_model(Model *m) {
+  if (m->D != 2 * m->nsf)
+    return fail(GAPX_ERR_MODEL, "des_len must be twice the number of symmetry-function lines");
+  if (m->D > 1024) return fail(GAPX_ERR_MODEL, "des_len above 1024 is not supported");
+  m->Dp = round_up(m->D, 32);
+  if (m->Dp > 256) return fail(GAPX_ERR_MODEL, "des_len above 256 is not supported by the contraction kernels");
+  m->Mp = round_up(m->M, 64);
+  int rc = build_tables(*m);
+  if (rc) return rc;
+  return upload_model(*m);
+}
+
+}  // namespace gapx
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace gapx;
+
+struct gapx_model : Model {};
+struct gapx_ctx : Ctx {};
+
+extern "C" {
+
+int gapx_version(void) { return 100; }
+
+const char *gapx_last_error(void) { return g_err.c_str(); }
+
+int gapx_device_count(int *n) {
+  if (!n) return fail(GAPX_ERR_ARG, "null pointer");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *n = c;
+  return GAPX_OK;
+}
+
+int gapx_model_create(int nspecies, const int *z, const double *w, int nsf, const int *sf_type,
+                      const double *sf_alpha, const double *sf_cut, const double *sf_zeta, int des_len,
+                      int nsparse, const double *theta, const double *mm, int mm_layout, int ld_mm,
+                      const double *coeff, gapx_model **out) {
+  if (!out || nspecies <= 0 || nsf <= 0 || des_len <= 0 || nsparse <= 0 || !z || !w || !sf_type || !sf_alpha ||
+      !sf_cut || !theta || !mm || !coeff)
+    return fail(GAPX_ERR_ARG, "gapx_model_create: invalid argument");
+  gapx_model *m = new gapx_model();
+  m->nspecies = nspecies;
+  m->z.assign(z, z + nspecies);
+  m->w.assign(w, w + nspecies);
+  m->nsf = nsf;
+  m->sf_type.assign(sf_type, sf_type + nsf);
+  m->sf_alpha.assign(sf_alpha, sf_alpha + nsf);
+  m->sf_cut.assign(sf_cut, sf_cut + nsf);
+  if (sf_zeta)
+    m->sf_zeta.assign(sf_zeta, sf_zeta + nsf);
+  else
+    m->sf_zeta.assign(nsf, 1.0);
+  m->D = des_len;
+  m->M = nsparse;
+  m->theta.assign(theta, theta + des_len);
+  m->mm.resize((size_t)nsparse * des_len);
+  const int ld = (mm_layout == 1) ? std::max(ld_mm, nsparse) : des_len;
+  for (int k = 0; k < nsparse; ++k)
+    for (int d = 0; d < des_len; ++d)
+      m->mm[(size_t)k * des_len + d] = (mm_layout == 1) ? mm[k + (size_t)ld * d] : mm[(size_t)k * des_len + d];
+  m->coeff.assign(coeff, coeff + nsparse);
+  int rc = finish_model(m);
+  if (rc) {
+    if (m->dev_block) hipFree(m->dev_block);
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return GAPX_OK;
+}
+
+int gapx_model_read(const char *path, gapx_model **out) {
+  if (!path || !out) return fail(GAPX_ERR_ARG, "gapx_model_read: null argument");
+  gapx_model *m = new gapx_model();
+  int rc = read_model_file(path, *m);
+  if (!rc) rc = finish_model(m);
+  if (rc) {
+    if (m->dev_block) hipFree(m->dev_block);
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return GAPX_OK;
+}
+
+int gapx_model_dims(const gapx_model *m, int *nspecies, int *nsf, int *des_len, int *nsparse) {
+  if (!m) return fail(GAPX_ERR_ARG, "null model");
+  if (nspecies) *nspecies = m->nspecies;
+  if (nsf) *nsf = m->nsf;
+  if (des_len) *des_len = m->D;
+  if (nsparse) *nsparse = m->M;
+  return GAPX_OK;
+}
+
+int gapx_model_arrays(const gapx_model *m, double *theta, double *mm, int mm_layout, int ld_mm, double *coeff) {
+  if (!m) return fail(GAPX_ERR_ARG, "null model");
+  if (theta) std::copy(m->theta.begin(), m->theta.end(), theta);
+  if (coeff) std::copy(m->coeff.begin(), m->coeff.end(), coeff);
+  if (mm) {
+    const int ld = (mm_layout == 1) ? std::max(ld_mm, m->M) : m->D;
+    for (int k = 0; k < m->M; ++k)
+      for (int d = 0; d < m->D; ++d) {
+        double v = m->mm[(size_t)k * m->D + d];
+        if (mm_layout == 1)
+          mm[k + (size_t)ld * d] = v;
+        else
+          mm[(size_t)k * m->D + d] = v;
+      }
+  }
+  return GAPX_OK;
+}
+
+int gapx_model_header(const gapx_model *m, int *z, double *w, int *sf_type, double *sf_alpha, double *sf_cut,
+                      double *sf_zeta) {
+  if (!m) return fail(GAPX_ERR_ARG, "null model");
+  if (z) std::copy(m->z.begin(), m->z.end(), z);
+  if (w) std::copy(m->w.begin(), m->w.end(), w);
+  if (sf_type) std::copy(m->sf_type.begin(), m->sf_type.end(), sf_type);
+  if (sf_alpha) std::copy(m->sf_alpha.begin(), m->sf_alpha.end(), sf_alpha);
+  if (sf_cut) std::copy(m->sf_cut.begin(), m->sf_cut.end(), sf_cut);
+  if (sf_zeta) std::copy(m->sf_zeta.begin(), m->sf_zeta.end(), sf_zeta);
+  return GAPX_OK;
+}
+
+void gapx_model_free(gapx_model *m) {
+  if (!m) return;
+  if (m->cached_ctx) ctx_free(m->cached_ctx);
+  if (m->dev_block) {
+    hipSetDevice(m->device);
+    hipFree(m->dev_block);
+  }
+  delete m;
+}
+
+int gapx_file_stamp(const char *path, long long *size, long long *mtime_ns) {
+  struct stat st;
+  if (!path || stat(path, &st) != 0) return fail(GAPX_ERR_FILE, std::string(path ? path : "(null)") + " does not exist");
+  if (size) *size = (long long)st.st_size;
+  if (mtime_ns) *mtime_ns = (long long)st.st_mtim.tv_sec * 1000000000LL + st.st_mtim.tv_nsec;
+  return GAPX_OK;
+}
+
+static int cached(gapx_model *m, Ctx **c) {
+  if (!m) return fail(GAPX_ERR_ARG, "null model");
+  if (!m->cached_ctx) {
+    int rc = ctx_create(m, &m->cached_ctx);
+    if (rc) return rc;
+  }
+  *c = m->cached_ctx;
+  return GAPX_OK;
+}
+
+int gapx_calc(gapx_model *m, int na, const int *species, const double *lat, const double *pos, double rcut, int flags,
+              double *ene, double *force, double *stress6, double *variance) {
+  Ctx *c;
+  int rc = cached(m, &c);
+  if (rc) return rc;
+  if (na <= 0) return fail(GAPX_ERR_ARG, "na must be positive");
+  const bool fortran = (flags & GAPX_LAYOUT_F) != 0;
+  if ((rc = ctx_load(*c, 1, &na, species, lat, pos, rcut, fortran))) return rc;
+  if ((rc = ctx_run(*c, flags))) return rc;
+  if ((rc = ctx_results(*c, ene, force, stress6, nullptr, fortran))) return rc;
+  if (variance) *variance = 0.0;  // gap_calc.f90:206
+  return GAPX_OK;
+}
+
+int gapx_calc_batch(gapx_model *m, int nstruct, const int *natoms, const int *species, const double *lat,
+                    const double *pos, double rcut, int flags, double *ene, double *force, double *stress6) {
+  Ctx *c;
+  int rc = cached(m, &c);
+  if (rc) return rc;
+  if ((rc = ctx_load(*c, nstruct, natoms, species, lat, pos, rcut, false))) return rc;
+  if ((rc = ctx_run(*c, flags & ~GAPX_LAYOUT_F))) return rc;
+  return ctx_results(*c, ene, force, stress6, nullptr, false);
+}
+
+int gapx_descriptors(gapx_model *m, int na, const int *species, const double *lat, const double *pos, double rcut,
+                     int flags, double *x) {
+  Ctx *c;
+  int rc = cached(m, &c);
+  if (rc) return rc;
+  const bool fortran = (flags & GAPX_LAYOUT_F) != 0;
+  if ((rc = ctx_load(*c, 1, &na, species, lat, pos, rcut, fortran))) return rc;
+  if ((rc = ctx_run(*c, 0))) return rc;
+  GAPX_HIP(hipStreamSynchronize(c->stream));
+  std::vector<double> tmp((size_t)c->N * m->Dp);
+  GAPX_HIP(hipMemcpy(tmp.data(), c->ptr<double>("x64"), tmp.size() * 8, hipMemcpyDeviceToHost));
+  for (int i = 0; i < na; ++i)
+    for (int d = 0; d < m->D; ++d) x[(size_t)i * m->D + d] = tmp[(size_t)i * m->Dp + d];
+  return GAPX_OK;
+}
+
+int gapx_ctx_create(gapx_model *m, gapx_ctx **out) {
+  if (!m || !out) return fail(GAPX_ERR_ARG, "null argument");
+  Ctx *c = nullptr;
+  int rc = ctx_create(m, &c);
+  if (rc) return rc;
+  *out = static_cast<gapx_ctx *>(c);
+  return GAPX_OK;
+}
+
+int gapx_ctx_load(gapx_ctx *c, int nstruct, const int *natoms, const int *species, const double *lat,
+                  const double *pos, double rcut) {
+  if (!c) return fail(GAPX_ERR_ARG, "null context");
+  return ctx_load(*c, nstruct, natoms, species, lat, pos, rcut, false);
+}
+
+int gapx_ctx_update(gapx_ctx *c, const double *lat, const double *pos) {
+  if (!c || !pos || c->N <= 0) return fail(GAPX_ERR_ARG, "gapx_ctx_update: nothing loaded");
+  GAPX_HIP(hipSetDevice(c->device));
+  if (lat) {
+    std::memcpy(c->lat.data(), lat, 9 * sizeof(double) * c->nstruct);
+    int nb_before = c->nbins;
+    int rc = struct_params(*c);
+    if (rc) return rc;
+    if (c->nbins != nb_before) c->invalidate_graph();
+    GAPX_HIP(hipMemcpyAsync(c->ptr<StructParams>("sp"), c->sp.data(), sizeof(StructParams) * c->nstruct,
+                            hipMemcpyHostToDevice, c->stream));
+  }
+  GAPX_HIP(hipMemcpyAsync(c->ptr<double>("pos"), pos, sizeof(double) * 3 * c->N, hipMemcpyHostToDevice, c->stream));
+  GAPX_HIP(hipStreamSynchronize(c->stream));
+  return GAPX_OK;
+}
+
+int gapx_ctx_run(gapx_ctx *c, int flags) {
+  if (!c) return fail(GAPX_ERR_ARG, "null context");
+  collect_timing(*c);
+  return ctx_run(*c, flags & ~GAPX_LAYOUT_F);
+}
+
+int gapx_ctx_sync(gapx_ctx *c) {
+  if (!c) return fail(GAPX_ERR_ARG, "null context");
+  GAPX_HIP(hipStreamSynchronize(c->stream));
+  collect_timing(*c);
+  return GAPX_OK;
+}
+
+int gapx_ctx_results(gapx_ctx *c, double *ene, double *force, double *stress6, double *eatom) {
+  if (!c) return fail(GAPX_ERR_ARG, "null context");
+  return ctx_results(*c, ene, force, stress6, eatom, false);
+}
+
+int gapx_ctx_features(gapx_ctx *c, double *x, double *dedx) {
+  if (!c || !c->ran) return fail(GAPX_ERR_ARG, "no evaluation has run on this context");
+  GAPX_HIP(hipStreamSynchronize(c->stream));
+  const Model &m = *c->m;
+  const bool fp64 = !(c->last_flags & GAPX_MIXED);
+  const size_t n = (size_t)c->N * m.Dp;
+  std::vector<double> t(n);
+  auto out = [&](double *dst) {
+    for (int i = 0; i < c->N; ++i)
+      for (int d = 0; d < m.D; ++d) dst[(size_t)i * m.D + d] = t[(size_t)i * m.Dp + d];
+  };
+  if (x) {
+    if (fp64) {
+      GAPX_HIP(hipMemcpy(t.data(), c->ptr<double>("x64"), n * 8, hipMemcpyDeviceToHost));
+    } else {
+      std::vector<float> f(n);
+      GAPX_HIP(hipMemcpy(f.data(), c->ptr<float>("x32"), n * 4, hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < n; ++k) t[k] = f[k];
+    }
+    out(x);
+  }
+  if (dedx) {
+    if (!(c->last_flags & GAPX_GRAD)) return fail(GAPX_ERR_ARG, "dE/dx needs a run with GAPX_GRAD");
+    if (fp64) {
+      GAPX_HIP(hipMemcpy(t.data(), c->ptr<double>("g64"), n * 8, hipMemcpyDeviceToHost));
+    } else {
+      std::vector<float> f(n);
+      GAPX_HIP(hipMemcpy(f.data(), c->ptr<float>("g32"), n * 4, hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < n; ++k) t[k] = f[k];
+    }
+    out(dedx);
+  }
+  return GAPX_OK;
+}
+
+int gapx_ctx_timing(gapx_ctx *c, int enable) {
+  if (!c) return fail(GAPX_ERR_ARG, "null context");
+  GAPX_HIP(hipStreamSynchronize(c->stream));
+  collect_timing(*c);
+  c->timing = enable != 0;
+  for (int s = 0; s < kNumStages; ++s) {
+    c->stage_ms[s] = 0;
+    c->stage_launches[s] = 0;
+    c->stage_pending[s] = false;
+  }
+  return GAPX_OK;
+}
+
+int gapx_ctx_stage_count(gapx_ctx *c, int *n) {
+  if (!c || !n) return fail(GAPX_ERR_ARG, "null argument");
+  *n = kNumStages;
+  return GAPX_OK;
+}
+
+int gapx_ctx_stage_info(gapx_ctx *c, int k, const char **name, double *ms, int *launches) {
+  if (!c || k < 0 || k >= kNumStages) return fail(GAPX_ERR_ARG, "bad stage index");
+  GAPX_HIP(hipStreamSynchronize(c->stream));
+  collect_timing(*c);
+  if (name) *name = kStageNames[k];
+  if (ms) *ms = c->stage_ms[k];
+  if (launches) *launches = c->stage_launches[k];
+  return GAPX_OK;
+}
+
+int gapx_ctx_use_graph(gapx_ctx *c, int on) {
+  if (!c) return fail(GAPX_ERR_ARG, "null context");
+  c->use_graph = on != 0;
+  if (!on) c->invalidate_graph();
+  return GAPX_OK;
+}
+
+int gapx_ctx_device_bytes(gapx_ctx *c, long long *bytes) {
+  if (!c || !bytes) return fail(GAPX_ERR_ARG, "null argument");
+  *bytes = (long long)c->bytes;
+  return GAPX_OK;
+}
+
+void gapx_ctx_free(gapx_ctx *c) { ctx_free(c); }
+
+}  // extern "C"

@@ -1,0 +1,175 @@
+// Internal definitions shared by the gapx host code and its HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace gapx {
+
+constexpr double kPi = 3.141592654;       // truncated pi of wacsf.f90:31
+constexpr double kGPa = 6.24219e-3;       // eV/A^3 per GPa, gap_calc.f90:9
+constexpr int kMaxNeighbors = 256;        // per-atom list held in LDS
+constexpr int kRefNeighborCap = 1000;     // gap_calc.f90:68
+constexpr int kMaxGroups = 8;             // distinct per-line cutoffs
+constexpr int kMaxSpeciesZ = 128;
+
+// Symmetry-function tables as the kernels read them.  Radial lines (types 1
+// and 3) are exp(-a (r - rs)^2) fc_g(r); angular lines (types 2 and 4) are
+// pref (1 + lam cos)^zeta exp(-eta (rij^2 + rik^2 + rjk^2)) fc_g fc_g fc_g.
+// Angular lines are sorted by (group, eta, lam, zeta) so that consecutive
+// lines share exponentials and cutoff products.
+struct SFTables {
+  int n_rad = 0, n_ang = 0, n_groups = 0;
+  double rc_max_rad = 0, rc_max_ang = 0;
+  // radial
+  std::vector<double> rad_a, rad_rs;
+  std::vector<int> rad_group, rad_feat;
+  // angular
+  std::vector<double> ang_eta, ang_lam, ang_pref;
+  std::vector<int> ang_zeta, ang_group, ang_feat;
+  std::vector<double> ang_zeta_real;   // non-integer zeta support (pow path)
+  std::vector<double> group_rc;        // cutoff per group
+};
+
+// Device-side pointers of the SF tables (uniform loads inside kernels).
+struct SFDev {
+  int n_rad, n_ang, n_groups, nsf;
+  double rc_max_rad, rc_max_ang;
+  const double *rad_a, *rad_rs;
+  const int *rad_group, *rad_feat;
+  const double *ang_eta, *ang_lam, *ang_pref, *ang_zeta_real;
+  const int *ang_zeta, *ang_group, *ang_feat;
+  const double *group_rc;
+};
+
+struct Model {
+  int device = 0;
+  int nspecies = 0, nsf = 0, D = 0, M = 0;
+  int Dp = 0, Mp = 0;                    // padded (multiples of 32 / 64)
+  std::vector<int> z;
+  std::vector<double> w;
+  std::vector<int> sf_type;
+  std::vector<double> sf_alpha, sf_cut, sf_zeta;
+  std::vector<double> theta, mm, coeff;  // host copies, mm row-major [M][D]
+  SFTables sf;
+  // device
+  void *dev_block = nullptr;             // one allocation for everything below
+  SFDev sfd{};
+  double *d_wz = nullptr;                // weight by atomic number [kMaxSpeciesZ]
+  int *d_hasz = nullptr;                 // 1 if listed
+  double *d_theta_inv = nullptr;         // [Dp] (0 in padding)
+  double *d_mu = nullptr;                // [Dp] column mean of mm/theta
+  double *d_mt = nullptr;                // [Mp][Dp] centred scaled sparse set
+  double *d_mn = nullptr;                // [Mp] |mt|^2
+  double *d_coeff = nullptr;             // [Mp] (0 in padding)
+  uint16_t *d_mt_hi = nullptr, *d_mt_lo = nullptr;     // FP16 split [Mp][Dp]
+  uint16_t *d_mt_bhi = nullptr, *d_mt_blo = nullptr;   // BF16 split [Mp][Dp]
+  uint16_t *d_mtT_hi = nullptr, *d_mtT_lo = nullptr;   // FP16 split, transposed [Dp][Mp]
+  uint16_t *d_mtT_bhi = nullptr, *d_mtT_blo = nullptr; // BF16 split, transposed
+  float *d_mn_f = nullptr;               // unused placeholder for alignment
+  size_t dev_bytes = 0;
+  struct Ctx *cached_ctx = nullptr;      // workspace for gapx_calc
+};
+
+// Per-structure cell-list parameters (host computes, device reads).
+struct StructParams {
+  double L[9];       // rows = lattice vectors
+  double Linv[9];    // inverse
+  double vol;
+  int nb[3];         // bins per axis
+  int S[3];          // search half-width in bins
+  int bin_base;      // offset into the global bin array
+  int atom_base;     // first atom
+  int natoms;
+  int pad;
+};
+
+// Thread-local error message.
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define GAPX_HIP(call)                                                          \
+  do {                                                                          \
+    hipError_t e_ = (call);                                                     \
+    if (e_ != hipSuccess)                                                       \
+      return ::gapx::fail(GAPX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---- kernel launchers (kernels live in the .hip files) ----------------------
+struct NeighborBufs {
+  int *bin_count, *bin_start, *bin_cursor, *bin_atoms;  // [nbins_total (+1)]
+  int *atom_bin;        // [N]
+  double *wpos;         // [N*3] wrapped positions
+  int *shift;           // [N*3] floor(frac)
+  int *nb_count;        // [N]
+  int *nb_start;        // [N+1]
+  int *nb_j;            // [cap]
+  int *nb_img;          // [cap*3] image relative to given positions
+  double *nb_d;         // [cap*4] dx dy dz r
+  int *nb_rev;          // [cap]
+  int *overflow;        // [1] atoms above the neighbour cap
+  int *total;           // [1] total neighbours
+};
+
+hipError_t launch_wrap_bin(hipStream_t s, int N, const double *pos, const int *atom_struct,
+                           const StructParams *sp, NeighborBufs nb);
+hipError_t launch_scan(hipStream_t s, const int *in, int *out, int n, int *total);
+hipError_t launch_scatter_bins(hipStream_t s, int N, int nbins, NeighborBufs nb);
+hipError_t launch_count_neighbors(hipStream_t s, int N, const int *atom_struct,
+                                  const StructParams *sp, double rcut, int cap, NeighborBufs nb);
+hipError_t launch_fill_neighbors(hipStream_t s, int N, const int *atom_struct,
+                                 const StructParams *sp, double rcut, NeighborBufs nb);
+hipError_t launch_reverse_slots(hipStream_t s, int N, NeighborBufs nb);
+hipError_t launch_fill_int(hipStream_t s, int *p, int n, int v);
+hipError_t launch_fill_double(hipStream_t s, double *p, size_t n, double v);
+
+struct DescOut {
+  int N, Dp;
+  double *x64;          // [N][Dp] raw features (FP64 run) or null
+  float *x32;           // [N][Dp] raw features (FP32 run) or null
+  double *xt64;         // [N][Dp] centred scaled (FP64)
+  float *xt32;          // [N][Dp] centred scaled (FP32, mixed run)
+  uint16_t *xt_hi, *xt_lo;   // [N][Dp] split operands (mixed)
+  double *xn;           // [N] |xt|^2 (FP64)
+};
+
+hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *species,
+                                  const double *wz, const NeighborBufs &nb, const SFDev &sf,
+                                  const double *theta_inv, const double *mu, int split_mode,
+                                  DescOut out, int chunk, int KP);
+hipError_t launch_descriptors_bwd(hipStream_t s, bool fp64, int N, const int *species,
+                                  const double *wz, const NeighborBufs &nb, const SFDev &sf,
+                                  int Dp, const double *g64, const float *g32, double *slot_f,
+                                  double *atom_vir, int KP);
+// Largest per-atom neighbour count the descriptor kernels' LDS layout holds.
+int max_neighbors_for_lds(bool fp64, int G, int Dp);
+hipError_t launch_gather_forces(hipStream_t s, int N, const NeighborBufs &nb, const double *slot_f,
+                                double *force);
+hipError_t launch_struct_reduce(hipStream_t s, int nstruct, const StructParams *sp,
+                                const double *eatom, const double *atom_vir, int grad,
+                                double *ene, double *stress6);
+
+// Contraction: E_i = sum_k exp(-|xt_i - mt_k|^2 / 2) c_k and
+// g_i = -(xt_i E_i - sum_k W_ik mt_k) * theta_inv.
+struct GprArgs {
+  int N, Np, Dp, Mp, splits, pts_per_split;
+  const double *xt64; const double *xn;
+  const uint16_t *xt_hi, *xt_lo;
+  const float *xt32;
+  const double *mt, *mn, *coeff;
+  const uint16_t *mt_hi, *mt_lo, *mtT_hi, *mtT_lo;
+  const double *theta_inv;
+  double *Gpart; double *Epart;   // [splits][Np][Dp], [splits][Np]
+  double *eatom;                  // [N]
+  double *g64; float *g32;        // [N][Dp]
+  int grad;
+  int bf16;
+};
+hipError_t launch_gpr_fp64(hipStream_t s, const GprArgs &a);
+hipError_t launch_gpr_mixed(hipStream_t s, const GprArgs &a);
+hipError_t launch_gpr_finish(hipStream_t s, const GprArgs &a, bool fp64);
+
+}  // namespace gapx

@@ -1,0 +1,315 @@
+// Periodic neighbour lists on the GPU: cell-list binning in fractional
+// space, then a CSR list of every (j, image) within rcut of each atom.
+//
+// Replaces the brute-force O(N^2 * images) search of FGAP_CALC
+// (gappy/libgap/gap_calc.f90:85-120).  The neighbour set is the same: every
+// image R_j + n.L with |R_j + n.L - R_i| <= rcut, (j, n) != (i, 0), self-images
+// included (gap_calc.f90:97-101).  Positions are wrapped into the cell first;
+// the stored image n refers to the positions as given.
+//
+// Bins: nb_k = max(1, floor(d_k / rcut)) along each lattice direction (d_k the
+// interplanar spacing); a neighbour can be at most S_k = ceil(rcut nb_k / d_k)
+// bins away, and offsets beyond the cell wrap with an image shift, so cells
+// smaller than rcut need no special case.  Atoms inside a bin are kept in
+// ascending index order so the lists (and every sum over them) are
+// deterministic.
+#include "gapx_internal.h"
+
+namespace gapx {
+
+namespace {
+
+__device__ inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
+
+__global__ void k_fill_int(int *p, int n, int v) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+__global__ void k_fill_double(double *p, size_t n, double v) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+__global__ void k_wrap_bin(int N, const double *__restrict__ pos, const int *__restrict__ atom_struct,
+                           const StructParams *__restrict__ sp, NeighborBufs nb) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const StructParams &P = sp[atom_struct[i]];
+  double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  int b[3], sh[3];
+  for (int k = 0; k < 3; ++k) {
+    double f = r[0] * P.Linv[k] + r[1] * P.Linv[3 + k] + r[2] * P.Linv[6 + k];
+    double fl = floor(f);
+    sh[k] = (int)fl;
+    int bk = (int)floor((f - fl) * P.nb[k]);
+    b[k] = bk < 0 ? 0 : (bk >= P.nb[k] ? P.nb[k] - 1 : bk);
+  }
+  for (int c = 0; c < 3; ++c) {
+    nb.wpos[3 * i + c] = r[c] - sh[0] * P.L[c] - sh[1] * P.L[3 + c] - sh[2] * P.L[6 + c];
+    nb.shift[3 * i + c] = sh[c];
+  }
+  int bin = P.bin_base + (b[0] * P.nb[1] + b[1]) * P.nb[2] + b[2];
+  nb.atom_bin[i] = bin;
+  atomicAdd(&nb.bin_count[bin], 1);
+}
+
+// Exclusive scan of n ints by one workgroup of 1024 threads; out[n] = total.
+__global__ __launch_bounds__(1024) void k_scan(const int *__restrict__ in, int *__restrict__ out, int n,
+                                               int *total) {
+  __shared__ long long part[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int lo = t * per, hi = min(n, lo + per);
+  long long s = 0;
+  for (int k = lo; k < hi; ++k) s += in[k];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    long long v = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  long long run = part[t] - s;  // exclusive prefix of this thread's segment
+  for (int k = lo; k < hi; ++k) {
+    out[k] = (int)run;
+    run += in[k];
+  }
+  if (t == 1023) {
+    out[n] = (int)part[1023];
+    if (total) total[0] = (int)part[1023];
+  }
+}
+
+__global__ void k_scatter(int N, NeighborBufs nb) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int b = nb.atom_bin[i];
+  int slot = nb.bin_start[b] + atomicAdd(&nb.bin_cursor[b], 1);
+  nb.bin_atoms[slot] = i;
+}
+
+// Sort each bin's atoms by index (insertion sort; bins hold tens of atoms).
+__global__ void k_sort_bins(int nbins, NeighborBufs nb) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  int lo = nb.bin_start[b], hi = nb.bin_start[b + 1];
+  for (int p = lo + 1; p < hi; ++p) {
+    int v = nb.bin_atoms[p];
+    int q = p - 1;
+    while (q >= lo && nb.bin_atoms[q] > v) {
+      nb.bin_atoms[q + 1] = nb.bin_atoms[q];
+      --q;
+    }
+    nb.bin_atoms[q + 1] = v;
+  }
+}
+
+// Visit every candidate (j, image) of atom i in a fixed order; fn(j, t, d, r)
+// is called for each one within rcut.
+template <typename Fn>
+__device__ inline void for_each_neighbor(int i, const StructParams &P, const NeighborBufs &nb, double rcut,
+                                         Fn fn) {
+  int b = nb.atom_bin[i] - P.bin_base;
+  int bi[3];
+  bi[2] = b % P.nb[2];
+  bi[1] = (b / P.nb[2]) % P.nb[1];
+  bi[0] = b / (P.nb[1] * P.nb[2]);
+  const double xi = nb.wpos[3 * i], yi = nb.wpos[3 * i + 1], zi = nb.wpos[3 * i + 2];
+  for (int o0 = -P.S[0]; o0 <= P.S[0]; ++o0) {
+    int bb0 = bi[0] + o0, t0 = floordiv(bb0, P.nb[0]), w0 = bb0 - t0 * P.nb[0];
+    for (int o1 = -P.S[1]; o1 <= P.S[1]; ++o1) {
+      int bb1 = bi[1] + o1, t1 = floordiv(bb1, P.nb[1]), w1 = bb1 - t1 * P.nb[1];
+      for (int o2 = -P.S[2]; o2 <= P.S[2]; ++o2) {
+        int bb2 = bi[2] + o2, t2 = floordiv(bb2, P.nb[2]), w2 = bb2 - t2 * P.nb[2];
+        const double sx = t0 * P.L[0] + t1 * P.L[3] + t2 * P.L[6];
+        const double sy = t0 * P.L[1] + t1 * P.L[4] + t2 * P.L[7];
+        const double sz = t0 * P.L[2] + t1 * P.L[5] + t2 * P.L[8];
+        const bool zero = (t0 == 0) && (t1 == 0) && (t2 == 0);
+        int bin = P.bin_base + (w0 * P.nb[1] + w1) * P.nb[2] + w2;
+        int lo = nb.bin_start[bin], hi = nb.bin_start[bin + 1];
+        for (int q = lo; q < hi; ++q) {
+          int j = nb.bin_atoms[q];
+          if (zero && j == i) continue;
+          double dx = nb.wpos[3 * j] + sx - xi;
+          double dy = nb.wpos[3 * j + 1] + sy - yi;
+          double dz = nb.wpos[3 * j + 2] + sz - zi;
+          double r = sqrt(dx * dx + dy * dy + dz * dz);
+          if (r > rcut) continue;
+          fn(j, t0, t1, t2, dx, dy, dz, r);
+        }
+      }
+    }
+  }
+}
+
+__global__ void k_count(int N, const int *__restrict__ atom_struct, const StructParams *__restrict__ sp,
+                        double rcut, int cap, NeighborBufs nb) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const StructParams &P = sp[atom_struct[i]];
+  int count = 0;
+  for_each_neighbor(i, P, nb, rcut, [&](int, int, int, int, double, double, double, double) { ++count; });
+  nb.nb_count[i] = count;
+  if (count > cap) atomicAdd(nb.overflow, 1);
+}
+
+__global__ void k_fill(int N, const int *__restrict__ atom_struct, const StructParams *__restrict__ sp,
+                       double rcut, NeighborBufs nb) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const StructParams &P = sp[atom_struct[i]];
+  int slot = nb.nb_start[i];
+  const int si0 = nb.shift[3 * i], si1 = nb.shift[3 * i + 1], si2 = nb.shift[3 * i + 2];
+  for_each_neighbor(i, P, nb, rcut, [&](int j, int t0, int t1, int t2, double dx, double dy, double dz, double r) {
+    nb.nb_j[slot] = j;
+    nb.nb_img[3 * slot] = t0 - nb.shift[3 * j] + si0;
+    nb.nb_img[3 * slot + 1] = t1 - nb.shift[3 * j + 1] + si1;
+    nb.nb_img[3 * slot + 2] = t2 - nb.shift[3 * j + 2] + si2;
+    nb.nb_d[4 * slot] = dx;
+    nb.nb_d[4 * slot + 1] = dy;
+    nb.nb_d[4 * slot + 2] = dz;
+    nb.nb_d[4 * slot + 3] = r;
+    ++slot;
+  });
+}
+
+// For slot (i -> j, n) find the slot (j -> i, -n); -1 if absent (a pair within
+// one rounding step of rcut can be seen from one side only).
+__global__ __launch_bounds__(64) void k_reverse(int N, NeighborBufs nb) {
+  int i = blockIdx.x;
+  if (i >= N) return;
+  int lo = nb.nb_start[i], hi = nb.nb_start[i + 1];
+  for (int s = lo + (int)threadIdx.x; s < hi; s += 64) {
+    int j = nb.nb_j[s];
+    int n0 = -nb.nb_img[3 * s], n1 = -nb.nb_img[3 * s + 1], n2 = -nb.nb_img[3 * s + 2];
+    int found = -1;
+    for (int q = nb.nb_start[j]; q < nb.nb_start[j + 1]; ++q) {
+      if (nb.nb_j[q] == i && nb.nb_img[3 * q] == n0 && nb.nb_img[3 * q + 1] == n1 && nb.nb_img[3 * q + 2] == n2) {
+        found = q;
+        break;
+      }
+    }
+    nb.nb_rev[s] = found;
+  }
+}
+
+// F_m = sum_s f^(m)_s - sum_s f^(j_s)_{rev(s)}: slot_f[s] holds dE_i/dR of the
+// atom in slot s of centre i, so the force on m collects minus its own
+// centre's derivative (the -sum over its slots) and minus what every centre
+// that sees m puts on it (the reverse slots).
+__global__ void k_gather(int N, NeighborBufs nb, const double *__restrict__ slot_f, double *__restrict__ force) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= N) return;
+  double f0 = 0, f1 = 0, f2 = 0;
+  for (int s = nb.nb_start[m]; s < nb.nb_start[m + 1]; ++s) {
+    f0 += slot_f[3 * s];
+    f1 += slot_f[3 * s + 1];
+    f2 += slot_f[3 * s + 2];
+    int r = nb.nb_rev[s];
+    if (r >= 0) {
+      f0 -= slot_f[3 * r];
+      f1 -= slot_f[3 * r + 1];
+      f2 -= slot_f[3 * r + 2];
+    }
+  }
+  force[3 * m] = f0;
+  force[3 * m + 1] = f1;
+  force[3 * m + 2] = f2;
+}
+
+// Per-structure energy and stress: deterministic tree sums over the atoms.
+__global__ __launch_bounds__(256) void k_struct_reduce(const StructParams *__restrict__ sp,
+                                                       const double *__restrict__ eatom,
+                                                       const double *__restrict__ vir, int grad,
+                                                       double *__restrict__ ene, double *__restrict__ stress6) {
+  const int s = blockIdx.x, t = threadIdx.x;
+  const StructParams &P = sp[s];
+  __shared__ double red[7][256];
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int a = t; a < P.natoms; a += 256) {
+    int i = P.atom_base + a;
+    acc[0] += eatom[i];
+    if (grad)
+      for (int c = 0; c < 6; ++c) acc[1 + c] += vir[6 * i + c];
+  }
+  for (int c = 0; c < 7; ++c) red[c][t] = acc[c];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off)
+      for (int c = 0; c < 7; ++c) red[c][t] += red[c][t + off];
+    __syncthreads();
+  }
+  if (t == 0) {
+    ene[s] = red[0][0];
+    // stress_cal = -virial / V converted to GPa (gap_calc.f90:189-203);
+    // virial order xx yy zz xy yz xz is already libgap's output order.
+    double f = -(1.0 / kGPa) / P.vol;
+    for (int c = 0; c < 6; ++c) stress6[6 * s + c] = grad ? red[1 + c][0] * f : 0.0;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_fill_int(hipStream_t s, int *p, int n, int v) {
+  if (n <= 0) return hipSuccess;
+  k_fill_int<<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_double(hipStream_t s, double *p, size_t n, double v) {
+  if (n == 0) return hipSuccess;
+  k_fill_double<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_wrap_bin(hipStream_t s, int N, const double *pos, const int *atom_struct,
+                           const StructParams *sp, NeighborBufs nb) {
+  k_wrap_bin<<<(N + 255) / 256, 256, 0, s>>>(N, pos, atom_struct, sp, nb);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(hipStream_t s, const int *in, int *out, int n, int *total) {
+  k_scan<<<1, 1024, 0, s>>>(in, out, n, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_bins(hipStream_t s, int N, int nbins, NeighborBufs nb) {
+  k_scatter<<<(N + 255) / 256, 256, 0, s>>>(N, nb);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_sort_bins<<<(nbins + 255) / 256, 256, 0, s>>>(nbins, nb);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_neighbors(hipStream_t s, int N, const int *atom_struct, const StructParams *sp,
+                                  double rcut, int cap, NeighborBufs nb) {
+  k_count<<<(N + 127) / 128, 128, 0, s>>>(N, atom_struct, sp, rcut, cap, nb);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_neighbors(hipStream_t s, int N, const int *atom_struct, const StructParams *sp,
+                                 double rcut, NeighborBufs nb) {
+  k_fill<<<(N + 127) / 128, 128, 0, s>>>(N, atom_struct, sp, rcut, nb);
+  return hipGetLastError();
+}
+
+hipError_t launch_reverse_slots(hipStream_t s, int N, NeighborBufs nb) {
+  k_reverse<<<N, 64, 0, s>>>(N, nb);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_forces(hipStream_t s, int N, const NeighborBufs &nb, const double *slot_f,
+                                double *force) {
+  k_gather<<<(N + 255) / 256, 256, 0, s>>>(N, nb, slot_f, force);
+  return hipGetLastError();
+}
+
+hipError_t launch_struct_reduce(hipStream_t s, int nstruct, const StructParams *sp, const double *eatom,
+                                const double *atom_vir, int grad, double *ene, double *stress6) {
+  k_struct_reduce<<<nstruct, 256, 0, s>>>(sp, eatom, atom_vir, grad, ene, stress6);
+  return hipGetLastError();
+}
+
+}  // namespace gapx

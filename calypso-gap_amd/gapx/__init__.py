@@ -1,0 +1,288 @@
+"""Python binding of the gapx C ABI (include/gapx.h) over ctypes.
+
+``gapx`` loads ``calypso-gap_amd/lib/libgapx.so`` (built in-tree by
+``calypso-gap_amd/build.py``) and raises ``GapxError`` when it is missing:
+there is no CPU fallback.
+
+    model = gapx.Model.read("gap_parameters")           # or Model.from_arrays(...)
+    e, f, s, var = model.calc(species, lat, pos, rcut=6.0)   # FGAP_CALC semantics
+
+    ctx = gapx.Context(model)                            # device-resident batch
+    ctx.load([species...], [lat...], [pos...], rcut)
+    ctx.run(gapx.GRAD | gapx.MIXED); ctx.results()
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libgapx.so")
+
+OK = 0
+GRAD = 1
+MIXED = 2
+LAYOUT_F = 4
+REF_NEIGHBOR_CAP = 8
+BF16 = 16
+
+ERR_NAMES = {1: "GAPX_ERR_ARG", 2: "GAPX_ERR_FILE", 3: "GAPX_ERR_MODEL", 4: "GAPX_ERR_SPECIES",
+             5: "GAPX_ERR_NEIGHBORS", 6: "GAPX_ERR_HIP", 7: "GAPX_ERR_NOMEM", 8: "GAPX_ERR_CAP"}
+
+# every symbol include/gapx.h declares
+EXPORTS = [
+    "gapx_version", "gapx_last_error", "gapx_device_count", "gapx_model_create", "gapx_model_read",
+    "gapx_model_dims", "gapx_model_arrays", "gapx_model_header", "gapx_model_free", "gapx_file_stamp",
+    "gapx_calc", "gapx_calc_batch", "gapx_descriptors", "gapx_ctx_create", "gapx_ctx_load",
+    "gapx_ctx_update", "gapx_ctx_run", "gapx_ctx_sync", "gapx_ctx_results", "gapx_ctx_features",
+    "gapx_ctx_timing", "gapx_ctx_stage_count", "gapx_ctx_stage_info", "gapx_ctx_use_graph",
+    "gapx_ctx_device_bytes", "gapx_ctx_free",
+]
+
+
+class GapxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (ERR_NAMES.get(code, str(code)), msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GapxError(6, "HIP library not built: %s (run calypso-gap_amd/build.py)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.gapx_last_error.restype = ctypes.c_char_p
+        L.gapx_model_create.argtypes = [ci, vp, vp, ci, vp, vp, vp, vp, ci, ci, vp, vp, ci, ci, vp, vp]
+        L.gapx_model_read.argtypes = [ctypes.c_char_p, vp]
+        L.gapx_model_dims.argtypes = [vp, vp, vp, vp, vp]
+        L.gapx_model_arrays.argtypes = [vp, vp, vp, ci, ci, vp]
+        L.gapx_model_header.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.gapx_model_free.argtypes = [vp]
+        L.gapx_model_free.restype = None
+        L.gapx_file_stamp.argtypes = [ctypes.c_char_p, vp, vp]
+        L.gapx_calc.argtypes = [vp, ci, vp, vp, vp, cd, ci, vp, vp, vp, vp]
+        L.gapx_calc_batch.argtypes = [vp, ci, vp, vp, vp, vp, cd, ci, vp, vp, vp]
+        L.gapx_descriptors.argtypes = [vp, ci, vp, vp, vp, cd, ci, vp]
+        L.gapx_ctx_create.argtypes = [vp, vp]
+        L.gapx_ctx_load.argtypes = [vp, ci, vp, vp, vp, vp, cd]
+        L.gapx_ctx_update.argtypes = [vp, vp, vp]
+        L.gapx_ctx_run.argtypes = [vp, ci]
+        L.gapx_ctx_sync.argtypes = [vp]
+        L.gapx_ctx_results.argtypes = [vp, vp, vp, vp, vp]
+        L.gapx_ctx_features.argtypes = [vp, vp, vp]
+        L.gapx_ctx_timing.argtypes = [vp, ci]
+        L.gapx_ctx_stage_count.argtypes = [vp, vp]
+        L.gapx_ctx_stage_info.argtypes = [vp, ci, vp, vp, vp]
+        L.gapx_ctx_use_graph.argtypes = [vp, ci]
+        L.gapx_ctx_device_bytes.argtypes = [vp, vp]
+        L.gapx_ctx_free.argtypes = [vp]
+        L.gapx_ctx_free.restype = None
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != OK:
+        raise GapxError(rc, lib().gapx_last_error().decode(errors="replace"))
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    _check(lib().gapx_device_count(ctypes.byref(n)))
+    return n.value
+
+
+class Model:
+    """Device-resident GAP model (species weights, SF table, theta, MM, coeff)."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle)
+        ns, nsf, D, M = (ctypes.c_int() for _ in range(4))
+        _check(lib().gapx_model_dims(self._h, ctypes.byref(ns), ctypes.byref(nsf), ctypes.byref(D),
+                                     ctypes.byref(M)))
+        self.nspecies, self.nsf, self.des_len, self.nsparse = ns.value, nsf.value, D.value, M.value
+
+    @classmethod
+    def read(cls, path):
+        h = ctypes.c_void_p()
+        _check(lib().gapx_model_read(os.fsencode(path), ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_arrays(cls, z, w, sf_type, sf_alpha, sf_cut, theta, mm, coeff, sf_zeta=None):
+        z = np.ascontiguousarray(z, np.int32)
+        w = np.ascontiguousarray(w, np.float64)
+        t = np.ascontiguousarray(sf_type, np.int32)
+        a = np.ascontiguousarray(sf_alpha, np.float64)
+        c = np.ascontiguousarray(sf_cut, np.float64)
+        zt = None if sf_zeta is None else np.ascontiguousarray(sf_zeta, np.float64)
+        th = np.ascontiguousarray(theta, np.float64)
+        mm = np.ascontiguousarray(mm, np.float64)
+        cf = np.ascontiguousarray(coeff, np.float64)
+        h = ctypes.c_void_p()
+        _check(lib().gapx_model_create(len(z), _p(z), _p(w), len(t), _p(t), _p(a), _p(c), _p(zt), mm.shape[1],
+                                       mm.shape[0], _p(th), _p(mm), 0, 0, _p(cf), ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_model(cls, m):
+        """From any object with the oracle-style fields (z, w, sf_type, ...)."""
+        zeta = getattr(m, "sf_zeta", None)
+        if isinstance(m, dict):
+            return cls.from_arrays(m["z"], m["w"], m["sf_type"], m["sf_alpha"], m["sf_cut"], m["theta"], m["mm"],
+                                   m["coeff"], m.get("sf_zeta"))
+        return cls.from_arrays(m.z, m.w, m.sf_type, m.sf_alpha, m.sf_cut, m.theta, m.mm, m.coeff, zeta)
+
+    def arrays(self):
+        th = np.zeros(self.des_len)
+        mm = np.zeros((self.nsparse, self.des_len))
+        cf = np.zeros(self.nsparse)
+        _check(lib().gapx_model_arrays(self._h, _p(th), _p(mm), 0, 0, _p(cf)))
+        return th, mm, cf
+
+    def header(self):
+        z = np.zeros(self.nspecies, np.int32)
+        w = np.zeros(self.nspecies)
+        t = np.zeros(self.nsf, np.int32)
+        a, c, zt = np.zeros(self.nsf), np.zeros(self.nsf), np.zeros(self.nsf)
+        _check(lib().gapx_model_header(self._h, _p(z), _p(w), _p(t), _p(a), _p(c), _p(zt)))
+        return dict(z=z, w=w, sf_type=t, sf_alpha=a, sf_cut=c, sf_zeta=zt)
+
+    def calc(self, species, lat, pos, rcut=6.0, grad=True, mixed=False, flags=0):
+        """FGAP_CALC: (energy eV, forces (N,3) eV/A, stress (6,) GPa, variance)."""
+        species = np.ascontiguousarray(species, np.int32)
+        lat = np.ascontiguousarray(lat, np.float64)
+        pos = np.ascontiguousarray(pos, np.float64)
+        na = len(species)
+        ene = np.zeros(1)
+        force = np.zeros((na, 3))
+        stress = np.zeros(6)
+        var = np.zeros(1)
+        fl = flags | (GRAD if grad else 0) | (MIXED if mixed else 0)
+        _check(lib().gapx_calc(self._h, na, _p(species), _p(lat), _p(pos), float(rcut), fl, _p(ene), _p(force),
+                               _p(stress), _p(var)))
+        return float(ene[0]), force, stress, float(var[0])
+
+    def calc_batch(self, structures, rcut=6.0, grad=True, mixed=False, flags=0):
+        natoms = np.array([len(s["species"]) for s in structures], np.int32)
+        species = np.ascontiguousarray(np.concatenate([s["species"] for s in structures]), np.int32)
+        lat = np.ascontiguousarray(np.stack([s["lat"] for s in structures]), np.float64)
+        pos = np.ascontiguousarray(np.concatenate([s["pos"] for s in structures]), np.float64)
+        ene = np.zeros(len(structures))
+        force = np.zeros((int(natoms.sum()), 3))
+        stress = np.zeros((len(structures), 6))
+        fl = flags | (GRAD if grad else 0) | (MIXED if mixed else 0)
+        _check(lib().gapx_calc_batch(self._h, len(structures), _p(natoms), _p(species), _p(lat), _p(pos),
+                                     float(rcut), fl, _p(ene), _p(force), _p(stress)))
+        offs = np.concatenate([[0], np.cumsum(natoms)])
+        return ene, [force[offs[k]:offs[k + 1]] for k in range(len(structures))], stress
+
+    def descriptors(self, species, lat, pos, rcut=6.0):
+        species = np.ascontiguousarray(species, np.int32)
+        x = np.zeros((len(species), self.des_len))
+        _check(lib().gapx_descriptors(self._h, len(species), _p(species), _p(np.ascontiguousarray(lat, np.float64)),
+                                      _p(np.ascontiguousarray(pos, np.float64)), float(rcut), 0, _p(x)))
+        return x
+
+    def close(self):
+        if self._h:
+            lib().gapx_model_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """Device-resident batch of structures and its workspace."""
+
+    def __init__(self, model):
+        self.model = model
+        h = ctypes.c_void_p()
+        _check(lib().gapx_ctx_create(model._h, ctypes.byref(h)))
+        self._h = h
+        self.nstruct = 0
+        self.natoms = None
+
+    def load(self, structures, rcut=6.0):
+        self.natoms = np.array([len(s["species"]) for s in structures], np.int32)
+        species = np.ascontiguousarray(np.concatenate([s["species"] for s in structures]), np.int32)
+        lat = np.ascontiguousarray(np.stack([s["lat"] for s in structures]), np.float64)
+        pos = np.ascontiguousarray(np.concatenate([s["pos"] for s in structures]), np.float64)
+        self.nstruct = len(structures)
+        _check(lib().gapx_ctx_load(self._h, self.nstruct, _p(self.natoms), _p(species), _p(lat), _p(pos),
+                                   float(rcut)))
+
+    def update(self, pos, lat=None):
+        pos = np.ascontiguousarray(pos, np.float64)
+        lat = None if lat is None else np.ascontiguousarray(lat, np.float64)
+        _check(lib().gapx_ctx_update(self._h, _p(lat), _p(pos)))
+
+    def run(self, flags=GRAD):
+        _check(lib().gapx_ctx_run(self._h, int(flags)))
+
+    def sync(self):
+        _check(lib().gapx_ctx_sync(self._h))
+
+    def results(self):
+        n = int(self.natoms.sum())
+        ene = np.zeros(self.nstruct)
+        force = np.zeros((n, 3))
+        stress = np.zeros((self.nstruct, 6))
+        eatom = np.zeros(n)
+        _check(lib().gapx_ctx_results(self._h, _p(ene), _p(force), _p(stress), _p(eatom)))
+        return dict(energy=ene, forces=force, stress=stress, eatom=eatom)
+
+    def features(self, dedx=True):
+        n = int(self.natoms.sum())
+        x = np.zeros((n, self.model.des_len))
+        g = np.zeros((n, self.model.des_len)) if dedx else None
+        _check(lib().gapx_ctx_features(self._h, _p(x), _p(g)))
+        return x, g
+
+    def timing(self, enable=True):
+        _check(lib().gapx_ctx_timing(self._h, int(enable)))
+
+    def stages(self):
+        n = ctypes.c_int()
+        _check(lib().gapx_ctx_stage_count(self._h, ctypes.byref(n)))
+        out = {}
+        for k in range(n.value):
+            name = ctypes.c_char_p()
+            ms = ctypes.c_double()
+            cnt = ctypes.c_int()
+            _check(lib().gapx_ctx_stage_info(self._h, k, ctypes.byref(name), ctypes.byref(ms), ctypes.byref(cnt)))
+            out[name.value.decode()] = (ms.value, cnt.value)
+        return out
+
+    def use_graph(self, on=True):
+        _check(lib().gapx_ctx_use_graph(self._h, int(on)))
+
+    def device_bytes(self):
+        b = ctypes.c_longlong()
+        _check(lib().gapx_ctx_device_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    def close(self):
+        if self._h:
+            lib().gapx_ctx_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

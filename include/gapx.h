@@ -1,0 +1,143 @@
+/*
+ * gapx.h -- C ABI of the MI355X (gfx950) GAP energy/force/stress evaluator.
+ *
+ * Every entry point takes plain pointers and sizes.  Host arrays belong to the
+ * caller; device memory belongs to the handle that allocated it.  A handle is
+ * not thread-safe; separate handles are independent.  Every function returns
+ * GAPX_OK (0) or a GAPX_ERR_* code; gapx_last_error() gives the message of
+ * the calling thread's last failure.
+ *
+ * Units and conventions are libgap's (gappy/README.md:43-45,
+ * gappy/libgap/gap_calc.f90:1-46): positions and cell in Angstrom, energy in
+ * eV, forces in eV/Angstrom, stress in GPa in the order xx yy zz xy yz zx with
+ * VASP's sign (positive = compressive load on the cell, the negative of ASE's
+ * stress).  Lattice vectors are the rows of LAT (gap_calc.f90:98).
+ *
+ * Array layouts:
+ *   GAPX_LAYOUT_C (default)  lat[3*k + c] = component c of vector k,
+ *                            pos[3*i + c], force[3*i + c]
+ *   GAPX_LAYOUT_F            Fortran LAT(3,3), POS(NA,3), FORCE(NA,3)
+ *                            (column-major: pos[i + NA*c])
+ */
+#ifndef GAPX_H
+#define GAPX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define GAPX_OK 0
+#define GAPX_ERR_ARG 1        /* invalid argument or size                      */
+#define GAPX_ERR_FILE 2       /* model file missing or unreadable              */
+#define GAPX_ERR_MODEL 3      /* inconsistent model (des_len != 2*nsf, ...)     */
+#define GAPX_ERR_SPECIES 4    /* atomic number not listed in the model header   */
+#define GAPX_ERR_NEIGHBORS 5  /* neighbour count above the configured cap       */
+#define GAPX_ERR_HIP 6        /* HIP runtime failure                           */
+#define GAPX_ERR_NOMEM 7      /* device or host allocation failed              */
+#define GAPX_ERR_CAP 8        /* FGAP_READ fixed-size caps exceeded             */
+
+/* ---- flags -------------------------------------------------------------- */
+#define GAPX_GRAD 1           /* lgrad: forces and stress (else energy only)    */
+#define GAPX_MIXED 2          /* FP32 descriptors + FP16 hi/lo MFMA contraction */
+#define GAPX_LAYOUT_F 4       /* Fortran column-major lat/pos/force             */
+#define GAPX_REF_NEIGHBOR_CAP 8 /* fail above 1000 neighbours like the reference */
+#define GAPX_BF16 16          /* with GAPX_MIXED: BF16 hi/lo instead of FP16     */
+
+typedef struct gapx_model gapx_model;
+typedef struct gapx_ctx gapx_ctx;
+
+int gapx_version(void);
+const char *gapx_last_error(void);
+/* Number of visible GPUs (0 when no GPU or no driver). */
+int gapx_device_count(int *n);
+
+/* ---- model ----------------------------------------------------------------
+ * gapx_model_create replaces the model half of FGAP_CALC's inputs
+ * (gap_calc.f90:1-46: theta, MM, coeff) together with the header FGAP_CALC
+ * and CAR2ACSF re-read from ./gap_parameters on every call
+ * (gap_calc.f90:75-83 species weights, wacsf.f90:40-56 SF table).
+ * mm_layout: 0 = row-major [nsparse][des_len]; 1 = Fortran MM(nsparse,des_len)
+ * with leading dimension ld_mm (>= nsparse).  sf_zeta may be NULL (all 1).
+ * The model is uploaded once and stays resident on the current device. */
+int gapx_model_create(int nspecies, const int *z, const double *w, int nsf, const int *sf_type,
+                      const double *sf_alpha, const double *sf_cut, const double *sf_zeta,
+                      int des_len, int nsparse, const double *theta, const double *mm,
+                      int mm_layout, int ld_mm, const double *coeff, gapx_model **out);
+
+/* Parse a gap_parameters file (format of gap_calc.f90:303-364 and
+ * wacsf.f90:40-56, read with Fortran list-directed rules) with no fixed caps;
+ * replaces FGAP_READ (gap_calc.f90:303) plus the header reads. */
+int gapx_model_read(const char *path, gapx_model **out);
+
+int gapx_model_dims(const gapx_model *m, int *nspecies, int *nsf, int *des_len, int *nsparse);
+
+/* Copy the model's GPR arrays back (FGAP_READ outputs).  mm_layout/ld_mm as
+ * in gapx_model_create.  Any pointer may be NULL. */
+int gapx_model_arrays(const gapx_model *m, double *theta, double *mm, int mm_layout, int ld_mm,
+                      double *coeff);
+
+/* Model header (species table and SF lines); arrays sized by gapx_model_dims. */
+int gapx_model_header(const gapx_model *m, int *z, double *w, int *sf_type, double *sf_alpha,
+                      double *sf_cut, double *sf_zeta);
+
+void gapx_model_free(gapx_model *m);
+
+/* Size and modification time of a file, for callers that cache models keyed
+ * on the file (the reference re-reads ./gap_parameters on every call). */
+int gapx_file_stamp(const char *path, long long *size, long long *mtime_ns);
+
+/* ---- one-shot evaluation ---------------------------------------------------
+ * FGAP_CALC (gap_calc.f90:1-301): energy, forces, stress and variance (always
+ * 0, gap_calc.f90:206) of one periodic structure.  rcut is the neighbour
+ * cutoff (per-line cutoffs above it are clipped by the neighbour list, as in
+ * the reference).  Uses a workspace cached inside the model handle. */
+int gapx_calc(gapx_model *m, int na, const int *species, const double *lat, const double *pos,
+              double rcut, int flags, double *ene, double *force, double *stress6,
+              double *variance);
+
+/* Batched FGAP_CALC over nstruct independent structures packed back to back
+ * (natoms[s] atoms each; species/pos concatenated; lat is nstruct*9).
+ * Always GAPX_LAYOUT_C.  ene[nstruct], force[3*sum(natoms)], stress6[6*nstruct]. */
+int gapx_calc_batch(gapx_model *m, int nstruct, const int *natoms, const int *species,
+                    const double *lat, const double *pos, double rcut, int flags, double *ene,
+                    double *force, double *stress6);
+
+/* Descriptors only (CAR2ACSF's xx, wacsf.f90:2-12; FP64): x[na*des_len],
+ * row = atom, column order = the SF table's plain features then weighted. */
+int gapx_descriptors(gapx_model *m, int na, const int *species, const double *lat,
+                     const double *pos, double rcut, int flags, double *x);
+
+/* ---- device-resident contexts ----------------------------------------------
+ * A context owns a HIP stream and device buffers for a batch of structures.
+ * Load once, then run repeatedly on the inputs already in HBM; results stay
+ * on the device until fetched. */
+int gapx_ctx_create(gapx_model *m, gapx_ctx **out);
+int gapx_ctx_load(gapx_ctx *c, int nstruct, const int *natoms, const int *species,
+                  const double *lat, const double *pos, double rcut);
+/* New positions (and cells, may be NULL) for the loaded batch, same sizes. */
+int gapx_ctx_update(gapx_ctx *c, const double *lat, const double *pos);
+/* Enqueue one full evaluation on the context's stream (asynchronous). */
+int gapx_ctx_run(gapx_ctx *c, int flags);
+int gapx_ctx_sync(gapx_ctx *c);
+/* Fetch results of the last run (synchronises).  Any pointer may be NULL.
+ * eatom gives the per-atom (local) energies. */
+int gapx_ctx_results(gapx_ctx *c, double *ene, double *force, double *stress6, double *eatom);
+/* Per-atom descriptors (raw, FP64 or FP32 run) and dE/dx of the last run. */
+int gapx_ctx_features(gapx_ctx *c, double *x, double *dedx);
+/* Stage timing with HIP events on the context's stream: enable, then read
+ * the summed milliseconds and launch counts per stage since the last reset. */
+int gapx_ctx_timing(gapx_ctx *c, int enable);
+int gapx_ctx_stage_count(gapx_ctx *c, int *n);
+int gapx_ctx_stage_info(gapx_ctx *c, int k, const char **name, double *ms, int *launches);
+/* Capture gapx_ctx_run(flags) into a HIP graph and replay it on later runs
+ * with the same flags (on = 1), or go back to eager launches (on = 0). */
+int gapx_ctx_use_graph(gapx_ctx *c, int on);
+/* Bytes of device memory held by the context. */
+int gapx_ctx_device_bytes(gapx_ctx *c, long long *bytes);
+void gapx_ctx_free(gapx_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
