@@ -18,18 +18,18 @@ def small_model(rng, zeta=(1, 2, 4), M=12, nrad=4, theta_scale=1.0, cuts=(6.0, 4
     zl += [1.0, 1.0]
     for eta in (0.01, 0.1):
         for t in (2, 4):
-            for z in zeta:
+            for zz in zeta:
                 types.append(t)
                 alphas.append(eta)
                 cutl.append(cuts[0])
-                zl.append(float(z))
+                zl.append(float(zz))
     nsf = len(types)
     D = 2 * nsf
     xs = rng.normal(size=(M, D))
     m = oracle.Model(z=np.array(z, np.int32), w=np.array(w, float),
                      sf_type=np.array(types, np.int32), sf_alpha=np.array(alphas), sf_cut=np.array(cutl),
                      theta=np.ones(D), mm=np.zeros((M, D)), coeff=rng.normal(0, 0.5, M),
-                     sf_zeta=np.array(zl) if any(z != 1 for z in zl) else None)
+                     sf_zeta=np.array(zl) if any(v != 1 for v in zl) else None)
     return m, xs
 
 
