@@ -613,6 +613,10 @@ static int ctx_run(Ctx &c, int flags) {
   stage_end(c, kForces);
   c.last_flags = flags;
   c.ran = true;
+  if (c.timing) {  // one event pair per stage: read them before the next run
+    GAPX_HIP(hipStreamSynchronize(c.stream));
+    collect_timing(c);
+  }
   return GAPX_OK;
 }
 
@@ -620,6 +624,9 @@ static int ctx_results(Ctx &c, double *ene, double *force, double *stress6, doub
   if (!c.ran) return fail(GAPX_ERR_ARG, "no evaluation has run on this context");
   GAPX_HIP(hipStreamSynchronize(c.stream));
   collect_timing(c);
+  int guard = 0;
+  GAPX_HIP(hipMemcpy(&guard, c.ptr<int>("stats") + 3, sizeof(int), hipMemcpyDeviceToHost));
+  if (guard) return fail(GAPX_ERR_NEIGHBORS, "descriptor kernels saw more neighbours than the LDS layout holds");
   if (ene) GAPX_HIP(hipMemcpy(ene, c.ptr<double>("ene"), sizeof(double) * c.nstruct, hipMemcpyDeviceToHost));
   if (stress6) GAPX_HIP(hipMemcpy(stress6, c.ptr<double>("stress"), sizeof(double) * 6 * c.nstruct, hipMemcpyDeviceToHost));
   if (eatom) GAPX_HIP(hipMemcpy(eatom, c.ptr<double>("eatom"), sizeof(double) * c.N, hipMemcpyDeviceToHost));

@@ -191,6 +191,10 @@ __global__ __launch_bounds__(64) void k_desc_fwd(int N, const int *__restrict__ 
 
   const int start = nb.nb_start[i];
   const int K = nb.nb_start[i + 1] - start;
+  if (K > KP) {  // the host sizes KP from the largest count; never index LDS past it
+    if (lane == 0) atomicAdd(&nb.total[3], 1);
+    return;
+  }
   stage_neighbors<T>(start, K, lane, nb, species, wz, sf, KP, s_d, s_w, s_fc, (T *)nullptr);
   for (int d = lane; d < Dp; d += 64) s_feat[d] = T(0);
   __syncthreads();
@@ -406,6 +410,10 @@ __global__ __launch_bounds__(64) void k_desc_bwd(int N, const int *__restrict__ 
 
   const int start = nb.nb_start[i];
   const int K = nb.nb_start[i + 1] - start;
+  if (K > KP) {
+    if (lane == 0) atomicAdd(&nb.total[3], 1);
+    return;
+  }
   stage_neighbors<T>(start, K, lane, nb, species, wz, sf, KP, s_d, s_w, s_fc, s_dfc);
   const size_t row = (size_t)i * Dp;
   for (int d = lane; d < Dp; d += 64) s_g[d] = g64 ? (T)g64[row + d] : (T)g32[row + d];

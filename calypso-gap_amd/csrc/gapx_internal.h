@@ -110,8 +110,8 @@ struct NeighborBufs {
   int *nb_img;          // [cap*3] image relative to given positions
   double *nb_d;         // [cap*4] dx dy dz r
   int *nb_rev;          // [cap]
-  int *overflow;        // [1] atoms above the neighbour cap
-  int *total;           // [1] total neighbours
+  int *overflow;        // [1] atoms above the neighbour cap (= total + 2)
+  int *total;           // [4] total neighbours, largest count, overflow, guard
 };
 
 hipError_t launch_wrap_bin(hipStream_t s, int N, const double *pos, const int *atom_struct,

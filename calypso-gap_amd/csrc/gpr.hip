@@ -26,7 +26,8 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int kPC = 32;  // sparse points per LDS chunk
 
 // FP64: v_mfma_f64_16x16x4_f64.  Operand layout per lane l (l16 = l % 16,
-// lq = l / 16): A[l16][lq], B[lq][l16]; result D[4 lq + r][l16], r = 0..3.
+// lq = l / 16): A[l16][lq], B[lq][l16]; result D[lq + 4 r][l16], r = 0..3
+// (rows strided by 4, unlike the FP16/BF16 forms; measured on gfx950).
 template <int DP>
 __global__ __launch_bounds__(256) void k_gpr_f64(GprArgs a) {
   __shared__ double s_m[kPC][DP + 4];
@@ -44,7 +45,7 @@ __global__ __launch_bounds__(256) void k_gpr_f64(GprArgs a) {
   for (int t = 0; t < DP / 4; ++t) xa[t] = a.xt64[(size_t)(row0 + l16) * DP + 4 * t + lq];
   double xnr[4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) xnr[r] = a.xn[row0 + 4 * lq + r];
+  for (int r = 0; r < 4; ++r) xnr[r] = a.xn[row0 + lq + 4 * r];
 
   v4d G[DP / 16];
 #pragma unroll
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(256) void k_gpr_f64(GprArgs a) {
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int rr = 4 * lq + r;
+      const int rr = lq + 4 * r;
       const double w0 = exp(-0.5 * (xnr[r] + s_mn[l16] - 2.0 * S0[r])) * s_c[l16];
       const double w1 = exp(-0.5 * (xnr[r] + s_mn[16 + l16] - 2.0 * S1[r])) * s_c[16 + l16];
       er[r] += w0 + w1;
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(256) void k_gpr_f64(GprArgs a) {
 #pragma unroll
     for (int b = 0; b < DP / 16; ++b)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) a.Gpart[(base + row0 + 4 * lq + r) * DP + 16 * b + l16] = G[b][r];
+      for (int r = 0; r < 4; ++r) a.Gpart[(base + row0 + lq + 4 * r) * DP + 16 * b + l16] = G[b][r];
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -103,7 +104,7 @@ __global__ __launch_bounds__(256) void k_gpr_f64(GprArgs a) {
     v += __shfl_xor(v, 2, 64);
     v += __shfl_xor(v, 4, 64);
     v += __shfl_xor(v, 8, 64);
-    if (l16 == 0) a.Epart[base + row0 + 4 * lq + r] = v;
+    if (l16 == 0) a.Epart[base + row0 + lq + 4 * r] = v;
   }
 }
 

@@ -152,6 +152,7 @@ __global__ void k_count(int N, const int *__restrict__ atom_struct, const Struct
   int count = 0;
   for_each_neighbor(i, P, nb, rcut, [&](int, int, int, int, double, double, double, double) { ++count; });
   nb.nb_count[i] = count;
+  atomicMax(&nb.total[1], count);
   if (count > cap) atomicAdd(nb.overflow, 1);
 }
 

@@ -29,6 +29,16 @@ def check(res, ref, n, te=TOL_E, tf=TOL_F, ts=TOL_S):
     assert de < te and df < tf and ds < ts, (de, df, ds)
 
 
+def test_descriptors_match_oracle():
+    rng = np.random.default_rng(21)
+    s = ternary_structure(150, 11.0, 211)
+    om = fitted_small_model(rng, s, zeta=(1, 2, 4, 16))
+    m = gapx.Model.from_model(om)
+    x = m.descriptors(s["species"], s["lat"], s["pos"], 6.0)
+    xo = oracle.descriptors(om, s["species"], s["lat"], s["pos"], 6.0)
+    np.testing.assert_allclose(x, xo, rtol=1e-11, atol=1e-13)
+
+
 def test_ase_traj_frames(frames):
     import os
     from conftest import GOLDEN
@@ -49,16 +59,6 @@ def test_reference_vectors(ref_vectors, case):
     check(res, c, len(c["species"]))
     if not c["lgrad"]:
         assert not np.any(res[1]) and not np.any(res[2])
-
-
-def test_descriptors_match_oracle():
-    rng = np.random.default_rng(21)
-    s = ternary_structure(150, 11.0, 211)
-    om = fitted_small_model(rng, s, zeta=(1, 2, 4, 16))
-    m = gapx.Model.from_model(om)
-    x = m.descriptors(s["species"], s["lat"], s["pos"], 6.0)
-    xo = oracle.descriptors(om, s["species"], s["lat"], s["pos"], 6.0)
-    np.testing.assert_allclose(x, xo, rtol=1e-11, atol=1e-13)
 
 
 @pytest.mark.parametrize("n,vpa,seed", [(40, 10.0, 1), (300, 12.0, 2), (7, 13.0, 3)])
