@@ -502,7 +502,9 @@ static int gpr_layout(Ctx &c, GprArgs &g) {
   const int tiles = c.Np / 64;
   int splits = std::max(1, (1024 + tiles - 1) / tiles);
   splits = std::min(splits, m.Mp / 32);
-  int per = round_up((m.Mp + splits - 1) / splits, 32);
+  // 64 points per split piece: a multiple of both kernels' LDS chunks
+  splits = std::min(splits, m.Mp / 64);
+  int per = round_up((m.Mp + splits - 1) / splits, 64);
   splits = (m.Mp + per - 1) / per;
   g.splits = splits;
   g.pts_per_split = per;
@@ -550,6 +552,8 @@ static int ctx_run(Ctx &c, int flags) {
       GAPX_HIP(launch_fill_int(c.stream, (int *)(o.xt_lo + (size_t)c.N * Dp), (c.Np - c.N) * Dp / 2, 0));
     }
   }
+  if ((rc = c.get("counters", 2, &o.triplets))) return rc;
+  GAPX_HIP(launch_fill_int(c.stream, (int *)o.triplets, 4, 0));
   const int split_mode = fp64 ? 0 : ((flags & GAPX_BF16) ? 2 : 1);
   stage_begin(c, kDescFwd);
   GAPX_HIP(launch_descriptors_fwd(c.stream, fp64, c.N, d_species, m.d_wz, nb, m.sfd, m.d_theta_inv, m.d_mu,
@@ -1080,6 +1084,18 @@ int gapx_ctx_use_graph(gapx_ctx *c, int on) {
   if (!c) return fail(GAPX_ERR_ARG, "null context");
   c->use_graph = on != 0;
   if (!on) c->invalidate_graph();
+  return GAPX_OK;
+}
+
+int gapx_ctx_counts(gapx_ctx *c, long long *pairs, long long *triplets) {
+  if (!c || !c->ran) return fail(GAPX_ERR_ARG, "no evaluation has run on this context");
+  GAPX_HIP(hipStreamSynchronize(c->stream));
+  if (pairs) *pairs = c->nb_cap;
+  if (triplets) {
+    unsigned long long t = 0;
+    GAPX_HIP(hipMemcpy(&t, c->ptr<unsigned long long>("counters"), sizeof t, hipMemcpyDeviceToHost));
+    *triplets = (long long)t;
+  }
   return GAPX_OK;
 }
 

@@ -232,6 +232,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd(int N, const int *__restrict__ 
   // angular lines, NCH at a time, over compacted triplets
   const int npairs = K * (K - 1) / 2;
   const T rcA = (T)sf.rc_max_ang;
+  unsigned long long ntrip = 0;
   for (int c0 = 0; c0 < sf.n_ang; c0 += NCH) {
     T acc[NCH], accw[NCH];
 #pragma unroll
@@ -252,6 +253,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd(int N, const int *__restrict__ 
       const unsigned long long m = __ballot(ok);
       if (ok) s_q[qn + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)a | ((uint32_t)b << 16);
       qn += __popcll(m);
+      ntrip += __popcll(m);
       __syncthreads();
       if (qn >= 64) {
         ang_fwd_term<T, NCH>(s_q[lane], true, c0, sf, s_d, s_w, s_fc, KP, acc, accw);
@@ -280,6 +282,10 @@ __global__ __launch_bounds__(64) void k_desc_fwd(int N, const int *__restrict__ 
     }
   }
   __syncthreads();
+  if (lane == 0 && out.triplets && ntrip) {
+    const int chunks = (sf.n_ang + NCH - 1) / NCH;
+    atomicAdd(out.triplets, ntrip / (unsigned long long)chunks);
+  }
 
   // epilogue: raw features, centred/scaled features, FP64 norm, split operands
   double xn = 0.0;

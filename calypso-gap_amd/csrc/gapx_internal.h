@@ -134,6 +134,7 @@ struct DescOut {
   float *xt32;          // [N][Dp] centred scaled (FP32, mixed run)
   uint16_t *xt_hi, *xt_lo;   // [N][Dp] split operands (mixed)
   double *xn;           // [N] |xt|^2 (FP64)
+  unsigned long long *triplets;  // [1] neighbour pairs (j<k) within the angular cutoff
 };
 
 hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *species,

@@ -147,8 +147,6 @@ hipError_t launch_gpr_fp64(hipStream_t s, const GprArgs &a) {
   return hipGetLastError();
 }
 
-hipError_t launch_gpr_mixed(hipStream_t, const GprArgs &) { return hipErrorNotSupported; }
-
 hipError_t launch_gpr_finish(hipStream_t s, const GprArgs &a, bool fp64) {
   k_gpr_finish<<<a.N, 128, 0, s>>>(a, fp64 ? 1 : 0);
   return hipGetLastError();

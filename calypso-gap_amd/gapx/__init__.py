@@ -37,7 +37,7 @@ EXPORTS = [
     "gapx_calc", "gapx_calc_batch", "gapx_descriptors", "gapx_ctx_create", "gapx_ctx_load",
     "gapx_ctx_update", "gapx_ctx_run", "gapx_ctx_sync", "gapx_ctx_results", "gapx_ctx_features",
     "gapx_ctx_timing", "gapx_ctx_stage_count", "gapx_ctx_stage_info", "gapx_ctx_use_graph",
-    "gapx_ctx_device_bytes", "gapx_ctx_free",
+    "gapx_ctx_device_bytes", "gapx_ctx_free", "gapx_ctx_counts",
 ]
 
 
@@ -81,6 +81,7 @@ def lib():
         L.gapx_ctx_stage_info.argtypes = [vp, ci, vp, vp, vp]
         L.gapx_ctx_use_graph.argtypes = [vp, ci]
         L.gapx_ctx_device_bytes.argtypes = [vp, vp]
+        L.gapx_ctx_counts.argtypes = [vp, vp, vp]
         L.gapx_ctx_free.argtypes = [vp]
         L.gapx_ctx_free.restype = None
         _lib = L
@@ -270,6 +271,13 @@ class Context:
 
     def use_graph(self, on=True):
         _check(lib().gapx_ctx_use_graph(self._h, int(on)))
+
+    def counts(self):
+        """(pairs, triplets) of the last run."""
+        p = ctypes.c_longlong()
+        t = ctypes.c_longlong()
+        _check(lib().gapx_ctx_counts(self._h, ctypes.byref(p), ctypes.byref(t)))
+        return p.value, t.value
 
     def device_bytes(self):
         b = ctypes.c_longlong()

@@ -133,6 +133,9 @@ int gapx_ctx_stage_info(gapx_ctx *c, int k, const char **name, double *ms, int *
 /* Capture gapx_ctx_run(flags) into a HIP graph and replay it on later runs
  * with the same flags (on = 1), or go back to eager launches (on = 0). */
 int gapx_ctx_use_graph(gapx_ctx *c, int on);
+/* Work counts of the last run: (centre, neighbour) pairs within rcut and
+ * neighbour pairs j < k of a centre within the largest angular cutoff. */
+int gapx_ctx_counts(gapx_ctx *c, long long *pairs, long long *triplets);
 /* Bytes of device memory held by the context. */
 int gapx_ctx_device_bytes(gapx_ctx *c, long long *bytes);
 void gapx_ctx_free(gapx_ctx *c);

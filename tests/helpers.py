@@ -47,3 +47,24 @@ def fitted_small_model(rng, s, **kw):
 def ternary_structure(n, vpa, seed):
     from gapx import synth
     return synth.structure(n, n * vpa, (1, 8, 22), np.random.default_rng(seed))
+
+
+_WL_CACHE = {}
+
+
+def workload_model(name):
+    """The benchmark workload's model, with descriptors from the CPU oracle (cached)."""
+    if name in _WL_CACHE:
+        return _WL_CACHE[name]
+    from gapx import synth
+
+    def desc(header, species, lat, pos, rc):
+        t, a, c, z = header["sf"]
+        hm = oracle.Model(z=header["z"], w=header["w"], sf_type=t, sf_alpha=a, sf_cut=c,
+                          theta=np.ones(2 * len(t)), mm=np.zeros((1, 2 * len(t))), coeff=np.zeros(1),
+                          sf_zeta=None if np.all(z == 1) else z)
+        return oracle.descriptors(hm, species, lat, pos, rc, nthreads=16)
+    d = synth.workload_model(name, desc)
+    _WL_CACHE[name] = oracle.Model(z=d["z"], w=d["w"], sf_type=d["sf_type"], sf_alpha=d["sf_alpha"], sf_cut=d["sf_cut"],
+                        theta=d["theta"], mm=d["mm"], coeff=d["coeff"], sf_zeta=d["sf_zeta"])
+    return _WL_CACHE[name]

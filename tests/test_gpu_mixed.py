@@ -1,0 +1,82 @@
+"""Mixed-precision path (FP32 descriptors, FP16 or BF16 hi+lo MFMA contraction
+with FP32 accumulation, FP64 norms and reductions) against the FP64 GPU path
+and the CPU oracle.
+
+Bar: |dE| <= 1 meV/atom and |dF| <= 20 meV/A against FP64 (the stated mixed
+precision target); the stress bar here is 0.05 GPa.
+"""
+import numpy as np
+import pytest
+
+import gapx
+import oracle
+from conftest import GOLDEN_CASE_NAMES, golden_cases, golden_model
+from helpers import fitted_small_model, ternary_structure, workload_model
+
+pytestmark = pytest.mark.gpu
+
+TOL_E, TOL_F, TOL_S = 1e-3, 2e-2, 5e-2
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if gapx.device_count() == 0:
+        pytest.skip("no GPU visible")
+
+
+def errors(a, b, n):
+    return (abs(a[0] - b[0]) / n, float(np.abs(a[1] - b[1]).max()), float(np.abs(a[2] - b[2]).max()))
+
+
+@pytest.mark.parametrize("bf16", [False, True], ids=["fp16x3", "bf16x3"])
+def test_mixed_vs_fp64_ternary4k(bf16):
+    from gapx import synth
+    om = workload_model("ternary4k")
+    s = synth.workload_structure("ternary4k")
+    m = gapx.Model.from_model(om)
+    r64 = m.calc(s["species"], s["lat"], s["pos"], 6.0)
+    rmx = m.calc(s["species"], s["lat"], s["pos"], 6.0, mixed=True, flags=gapx.BF16 if bf16 else 0)
+    de, df, ds = errors(rmx, r64, len(s["species"]))
+    print("mixed %s vs fp64: dE/atom %.3e eV  dF %.3e eV/A  dS %.3e GPa" % ("bf16" if bf16 else "fp16", de, df, ds))
+    assert de <= TOL_E and df <= TOL_F and ds <= TOL_S
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASE_NAMES)
+def test_mixed_reference_vectors(ref_vectors, case):
+    c = [x for x in golden_cases(ref_vectors) if x["name"] == case][0]
+    m = gapx.Model.from_model(golden_model(ref_vectors, c["model"]))
+    r = m.calc(c["species"], c["lat"], c["pos"], c["rcut"], grad=c["lgrad"], mixed=True)
+    de, df, ds = errors(r, (c["energy"], c["forces"], c["stress"]), len(c["species"]))
+    print(case, de, df, ds)
+    # relative bar: these models have features up to 1e7 and energies of
+    # hundreds of eV, so scale the absolute bar by the force/stress magnitude
+    fs = max(1.0, float(np.abs(c["forces"]).max()))
+    ss = max(1.0, float(np.abs(c["stress"]).max()))
+    assert de <= TOL_E and df <= TOL_F * fs and ds <= TOL_S * ss
+
+
+def test_mixed_small_model_vs_oracle():
+    rng = np.random.default_rng(31)
+    s = ternary_structure(200, 12.0, 331)
+    om = fitted_small_model(rng, s, zeta=(1, 2, 4, 16))
+    m = gapx.Model.from_model(om)
+    r = m.calc(s["species"], s["lat"], s["pos"], 6.0, mixed=True)
+    ref = oracle.gap_calc(om, s["species"], s["lat"], s["pos"], 6.0, mode="sparse", nthreads=8)
+    de, df, ds = errors(r, (ref["energy"], ref["forces"], ref["stress"]), 200)
+    assert de <= TOL_E and df <= TOL_F and ds <= TOL_S
+    assert np.abs(r[1].sum(axis=0)).max() < 1e-4
+
+
+def test_mixed_batch_matches_single():
+    from gapx import synth
+    rng = np.random.default_rng(9)
+    pop = synth.population(n_struct=10, nmin=8, nmax=48, seed=99)
+    om = fitted_small_model(rng, pop[0], zeta=(1, 2), z=(1, 6, 8, 22), w=(1.0, 6.0, 8.0, 22.0))
+    m = gapx.Model.from_model(om)
+    ene, forces, stress = m.calc_batch(pop, 6.0, mixed=True)
+    for k, s in enumerate(pop):
+        e, f, st, _ = m.calc(s["species"], s["lat"], s["pos"], 6.0, mixed=True)
+        # split boundaries of the contraction differ between a batch and a
+        # single structure, so FP32 partial sums round differently
+        assert abs(e - ene[k]) < 1e-6 * max(1, abs(e))
+        np.testing.assert_allclose(forces[k], f, atol=1e-5)

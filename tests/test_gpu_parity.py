@@ -8,7 +8,7 @@ import pytest
 import gapx
 import oracle
 from conftest import GOLDEN_CASE_NAMES, golden_cases, golden_model
-from helpers import fitted_small_model, ternary_structure
+from helpers import fitted_small_model, ternary_structure, workload_model
 
 pytestmark = pytest.mark.gpu
 
@@ -73,24 +73,10 @@ def test_small_models_match_oracle(n, vpa, seed):
     assert np.abs(res[1].sum(axis=0)).max() < 1e-9
 
 
-def _workload_model(name):
-    from gapx import synth
-
-    def desc(header, species, lat, pos, rc):
-        t, a, c, z = header["sf"]
-        hm = oracle.Model(z=header["z"], w=header["w"], sf_type=t, sf_alpha=a, sf_cut=c,
-                          theta=np.ones(2 * len(t)), mm=np.zeros((1, 2 * len(t))), coeff=np.zeros(1),
-                          sf_zeta=None if np.all(z == 1) else z)
-        return oracle.descriptors(hm, species, lat, pos, rc, nthreads=16)
-    d = synth.workload_model(name, desc)
-    return oracle.Model(z=d["z"], w=d["w"], sf_type=d["sf_type"], sf_alpha=d["sf_alpha"], sf_cut=d["sf_cut"],
-                        theta=d["theta"], mm=d["mm"], coeff=d["coeff"], sf_zeta=d["sf_zeta"])
-
-
 def test_ternary4k_full_size_matches_oracle():
     """4096 atoms, D = 128, M = 4000 (the benchmark workload) against the sparse oracle."""
     from gapx import synth
-    om = _workload_model("ternary4k")
+    om = workload_model("ternary4k")
     s = synth.workload_structure("ternary4k")
     m = gapx.Model.from_model(om)
     res = m.calc(s["species"], s["lat"], s["pos"], 6.0)
