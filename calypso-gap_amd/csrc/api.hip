@@ -157,6 +157,13 @@ static int upload_model(Model &m) {
     mn[k] = n;
     cf[k] = m.coeff[k];
   }
+  // W = K c with 0 < K <= 1: scale it by a power of two that keeps |W| below
+  // 2^14 so its FP16 hi part cannot overflow and its lo part stays normal
+  double cmax = 0.0;
+  for (int k = 0; k < M; ++k) cmax = std::max(cmax, std::fabs(m.coeff[k]));
+  int e2 = (cmax > 0) ? (int)std::floor(std::log2(16384.0 / cmax)) : 10;
+  e2 = std::max(-24, std::min(e2, 10));
+  m.w_scale = std::ldexp(1.0f, e2);
   std::vector<uint16_t> hi((size_t)Mp * Dp), lo((size_t)Mp * Dp), bhi((size_t)Mp * Dp), blo((size_t)Mp * Dp);
   std::vector<uint16_t> thi((size_t)Mp * Dp), tlo((size_t)Mp * Dp), tbhi((size_t)Mp * Dp), tblo((size_t)Mp * Dp);
   for (int k = 0; k < Mp; ++k)
@@ -263,7 +270,8 @@ struct Ctx {
   std::vector<StructParams> sp;
   std::map<std::string, DevMem> mem;
   size_t bytes = 0;
-  int nb_cap = 0, KP = 8;
+  int kcap = 64, KP = 64;
+  int *host_stats = nullptr;  // pinned copy of the last run's neighbour stats
   int last_flags = -1;
   bool ran = false;
   int chunk = 16;
@@ -359,6 +367,14 @@ static int struct_params(Ctx &c) {
   return GAPX_OK;
 }
 
+// Initial slots per atom from the densest structure of the batch.
+static int initial_kcap(const Ctx &c) {
+  double kest = 0;
+  for (int s = 0; s < c.nstruct; ++s)
+    kest = std::max(kest, 4.0 / 3.0 * 3.141592653589793 * c.rcut * c.rcut * c.rcut * c.natoms[s] / c.sp[s].vol);
+  return round_up(std::max(32, (int)(1.25 * kest) + 24), 8);
+}
+
 static int ctx_load(Ctx &c, int nstruct, const int *natoms, const int *species, const double *lat,
                     const double *pos, double rcut, bool fortran) {
   if (nstruct <= 0 || !natoms || !species || !lat || !pos) return fail(GAPX_ERR_ARG, "empty batch");
@@ -411,6 +427,7 @@ static int ctx_load(Ctx &c, int nstruct, const int *natoms, const int *species, 
   GAPX_HIP(hipMemcpyAsync(d_pos, hpos.data(), sizeof(double) * 3 * c.N, hipMemcpyHostToDevice, c.stream));
   GAPX_HIP(hipMemcpyAsync(d_sp, c.sp.data(), sizeof(StructParams) * nstruct, hipMemcpyHostToDevice, c.stream));
   GAPX_HIP(hipStreamSynchronize(c.stream));
+  c.kcap = initial_kcap(c);
   c.ran = false;
   c.invalidate_graph();
   return GAPX_OK;
@@ -447,49 +464,44 @@ static void collect_timing(Ctx &c) {
   }
 }
 
-// Neighbour list up to the counts; returns total and max count (synchronises).
-static int run_neighbors(Ctx &c, int flags, Bufs &b) {
+// Largest neighbour count the run accepts (LDS layout, or the reference's cap).
+static int neighbor_cap(Ctx &c, int flags) {
+  const int lds = std::min(1024, max_neighbors_for_lds(!(flags & GAPX_MIXED), std::max(1, c.m->sf.n_groups), c.m->Dp));
+  return (flags & GAPX_REF_NEIGHBOR_CAP) ? std::min(lds, kRefNeighborCap) : lds;
+}
+
+// Neighbour list: bins, then one wavefront per atom writing up to kcap slots.
+// Nothing here waits for the device; counts above kcap are detected when the
+// results are fetched (the run is then repeated with a larger kcap).
+static int run_neighbors(Ctx &c, int flags, Bufs &b, bool grad) {
   NeighborBufs &nb = b.nb;
   int rc;
   int *stats;
+  const size_t slots = (size_t)c.N * c.kcap;
   if ((rc = c.get("bin_count", c.nbins, &nb.bin_count)) || (rc = c.get("bin_start", c.nbins + 1, &nb.bin_start)) ||
-      (rc = c.get("bin_cursor", c.nbins, &nb.bin_cursor)) || (rc = c.get("bin_atoms", c.N, &nb.bin_atoms)) ||
-      (rc = c.get("atom_bin", c.N, &nb.atom_bin)) || (rc = c.get("wpos", 3 * (size_t)c.N, &nb.wpos)) ||
-      (rc = c.get("shift", 3 * (size_t)c.N, &nb.shift)) || (rc = c.get("nb_count", c.N, &nb.nb_count)) ||
-      (rc = c.get("nb_start", c.N + 1, &nb.nb_start)) || (rc = c.get("stats", 4, &stats)))
+      (rc = c.get("bin_cursor", std::max(c.nbins, c.N), &nb.bin_cursor)) ||
+      (rc = c.get("bin_atoms", c.N, &nb.bin_atoms)) || (rc = c.get("atom_bin", c.N, &nb.atom_bin)) ||
+      (rc = c.get("wpos", 3 * (size_t)c.N, &nb.wpos)) || (rc = c.get("shift", 3 * (size_t)c.N, &nb.shift)) ||
+      (rc = c.get("nb_count", c.N, &nb.nb_count)) || (rc = c.get("stats", 4, &stats)) ||
+      (rc = c.get("nb_j", slots, &nb.nb_j)) || (rc = c.get("nb_key", slots, &nb.nb_key)) ||
+      (rc = c.get("nb_d", 4 * slots, &nb.nb_d)) || (rc = c.get("nb_rev", slots, &nb.nb_rev)))
     return rc;
   nb.total = stats;
   nb.overflow = stats + 2;
-  const int *d_species = c.ptr<int>("species");
+  nb.kcap = c.kcap;
   const int *d_astruct = c.ptr<int>("astruct");
   const double *d_pos = c.ptr<double>("pos");
   const StructParams *d_sp = c.ptr<StructParams>("sp");
-  (void)d_species;
-  const int cap = (flags & GAPX_REF_NEIGHBOR_CAP)
-                      ? kRefNeighborCap
-                      : std::min(1024, max_neighbors_for_lds(!(flags & GAPX_MIXED), std::max(1, c.m->sf.n_groups), c.m->Dp));
   GAPX_HIP(launch_fill_int(c.stream, nb.bin_count, c.nbins, 0));
   GAPX_HIP(launch_fill_int(c.stream, nb.bin_cursor, c.nbins, 0));
   GAPX_HIP(launch_fill_int(c.stream, stats, 4, 0));
   GAPX_HIP(launch_wrap_bin(c.stream, c.N, d_pos, d_astruct, d_sp, nb));
   GAPX_HIP(launch_scan(c.stream, nb.bin_count, nb.bin_start, c.nbins, nullptr));
   GAPX_HIP(launch_scatter_bins(c.stream, c.N, c.nbins, nb));
-  GAPX_HIP(launch_count_neighbors(c.stream, c.N, d_astruct, d_sp, c.rcut, cap, nb));
-  GAPX_HIP(launch_scan(c.stream, nb.nb_count, nb.nb_start, c.N, stats));
-  int h[4];
-  GAPX_HIP(hipMemcpyAsync(h, stats, sizeof h, hipMemcpyDeviceToHost, c.stream));
-  GAPX_HIP(hipStreamSynchronize(c.stream));
-  if (h[2] > 0) {
-    char buf[160];
-    std::snprintf(buf, sizeof buf, "%d atoms have more than %d neighbours (largest %d)", h[2], cap, h[1]);
-    return fail(GAPX_ERR_NEIGHBORS, buf);
-  }
-  const int total = h[0];
-  c.KP = std::max(8, round_up(h[1], 8));
-  if ((rc = c.get("nb_j", total, &nb.nb_j)) || (rc = c.get("nb_img", 3 * (size_t)total, &nb.nb_img)) ||
-      (rc = c.get("nb_d", 4 * (size_t)total, &nb.nb_d)) || (rc = c.get("nb_rev", total, &nb.nb_rev)))
-    return rc;
-  c.nb_cap = total;
+  GAPX_HIP(launch_build_neighbors(c.stream, c.N, d_astruct, d_sp, c.rcut, nb));
+  if (grad) GAPX_HIP(launch_reverse_slots(c.stream, c.N, nb));
+  c.KP = c.kcap;
+  (void)flags;
   return GAPX_OK;
 }
 
@@ -521,13 +533,10 @@ static int ctx_run(Ctx &c, int flags) {
   Bufs b;
   int rc;
   stage_begin(c, kNeighbors);
-  if ((rc = run_neighbors(c, flags, b))) return rc;
+  if ((rc = run_neighbors(c, flags, b, grad))) return rc;
   NeighborBufs &nb = b.nb;
   const int *d_species = c.ptr<int>("species");
-  const int *d_astruct = c.ptr<int>("astruct");
   const StructParams *d_sp = c.ptr<StructParams>("sp");
-  GAPX_HIP(launch_fill_neighbors(c.stream, c.N, d_astruct, d_sp, c.rcut, nb));
-  if (grad) GAPX_HIP(launch_reverse_slots(c.stream, c.N, nb));
   stage_end(c, kNeighbors);
 
   // descriptors
@@ -579,6 +588,7 @@ static int ctx_run(Ctx &c, int flags) {
   g.theta_inv = m.d_theta_inv;
   g.grad = grad ? 1 : 0;
   g.bf16 = bf ? 1 : 0;
+  g.wscale = m.w_scale;
   if ((rc = c.get("Epart", (size_t)g.splits * c.Np, &g.Epart)) || (rc = c.get("eatom", c.Np, &g.eatom))) return rc;
   if (grad) {
     if ((rc = c.get("Gpart", (size_t)g.splits * ND, &g.Gpart))) return rc;
@@ -602,7 +612,7 @@ static int ctx_run(Ctx &c, int flags) {
       (rc = c.get("stress", 6 * (size_t)c.nstruct, &b.stress)) || (rc = c.get("atom_vir", 6 * (size_t)c.N, &b.atom_vir)))
     return rc;
   if (grad) {
-    if ((rc = c.get("slot_f", 3 * (size_t)std::max(1, c.nb_cap), &b.slot_f))) return rc;
+    if ((rc = c.get("slot_f", 3 * (size_t)c.N * c.kcap, &b.slot_f))) return rc;
     stage_begin(c, kDescBwd);
     GAPX_HIP(launch_descriptors_bwd(c.stream, fp64, c.N, d_species, m.d_wz, nb, m.sfd, Dp, g.g64, g.g32, b.slot_f,
                                     b.atom_vir, c.KP));
@@ -615,6 +625,7 @@ static int ctx_run(Ctx &c, int flags) {
     GAPX_HIP(launch_fill_double(c.stream, b.force, 3 * (size_t)c.N, 0.0));
   GAPX_HIP(launch_struct_reduce(c.stream, c.nstruct, d_sp, g.eatom, b.atom_vir, grad ? 1 : 0, b.ene, b.stress));
   stage_end(c, kForces);
+  GAPX_HIP(hipMemcpyAsync(c.host_stats, nb.total, 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
   c.last_flags = flags;
   c.ran = true;
   if (c.timing) {  // one event pair per stage: read them before the next run
@@ -624,13 +635,35 @@ static int ctx_run(Ctx &c, int flags) {
   return GAPX_OK;
 }
 
-static int ctx_results(Ctx &c, double *ene, double *force, double *stress6, double *eatom, bool fortran) {
-  if (!c.ran) return fail(GAPX_ERR_ARG, "no evaluation has run on this context");
+// Wait for the last run and check its neighbour counts: above kcap the run
+// is repeated with room for the largest count; above the cap it is an error.
+static int ctx_settle(Ctx &c) {
   GAPX_HIP(hipStreamSynchronize(c.stream));
   collect_timing(c);
-  int guard = 0;
-  GAPX_HIP(hipMemcpy(&guard, c.ptr<int>("stats") + 3, sizeof(int), hipMemcpyDeviceToHost));
-  if (guard) return fail(GAPX_ERR_NEIGHBORS, "descriptor kernels saw more neighbours than the LDS layout holds");
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    const int *h = c.host_stats;
+    const int cap = neighbor_cap(c, c.last_flags);
+    if (h[1] > cap) {
+      char buf[160];
+      std::snprintf(buf, sizeof buf, "an atom has %d neighbours within rcut, above the limit of %d", h[1], cap);
+      return fail(GAPX_ERR_NEIGHBORS, buf);
+    }
+    if (h[3]) return fail(GAPX_ERR_NEIGHBORS, "descriptor kernels saw more neighbours than their LDS layout holds");
+    if (h[2] == 0) return GAPX_OK;
+    c.kcap = round_up(h[1] + 8, 8);
+    c.invalidate_graph();
+    int rc = ctx_run(c, c.last_flags);
+    if (rc) return rc;
+    GAPX_HIP(hipStreamSynchronize(c.stream));
+    collect_timing(c);
+  }
+  return fail(GAPX_ERR_NEIGHBORS, "neighbour capacity did not converge");
+}
+
+static int ctx_results(Ctx &c, double *ene, double *force, double *stress6, double *eatom, bool fortran) {
+  if (!c.ran) return fail(GAPX_ERR_ARG, "no evaluation has run on this context");
+  int rc = ctx_settle(c);
+  if (rc) return rc;
   if (ene) GAPX_HIP(hipMemcpy(ene, c.ptr<double>("ene"), sizeof(double) * c.nstruct, hipMemcpyDeviceToHost));
   if (stress6) GAPX_HIP(hipMemcpy(stress6, c.ptr<double>("stress"), sizeof(double) * 6 * c.nstruct, hipMemcpyDeviceToHost));
   if (eatom) GAPX_HIP(hipMemcpy(eatom, c.ptr<double>("eatom"), sizeof(double) * c.N, hipMemcpyDeviceToHost));
@@ -656,6 +689,12 @@ static int ctx_create(Model *m, Ctx **out) {
     return fail(GAPX_ERR_HIP, "could not create a HIP stream");
   }
   for (auto &e : c->ev) hipEventCreate(&e);
+  if (hipHostMalloc((void **)&c->host_stats, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    hipStreamDestroy(c->stream);
+    delete c;
+    return fail(GAPX_ERR_NOMEM, "pinned host allocation failed");
+  }
+  std::memset(c->host_stats, 0, 4 * sizeof(int));
   const char *ch = std::getenv("GAPX_DESC_CHUNK");
   if (ch) c->chunk = std::atoi(ch);
   *out = c;
@@ -672,6 +711,7 @@ static void ctx_free(Ctx *c) {
   for (auto &e : c->ev)
     if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
+  if (c->host_stats) hipHostFree(c->host_stats);
   delete c;
 }
 
@@ -958,7 +998,7 @@ int gapx_descriptors(gapx_model *m, int na, const int *species, const double *la
   const bool fortran = (flags & GAPX_LAYOUT_F) != 0;
   if ((rc = ctx_load(*c, 1, &na, species, lat, pos, rcut, fortran))) return rc;
   if ((rc = ctx_run(*c, 0))) return rc;
-  GAPX_HIP(hipStreamSynchronize(c->stream));
+  if ((rc = ctx_settle(*c))) return rc;
   std::vector<double> tmp((size_t)c->N * m->Dp);
   GAPX_HIP(hipMemcpy(tmp.data(), c->ptr<double>("x64"), tmp.size() * 8, hipMemcpyDeviceToHost));
   for (int i = 0; i < na; ++i)
@@ -1006,6 +1046,7 @@ int gapx_ctx_run(gapx_ctx *c, int flags) {
 
 int gapx_ctx_sync(gapx_ctx *c) {
   if (!c) return fail(GAPX_ERR_ARG, "null context");
+  if (c->ran) return ctx_settle(*c);
   GAPX_HIP(hipStreamSynchronize(c->stream));
   collect_timing(*c);
   return GAPX_OK;
@@ -1018,7 +1059,8 @@ int gapx_ctx_results(gapx_ctx *c, double *ene, double *force, double *stress6, d
 
 int gapx_ctx_features(gapx_ctx *c, double *x, double *dedx) {
   if (!c || !c->ran) return fail(GAPX_ERR_ARG, "no evaluation has run on this context");
-  GAPX_HIP(hipStreamSynchronize(c->stream));
+  int rc = ctx_settle(*c);
+  if (rc) return rc;
   const Model &m = *c->m;
   const bool fp64 = !(c->last_flags & GAPX_MIXED);
   const size_t n = (size_t)c->N * m.Dp;
@@ -1090,7 +1132,9 @@ int gapx_ctx_use_graph(gapx_ctx *c, int on) {
 int gapx_ctx_counts(gapx_ctx *c, long long *pairs, long long *triplets) {
   if (!c || !c->ran) return fail(GAPX_ERR_ARG, "no evaluation has run on this context");
   GAPX_HIP(hipStreamSynchronize(c->stream));
-  if (pairs) *pairs = c->nb_cap;
+  int rc = ctx_settle(*c);
+  if (rc) return rc;
+  if (pairs) *pairs = c->host_stats[0];
   if (triplets) {
     unsigned long long t = 0;
     GAPX_HIP(hipMemcpy(&t, c->ptr<unsigned long long>("counters"), sizeof t, hipMemcpyDeviceToHost));

@@ -189,8 +189,8 @@ __global__ __launch_bounds__(64) void k_desc_fwd(int N, const int *__restrict__ 
   T *s_feat = s_fc + L.G * KP;
   uint32_t *s_q = (uint32_t *)(s_feat + Dp);
 
-  const int start = nb.nb_start[i];
-  const int K = nb.nb_start[i + 1] - start;
+  const int start = i * nb.kcap;
+  const int K = nb.nb_count[i];
   if (K > KP) {  // the host sizes KP from the largest count; never index LDS past it
     if (lane == 0) atomicAdd(&nb.total[3], 1);
     return;
@@ -414,8 +414,8 @@ __global__ __launch_bounds__(64) void k_desc_bwd(int N, const int *__restrict__ 
   T *s_F = s_T + KP;
   uint32_t *s_q = (uint32_t *)(s_F + 3 * KP);
 
-  const int start = nb.nb_start[i];
-  const int K = nb.nb_start[i + 1] - start;
+  const int start = i * nb.kcap;
+  const int K = nb.nb_count[i];
   if (K > KP) {
     if (lane == 0) atomicAdd(&nb.total[3], 1);
     return;

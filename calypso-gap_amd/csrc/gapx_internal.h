@@ -69,7 +69,7 @@ struct Model {
   uint16_t *d_mt_bhi = nullptr, *d_mt_blo = nullptr;   // BF16 split [Mp][Dp]
   uint16_t *d_mtT_hi = nullptr, *d_mtT_lo = nullptr;   // FP16 split, transposed [Dp][Mp]
   uint16_t *d_mtT_bhi = nullptr, *d_mtT_blo = nullptr; // BF16 split, transposed
-  float *d_mn_f = nullptr;               // unused placeholder for alignment
+  float w_scale = 1024.f;                // power of 2 applied to W before its FP16 split
   size_t dev_bytes = 0;
   struct Ctx *cached_ctx = nullptr;      // workspace for gapx_calc
 };
@@ -100,28 +100,26 @@ int fail(int code, const std::string &msg);
 
 // ---- kernel launchers (kernels live in the .hip files) ----------------------
 struct NeighborBufs {
-  int *bin_count, *bin_start, *bin_cursor, *bin_atoms;  // [nbins_total (+1)]
+  int *bin_count, *bin_start, *bin_cursor, *bin_atoms;  // [nbins_total (+1)], bin_atoms [N]
   int *atom_bin;        // [N]
   double *wpos;         // [N*3] wrapped positions
   int *shift;           // [N*3] floor(frac)
+  int kcap;             // slots per atom: atom i's neighbours are slots i*kcap + [0, nb_count[i])
   int *nb_count;        // [N]
-  int *nb_start;        // [N+1]
-  int *nb_j;            // [cap]
-  int *nb_img;          // [cap*3] image relative to given positions
-  double *nb_d;         // [cap*4] dx dy dz r
-  int *nb_rev;          // [cap]
-  int *overflow;        // [1] atoms above the neighbour cap (= total + 2)
-  int *total;           // [4] total neighbours, largest count, overflow, guard
+  int *nb_j;            // [N*kcap]
+  unsigned long long *nb_key;  // [N*kcap] packed (j, image) for the reverse-slot search
+  double *nb_d;         // [N*kcap*4] dx dy dz r (relative to the centre)
+  int *nb_rev;          // [N*kcap] slot (j -> i, -n) of slot (i -> j, n), or -1
+  int *overflow;        // = total + 2
+  int *total;           // [4] kept slots, largest count, atoms above kcap, kernel guard
 };
 
 hipError_t launch_wrap_bin(hipStream_t s, int N, const double *pos, const int *atom_struct,
                            const StructParams *sp, NeighborBufs nb);
 hipError_t launch_scan(hipStream_t s, const int *in, int *out, int n, int *total);
 hipError_t launch_scatter_bins(hipStream_t s, int N, int nbins, NeighborBufs nb);
-hipError_t launch_count_neighbors(hipStream_t s, int N, const int *atom_struct,
-                                  const StructParams *sp, double rcut, int cap, NeighborBufs nb);
-hipError_t launch_fill_neighbors(hipStream_t s, int N, const int *atom_struct,
-                                 const StructParams *sp, double rcut, NeighborBufs nb);
+hipError_t launch_build_neighbors(hipStream_t s, int N, const int *atom_struct,
+                                  const StructParams *sp, double rcut, NeighborBufs nb);
 hipError_t launch_reverse_slots(hipStream_t s, int N, NeighborBufs nb);
 hipError_t launch_fill_int(hipStream_t s, int *p, int n, int v);
 hipError_t launch_fill_double(hipStream_t s, double *p, size_t n, double v);
@@ -168,6 +166,7 @@ struct GprArgs {
   double *g64; float *g32;        // [N][Dp]
   int grad;
   int bf16;
+  float wscale;                   // W multiplier before the 16-bit split (power of 2)
 };
 hipError_t launch_gpr_fp64(hipStream_t s, const GprArgs &a);
 hipError_t launch_gpr_mixed(hipStream_t s, const GprArgs &a);

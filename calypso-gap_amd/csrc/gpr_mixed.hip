@@ -8,8 +8,9 @@
 //   W = exp(-d2 / 2) c,  E_i += sum_k W_ik,  G += W mt.
 // Every operand x is carried as hi = round(x), lo = round(x - hi) in the
 // 16-bit type and each product as hi*hi + hi*lo + lo*hi (the lo*lo term is
-// below FP32 rounding).  W is scaled by 2^10 before its split so that lo stays
-// in the normal FP16 range; G is unscaled when it is written.
+// below FP32 rounding).  W is scaled by a power of two chosen from the
+// largest |alpha| (GprArgs::wscale) before its split, so that hi cannot
+// overflow and lo stays in the normal FP16 range; G is unscaled on output.
 //
 // Operand layout of the 16x16x32 forms per lane l (l16 = l % 16, lq = l / 16):
 // A[l16][8 lq + e], B[8 lq + e][l16] (e = 0..7, one 16-byte register group),
@@ -23,8 +24,6 @@ namespace {
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef __bf16 v8b __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
-
-constexpr float kWScale = 1024.0f;
 
 template <bool BF>
 struct Half;
@@ -125,7 +124,7 @@ __global__ __launch_bounds__(256) void k_gpr_mixed(GprArgs a) {
         const float d2 = (float)(xnr[r] + mnp - 2.0 * (double)S[r]);
         const float w = __expf(-0.5f * d2) * cp;
         er[r] += (double)w;
-        const float ws = w * kWScale;
+        const float ws = w * a.wscale;
         const hs hi = (hs)ws;
         const hs lo = (hs)(ws - (float)hi);
         const int o = (4 * lq + r) * ST + pt;
@@ -154,7 +153,7 @@ __global__ __launch_bounds__(256) void k_gpr_mixed(GprArgs a) {
 
   const size_t base = (size_t)split * a.Np;
   if (a.grad) {
-    const double inv = 1.0 / (double)kWScale;
+    const double inv = 1.0 / (double)a.wscale;
 #pragma unroll
     for (int b = 0; b < DP / 16; ++b)
 #pragma unroll

@@ -90,134 +90,163 @@ __global__ void k_scatter(int N, NeighborBufs nb) {
   nb.bin_atoms[slot] = i;
 }
 
-// Sort each bin's atoms by index (insertion sort; bins hold tens of atoms).
-__global__ void k_sort_bins(int nbins, NeighborBufs nb) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
+// Order each bin's atoms by index: one wavefront per bin, every element's
+// rank is the number of smaller indices in the bin (bins hold tens of atoms).
+__global__ __launch_bounds__(64) void k_rank_bins(int nbins, NeighborBufs nb) {
+  const int b = blockIdx.x, lane = threadIdx.x;
   if (b >= nbins) return;
-  int lo = nb.bin_start[b], hi = nb.bin_start[b + 1];
-  for (int p = lo + 1; p < hi; ++p) {
-    int v = nb.bin_atoms[p];
-    int q = p - 1;
-    while (q >= lo && nb.bin_atoms[q] > v) {
-      nb.bin_atoms[q + 1] = nb.bin_atoms[q];
-      --q;
+  const int lo = nb.bin_start[b], n = nb.bin_start[b + 1] - lo;
+  if (n <= 64) {
+    const int v = (lane < n) ? nb.bin_atoms[lo + lane] : 0x7fffffff;
+    int rank = 0;
+    for (int k = 0; k < n; ++k) rank += (__shfl(v, k, 64) < v) ? 1 : 0;
+    __syncthreads();
+    if (lane < n) nb.bin_cursor[lo + rank] = v;  // bin_cursor doubles as scratch
+    __syncthreads();
+    if (lane < n) nb.bin_atoms[lo + lane] = nb.bin_cursor[lo + lane];
+  } else {
+    for (int e = lane; e < n; e += 64) {
+      const int v = nb.bin_atoms[lo + e];
+      int rank = 0;
+      for (int k = 0; k < n; ++k) rank += (nb.bin_atoms[lo + k] < v) ? 1 : 0;
+      nb.bin_cursor[lo + rank] = v;
     }
-    nb.bin_atoms[q + 1] = v;
+    __syncthreads();
+    for (int e = lane; e < n; e += 64) nb.bin_atoms[lo + e] = nb.bin_cursor[lo + e];
   }
 }
 
-// Visit every candidate (j, image) of atom i in a fixed order; fn(j, t, d, r)
-// is called for each one within rcut.
-template <typename Fn>
-__device__ inline void for_each_neighbor(int i, const StructParams &P, const NeighborBufs &nb, double rcut,
-                                         Fn fn) {
-  int b = nb.atom_bin[i] - P.bin_base;
-  int bi[3];
-  bi[2] = b % P.nb[2];
-  bi[1] = (b / P.nb[2]) % P.nb[1];
-  bi[0] = b / (P.nb[1] * P.nb[2]);
+__device__ __forceinline__ unsigned long long nb_key(int j, int n0, int n1, int n2) {
+  return ((unsigned long long)(unsigned)j << 24) | ((unsigned long long)((n0 + 128) & 255) << 16) |
+         ((unsigned long long)((n1 + 128) & 255) << 8) | (unsigned long long)((n2 + 128) & 255);
+}
+
+// One wavefront per centre atom: visit the (2S+1)^3 bin images in a fixed
+// order, test the bin's atoms one per lane and append the ones within rcut
+// (ballot + prefix count), so the list order is bin order, then index order.
+// Slots live at i * kcap + a; counts above kcap are reported, not written.
+__global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ atom_struct,
+                                              const StructParams *__restrict__ sp, double rcut, NeighborBufs nb) {
+  const int i = blockIdx.x, lane = threadIdx.x;
+  if (i >= N) return;
+  const StructParams &P = sp[atom_struct[i]];
+  const int b = nb.atom_bin[i] - P.bin_base;
+  const int bi2 = b % P.nb[2], bi1 = (b / P.nb[2]) % P.nb[1], bi0 = b / (P.nb[1] * P.nb[2]);
   const double xi = nb.wpos[3 * i], yi = nb.wpos[3 * i + 1], zi = nb.wpos[3 * i + 2];
+  const int si0 = nb.shift[3 * i], si1 = nb.shift[3 * i + 1], si2 = nb.shift[3 * i + 2];
+  const size_t base = (size_t)i * nb.kcap;
+  int count = 0;
   for (int o0 = -P.S[0]; o0 <= P.S[0]; ++o0) {
-    int bb0 = bi[0] + o0, t0 = floordiv(bb0, P.nb[0]), w0 = bb0 - t0 * P.nb[0];
+    const int bb0 = bi0 + o0, t0 = floordiv(bb0, P.nb[0]), w0 = bb0 - t0 * P.nb[0];
     for (int o1 = -P.S[1]; o1 <= P.S[1]; ++o1) {
-      int bb1 = bi[1] + o1, t1 = floordiv(bb1, P.nb[1]), w1 = bb1 - t1 * P.nb[1];
+      const int bb1 = bi1 + o1, t1 = floordiv(bb1, P.nb[1]), w1 = bb1 - t1 * P.nb[1];
       for (int o2 = -P.S[2]; o2 <= P.S[2]; ++o2) {
-        int bb2 = bi[2] + o2, t2 = floordiv(bb2, P.nb[2]), w2 = bb2 - t2 * P.nb[2];
+        const int bb2 = bi2 + o2, t2 = floordiv(bb2, P.nb[2]), w2 = bb2 - t2 * P.nb[2];
         const double sx = t0 * P.L[0] + t1 * P.L[3] + t2 * P.L[6];
         const double sy = t0 * P.L[1] + t1 * P.L[4] + t2 * P.L[7];
         const double sz = t0 * P.L[2] + t1 * P.L[5] + t2 * P.L[8];
         const bool zero = (t0 == 0) && (t1 == 0) && (t2 == 0);
-        int bin = P.bin_base + (w0 * P.nb[1] + w1) * P.nb[2] + w2;
-        int lo = nb.bin_start[bin], hi = nb.bin_start[bin + 1];
-        for (int q = lo; q < hi; ++q) {
-          int j = nb.bin_atoms[q];
-          if (zero && j == i) continue;
-          double dx = nb.wpos[3 * j] + sx - xi;
-          double dy = nb.wpos[3 * j + 1] + sy - yi;
-          double dz = nb.wpos[3 * j + 2] + sz - zi;
-          double r = sqrt(dx * dx + dy * dy + dz * dz);
-          if (r > rcut) continue;
-          fn(j, t0, t1, t2, dx, dy, dz, r);
+        const int bin = P.bin_base + (w0 * P.nb[1] + w1) * P.nb[2] + w2;
+        const int lo = nb.bin_start[bin], hi = nb.bin_start[bin + 1];
+        for (int q0 = lo; q0 < hi; q0 += 64) {
+          const int q = q0 + lane;
+          bool ok = false;
+          int j = 0;
+          double dx = 0, dy = 0, dz = 0, r = 0;
+          if (q < hi) {
+            j = nb.bin_atoms[q];
+            if (!(zero && j == i)) {
+              dx = nb.wpos[3 * j] + sx - xi;
+              dy = nb.wpos[3 * j + 1] + sy - yi;
+              dz = nb.wpos[3 * j + 2] + sz - zi;
+              r = sqrt(dx * dx + dy * dy + dz * dz);
+              ok = !(r > rcut);
+            }
+          }
+          const unsigned long long m = __ballot(ok);
+          const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+          if (ok && pos < nb.kcap) {
+            const size_t sl = base + pos;
+            nb.nb_j[sl] = j;
+            nb.nb_key[sl] = nb_key(j, t0 - nb.shift[3 * j] + si0, t1 - nb.shift[3 * j + 1] + si1,
+                                   t2 - nb.shift[3 * j + 2] + si2);
+            nb.nb_d[4 * sl] = dx;
+            nb.nb_d[4 * sl + 1] = dy;
+            nb.nb_d[4 * sl + 2] = dz;
+            nb.nb_d[4 * sl + 3] = r;
+          }
+          count += __popcll(m);
         }
       }
     }
   }
-}
-
-__global__ void k_count(int N, const int *__restrict__ atom_struct, const StructParams *__restrict__ sp,
-                        double rcut, int cap, NeighborBufs nb) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const StructParams &P = sp[atom_struct[i]];
-  int count = 0;
-  for_each_neighbor(i, P, nb, rcut, [&](int, int, int, int, double, double, double, double) { ++count; });
-  nb.nb_count[i] = count;
-  atomicMax(&nb.total[1], count);
-  if (count > cap) atomicAdd(nb.overflow, 1);
-}
-
-__global__ void k_fill(int N, const int *__restrict__ atom_struct, const StructParams *__restrict__ sp,
-                       double rcut, NeighborBufs nb) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const StructParams &P = sp[atom_struct[i]];
-  int slot = nb.nb_start[i];
-  const int si0 = nb.shift[3 * i], si1 = nb.shift[3 * i + 1], si2 = nb.shift[3 * i + 2];
-  for_each_neighbor(i, P, nb, rcut, [&](int j, int t0, int t1, int t2, double dx, double dy, double dz, double r) {
-    nb.nb_j[slot] = j;
-    nb.nb_img[3 * slot] = t0 - nb.shift[3 * j] + si0;
-    nb.nb_img[3 * slot + 1] = t1 - nb.shift[3 * j + 1] + si1;
-    nb.nb_img[3 * slot + 2] = t2 - nb.shift[3 * j + 2] + si2;
-    nb.nb_d[4 * slot] = dx;
-    nb.nb_d[4 * slot + 1] = dy;
-    nb.nb_d[4 * slot + 2] = dz;
-    nb.nb_d[4 * slot + 3] = r;
-    ++slot;
-  });
+  if (lane == 0) {
+    nb.nb_count[i] = min(count, nb.kcap);
+    atomicMax(&nb.total[1], count);
+    atomicAdd(&nb.total[0], min(count, nb.kcap));
+    if (count > nb.kcap) atomicAdd(nb.overflow, 1);
+  }
 }
 
 // For slot (i -> j, n) find the slot (j -> i, -n); -1 if absent (a pair within
-// one rounding step of rcut can be seen from one side only).
+// one rounding step of rcut can be seen from one side only).  One wavefront
+// per centre atom, one slot per lane, a linear scan of j's packed keys.
 __global__ __launch_bounds__(64) void k_reverse(int N, NeighborBufs nb) {
-  int i = blockIdx.x;
+  const int i = blockIdx.x;
   if (i >= N) return;
-  int lo = nb.nb_start[i], hi = nb.nb_start[i + 1];
-  for (int s = lo + (int)threadIdx.x; s < hi; s += 64) {
-    int j = nb.nb_j[s];
-    int n0 = -nb.nb_img[3 * s], n1 = -nb.nb_img[3 * s + 1], n2 = -nb.nb_img[3 * s + 2];
+  const int K = nb.nb_count[i];
+  const size_t base = (size_t)i * nb.kcap;
+  for (int a = (int)threadIdx.x; a < K; a += 64) {
+    const unsigned long long key = nb.nb_key[base + a];
+    const int j = (int)(key >> 24);
+    const int n0 = (int)((key >> 16) & 255) - 128, n1 = (int)((key >> 8) & 255) - 128, n2 = (int)(key & 255) - 128;
+    const unsigned long long want = nb_key(i, -n0, -n1, -n2);
+    const size_t jb = (size_t)j * nb.kcap;
+    const int Kj = nb.nb_count[j];
     int found = -1;
-    for (int q = nb.nb_start[j]; q < nb.nb_start[j + 1]; ++q) {
-      if (nb.nb_j[q] == i && nb.nb_img[3 * q] == n0 && nb.nb_img[3 * q + 1] == n1 && nb.nb_img[3 * q + 2] == n2) {
-        found = q;
+    for (int q = 0; q < Kj; ++q)
+      if (nb.nb_key[jb + q] == want) {
+        found = (int)(jb + q);
         break;
       }
-    }
-    nb.nb_rev[s] = found;
+    nb.nb_rev[base + a] = found;
   }
 }
 
 // F_m = sum_s f^(m)_s - sum_s f^(j_s)_{rev(s)}: slot_f[s] holds dE_i/dR of the
 // atom in slot s of centre i, so the force on m collects minus its own
 // centre's derivative (the -sum over its slots) and minus what every centre
-// that sees m puts on it (the reverse slots).
-__global__ void k_gather(int N, NeighborBufs nb, const double *__restrict__ slot_f, double *__restrict__ force) {
-  int m = blockIdx.x * blockDim.x + threadIdx.x;
+// that sees m puts on it (the reverse slots).  One wavefront per atom; the
+// lane sums are combined in a fixed order (deterministic).
+__global__ __launch_bounds__(64) void k_gather(int N, NeighborBufs nb, const double *__restrict__ slot_f,
+                                               double *__restrict__ force) {
+  const int m = blockIdx.x, lane = threadIdx.x;
   if (m >= N) return;
+  const int K = nb.nb_count[m];
+  const size_t base = (size_t)m * nb.kcap;
   double f0 = 0, f1 = 0, f2 = 0;
-  for (int s = nb.nb_start[m]; s < nb.nb_start[m + 1]; ++s) {
+  for (int a = lane; a < K; a += 64) {
+    const size_t s = base + a;
     f0 += slot_f[3 * s];
     f1 += slot_f[3 * s + 1];
     f2 += slot_f[3 * s + 2];
-    int r = nb.nb_rev[s];
+    const int r = nb.nb_rev[s];
     if (r >= 0) {
-      f0 -= slot_f[3 * r];
-      f1 -= slot_f[3 * r + 1];
-      f2 -= slot_f[3 * r + 2];
+      f0 -= slot_f[3 * (size_t)r];
+      f1 -= slot_f[3 * (size_t)r + 1];
+      f2 -= slot_f[3 * (size_t)r + 2];
     }
   }
-  force[3 * m] = f0;
-  force[3 * m + 1] = f1;
-  force[3 * m + 2] = f2;
+  for (int o = 32; o > 0; o >>= 1) {
+    f0 += __shfl_xor(f0, o, 64);
+    f1 += __shfl_xor(f1, o, 64);
+    f2 += __shfl_xor(f2, o, 64);
+  }
+  if (lane == 0) {
+    force[3 * m] = f0;
+    force[3 * m + 1] = f1;
+    force[3 * m + 2] = f2;
+  }
 }
 
 // Per-structure energy and stress: deterministic tree sums over the atoms.
@@ -280,19 +309,13 @@ hipError_t launch_scatter_bins(hipStream_t s, int N, int nbins, NeighborBufs nb)
   k_scatter<<<(N + 255) / 256, 256, 0, s>>>(N, nb);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_sort_bins<<<(nbins + 255) / 256, 256, 0, s>>>(nbins, nb);
+  k_rank_bins<<<nbins, 64, 0, s>>>(nbins, nb);
   return hipGetLastError();
 }
 
-hipError_t launch_count_neighbors(hipStream_t s, int N, const int *atom_struct, const StructParams *sp,
-                                  double rcut, int cap, NeighborBufs nb) {
-  k_count<<<(N + 127) / 128, 128, 0, s>>>(N, atom_struct, sp, rcut, cap, nb);
-  return hipGetLastError();
-}
-
-hipError_t launch_fill_neighbors(hipStream_t s, int N, const int *atom_struct, const StructParams *sp,
-                                 double rcut, NeighborBufs nb) {
-  k_fill<<<(N + 127) / 128, 128, 0, s>>>(N, atom_struct, sp, rcut, nb);
+hipError_t launch_build_neighbors(hipStream_t s, int N, const int *atom_struct, const StructParams *sp,
+                                  double rcut, NeighborBufs nb) {
+  k_build<<<N, 64, 0, s>>>(N, atom_struct, sp, rcut, nb);
   return hipGetLastError();
 }
 
@@ -303,7 +326,7 @@ hipError_t launch_reverse_slots(hipStream_t s, int N, NeighborBufs nb) {
 
 hipError_t launch_gather_forces(hipStream_t s, int N, const NeighborBufs &nb, const double *slot_f,
                                 double *force) {
-  k_gather<<<(N + 255) / 256, 256, 0, s>>>(N, nb, slot_f, force);
+  k_gather<<<N, 64, 0, s>>>(N, nb, slot_f, force);
   return hipGetLastError();
 }
 
