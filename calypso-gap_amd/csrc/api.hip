@@ -477,6 +477,9 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b, bool grad) {
   NeighborBufs &nb = b.nb;
   int rc;
   int *stats;
+  // never size the descriptor kernels' LDS past the cap: a count above it is
+  // reported by ctx_settle as GAPX_ERR_NEIGHBORS
+  c.kcap = std::min(c.kcap, round_up(neighbor_cap(c, flags), 8));
   const size_t slots = (size_t)c.N * c.kcap;
   if ((rc = c.get("bin_count", c.nbins, &nb.bin_count)) || (rc = c.get("bin_start", c.nbins + 1, &nb.bin_start)) ||
       (rc = c.get("bin_cursor", std::max(c.nbins, c.N), &nb.bin_cursor)) ||
@@ -501,7 +504,6 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b, bool grad) {
   GAPX_HIP(launch_build_neighbors(c.stream, c.N, d_astruct, d_sp, c.rcut, nb));
   if (grad) GAPX_HIP(launch_reverse_slots(c.stream, c.N, nb));
   c.KP = c.kcap;
-  (void)flags;
   return GAPX_OK;
 }
 
