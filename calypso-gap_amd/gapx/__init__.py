@@ -37,7 +37,7 @@ EXPORTS = [
     "gapx_calc", "gapx_calc_batch", "gapx_descriptors", "gapx_ctx_create", "gapx_ctx_load",
     "gapx_ctx_update", "gapx_ctx_run", "gapx_ctx_sync", "gapx_ctx_results", "gapx_ctx_features",
     "gapx_ctx_timing", "gapx_ctx_stage_count", "gapx_ctx_stage_info", "gapx_ctx_use_graph",
-    "gapx_ctx_device_bytes", "gapx_ctx_free", "gapx_ctx_counts",
+    "gapx_ctx_device_bytes", "gapx_ctx_free", "gapx_ctx_counts", "gapx_min_distance",
 ]
 
 
@@ -82,6 +82,7 @@ def lib():
         L.gapx_ctx_use_graph.argtypes = [vp, ci]
         L.gapx_ctx_device_bytes.argtypes = [vp, vp]
         L.gapx_ctx_counts.argtypes = [vp, vp, vp]
+        L.gapx_min_distance.argtypes = [ci, vp, vp, cd, ci, vp]
         L.gapx_ctx_free.argtypes = [vp]
         L.gapx_ctx_free.restype = None
         _lib = L
@@ -101,6 +102,15 @@ def device_count():
     n = ctypes.c_int(0)
     _check(lib().gapx_device_count(ctypes.byref(n)))
     return n.value
+
+
+def min_distance(lat, pos, rcut=6.0):
+    """FGET_BOND: min(10, shortest interatomic distance within rcut over images)."""
+    lat = np.ascontiguousarray(lat, np.float64)
+    pos = np.ascontiguousarray(pos, np.float64)
+    out = np.zeros(1)
+    _check(lib().gapx_min_distance(len(pos), _p(lat), _p(pos), float(rcut), 0, _p(out)))
+    return float(out[0])
 
 
 class Model:

@@ -108,6 +108,12 @@ int gapx_calc_batch(gapx_model *m, int nstruct, const int *natoms, const int *sp
 int gapx_descriptors(gapx_model *m, int na, const int *species, const double *lat,
                      const double *pos, double rcut, int flags, double *x);
 
+/* FGET_BOND (gappy/libgap/get_bond.f90:4-117): min(10, shortest distance
+ * within rcut between any atom and any periodic image of any atom, the
+ * atom's own zero image excluded), over the reference's image range.
+ * lat/pos in the layout given by flags (GAPX_LAYOUT_F or C). */
+int gapx_min_distance(int na, const double *lat, const double *pos, double rcut, int flags, double *dmin);
+
 /* ---- device-resident contexts ----------------------------------------------
  * A context owns a HIP stream and device buffers for a batch of structures.
  * Load once, then run repeatedly on the inputs already in HBM; results stay

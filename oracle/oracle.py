@@ -324,3 +324,17 @@ def ref_gap_calc(model_path, species, lat, pos, theta, mm, coeff, rcut, lgrad=Tr
         os.chdir(cwd)
     return dict(energy=args["ENE"].value, forces=np.array(args["FORCE"]), stress=args["STRESS"].copy(),
                 variance=args["VAR"].value)
+
+
+def ref_get_bond(lat, species, pos, rcut):
+    """The reference FGET_BOND (compiled Fortran, get_bond.f90:4-117)."""
+    ref = ctypes.CDLL(REF_PATH)
+    na = ctypes.c_int(len(species))
+    L = np.asfortranarray(lat, dtype=np.float64)
+    P = np.asfortranarray(pos, dtype=np.float64)
+    E = np.asfortranarray(species, dtype=np.int32)
+    rc = ctypes.c_double(rcut)
+    out = ctypes.c_double(0.0)
+    ref.fget_bond_(ctypes.byref(na), L.ctypes.data_as(ctypes.c_void_p), E.ctypes.data_as(ctypes.c_void_p),
+                   P.ctypes.data_as(ctypes.c_void_p), ctypes.byref(rc), ctypes.byref(out))
+    return out.value

@@ -14,6 +14,8 @@ cutoff (more than one image per axis, self-images), lgrad = .false., SF
 type 3, per-line cutoffs below Rcut, and the 64-atom / D = 96 / M = 500
 single-species workload itself.
 
+FGET_BOND (get_bond.f90:4-117) is run on the same structures at two cutoffs.
+
 Output: tests/golden/ref_vectors.npz (inputs, model arrays, reference outputs).
 Usage: python tests/golden/make_ref_vectors.py
 """
@@ -147,6 +149,9 @@ def main():
         out[pre + "energy"] = np.array(res["energy"])
         out[pre + "forces"] = res["forces"]
         out[pre + "stress"] = res["stress"]
+        # FGET_BOND at the evaluation cutoff and at a cutoff below every distance
+        out[pre + "min_bond"] = np.array([oracle.ref_get_bond(lat, sp, pos, rcut),
+                                          oracle.ref_get_bond(lat, sp, pos, 0.9)])
         print("%-24s N=%3d E=%.10f |F|max=%.6f S0=%.6f" % (cname, len(sp), res["energy"],
                                                           np.abs(res["forces"]).max(), res["stress"][0]))
     path = os.path.join(HERE, "ref_vectors.npz")

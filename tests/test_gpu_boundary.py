@@ -1,0 +1,120 @@
+"""The drop-in boundary on the GPU: libgap.libgap.fgap_calc (reference
+signature), libgap.GAP.Calculator, the ASE adapter and FGET_BOND against the
+reference's own outputs; the batched and status entry points."""
+import os
+import shutil
+import sys
+
+import numpy as np
+import pytest
+
+import gapx
+import oracle
+from conftest import GOLDEN, GOLDEN_CASE_NAMES, ROOT, golden_cases, golden_model
+
+pytestmark = pytest.mark.gpu
+PKG = os.path.join(ROOT, "calypso-gap_amd")
+MODEL = os.path.join(GOLDEN, "c64_diamond", "gap_parameters")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if gapx.device_count() == 0:
+        pytest.skip("no GPU visible")
+
+
+@pytest.fixture
+def in_model_dir(tmp_path, monkeypatch):
+    shutil.copy(MODEL, tmp_path / "gap_parameters")
+    monkeypatch.chdir(tmp_path)
+    return tmp_path
+
+
+def test_fgap_calc_reference_signature_on_traj_frames(in_model_dir, frames):
+    import libgap.libgap as L
+    n, d, theta, mm, invcmm, coeff = L.fgap_read()
+    for k in (0, 5, 10):
+        ene, force, stress, var = L.fgap_calc(frames["numbers"], frames["cells"][k], frames["positions"][k],
+                                              theta[:d], mm[:n, :d], invcmm[:n, :n], coeff[:n], 6.0, True)
+        assert abs(ene - frames["energies"][k]) / 64 < 1e-9
+        assert np.abs(force - frames["forces"][k]).max() < 1e-7
+        assert np.abs(stress - frames["stresses"][k]).max() < 1e-6
+        assert var == 0.0
+
+
+def test_gap_calculator_api(in_model_dir, frames):
+    from libgap.GAP import Calculator
+    gap = Calculator(rcut=6.0)
+    symbols = ["C"] * 64
+    ene, force, stress, _ = gap.gap_calc(symbols, frames["cells"][3], frames["positions"][3], True)
+    assert abs(ene - frames["energies"][3]) / 64 < 1e-9
+    assert np.abs(force - frames["forces"][3]).max() < 1e-7
+    ene0, force0, stress0, _ = gap.gap_calc(symbols, frames["cells"][3], frames["positions"][3], False)
+    assert ene0 == pytest.approx(ene, abs=1e-10) and not force0.any() and not stress0.any()
+    mixed = Calculator(rcut=6.0, precision="mixed")
+    em, fm, sm, _ = mixed.gap_calc(symbols, frames["cells"][3], frames["positions"][3], True)
+    assert abs(em - ene) / 64 < 5e-3
+    with pytest.raises(RuntimeError):
+        gap.gap_calc([79] * 64, frames["cells"][3], frames["positions"][3], True)
+
+
+def test_ase_adapter_on_traj_frames(in_model_dir, frames):
+    sys.path.insert(0, os.path.join(ROOT, "tests", "stub_ase"))
+    try:
+        import importlib.util
+        from ase import Atoms
+        spec = importlib.util.spec_from_file_location("gap_calc", os.path.join(PKG, "ASE", "gap_calc.py"))
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        atoms = Atoms(frames["numbers"], frames["positions"][7], frames["cells"][7])
+        atoms.calc = mod.GAP(rcut=6.0)
+        assert abs(atoms.get_potential_energy() - frames["energies"][7]) / 64 < 1e-9
+        assert np.abs(atoms.get_forces() - frames["forces"][7]).max() < 1e-7
+        assert np.abs(atoms.get_stress() - frames["stresses"][7]).max() < 1e-6
+        assert atoms.calc.results["free_energy"] == atoms.calc.results["energy"]
+        assert atoms.calc.results["variance"] == 0.0
+    finally:
+        sys.path.remove(os.path.join(ROOT, "tests", "stub_ase"))
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASE_NAMES)
+def test_fget_bond_matches_reference(ref_vectors, case):
+    import libgap.libgap as L
+    c = [x for x in golden_cases(ref_vectors) if x["name"] == case][0]
+    want = ref_vectors["case_%s_min_bond" % case]
+    got = [L.fget_bond(c["lat"], c["species"], c["pos"], c["rcut"]), L.fget_bond(c["lat"], c["species"], c["pos"], 0.9)]
+    np.testing.assert_allclose(got, want, rtol=1e-14, atol=0)
+
+
+def test_fgap_calc_batch_and_status(in_model_dir, ref_vectors):
+    import libgap.libgap as L
+    cases = [x for x in golden_cases(ref_vectors) if x["model"] == "example" and x["lgrad"]]
+    n, d, theta, mm, invcmm, coeff = L.fgap_read()
+    natoms = np.array([len(c["species"]) for c in cases], np.int32)
+    species = np.concatenate([c["species"] for c in cases]).astype(np.int32)
+    lat = np.stack([c["lat"] for c in cases], axis=2)        # LAT(3,3,nstruct)
+    pos = np.concatenate([c["pos"] for c in cases]).T.copy()  # pos(3,ntot)
+    ene, force, stress, status = L.fgap_calc_batch(natoms, species, lat, pos, theta[:d], mm[:n, :d], coeff[:n],
+                                                   6.0, True, 0)
+    assert status == 0
+    off = np.concatenate([[0], np.cumsum(natoms)])
+    for k, c in enumerate(cases):
+        assert abs(ene[k] - c["energy"]) / natoms[k] < 1e-9
+        assert np.abs(force[:, off[k]:off[k + 1]].T - c["forces"]).max() < 1e-7
+        assert np.abs(stress[:, k] - c["stress"]).max() < 1e-6
+    bad = cases[0]["species"].copy()
+    bad[0] = 3
+    out = L.fgap_calc_status(bad, cases[0]["lat"], cases[0]["pos"], theta[:d], mm[:n, :d], coeff[:n], 6.0, True, 0)
+    assert out[-1] == 4
+    assert b"atomic number 3" in L.fgap_last_error()
+
+
+def test_fgap_descriptors_match_oracle(in_model_dir, frames):
+    import libgap.libgap as L
+    n, d, theta, mm, invcmm, coeff = L.fgap_read()
+    xx, status = L.fgap_descriptors(frames["numbers"], frames["cells"][2], frames["positions"][2], theta[:d],
+                                    mm[:n, :d], coeff[:n], 6.0)
+    assert status == 0
+    om = oracle.parse_model("gap_parameters")
+    xo = oracle.descriptors(om, frames["numbers"], frames["cells"][2], frames["positions"][2], 6.0)
+    np.testing.assert_allclose(xx.T, xo, rtol=1e-11, atol=1e-9)
