@@ -105,6 +105,40 @@ static int build_tables(Model &m) {
   return GAPX_OK;
 }
 
+// Recognise an angular table of the form eta x (lambda = +1, -1) x zeta
+// (one cutoff group, the same zeta list for every eta, zeta a power of two
+// up to 16); the descriptor kernels then use their unrolled grid form.
+static void detect_grid(const SFTables &t, SFDev &sd) {
+  sd.grid = 0;
+  if (t.n_ang == 0 || std::getenv("GAPX_NO_GRID")) return;
+  const int g = t.ang_group[0];
+  for (int s = 0; s < t.n_ang; ++s)
+    if (t.ang_group[s] != g) return;
+  int nz = 0;
+  while (nz < t.n_ang && t.ang_eta[nz] == t.ang_eta[0] && t.ang_lam[nz] == 1.0) ++nz;
+  if (nz == 0 || nz > 5 || t.n_ang % (2 * nz) != 0) return;
+  const int ne = t.n_ang / (2 * nz);
+  for (int h = 0; h < ne; ++h)
+    for (int li = 0; li < 2; ++li)
+      for (int k = 0; k < nz; ++k) {
+        const int s = h * 2 * nz + li * nz + k;
+        if (t.ang_eta[s] != t.ang_eta[h * 2 * nz]) return;
+        if (t.ang_lam[s] != (li == 0 ? 1.0 : -1.0)) return;
+        if (t.ang_zeta[s] != t.ang_zeta[k]) return;
+      }
+  for (int k = 0; k < nz; ++k) {
+    const int z = t.ang_zeta[k];
+    if (z != 1 && z != 2 && z != 4 && z != 8 && z != 16) return;
+    sd.zlog[k] = (z == 1) ? 0 : (z == 2) ? 1 : (z == 4) ? 2 : (z == 8) ? 3 : 4;
+    sd.zeta_k[k] = z;
+    sd.pref_k[k] = t.ang_pref[k];
+  }
+  sd.nz = nz;
+  sd.n_eta = ne;
+  sd.g_ang = g;
+  sd.grid = 1;
+}
+
 static uint16_t f16_bits(double v) { return __builtin_bit_cast(uint16_t, (_Float16)v); }
 static double f16_val(uint16_t b) { return (double)__builtin_bit_cast(_Float16, b); }
 static uint16_t bf16_bits(double v) {
@@ -243,6 +277,7 @@ static int upload_model(Model &m) {
   sd.rad_a = rad_a; sd.rad_rs = rad_rs; sd.rad_group = rad_group; sd.rad_feat = rad_feat;
   sd.ang_eta = ang_eta; sd.ang_lam = ang_lam; sd.ang_pref = ang_pref; sd.ang_zeta_real = ang_zr;
   sd.ang_zeta = ang_zeta; sd.ang_group = ang_group; sd.ang_feat = ang_feat; sd.group_rc = group_rc;
+  detect_grid(t, sd);
   return GAPX_OK;
 }
 
@@ -270,11 +305,12 @@ struct Ctx {
   std::vector<StructParams> sp;
   std::map<std::string, DevMem> mem;
   size_t bytes = 0;
-  int kcap = 64, KP = 64;
+  int kcap = 64, KP = 64, tcap = 1024;
   int *host_stats = nullptr;  // pinned copy of the last run's neighbour stats
   int last_flags = -1;
   bool ran = false;
-  int chunk = 16;
+  int chunk = 0;  // lines per forward pass (0: 16)
+  int waves_fwd = 4, waves_bwd = 4;  // wavefronts per atom in the descriptor kernels
   // timing
   bool timing = false;
   hipEvent_t ev[2 * kNumStages] = {};
@@ -368,6 +404,16 @@ static int struct_params(Ctx &c) {
 }
 
 // Initial slots per atom from the densest structure of the batch.
+static int initial_tcap(const Ctx &c) {
+  // pairs within the angular cutoff: about (15/32) K (K - 1) / 2 at uniform density
+  const double ra = c.m->sf.rc_max_ang;
+  double kest = 0;
+  for (int s = 0; s < c.nstruct; ++s)
+    kest = std::max(kest, 4.0 / 3.0 * 3.141592653589793 * ra * ra * ra * c.natoms[s] / c.sp[s].vol);
+  const double t = 15.0 / 32.0 * kest * kest / 2.0;
+  return round_up(std::max(256, (int)(1.6 * t) + 128), 64);
+}
+
 static int initial_kcap(const Ctx &c) {
   double kest = 0;
   for (int s = 0; s < c.nstruct; ++s)
@@ -428,6 +474,7 @@ static int ctx_load(Ctx &c, int nstruct, const int *natoms, const int *species, 
   GAPX_HIP(hipMemcpyAsync(d_sp, c.sp.data(), sizeof(StructParams) * nstruct, hipMemcpyHostToDevice, c.stream));
   GAPX_HIP(hipStreamSynchronize(c.stream));
   c.kcap = initial_kcap(c);
+  c.tcap = initial_tcap(c);
   c.ran = false;
   c.invalidate_graph();
   return GAPX_OK;
@@ -435,6 +482,7 @@ static int ctx_load(Ctx &c, int nstruct, const int *natoms, const int *species, 
 
 struct Bufs {
   NeighborBufs nb{};
+  TripletBufs tb{};
   DescOut out{};
   GprArgs g{};
   double *slot_f = nullptr, *atom_vir = nullptr, *force = nullptr, *ene = nullptr, *stress = nullptr;
@@ -485,7 +533,7 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b, bool grad) {
       (rc = c.get("bin_cursor", std::max(c.nbins, c.N), &nb.bin_cursor)) ||
       (rc = c.get("bin_atoms", c.N, &nb.bin_atoms)) || (rc = c.get("atom_bin", c.N, &nb.atom_bin)) ||
       (rc = c.get("wpos", 3 * (size_t)c.N, &nb.wpos)) || (rc = c.get("shift", 3 * (size_t)c.N, &nb.shift)) ||
-      (rc = c.get("nb_count", c.N, &nb.nb_count)) || (rc = c.get("stats", 4, &stats)) ||
+      (rc = c.get("nb_count", c.N, &nb.nb_count)) || (rc = c.get("stats", 8, &stats)) ||
       (rc = c.get("nb_j", slots, &nb.nb_j)) || (rc = c.get("nb_key", slots, &nb.nb_key)) ||
       (rc = c.get("nb_d", 4 * slots, &nb.nb_d)) || (rc = c.get("nb_rev", slots, &nb.nb_rev)))
     return rc;
@@ -497,13 +545,24 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b, bool grad) {
   const StructParams *d_sp = c.ptr<StructParams>("sp");
   GAPX_HIP(launch_fill_int(c.stream, nb.bin_count, c.nbins, 0));
   GAPX_HIP(launch_fill_int(c.stream, nb.bin_cursor, c.nbins, 0));
-  GAPX_HIP(launch_fill_int(c.stream, stats, 4, 0));
+  GAPX_HIP(launch_fill_int(c.stream, stats, 8, 0));
   GAPX_HIP(launch_wrap_bin(c.stream, c.N, d_pos, d_astruct, d_sp, nb));
   GAPX_HIP(launch_scan(c.stream, nb.bin_count, nb.bin_start, c.nbins, nullptr));
   GAPX_HIP(launch_scatter_bins(c.stream, c.N, c.nbins, nb));
   GAPX_HIP(launch_build_neighbors(c.stream, c.N, d_astruct, d_sp, c.rcut, nb));
   if (grad) GAPX_HIP(launch_reverse_slots(c.stream, c.N, nb));
   c.KP = c.kcap;
+  // triplet lists for the angular lines (forward and backward share them)
+  TripletBufs &tb = b.tb;
+  tb.tcap = c.tcap;
+  tb.stats = stats + 4;
+  if ((rc = c.get("trip_list", (size_t)c.N * c.tcap, &tb.list)) || (rc = c.get("trip_count", c.N, &tb.count)))
+    return rc;
+  if (c.m->sf.n_ang > 0) {
+    GAPX_HIP(launch_triplets(c.stream, c.N, nb, c.m->sf.rc_max_ang, tb));
+  } else {
+    GAPX_HIP(launch_fill_int(c.stream, tb.count, c.N, 0));
+  }
   return GAPX_OK;
 }
 
@@ -563,12 +622,11 @@ static int ctx_run(Ctx &c, int flags) {
       GAPX_HIP(launch_fill_int(c.stream, (int *)(o.xt_lo + (size_t)c.N * Dp), (c.Np - c.N) * Dp / 2, 0));
     }
   }
-  if ((rc = c.get("counters", 2, &o.triplets))) return rc;
-  GAPX_HIP(launch_fill_int(c.stream, (int *)o.triplets, 4, 0));
   const int split_mode = fp64 ? 0 : ((flags & GAPX_BF16) ? 2 : 1);
   stage_begin(c, kDescFwd);
-  GAPX_HIP(launch_descriptors_fwd(c.stream, fp64, c.N, d_species, m.d_wz, nb, m.sfd, m.d_theta_inv, m.d_mu,
-                                  split_mode, o, c.chunk, c.KP));
+  const int chunk = c.chunk > 0 ? c.chunk : 16;
+  GAPX_HIP(launch_descriptors_fwd(c.stream, fp64, c.N, d_species, m.d_wz, nb, b.tb, m.sfd, m.d_theta_inv, m.d_mu,
+                                  split_mode, o, chunk, c.waves_fwd, c.KP));
   stage_end(c, kDescFwd);
 
   // contraction
@@ -616,8 +674,8 @@ static int ctx_run(Ctx &c, int flags) {
   if (grad) {
     if ((rc = c.get("slot_f", 3 * (size_t)c.N * c.kcap, &b.slot_f))) return rc;
     stage_begin(c, kDescBwd);
-    GAPX_HIP(launch_descriptors_bwd(c.stream, fp64, c.N, d_species, m.d_wz, nb, m.sfd, Dp, g.g64, g.g32, b.slot_f,
-                                    b.atom_vir, c.KP));
+    GAPX_HIP(launch_descriptors_bwd(c.stream, fp64, c.N, d_species, m.d_wz, nb, b.tb, m.sfd, Dp, g.g64, g.g32,
+                                    b.slot_f, b.atom_vir, c.waves_bwd, c.KP));
     stage_end(c, kDescBwd);
   }
   stage_begin(c, kForces);
@@ -627,7 +685,7 @@ static int ctx_run(Ctx &c, int flags) {
     GAPX_HIP(launch_fill_double(c.stream, b.force, 3 * (size_t)c.N, 0.0));
   GAPX_HIP(launch_struct_reduce(c.stream, c.nstruct, d_sp, g.eatom, b.atom_vir, grad ? 1 : 0, b.ene, b.stress));
   stage_end(c, kForces);
-  GAPX_HIP(hipMemcpyAsync(c.host_stats, nb.total, 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  GAPX_HIP(hipMemcpyAsync(c.host_stats, nb.total, 8 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
   c.last_flags = flags;
   c.ran = true;
   if (c.timing) {  // one event pair per stage: read them before the next run
@@ -651,8 +709,9 @@ static int ctx_settle(Ctx &c) {
       return fail(GAPX_ERR_NEIGHBORS, buf);
     }
     if (h[3]) return fail(GAPX_ERR_NEIGHBORS, "descriptor kernels saw more neighbours than their LDS layout holds");
-    if (h[2] == 0) return GAPX_OK;
-    c.kcap = round_up(h[1] + 8, 8);
+    if (h[2] == 0 && h[6] == 0) return GAPX_OK;
+    if (h[2]) c.kcap = round_up(h[1] + 8, 8);
+    if (h[6]) c.tcap = round_up(h[5] + 64, 64);
     c.invalidate_graph();
     int rc = ctx_run(c, c.last_flags);
     if (rc) return rc;
@@ -691,14 +750,18 @@ static int ctx_create(Model *m, Ctx **out) {
     return fail(GAPX_ERR_HIP, "could not create a HIP stream");
   }
   for (auto &e : c->ev) hipEventCreate(&e);
-  if (hipHostMalloc((void **)&c->host_stats, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void **)&c->host_stats, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     hipStreamDestroy(c->stream);
     delete c;
     return fail(GAPX_ERR_NOMEM, "pinned host allocation failed");
   }
-  std::memset(c->host_stats, 0, 4 * sizeof(int));
-  const char *ch = std::getenv("GAPX_DESC_CHUNK");
-  if (ch) c->chunk = std::atoi(ch);
+  std::memset(c->host_stats, 0, 8 * sizeof(int));
+  // Grid-form tables run fastest with one wavefront per atom (measured on the
+  // 4096-atom ternary cell, D = 128); the generic kernels with four.
+  if (c->m->sfd.grid) c->waves_fwd = c->waves_bwd = 1;
+  if (const char *ch = std::getenv("GAPX_DESC_CHUNK")) c->chunk = std::atoi(ch);
+  if (const char *w = std::getenv("GAPX_DESC_WAVES_FWD")) c->waves_fwd = std::atoi(w);
+  if (const char *w = std::getenv("GAPX_DESC_WAVES_BWD")) c->waves_bwd = std::atoi(w);
   *out = c;
   return GAPX_OK;
 }
@@ -1137,11 +1200,7 @@ int gapx_ctx_counts(gapx_ctx *c, long long *pairs, long long *triplets) {
   int rc = ctx_settle(*c);
   if (rc) return rc;
   if (pairs) *pairs = c->host_stats[0];
-  if (triplets) {
-    unsigned long long t = 0;
-    GAPX_HIP(hipMemcpy(&t, c->ptr<unsigned long long>("counters"), sizeof t, hipMemcpyDeviceToHost));
-    *triplets = (long long)t;
-  }
+  if (triplets) *triplets = c->host_stats[4];
   return GAPX_OK;
 }
 

@@ -21,11 +21,15 @@
 //
 // Work layout: the atom's neighbours live in LDS (positions relative to the
 // centre, distance, weight, cutoff function per cutoff group).  Radial sums
-// put one neighbour on each lane.  Angular sums enumerate neighbour pairs
-// j < k, compact the pairs that pass the cutoff test into an LDS queue with a
-// ballot, and process them 64 at a time, so every lane holds a live triplet.
-// The forward keeps NCH lines' partial sums in registers per lane and reduces
-// them across the wavefront after each chunk of lines.
+// put one neighbour on each lane.  Angular sums read the atom's triplet list,
+// built once per evaluation by k_triplets (neighbour pairs j < k with all
+// three distances within the largest angular cutoff, compacted with a
+// ballot), 64 triplets per step, so every lane holds a live triplet and
+// forward and backward share one enumeration.  The forward keeps NCH lines'
+// partial sums in registers per lane and reduces them across the wavefront
+// after each chunk of lines.  Lines are sorted by (cutoff group, eta, lambda,
+// zeta), so exponentials, cutoff products and the powers (1 + lambda cos)^zeta
+// are updated incrementally along a chunk (zeta = 2, 4, 16 by squaring).
 #include "gapx_internal.h"
 
 namespace gapx {
@@ -60,42 +64,54 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
-// base^n for an integer n >= 0 that is the same on every lane.
+// Powers (1 + lam cos)^zeta along lines sorted by (lam, zeta) within an eta
+// group.  pw = base^z and pm = base^(z-1) for the current z; a new lambda or
+// a smaller zeta restarts at z = 1; larger integer zetas are reached by
+// squaring (z -> 2z) or one more factor (z -> z+1), decisions that are the
+// same on every lane.
 template <typename T>
-__device__ __forceinline__ T ipow(T base, int n) {
-  T r = T(1);
-  while (n) {
-    if (n & 1) r *= base;
-    base *= base;
-    n >>= 1;
+struct PowState {
+  T lam = T(0), base = T(0), pw = T(0), pm = T(1);
+  int z = 0;
+  __device__ __forceinline__ void at(T new_lam, int zi, T cth) {
+    if (new_lam != lam || zi < z || z == 0) {
+      lam = new_lam;
+      base = T(1) + lam * cth;
+      pw = base;
+      pm = T(1);
+      z = 1;
+    }
+    while (z < zi) {
+      if (2 * z <= zi) {
+        pm = pw * pm;
+        pw = pw * pw;
+        z *= 2;
+      } else {
+        pm = pw;
+        pw = pw * base;
+        z += 1;
+      }
+    }
   }
-  return r;
-}
+};
 
-// (a, b), a < b, of the p-th pair in row-major order of the upper triangle.
-__device__ __forceinline__ void pair_index(int p, int K, int &a, int &b) {
-  const int n2 = 2 * K - 1;
-  double disc = (double)n2 * n2 - 8.0 * p;
-  int r = (int)((n2 - sqrt(disc)) * 0.5);
-  if (r < 0) r = 0;
-  while (r > 0 && r * (2 * K - r - 1) / 2 > p) --r;
-  while ((r + 1) * (2 * K - r - 2) / 2 <= p) ++r;
-  a = r;
-  b = p - r * (2 * K - r - 1) / 2 + r + 1;
-}
-
-// Angular power P = pref base^zeta and dP/dcos = pref zeta lam base^(zeta-1).
+// P = pref base^zeta and dP/dcos = pref zeta lam base^(zeta-1) of line s.
 template <typename T>
-__device__ __forceinline__ void ang_pow(T base, T lam, int zi, T zr, T pref, T &P, T &dP) {
+__device__ __forceinline__ void line_power(const SFDev &sf, int s, T cth, PowState<T> &ps, T &P, T &dP) {
+  const int zi = sf.ang_zeta[s];
+  const T lam = (T)sf.ang_lam[s];
   if (zi == 1) {
-    P = base;
+    ps.at(lam, 1, cth);
+    P = ps.pw;
     dP = lam;
   } else if (zi > 1) {
-    T b1 = ipow(base, zi - 1);
-    P = pref * b1 * base;
-    dP = pref * (T)zi * lam * b1;
-  } else {
-    T b1 = (base > T(0)) ? Math<T>::pw(base, zr - T(1)) : T(0);
+    ps.at(lam, zi, cth);
+    const T pref = (T)sf.ang_pref[s];
+    P = pref * ps.pw;
+    dP = pref * (T)zi * lam * ps.pm;
+  } else {  // non-integer zeta
+    const T base = T(1) + lam * cth, zr = (T)sf.ang_zeta_real[s], pref = (T)sf.ang_pref[s];
+    const T b1 = (base > T(0)) ? Math<T>::pw(base, zr - T(1)) : T(0);
     P = pref * b1 * base;
     dP = pref * zr * lam * b1;
   }
@@ -106,26 +122,56 @@ struct Lds {
 };
 
 // ---------------------------------------------------------------------------
-// shared neighbour staging
+// triplet lists
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ void stage_neighbors(int start, int K, int lane, const NeighborBufs &nb,
-                                                const int *__restrict__ species, const double *__restrict__ wz,
-                                                const SFDev &sf, int KP, T *s_d, T *s_w, T *s_fc, T *s_dfc) {
+// One wavefront per centre atom, neighbours staged in LDS.  Pairs (a, b),
+// a < b, with r_a, r_b, r_ab all within the largest angular cutoff are
+// appended (ballot + prefix count) to the atom's list at i * tcap, packed as
+// a | b << 16, in diagonal order: d = b - a outer, a inner.  Consecutive
+// entries then have distinct a and distinct b, so the backward pass's LDS
+// atomics on per-neighbour accumulators rarely collide within one
+// instruction.  The cutoff test is a squared comparison with a relative
+// margin of 1e-6; a triplet admitted by the margin contributes zero, since the
+// descriptor kernels apply each line's cutoff to every distance.
+__global__ __launch_bounds__(64) void k_triplets(int N, NeighborBufs nb, double rcA, TripletBufs tb) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float *s_d = (float *)smem;  // [4 * kcap]
+  const int i = blockIdx.x, lane = threadIdx.x;
+  if (i >= N) return;
+  const int K = nb.nb_count[i];
+  const size_t start = (size_t)i * nb.kcap;
   for (int a = lane; a < K; a += 64) {
-    const double *dd = nb.nb_d + 4 * (size_t)(start + a);
-    const double r = dd[3];
-    s_d[4 * a] = (T)dd[0];
-    s_d[4 * a + 1] = (T)dd[1];
-    s_d[4 * a + 2] = (T)dd[2];
-    s_d[4 * a + 3] = (T)r;
-    s_w[a] = (T)wz[species[nb.nb_j[start + a]]];
-    for (int g = 0; g < sf.n_groups; ++g) {
-      const double rc = sf.group_rc[g];
-      const bool in = r <= rc;
-      s_fc[g * KP + a] = in ? (T)(0.5 * (cos(kPi * r / rc) + 1.0)) : T(0);
-      if (s_dfc) s_dfc[g * KP + a] = in ? (T)(0.5 * (-sin(kPi * r / rc)) * (kPi / rc)) : T(0);
+    const double *dd = nb.nb_d + 4 * (start + a);
+    s_d[4 * a] = (float)dd[0];
+    s_d[4 * a + 1] = (float)dd[1];
+    s_d[4 * a + 2] = (float)dd[2];
+    s_d[4 * a + 3] = (float)dd[3];
+  }
+  __syncthreads();
+  const float rc = (float)rcA * (1.0f + 1e-6f);
+  const float rc2 = rc * rc;
+  const size_t out = (size_t)i * tb.tcap;
+  int count = 0;
+  for (int d = 1; d < K; ++d) {
+    for (int a0 = 0; a0 + d < K; a0 += 64) {
+      const int a = a0 + lane, b = a + d;
+      bool ok = false;
+      if (b < K && s_d[4 * a + 3] <= rc && s_d[4 * b + 3] <= rc) {
+        const float ex = s_d[4 * b] - s_d[4 * a], ey = s_d[4 * b + 1] - s_d[4 * a + 1],
+                    ez = s_d[4 * b + 2] - s_d[4 * a + 2];
+        ok = ex * ex + ey * ey + ez * ez <= rc2;
+      }
+      const unsigned long long m = __ballot(ok);
+      const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+      if (ok && pos < tb.tcap) tb.list[out + pos] = (uint32_t)a | ((uint32_t)b << 16);
+      count += __popcll(m);
     }
+  }
+  if (lane == 0) {
+    tb.count[i] = min(count, tb.tcap);
+    atomicAdd(&tb.stats[0], min(count, tb.tcap));
+    atomicMax(&tb.stats[1], count);
+    if (count > tb.tcap) atomicAdd(&tb.stats[2], 1);
   }
 }
 
@@ -148,6 +194,7 @@ __device__ __forceinline__ void ang_fwd_term(uint32_t q, bool active, int c0, co
   const T on = active ? T(1) : T(0);
   int cur_g = -1;
   T F3 = 0, E = 0, cur_eta = T(-1);
+  PowState<T> ps;
 #pragma unroll
   for (int l = 0; l < NCH; ++l) {
     const int s = c0 + l;
@@ -155,6 +202,7 @@ __device__ __forceinline__ void ang_fwd_term(uint32_t q, bool active, int c0, co
       const int g = sf.ang_group[s];
       if (g != cur_g) {
         cur_g = g;
+        cur_eta = T(-1);  // E below carries this group's cutoff product
         const T rc = (T)sf.group_rc[g];
         const T fe = (e <= rc) ? T(0.5) * (Math<T>::cs((T)kPi * e / rc) + T(1)) : T(0);
         F3 = s_fc[g * KP + a] * s_fc[g * KP + b] * fe * on;
@@ -162,41 +210,81 @@ __device__ __forceinline__ void ang_fwd_term(uint32_t q, bool active, int c0, co
       const T eta = (T)sf.ang_eta[s];
       if (eta != cur_eta) {
         cur_eta = eta;
-        E = Math<T>::ex(-eta * S2);
+        E = Math<T>::ex(-eta * S2) * F3;
       }
       T P, dP;
-      ang_pow<T>(T(1) + (T)sf.ang_lam[s] * cth, (T)sf.ang_lam[s], sf.ang_zeta[s], (T)sf.ang_zeta_real[s],
-                 (T)sf.ang_pref[s], P, dP);
-      const T t = P * E * F3;
+      line_power<T>(sf, s, cth, ps, P, dP);
+      const T t = P * E;
       acc[l] += t;
       accw[l] += t * wab;
     }
   }
 }
 
-template <typename T, int NCH>
-__global__ __launch_bounds__(64) void k_desc_fwd(int N, const int *__restrict__ species,
-                                                 const double *__restrict__ wz, NeighborBufs nb, SFDev sf,
-                                                 Lds L, const double *__restrict__ theta_inv,
-                                                 const double *__restrict__ mu, int split_mode, DescOut out) {
+// W wavefronts per atom (workgroup of 64 W threads): neighbours, triplets
+// and radial terms are split over the waves; each chunk's per-lane sums are
+// reduced within a wave and then across the waves through LDS.
+template <typename T, int NCH, int W>
+__device__ __forceinline__ void chunk_reduce(T (&acc)[NCH], T (&accw)[NCH], int wv, int lane, int tid, int c0,
+                                             int n, const int *__restrict__ feat, int nsf, T *s_part, T *s_feat) {
+#pragma unroll
+  for (int l = 0; l < NCH; ++l) {
+    const T v = wave_sum(acc[l]);
+    const T vw = wave_sum(accw[l]);
+    if (lane == 0) {
+      s_part[wv * 2 * NCH + l] = v;
+      s_part[wv * 2 * NCH + NCH + l] = vw;
+    }
+  }
+  __syncthreads();
+  if (tid < 2 * NCH) {
+    const int l = tid % NCH;
+    if (c0 + l < n) {
+      T sum = T(0);
+#pragma unroll
+      for (int w = 0; w < W; ++w) sum += s_part[w * 2 * NCH + tid];
+      s_feat[feat[c0 + l] + (tid >= NCH ? nsf : 0)] = sum;
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int NCH, int W>
+__global__ __launch_bounds__(256) void k_desc_fwd(int N, const int *__restrict__ species,
+                                                  const double *__restrict__ wz, NeighborBufs nb, TripletBufs tb,
+                                                  SFDev sf, Lds L, const double *__restrict__ theta_inv,
+                                                  const double *__restrict__ mu, int split_mode, DescOut out) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int i = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int KP = L.KP, Dp = L.Dp;
-  T *s_d = (T *)smem;
+  double *s_xn = (double *)smem;  // [W]
+  T *s_d = (T *)(s_xn + W);
   T *s_w = s_d + 4 * KP;
   T *s_fc = s_w + KP;
   T *s_feat = s_fc + L.G * KP;
-  uint32_t *s_q = (uint32_t *)(s_feat + Dp);
+  T *s_part = s_feat + Dp;  // [W * 2 * NCH]
 
   const int start = i * nb.kcap;
   const int K = nb.nb_count[i];
-  if (K > KP) {  // the host sizes KP from the largest count; never index LDS past it
-    if (lane == 0) atomicAdd(&nb.total[3], 1);
+  if (K > KP) {  // the host sizes KP from the capacity; never index LDS past it
+    if (tid == 0) atomicAdd(&nb.total[3], 1);
     return;
   }
-  stage_neighbors<T>(start, K, lane, nb, species, wz, sf, KP, s_d, s_w, s_fc, (T *)nullptr);
-  for (int d = lane; d < Dp; d += 64) s_feat[d] = T(0);
+  for (int a = tid; a < K; a += 64 * W) {
+    const double *dd = nb.nb_d + 4 * (size_t)(start + a);
+    const double r = dd[3];
+    s_d[4 * a] = (T)dd[0];
+    s_d[4 * a + 1] = (T)dd[1];
+    s_d[4 * a + 2] = (T)dd[2];
+    s_d[4 * a + 3] = (T)r;
+    s_w[a] = (T)wz[species[nb.nb_j[start + a]]];
+    for (int g = 0; g < sf.n_groups; ++g) {
+      const double rc = sf.group_rc[g];
+      s_fc[g * KP + a] = (r <= rc) ? (T)(0.5 * (cos(kPi * r / rc) + 1.0)) : T(0);
+    }
+  }
+  for (int d = tid; d < Dp; d += 64 * W) s_feat[d] = T(0);
   __syncthreads();
 
   // radial lines, NCH at a time
@@ -204,7 +292,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd(int N, const int *__restrict__ 
     T acc[NCH], accw[NCH];
 #pragma unroll
     for (int l = 0; l < NCH; ++l) acc[l] = accw[l] = T(0);
-    for (int a = lane; a < K; a += 64) {
+    for (int a = tid; a < K; a += 64 * W) {
       const T r = s_d[4 * a + 3], w = s_w[a];
 #pragma unroll
       for (int l = 0; l < NCH; ++l) {
@@ -217,80 +305,28 @@ __global__ __launch_bounds__(64) void k_desc_fwd(int N, const int *__restrict__ 
         }
       }
     }
-#pragma unroll
-    for (int l = 0; l < NCH; ++l) {
-      const T v = wave_sum(acc[l]);
-      const T vw = wave_sum(accw[l]);
-      if (lane == 0 && c0 + l < sf.n_rad) {
-        const int f = sf.rad_feat[c0 + l];
-        s_feat[f] = v;
-        s_feat[f + sf.nsf] = vw;
-      }
-    }
+    chunk_reduce<T, NCH, W>(acc, accw, wv, lane, tid, c0, sf.n_rad, sf.rad_feat, sf.nsf, s_part, s_feat);
   }
 
-  // angular lines, NCH at a time, over compacted triplets
-  const int npairs = K * (K - 1) / 2;
-  const T rcA = (T)sf.rc_max_ang;
-  unsigned long long ntrip = 0;
+  // angular lines, NCH at a time, over the atom's triplet list
+  const uint32_t *tl = tb.list + (size_t)i * tb.tcap;
+  const int nt = tb.count[i];
   for (int c0 = 0; c0 < sf.n_ang; c0 += NCH) {
     T acc[NCH], accw[NCH];
 #pragma unroll
     for (int l = 0; l < NCH; ++l) acc[l] = accw[l] = T(0);
-    int qn = 0;
-    for (int p0 = 0; p0 < npairs; p0 += 64) {
-      const int p = p0 + lane;
-      bool ok = false;
-      int a = 0, b = 0;
-      if (p < npairs) {
-        pair_index(p, K, a, b);
-        if (s_d[4 * a + 3] <= rcA && s_d[4 * b + 3] <= rcA) {
-          const T ex = s_d[4 * b] - s_d[4 * a], ey = s_d[4 * b + 1] - s_d[4 * a + 1],
-                  ez = s_d[4 * b + 2] - s_d[4 * a + 2];
-          ok = Math<T>::sq(ex * ex + ey * ey + ez * ez) <= rcA;
-        }
-      }
-      const unsigned long long m = __ballot(ok);
-      if (ok) s_q[qn + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)a | ((uint32_t)b << 16);
-      qn += __popcll(m);
-      ntrip += __popcll(m);
-      __syncthreads();
-      if (qn >= 64) {
-        ang_fwd_term<T, NCH>(s_q[lane], true, c0, sf, s_d, s_w, s_fc, KP, acc, accw);
-        const int rest = qn - 64;
-        const uint32_t mv = (lane < rest) ? s_q[64 + lane] : 0u;
-        __syncthreads();
-        if (lane < rest) s_q[lane] = mv;
-        qn = rest;
-        __syncthreads();
-      }
+    for (int t0 = 64 * wv; t0 < nt; t0 += 64 * W) {
+      const bool act = t0 + lane < nt;
+      const uint32_t q = act ? tl[t0 + lane] : (1u << 16);
+      ang_fwd_term<T, NCH>(q, act, c0, sf, s_d, s_w, s_fc, KP, acc, accw);
     }
-    if (qn > 0) {
-      const bool act = lane < qn;
-      ang_fwd_term<T, NCH>(act ? s_q[lane] : (1u << 16), act, c0, sf, s_d, s_w, s_fc, KP, acc, accw);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int l = 0; l < NCH; ++l) {
-      const T v = wave_sum(acc[l]);
-      const T vw = wave_sum(accw[l]);
-      if (lane == 0 && c0 + l < sf.n_ang) {
-        const int f = sf.ang_feat[c0 + l];
-        s_feat[f] = v;
-        s_feat[f + sf.nsf] = vw;
-      }
-    }
-  }
-  __syncthreads();
-  if (lane == 0 && out.triplets && ntrip) {
-    const int chunks = (sf.n_ang + NCH - 1) / NCH;
-    atomicAdd(out.triplets, ntrip / (unsigned long long)chunks);
+    chunk_reduce<T, NCH, W>(acc, accw, wv, lane, tid, c0, sf.n_ang, sf.ang_feat, sf.nsf, s_part, s_feat);
   }
 
   // epilogue: raw features, centred/scaled features, FP64 norm, split operands
   double xn = 0.0;
   const size_t row = (size_t)i * Dp;
-  for (int d = lane; d < Dp; d += 64) {
+  for (int d = tid; d < Dp; d += 64 * W) {
     const T x = s_feat[d];
     if (out.x64) out.x64[row + d] = (double)x;
     if (out.x32) out.x32[row + d] = (float)x;
@@ -311,7 +347,299 @@ __global__ __launch_bounds__(64) void k_desc_fwd(int N, const int *__restrict__ 
     }
   }
   xn = wave_sum(xn);
-  if (lane == 0) out.xn[i] = xn;
+  if (lane == 0) s_xn[wv] = xn;
+  __syncthreads();
+  if (tid == 0) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) v += s_xn[w];
+    out.xn[i] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// grid form of the angular table: eta x (lambda = +1, -1) x zeta, zeta in
+// {1, 2, 4, 8, 16}, one cutoff group (SFDev::grid).  Powers come from one
+// squaring chain per lambda, (lambda, zeta) products are unrolled, and only
+// the eta loop is a (uniform) runtime loop.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T pick5(const T (&v)[5], int j) {
+  T r = v[0];
+  r = (j == 1) ? v[1] : r;
+  r = (j == 2) ? v[2] : r;
+  r = (j == 3) ? v[3] : r;
+  r = (j == 4) ? v[4] : r;
+  return r;
+}
+
+// P_k = pref_k b^zeta_k and dP_k/dcos = pref_k zeta_k lam b^(zeta_k - 1) for
+// b = 1 + lam cos, lam = +1 (p) and -1 (m).
+template <typename T, int NZ, bool DERIV>
+__device__ __forceinline__ void grid_powers(T cth, const SFDev &sf, T (&Pp)[NZ], T (&Pm)[NZ], T (&dPp)[NZ],
+                                            T (&dPm)[NZ]) {
+  const T bp = T(1) + cth, bm = T(1) - cth;
+  T pp[5], pm[5], qp[5], qm[5];  // b^(2^j) and b^(2^j - 1)
+  pp[0] = bp;
+  pm[0] = bm;
+  qp[0] = T(1);
+  qm[0] = T(1);
+#pragma unroll
+  for (int j = 1; j < 5; ++j) {
+    qp[j] = pp[j - 1] * qp[j - 1];
+    qm[j] = pm[j - 1] * qm[j - 1];
+    pp[j] = pp[j - 1] * pp[j - 1];
+    pm[j] = pm[j - 1] * pm[j - 1];
+  }
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) {
+    const int j = sf.zlog[k];
+    const T pref = (T)sf.pref_k[k];
+    Pp[k] = pref * pick5(pp, j);
+    Pm[k] = pref * pick5(pm, j);
+    if (DERIV) {
+      const T pz = pref * (T)sf.zeta_k[k];
+      dPp[k] = pz * pick5(qp, j);
+      dPm[k] = -pz * pick5(qm, j);
+    }
+  }
+}
+
+// Sum NV per-lane values over the workgroup: s_out[j] = total of v[j].
+template <typename T, int NV, int W>
+__device__ __forceinline__ void group_sum(T (&v)[NV], int wv, int lane, int tid, T *s_part, T *s_out) {
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    const T t = wave_sum(v[j]);
+    if (lane == 0) s_part[wv * NV + j] = t;
+  }
+  __syncthreads();
+  for (int j = tid; j < NV; j += 64 * W) {
+    T sum = T(0);
+#pragma unroll
+    for (int w = 0; w < W; ++w) sum += s_part[w * NV + j];
+    s_out[j] = sum;
+  }
+  __syncthreads();
+}
+
+template <typename T, int NZ>
+struct GridShape {
+  static constexpr int CAP = sizeof(T) == 4 ? 32 : 16;             // (plain, weighted) pairs per pass
+  static constexpr int NE = (CAP / (2 * NZ)) > 0 ? CAP / (2 * NZ) : 1;  // etas per pass
+  static constexpr int NV = NE * 2 * NZ * 2;
+};
+
+template <typename T, int NZ, int W>
+__global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restrict__ species,
+                                                       const double *__restrict__ wz, NeighborBufs nb,
+                                                       TripletBufs tb, SFDev sf, Lds L,
+                                                       const double *__restrict__ theta_inv,
+                                                       const double *__restrict__ mu, int split_mode, DescOut out) {
+  constexpr int NE = GridShape<T, NZ>::NE, NV = GridShape<T, NZ>::NV;
+  constexpr int RCH = 16;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int i = blockIdx.x;
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int KP = L.KP, Dp = L.Dp;
+  double *s_xn = (double *)smem;  // [W]
+  T *s_d = (T *)(s_xn + W);
+  T *s_w = s_d + 4 * KP;
+  T *s_fc = s_w + KP;
+  T *s_feat = s_fc + L.G * KP;
+  T *s_part = s_feat + Dp;      // [W * NV]
+  T *s_sum = s_part + W * NV;   // [NV]
+
+  const int start = i * nb.kcap;
+  const int K = nb.nb_count[i];
+  if (K > KP) {
+    if (tid == 0) atomicAdd(&nb.total[3], 1);
+    return;
+  }
+  for (int a = tid; a < K; a += 64 * W) {
+    const double *dd = nb.nb_d + 4 * (size_t)(start + a);
+    const double r = dd[3];
+    s_d[4 * a] = (T)dd[0];
+    s_d[4 * a + 1] = (T)dd[1];
+    s_d[4 * a + 2] = (T)dd[2];
+    s_d[4 * a + 3] = (T)r;
+    s_w[a] = (T)wz[species[nb.nb_j[start + a]]];
+    for (int g = 0; g < sf.n_groups; ++g) {
+      const double rc = sf.group_rc[g];
+      s_fc[g * KP + a] = (r <= rc) ? (T)(0.5 * (cos(kPi * r / rc) + 1.0)) : T(0);
+    }
+  }
+  for (int d = tid; d < Dp; d += 64 * W) s_feat[d] = T(0);
+  __syncthreads();
+
+  // radial lines, RCH at a time
+  for (int c0 = 0; c0 < sf.n_rad; c0 += RCH) {
+    T acc[RCH], accw[RCH];
+#pragma unroll
+    for (int l = 0; l < RCH; ++l) acc[l] = accw[l] = T(0);
+    for (int a = tid; a < K; a += 64 * W) {
+      const T r = s_d[4 * a + 3], w = s_w[a];
+#pragma unroll
+      for (int l = 0; l < RCH; ++l) {
+        const int s = c0 + l;
+        if (s < sf.n_rad) {
+          const T dr = r - (T)sf.rad_rs[s];
+          const T t = Math<T>::ex(-(T)sf.rad_a[s] * dr * dr) * s_fc[sf.rad_group[s] * KP + a];
+          acc[l] += t;
+          accw[l] += t * w;
+        }
+      }
+    }
+    chunk_reduce<T, RCH, W>(acc, accw, wv, lane, tid, c0, sf.n_rad, sf.rad_feat, sf.nsf, s_part, s_feat);
+  }
+
+  // angular lines in grid form, NE etas per pass
+  const uint32_t *tl = tb.list + (size_t)i * tb.tcap;
+  const int nt = tb.count[i];
+  const int g = sf.g_ang;
+  const T rc = (T)sf.group_rc[g];
+  for (int h0 = 0; h0 < sf.n_eta; h0 += NE) {
+    T acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = T(0);
+    for (int t0 = 64 * wv; t0 < nt; t0 += 64 * W) {
+      const bool act = t0 + lane < nt;
+      const uint32_t q = act ? tl[t0 + lane] : (1u << 16);
+      const int a = q & 0xffff, b = q >> 16;
+      const T ax = s_d[4 * a], ay = s_d[4 * a + 1], az = s_d[4 * a + 2], ra = s_d[4 * a + 3];
+      const T bx = s_d[4 * b], by = s_d[4 * b + 1], bz = s_d[4 * b + 2], rb = s_d[4 * b + 3];
+      const T ex = bx - ax, ey = by - ay, ez = bz - az;
+      const T e2 = ex * ex + ey * ey + ez * ez;
+      const T e = Math<T>::sq(e2);
+      const T cth = (ra * ra + rb * rb - e2) / (T(2) * ra * rb);
+      const T S2 = ra * ra + rb * rb + e2;
+      const T wab = s_w[a] * s_w[b];
+      const T fe = (e <= rc) ? T(0.5) * (Math<T>::cs((T)kPi * e / rc) + T(1)) : T(0);
+      const T F3 = act ? s_fc[g * KP + a] * s_fc[g * KP + b] * fe : T(0);
+      T Pp[NZ], Pm[NZ], dPp[NZ], dPm[NZ];
+      grid_powers<T, NZ, false>(cth, sf, Pp, Pm, dPp, dPm);
+#pragma unroll
+      for (int hh = 0; hh < NE; ++hh) {
+        const int h = h0 + hh;
+        if (h < sf.n_eta) {
+          const T E = Math<T>::ex(-(T)sf.ang_eta[h * 2 * NZ] * S2) * F3;
+#pragma unroll
+          for (int k = 0; k < NZ; ++k) {
+            const T tp = E * Pp[k], tm = E * Pm[k];
+            acc[(hh * 2 + 0) * NZ + k] += tp;
+            acc[(hh * 2 + 1) * NZ + k] += tm;
+            acc[NV / 2 + (hh * 2 + 0) * NZ + k] += tp * wab;
+            acc[NV / 2 + (hh * 2 + 1) * NZ + k] += tm * wab;
+          }
+        }
+      }
+    }
+    group_sum<T, NV, W>(acc, wv, lane, tid, s_part, s_sum);
+    for (int j = tid; j < NV; j += 64 * W) {
+      const int jl = j % (NV / 2);            // (hh, li, k)
+      const int h = h0 + jl / (2 * NZ);
+      if (h < sf.n_eta) s_feat[sf.ang_feat[h0 * 2 * NZ + jl] + (j >= NV / 2 ? sf.nsf : 0)] = s_sum[j];
+    }
+    __syncthreads();
+  }
+
+  // epilogue
+  double xn = 0.0;
+  const size_t row = (size_t)i * Dp;
+  for (int d = tid; d < Dp; d += 64 * W) {
+    const T x = s_feat[d];
+    if (out.x64) out.x64[row + d] = (double)x;
+    if (out.x32) out.x32[row + d] = (float)x;
+    const double xt = (double)x * theta_inv[d] - mu[d];
+    xn += xt * xt;
+    if (out.xt64) out.xt64[row + d] = xt;
+    if (out.xt32) out.xt32[row + d] = (float)xt;
+    if (split_mode == 1) {
+      const _Float16 hi = (_Float16)xt;
+      const _Float16 lo = (_Float16)(xt - (double)hi);
+      out.xt_hi[row + d] = __builtin_bit_cast(uint16_t, hi);
+      out.xt_lo[row + d] = __builtin_bit_cast(uint16_t, lo);
+    } else if (split_mode == 2) {
+      const __bf16 hi = (__bf16)(float)xt;
+      const __bf16 lo = (__bf16)(float)(xt - (double)(float)hi);
+      out.xt_hi[row + d] = __builtin_bit_cast(uint16_t, hi);
+      out.xt_lo[row + d] = __builtin_bit_cast(uint16_t, lo);
+    }
+  }
+  xn = wave_sum(xn);
+  if (lane == 0) s_xn[wv] = xn;
+  __syncthreads();
+  if (tid == 0) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) v += s_xn[w];
+    out.xn[i] = v;
+  }
+}
+
+template <typename T, int NZ>
+__device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDev &sf, const T *s_d, const T *s_w,
+                                             const T *s_fc, const T *s_dfc, const T *s_gA, const T *s_gW, int KP,
+                                             T *s_T, T *s_F, T (&vir)[6]) {
+  if (!active) return;
+  const int a = q & 0xffff, b = q >> 16;
+  const T ax = s_d[4 * a], ay = s_d[4 * a + 1], az = s_d[4 * a + 2], ra = s_d[4 * a + 3];
+  const T bx = s_d[4 * b], by = s_d[4 * b + 1], bz = s_d[4 * b + 2], rb = s_d[4 * b + 3];
+  const T ex = bx - ax, ey = by - ay, ez = bz - az;
+  const T e2 = ex * ex + ey * ey + ez * ez;
+  const T e = Math<T>::sq(e2);
+  const T inv2ab = T(1) / (T(2) * ra * rb);
+  const T cth = (ra * ra + rb * rb - e2) * inv2ab;
+  const T S2 = ra * ra + rb * rb + e2;
+  const T dca = T(1) / rb - cth / ra, dcb = T(1) / ra - cth / rb, dce = -e * (T(2) * inv2ab);
+  const T wab = s_w[a] * s_w[b];
+  const int g = sf.g_ang;
+  const T rc = (T)sf.group_rc[g];
+  const T fa = s_fc[g * KP + a], dfa = s_dfc[g * KP + a], fb = s_fc[g * KP + b], dfb = s_dfc[g * KP + b];
+  const bool in = e <= rc;
+  const T fe = in ? T(0.5) * (Math<T>::cs((T)kPi * e / rc) + T(1)) : T(0);
+  const T dfe = in ? T(0.5) * (-Math<T>::sn((T)kPi * e / rc)) * ((T)kPi / rc) : T(0);
+  T Pp[NZ], Pm[NZ], dPp[NZ], dPm[NZ];
+  grid_powers<T, NZ, true>(cth, sf, Pp, Pm, dPp, dPm);
+  T SA = 0, SB = 0, SC = 0;
+  for (int h = 0; h < sf.n_eta; ++h) {
+    const T eta = (T)sf.ang_eta[h * 2 * NZ];
+    const T E = Math<T>::ex(-eta * S2);
+    const int s0 = h * 2 * NZ;
+    T sa = 0, sb = 0;
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      const T gp = s_gA[s0 + k] + wab * s_gW[s0 + k];
+      const T gm = s_gA[s0 + NZ + k] + wab * s_gW[s0 + NZ + k];
+      sa += gp * dPp[k] + gm * dPm[k];
+      sb += gp * Pp[k] + gm * Pm[k];
+    }
+    SA += sa * E;
+    SB += sb * E;
+    SC += eta * sb * E;
+  }
+  const T F = fa * fb * fe;
+  const T Ta = F * SA * dca - T(2) * ra * F * SC + SB * dfa * fb * fe;
+  const T Tb = F * SA * dcb - T(2) * rb * F * SC + SB * fa * dfb * fe;
+  const T Te = F * SA * dce - T(2) * e * F * SC + SB * fa * fb * dfe;
+  atomicAdd(&s_T[a], Ta);
+  atomicAdd(&s_T[b], Tb);
+  if (e > T(0)) {
+    const T sc = Te / e;
+    const T ux = sc * ex, uy = sc * ey, uz = sc * ez;
+    atomicAdd(&s_F[3 * b], ux);
+    atomicAdd(&s_F[3 * b + 1], uy);
+    atomicAdd(&s_F[3 * b + 2], uz);
+    atomicAdd(&s_F[3 * a], -ux);
+    atomicAdd(&s_F[3 * a + 1], -uy);
+    atomicAdd(&s_F[3 * a + 2], -uz);
+    vir[0] += ux * ex;
+    vir[1] += uy * ey;
+    vir[2] += uz * ez;
+    vir[3] += ux * ey;
+    vir[4] += uy * ez;
+    vir[5] += ux * ez;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -319,8 +647,8 @@ __global__ __launch_bounds__(64) void k_desc_fwd(int N, const int *__restrict__ 
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ void ang_bwd_term(uint32_t q, bool active, const SFDev &sf, const T *s_d,
-                                             const T *s_w, const T *s_fc, const T *s_dfc, const T *s_g,
-                                             int KP, T *s_T, T *s_F, T (&vir)[6]) {
+                                             const T *s_w, const T *s_fc, const T *s_dfc, const T *s_gA,
+                                             const T *s_gW, int KP, T *s_T, T *s_F, T (&vir)[6]) {
   if (!active) return;
   const int a = q & 0xffff, b = q >> 16;
   const T ax = s_d[4 * a], ay = s_d[4 * a + 1], az = s_d[4 * a + 2], ra = s_d[4 * a + 3];
@@ -338,6 +666,7 @@ __device__ __forceinline__ void ang_bwd_term(uint32_t q, bool active, const SFDe
   T fa = 0, dfa = 0, fb = 0, dfb = 0, fe = 0, dfe = 0;
   int cur_g = -1;
   T E = 0, cur_eta = T(-1);
+  PowState<T> ps;
   auto flush = [&]() {
     const T F = fa * fb * fe;
     Ta += F * SA * dca - T(2) * ra * F * SC + SB * dfa * fb * fe;
@@ -364,13 +693,11 @@ __device__ __forceinline__ void ang_bwd_term(uint32_t q, bool active, const SFDe
       cur_eta = eta;
       E = Math<T>::ex(-eta * S2);
     }
-    const int f = sf.ang_feat[s];
-    const T geff = s_g[f] + wab * s_g[f + sf.nsf];
+    const T gE = (s_gA[s] + wab * s_gW[s]) * E;
     T P, dP;
-    ang_pow<T>(T(1) + (T)sf.ang_lam[s] * cth, (T)sf.ang_lam[s], sf.ang_zeta[s], (T)sf.ang_zeta_real[s],
-               (T)sf.ang_pref[s], P, dP);
-    const T B = geff * P * E;
-    SA += geff * dP * E;
+    line_power<T>(sf, s, cth, ps, P, dP);
+    const T B = gE * P;
+    SA += gE * dP;
     SB += B;
     SC += eta * B;
   }
@@ -395,41 +722,61 @@ __device__ __forceinline__ void ang_bwd_term(uint32_t q, bool active, const SFDe
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(64) void k_desc_bwd(int N, const int *__restrict__ species,
-                                                 const double *__restrict__ wz, NeighborBufs nb, SFDev sf,
-                                                 Lds L, const double *__restrict__ g64,
-                                                 const float *__restrict__ g32, double *__restrict__ slot_f,
-                                                 double *__restrict__ atom_vir) {
+template <typename T, int W, int GNZ>
+__global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__ species,
+                                                  const double *__restrict__ wz, NeighborBufs nb, TripletBufs tb,
+                                                  SFDev sf, Lds L, const double *__restrict__ g64,
+                                                  const float *__restrict__ g32, double *__restrict__ slot_f,
+                                                  double *__restrict__ atom_vir) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int i = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int KP = L.KP, Dp = L.Dp, G = L.G;
-  T *s_d = (T *)smem;
+  double *s_vir = (double *)smem;  // [W * 6]
+  T *s_d = (T *)(s_vir + 6 * W);
   T *s_w = s_d + 4 * KP;
   T *s_fc = s_w + KP;
   T *s_dfc = s_fc + G * KP;
   T *s_g = s_dfc + G * KP;
-  T *s_T = s_g + Dp;
-  T *s_F = s_T + KP;
-  uint32_t *s_q = (uint32_t *)(s_F + 3 * KP);
+  T *s_gA = s_g + Dp;  // per angular line: g of the plain and of the weighted feature
+  T *s_gW = s_gA + sf.n_ang;
+  // per-wave accumulators (summed in wave order at the end: deterministic)
+  T *s_T = s_gA + Dp;     // [W][KP]
+  T *s_F = s_T + W * KP;  // [W][3 KP]
 
   const int start = i * nb.kcap;
   const int K = nb.nb_count[i];
   if (K > KP) {
-    if (lane == 0) atomicAdd(&nb.total[3], 1);
+    if (tid == 0) atomicAdd(&nb.total[3], 1);
     return;
   }
-  stage_neighbors<T>(start, K, lane, nb, species, wz, sf, KP, s_d, s_w, s_fc, s_dfc);
+  for (int a = tid; a < K; a += 64 * W) {
+    const double *dd = nb.nb_d + 4 * (size_t)(start + a);
+    const double r = dd[3];
+    s_d[4 * a] = (T)dd[0];
+    s_d[4 * a + 1] = (T)dd[1];
+    s_d[4 * a + 2] = (T)dd[2];
+    s_d[4 * a + 3] = (T)r;
+    s_w[a] = (T)wz[species[nb.nb_j[start + a]]];
+    for (int g = 0; g < sf.n_groups; ++g) {
+      const double rc = sf.group_rc[g];
+      const bool in = r <= rc;
+      s_fc[g * KP + a] = in ? (T)(0.5 * (cos(kPi * r / rc) + 1.0)) : T(0);
+      s_dfc[g * KP + a] = in ? (T)(0.5 * (-sin(kPi * r / rc)) * (kPi / rc)) : T(0);
+    }
+  }
+  for (int k = tid; k < 4 * W * KP; k += 64 * W) s_T[k] = T(0);  // s_T and s_F
   const size_t row = (size_t)i * Dp;
-  for (int d = lane; d < Dp; d += 64) s_g[d] = g64 ? (T)g64[row + d] : (T)g32[row + d];
-  for (int a = lane; a < K; a += 64) {
-    s_F[3 * a] = s_F[3 * a + 1] = s_F[3 * a + 2] = T(0);
+  for (int d = tid; d < Dp; d += 64 * W) s_g[d] = g64 ? (T)g64[row + d] : (T)g32[row + d];
+  for (int s = tid; s < sf.n_ang; s += 64 * W) {
+    const int f = sf.ang_feat[s];
+    s_gA[s] = g64 ? (T)g64[row + f] : (T)g32[row + f];
+    s_gW[s] = g64 ? (T)g64[row + f + sf.nsf] : (T)g32[row + f + sf.nsf];
   }
   __syncthreads();
 
-  // radial: one neighbour per lane
-  for (int a = lane; a < K; a += 64) {
+  // radial: one neighbour per thread
+  for (int a = tid; a < K; a += 64 * W) {
     const T r = s_d[4 * a + 3], w = s_w[a];
     T acc = 0;
     for (int s = 0; s < sf.n_rad; ++s) {
@@ -440,54 +787,39 @@ __global__ __launch_bounds__(64) void k_desc_bwd(int N, const int *__restrict__ 
       const T geff = s_g[f] + w * s_g[f + sf.nsf];
       acc += geff * Math<T>::ex(-A * dr * dr) * (T(-2) * A * dr * s_fc[g * KP + a] + s_dfc[g * KP + a]);
     }
-    s_T[a] = acc;
+    s_T[a] = acc;  // wave 0's row
   }
   __syncthreads();
 
   T vir[6] = {0, 0, 0, 0, 0, 0};
-  const int npairs = K * (K - 1) / 2;
-  const T rcA = (T)sf.rc_max_ang;
   if (sf.n_ang > 0) {
-    int qn = 0;
-    for (int p0 = 0; p0 < npairs; p0 += 64) {
-      const int p = p0 + lane;
-      bool ok = false;
-      int a = 0, b = 0;
-      if (p < npairs) {
-        pair_index(p, K, a, b);
-        if (s_d[4 * a + 3] <= rcA && s_d[4 * b + 3] <= rcA) {
-          const T ex = s_d[4 * b] - s_d[4 * a], ey = s_d[4 * b + 1] - s_d[4 * a + 1],
-                  ez = s_d[4 * b + 2] - s_d[4 * a + 2];
-          ok = Math<T>::sq(ex * ex + ey * ey + ez * ez) <= rcA;
-        }
-      }
-      const unsigned long long m = __ballot(ok);
-      if (ok) s_q[qn + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)a | ((uint32_t)b << 16);
-      qn += __popcll(m);
-      __syncthreads();
-      if (qn >= 64) {
-        ang_bwd_term<T>(s_q[lane], true, sf, s_d, s_w, s_fc, s_dfc, s_g, KP, s_T, s_F, vir);
-        const int rest = qn - 64;
-        const uint32_t mv = (lane < rest) ? s_q[64 + lane] : 0u;
-        __syncthreads();
-        if (lane < rest) s_q[lane] = mv;
-        qn = rest;
-        __syncthreads();
-      }
-    }
-    if (qn > 0) {
-      const bool act = lane < qn;
-      ang_bwd_term<T>(act ? s_q[lane] : (1u << 16), act, sf, s_d, s_w, s_fc, s_dfc, s_g, KP, s_T, s_F, vir);
+    const uint32_t *tl = tb.list + (size_t)i * tb.tcap;
+    const int nt = tb.count[i];
+    for (int t0 = 64 * wv; t0 < nt; t0 += 64 * W) {
+      const bool act = t0 + lane < nt;
+      const uint32_t q = act ? tl[t0 + lane] : (1u << 16);
+      if constexpr (GNZ > 0)
+        ang_bwd_grid<T, GNZ>(q, act, sf, s_d, s_w, s_fc, s_dfc, s_gA, s_gW, KP, s_T + wv * KP, s_F + 3 * wv * KP, vir);
+      else
+        ang_bwd_term<T>(q, act, sf, s_d, s_w, s_fc, s_dfc, s_gA, s_gW, KP, s_T + wv * KP, s_F + 3 * wv * KP, vir);
     }
   }
   __syncthreads();
 
   // per-slot force contributions dE_i/dR(slot atom) and the radial virial
-  for (int a = lane; a < K; a += 64) {
+  for (int a = tid; a < K; a += 64 * W) {
     const T r = s_d[4 * a + 3];
-    const T s = s_T[a] / r;
+    T Ts = T(0), Fx = T(0), Fy = T(0), Fz = T(0);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      Ts += s_T[w * KP + a];
+      Fx += s_F[3 * (w * KP + a)];
+      Fy += s_F[3 * (w * KP + a) + 1];
+      Fz += s_F[3 * (w * KP + a) + 2];
+    }
+    const T s = Ts / r;
     const T dx = s_d[4 * a], dy = s_d[4 * a + 1], dz = s_d[4 * a + 2];
-    const T fx = s * dx + s_F[3 * a], fy = s * dy + s_F[3 * a + 1], fz = s * dz + s_F[3 * a + 2];
+    const T fx = s * dx + Fx, fy = s * dy + Fy, fz = s * dz + Fz;
     double *o = slot_f + 3 * (size_t)(start + a);
     o[0] = (double)fx;
     o[1] = (double)fy;
@@ -502,67 +834,156 @@ __global__ __launch_bounds__(64) void k_desc_bwd(int N, const int *__restrict__ 
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     const double v = wave_sum((double)vir[c]);
-    if (lane == 0) atom_vir[6 * (size_t)i + c] = v;
+    if (lane == 0) s_vir[6 * wv + c] = v;
+  }
+  __syncthreads();
+  if (tid < 6) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) v += s_vir[6 * w + tid];
+    atom_vir[6 * (size_t)i + tid] = v;
   }
 }
 
-template <typename T>
+template <typename T, int NCH, int W>
 size_t fwd_lds(const Lds &L) {
-  return (size_t)(4 * L.KP + L.KP + L.G * L.KP + L.Dp) * sizeof(T) + 128 * sizeof(uint32_t);
+  return W * sizeof(double) + (size_t)(4 * L.KP + L.KP + L.G * L.KP + L.Dp + W * 2 * NCH) * sizeof(T);
 }
 
-template <typename T>
+template <typename T, int W>
 size_t bwd_lds(const Lds &L) {
-  return (size_t)(4 * L.KP + L.KP + 2 * L.G * L.KP + L.Dp + L.KP + 3 * L.KP) * sizeof(T) +
-         128 * sizeof(uint32_t);
+  return 6 * W * sizeof(double) + (size_t)(5 * L.KP + 2 * L.G * L.KP + 2 * L.Dp + 4 * W * L.KP) * sizeof(T);
+}
+
+constexpr size_t kLdsBudget = 64 * 1024;
+
+template <typename T, int NCH, int W>
+hipError_t fwd_launch(hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
+                      const TripletBufs &tb, const SFDev &sf, Lds L, const double *theta_inv, const double *mu,
+                      int split_mode, DescOut out) {
+  k_desc_fwd<T, NCH, W><<<N, 64 * W, fwd_lds<T, NCH, W>(L), s>>>(N, species, wz, nb, tb, sf, L, theta_inv, mu,
+                                                                  split_mode, out);
+  return hipGetLastError();
 }
 
 template <typename T, int NCH>
-hipError_t fwd_launch(hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
-                      const SFDev &sf, Lds L, const double *theta_inv, const double *mu, int split_mode,
-                      DescOut out) {
-  const size_t lds = fwd_lds<T>(L);
-  k_desc_fwd<T, NCH><<<N, 64, lds, s>>>(N, species, wz, nb, sf, L, theta_inv, mu, split_mode, out);
+hipError_t fwd_waves(int waves, hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
+                     const TripletBufs &tb, const SFDev &sf, Lds L, const double *theta_inv, const double *mu,
+                     int split_mode, DescOut out) {
+  if (waves <= 1) return fwd_launch<T, NCH, 1>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+  if (waves == 2) return fwd_launch<T, NCH, 2>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+  return fwd_launch<T, NCH, 4>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+}
+
+template <typename T, int W, int GNZ>
+hipError_t bwd_launch(hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
+                      const TripletBufs &tb, const SFDev &sf, Lds L, const double *g64, const float *g32,
+                      double *slot_f, double *atom_vir) {
+  k_desc_bwd<T, W, GNZ><<<N, 64 * W, bwd_lds<T, W>(L), s>>>(N, species, wz, nb, tb, sf, L, g64, g32, slot_f,
+                                                            atom_vir);
   return hipGetLastError();
+}
+
+template <typename T, int W>
+hipError_t bwd_grid(hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
+                    const TripletBufs &tb, const SFDev &sf, Lds L, const double *g64, const float *g32,
+                    double *slot_f, double *atom_vir) {
+  if (!sf.grid) return bwd_launch<T, W, 0>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+  switch (sf.nz) {
+    case 1: return bwd_launch<T, W, 1>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+    case 2: return bwd_launch<T, W, 2>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+    case 3: return bwd_launch<T, W, 3>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+    case 4: return bwd_launch<T, W, 4>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+    default: return bwd_launch<T, W, 5>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+  }
+}
+
+template <typename T, int NZ, int W>
+size_t fwd_grid_lds(const Lds &L) {
+  constexpr int NV = GridShape<T, NZ>::NV;
+  return W * sizeof(double) + (size_t)(4 * L.KP + L.KP + L.G * L.KP + L.Dp + W * NV + NV) * sizeof(T);
+}
+
+template <typename T, int NZ, int W>
+hipError_t fwd_grid_launch(hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
+                           const TripletBufs &tb, const SFDev &sf, Lds L, const double *theta_inv, const double *mu,
+                           int split_mode, DescOut out) {
+  k_desc_fwd_grid<T, NZ, W><<<N, 64 * W, fwd_grid_lds<T, NZ, W>(L), s>>>(N, species, wz, nb, tb, sf, L, theta_inv,
+                                                                         mu, split_mode, out);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t fwd_grid(int waves, hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
+                    const TripletBufs &tb, const SFDev &sf, Lds L, const double *theta_inv, const double *mu,
+                    int split_mode, DescOut out) {
+#define GAPX_FWD_GRID(NZ)                                                                                     \
+  return (waves >= 2) ? fwd_grid_launch<T, NZ, 2>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out) \
+                      : fwd_grid_launch<T, NZ, 1>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out)
+  switch (sf.nz) {
+    case 1: GAPX_FWD_GRID(1);
+    case 2: GAPX_FWD_GRID(2);
+    case 3: GAPX_FWD_GRID(3);
+    case 4: GAPX_FWD_GRID(4);
+    default: GAPX_FWD_GRID(5);
+  }
+#undef GAPX_FWD_GRID
 }
 
 }  // namespace
 
 // Largest neighbour count the LDS layout allows (checked by the host).
 int max_neighbors_for_lds(bool fp64, int G, int Dp) {
-  const size_t budget = 64 * 1024;
+  const size_t budget = kLdsBudget - 64;
   const size_t bytes = fp64 ? 8 : 4;
-  // backward is the larger layout: (9 + 2G) KP + Dp values + queue
-  long kp = (long)((budget - 512) / bytes - Dp) / (9 + 2 * G);
+  // the backward with one wave is the largest single-wave layout: (9 + 2G) KP + 2 Dp values
+  long kp = (long)(budget / bytes - 2 * Dp) / (9 + 2 * G);
   return (int)(kp / 8 * 8);
 }
 
+hipError_t launch_triplets(hipStream_t s, int N, const NeighborBufs &nb, double rcA, TripletBufs tb) {
+  if (N <= 0) return hipSuccess;
+  k_triplets<<<N, 64, 4 * sizeof(float) * (size_t)nb.kcap, s>>>(N, nb, rcA, tb);
+  return hipGetLastError();
+}
+
 hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *species, const double *wz,
-                                  const NeighborBufs &nb, const SFDev &sf, const double *theta_inv,
-                                  const double *mu, int split_mode, DescOut out, int chunk, int KP) {
+                                  const NeighborBufs &nb, const TripletBufs &tb, const SFDev &sf,
+                                  const double *theta_inv, const double *mu, int split_mode, DescOut out, int chunk,
+                                  int waves, int KP) {
   Lds L{KP, sf.n_groups > 0 ? sf.n_groups : 1, out.Dp};
   if (N <= 0) return hipSuccess;
-  if (fp64) {
-    if (chunk <= 8) return fwd_launch<double, 8>(s, N, species, wz, nb, sf, L, theta_inv, mu, split_mode, out);
-    if (chunk <= 16) return fwd_launch<double, 16>(s, N, species, wz, nb, sf, L, theta_inv, mu, split_mode, out);
-    return fwd_launch<double, 32>(s, N, species, wz, nb, sf, L, theta_inv, mu, split_mode, out);
+  if (sf.grid) {
+    if (fp64) return fwd_grid<double>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+    return fwd_grid<float>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
   }
-  if (chunk <= 8) return fwd_launch<float, 8>(s, N, species, wz, nb, sf, L, theta_inv, mu, split_mode, out);
-  if (chunk <= 16) return fwd_launch<float, 16>(s, N, species, wz, nb, sf, L, theta_inv, mu, split_mode, out);
-  return fwd_launch<float, 32>(s, N, species, wz, nb, sf, L, theta_inv, mu, split_mode, out);
+  if (fp64) {
+    if (chunk <= 8) return fwd_waves<double, 8>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+    return fwd_waves<double, 16>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+  }
+  if (chunk <= 8) return fwd_waves<float, 8>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+  return fwd_waves<float, 16>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
 }
 
 hipError_t launch_descriptors_bwd(hipStream_t s, bool fp64, int N, const int *species, const double *wz,
-                                  const NeighborBufs &nb, const SFDev &sf, int Dp, const double *g64,
-                                  const float *g32, double *slot_f, double *atom_vir, int KP) {
+                                  const NeighborBufs &nb, const TripletBufs &tb, const SFDev &sf, int Dp,
+                                  const double *g64, const float *g32, double *slot_f, double *atom_vir, int waves,
+                                  int KP) {
   Lds L{KP, sf.n_groups > 0 ? sf.n_groups : 1, Dp};
   if (N <= 0) return hipSuccess;
+  // fewer waves when the per-wave accumulators would not fit in LDS
   if (fp64) {
-    k_desc_bwd<double><<<N, 64, bwd_lds<double>(L), s>>>(N, species, wz, nb, sf, L, g64, g32, slot_f, atom_vir);
-  } else {
-    k_desc_bwd<float><<<N, 64, bwd_lds<float>(L), s>>>(N, species, wz, nb, sf, L, g64, g32, slot_f, atom_vir);
+    if (waves >= 4 && bwd_lds<double, 4>(L) <= kLdsBudget)
+      return bwd_grid<double, 4>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+    if (waves >= 2 && bwd_lds<double, 2>(L) <= kLdsBudget)
+      return bwd_grid<double, 2>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+    return bwd_grid<double, 1>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
   }
-  return hipGetLastError();
+  if (waves >= 4 && bwd_lds<float, 4>(L) <= kLdsBudget)
+    return bwd_grid<float, 4>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+  if (waves >= 2 && bwd_lds<float, 2>(L) <= kLdsBudget)
+    return bwd_grid<float, 2>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+  return bwd_grid<float, 1>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
 }
 
 }  // namespace gapx

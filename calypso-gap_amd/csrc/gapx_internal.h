@@ -38,6 +38,12 @@ struct SFTables {
 struct SFDev {
   int n_rad, n_ang, n_groups, nsf;
   double rc_max_rad, rc_max_ang;
+  // Grid form of the angular table (set when every angular line shares one
+  // cutoff group and the sorted lines are eta x (lambda = +1, -1) x zeta with
+  // the same zeta list, zeta in {1, 2, 4, 8, 16}): line s = h * 2 nz + li * nz + k.
+  int grid, n_eta, nz, g_ang;
+  int zlog[5];             // log2 zeta_k
+  double zeta_k[5], pref_k[5];
   const double *rad_a, *rad_rs;
   const int *rad_group, *rad_feat;
   const double *ang_eta, *ang_lam, *ang_pref, *ang_zeta_real;
@@ -132,17 +138,26 @@ struct DescOut {
   float *xt32;          // [N][Dp] centred scaled (FP32, mixed run)
   uint16_t *xt_hi, *xt_lo;   // [N][Dp] split operands (mixed)
   double *xn;           // [N] |xt|^2 (FP64)
-  unsigned long long *triplets;  // [1] neighbour pairs (j<k) within the angular cutoff
 };
 
+// Per-atom triplet lists: neighbour pairs (a, b), a < b, with r_a, r_b and
+// r_ab within the largest angular cutoff, packed a | b << 16 at i * tcap.
+struct TripletBufs {
+  int tcap;
+  uint32_t *list;       // [N*tcap]
+  int *count;           // [N]
+  int *stats;           // [3] kept, largest count, atoms above tcap
+};
+
+hipError_t launch_triplets(hipStream_t s, int N, const NeighborBufs &nb, double rcA, TripletBufs tb);
 hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *species,
-                                  const double *wz, const NeighborBufs &nb, const SFDev &sf,
-                                  const double *theta_inv, const double *mu, int split_mode,
-                                  DescOut out, int chunk, int KP);
+                                  const double *wz, const NeighborBufs &nb, const TripletBufs &tb,
+                                  const SFDev &sf, const double *theta_inv, const double *mu,
+                                  int split_mode, DescOut out, int chunk, int waves, int KP);
 hipError_t launch_descriptors_bwd(hipStream_t s, bool fp64, int N, const int *species,
-                                  const double *wz, const NeighborBufs &nb, const SFDev &sf,
-                                  int Dp, const double *g64, const float *g32, double *slot_f,
-                                  double *atom_vir, int KP);
+                                  const double *wz, const NeighborBufs &nb, const TripletBufs &tb,
+                                  const SFDev &sf, int Dp, const double *g64, const float *g32,
+                                  double *slot_f, double *atom_vir, int waves, int KP);
 // Largest per-atom neighbour count the descriptor kernels' LDS layout holds.
 int max_neighbors_for_lds(bool fp64, int G, int Dp);
 hipError_t launch_gather_forces(hipStream_t s, int N, const NeighborBufs &nb, const double *slot_f,
