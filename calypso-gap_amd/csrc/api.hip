@@ -306,7 +306,8 @@ struct Ctx {
   std::map<std::string, DevMem> mem;
   size_t bytes = 0;
   int kcap = 64, KP = 64, tcap = 1024;
-  int *host_stats = nullptr;  // pinned copy of the last run's neighbour stats
+  int *host_stats = nullptr;      // pinned: the last run's list statistics
+  int *host_stats_dev = nullptr;  // its device-side address
   int last_flags = -1;
   bool ran = false;
   int chunk = 0;  // lines per forward pass (0: 16)
@@ -534,7 +535,9 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b, bool grad) {
       (rc = c.get("bin_atoms", c.N, &nb.bin_atoms)) || (rc = c.get("atom_bin", c.N, &nb.atom_bin)) ||
       (rc = c.get("wpos", 3 * (size_t)c.N, &nb.wpos)) || (rc = c.get("shift", 3 * (size_t)c.N, &nb.shift)) ||
       (rc = c.get("nb_count", c.N, &nb.nb_count)) || (rc = c.get("stats", 8, &stats)) ||
-      (rc = c.get("nb_j", slots, &nb.nb_j)) || (rc = c.get("nb_key", slots, &nb.nb_key)) ||
+      (rc = c.get("nb_raw", c.N, &nb.nb_raw)) || (rc = c.get("spos", 4 * (size_t)c.N, &nb.spos)) ||
+      (rc = c.get("sinfo", 4 * (size_t)c.N, &nb.sinfo)) || (rc = c.get("nb_j", slots, &nb.nb_j)) ||
+      (rc = c.get("nb_key", slots + 16, &nb.nb_key)) ||  // 16 keys of padding for batched reads
       (rc = c.get("nb_d", 4 * slots, &nb.nb_d)) || (rc = c.get("nb_rev", slots, &nb.nb_rev)))
     return rc;
   nb.total = stats;
@@ -543,9 +546,7 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b, bool grad) {
   const int *d_astruct = c.ptr<int>("astruct");
   const double *d_pos = c.ptr<double>("pos");
   const StructParams *d_sp = c.ptr<StructParams>("sp");
-  GAPX_HIP(launch_fill_int(c.stream, nb.bin_count, c.nbins, 0));
-  GAPX_HIP(launch_fill_int(c.stream, nb.bin_cursor, c.nbins, 0));
-  GAPX_HIP(launch_fill_int(c.stream, stats, 8, 0));
+  GAPX_HIP(launch_zero_ints(c.stream, nb.bin_count, c.nbins, nb.bin_cursor, c.nbins, stats, 8));
   GAPX_HIP(launch_wrap_bin(c.stream, c.N, d_pos, d_astruct, d_sp, nb));
   GAPX_HIP(launch_scan(c.stream, nb.bin_count, nb.bin_start, c.nbins, nullptr));
   GAPX_HIP(launch_scatter_bins(c.stream, c.N, c.nbins, nb));
@@ -555,13 +556,14 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b, bool grad) {
   // triplet lists for the angular lines (forward and backward share them)
   TripletBufs &tb = b.tb;
   tb.tcap = c.tcap;
-  tb.stats = stats + 4;
-  if ((rc = c.get("trip_list", (size_t)c.N * c.tcap, &tb.list)) || (rc = c.get("trip_count", c.N, &tb.count)))
+  if ((rc = c.get("trip_list", (size_t)c.N * c.tcap, &tb.list)) || (rc = c.get("trip_count", c.N, &tb.count)) ||
+      (rc = c.get("trip_raw", c.N, &tb.raw)))
     return rc;
   if (c.m->sf.n_ang > 0) {
     GAPX_HIP(launch_triplets(c.stream, c.N, nb, c.m->sf.rc_max_ang, tb));
   } else {
     GAPX_HIP(launch_fill_int(c.stream, tb.count, c.N, 0));
+    tb.raw = nullptr;  // no triplet statistics
   }
   return GAPX_OK;
 }
@@ -685,7 +687,8 @@ static int ctx_run(Ctx &c, int flags) {
     GAPX_HIP(launch_fill_double(c.stream, b.force, 3 * (size_t)c.N, 0.0));
   GAPX_HIP(launch_struct_reduce(c.stream, c.nstruct, d_sp, g.eatom, b.atom_vir, grad ? 1 : 0, b.ene, b.stress));
   stage_end(c, kForces);
-  GAPX_HIP(hipMemcpyAsync(c.host_stats, nb.total, 8 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  // list statistics straight into pinned host memory (read by ctx_settle)
+  GAPX_HIP(launch_list_stats(c.stream, c.N, nb, b.tb, c.host_stats_dev));
   c.last_flags = flags;
   c.ran = true;
   if (c.timing) {  // one event pair per stage: read them before the next run
@@ -750,7 +753,10 @@ static int ctx_create(Model *m, Ctx **out) {
     return fail(GAPX_ERR_HIP, "could not create a HIP stream");
   }
   for (auto &e : c->ev) hipEventCreate(&e);
-  if (hipHostMalloc((void **)&c->host_stats, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void **)&c->host_stats, 8 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) !=
+          hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->host_stats_dev, c->host_stats, 0) != hipSuccess) {
+    if (c->host_stats) hipHostFree(c->host_stats);
     hipStreamDestroy(c->stream);
     delete c;
     return fail(GAPX_ERR_NOMEM, "pinned host allocation failed");

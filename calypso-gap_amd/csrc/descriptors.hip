@@ -169,9 +169,7 @@ __global__ __launch_bounds__(64) void k_triplets(int N, NeighborBufs nb, double 
   }
   if (lane == 0) {
     tb.count[i] = min(count, tb.tcap);
-    atomicAdd(&tb.stats[0], min(count, tb.tcap));
-    atomicMax(&tb.stats[1], count);
-    if (count > tb.tcap) atomicAdd(&tb.stats[2], 1);
+    tb.raw[i] = count;
   }
 }
 

@@ -116,8 +116,11 @@ struct NeighborBufs {
   unsigned long long *nb_key;  // [N*kcap] packed (j, image) for the reverse-slot search
   double *nb_d;         // [N*kcap*4] dx dy dz r (relative to the centre)
   int *nb_rev;          // [N*kcap] slot (j -> i, -n) of slot (i -> j, n), or -1
+  int *nb_raw;          // [N] neighbour count before the kcap clamp
+  double *spos;         // [N*4] wrapped positions in bin order (x y z, pad)
+  int *sinfo;           // [N*4] in bin order: atom index, its three cell shifts
   int *overflow;        // = total + 2
-  int *total;           // [4] kept slots, largest count, atoms above kcap, kernel guard
+  int *total;           // [8] see launch_list_stats; [3] is the descriptor kernels' guard
 };
 
 hipError_t launch_wrap_bin(hipStream_t s, int N, const double *pos, const int *atom_struct,
@@ -128,6 +131,7 @@ hipError_t launch_build_neighbors(hipStream_t s, int N, const int *atom_struct,
                                   const StructParams *sp, double rcut, NeighborBufs nb);
 hipError_t launch_reverse_slots(hipStream_t s, int N, NeighborBufs nb);
 hipError_t launch_fill_int(hipStream_t s, int *p, int n, int v);
+hipError_t launch_zero_ints(hipStream_t s, int *a, int na, int *b, int nb, int *c, int nc);
 hipError_t launch_fill_double(hipStream_t s, double *p, size_t n, double v);
 
 struct DescOut {
@@ -145,11 +149,15 @@ struct DescOut {
 struct TripletBufs {
   int tcap;
   uint32_t *list;       // [N*tcap]
-  int *count;           // [N]
-  int *stats;           // [3] kept, largest count, atoms above tcap
+  int *count;           // [N] kept triplets (<= tcap)
+  int *raw;             // [N] triplet count before the tcap clamp
 };
 
 hipError_t launch_triplets(hipStream_t s, int N, const NeighborBufs &nb, double rcA, TripletBufs tb);
+// out[0..7] = kept pairs, largest raw pair count, atoms above kcap, descriptor
+// guard (nb.total[3]), kept triplets, largest raw triplet count, atoms above
+// tcap, 0.  `out` may be pinned host memory (device-visible).
+hipError_t launch_list_stats(hipStream_t s, int N, const NeighborBufs &nb, const TripletBufs &tb, int *out);
 hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *species,
                                   const double *wz, const NeighborBufs &nb, const TripletBufs &tb,
                                   const SFDev &sf, const double *theta_inv, const double *mu,

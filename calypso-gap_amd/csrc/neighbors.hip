@@ -7,12 +7,18 @@
 // included (gap_calc.f90:97-101).  Positions are wrapped into the cell first;
 // the stored image n refers to the positions as given.
 //
+// Statistics (kept pairs, largest count, overflow) are per-atom values folded
+// by one small kernel at the end of a run: same-address atomics from every
+// wavefront serialise in the memory system and cost more than the search.
+//
 // Bins: nb_k = max(1, floor(d_k / rcut)) along each lattice direction (d_k the
 // interplanar spacing); a neighbour can be at most S_k = ceil(rcut nb_k / d_k)
 // bins away, and offsets beyond the cell wrap with an image shift, so cells
 // smaller than rcut need no special case.  Atoms inside a bin are kept in
 // ascending index order so the lists (and every sum over them) are
 // deterministic.
+#include <algorithm>
+
 #include "gapx_internal.h"
 
 namespace gapx {
@@ -24,6 +30,13 @@ __device__ inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a +
 __global__ void k_fill_int(int *p, int n, int v) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = v;
+}
+
+__global__ void k_zero3(int *a, int na, int *b, int nb, int *c, int nc) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < na) a[t] = 0;
+  if (t < nb) b[t] = 0;
+  if (t < nc) c[t] = 0;
 }
 
 __global__ void k_fill_double(double *p, size_t n, double v) {
@@ -90,6 +103,16 @@ __global__ void k_scatter(int N, NeighborBufs nb) {
   nb.bin_atoms[slot] = i;
 }
 
+// Copy atom j's wrapped position and (index, cell shift) to bin slot q, so
+// the pair search reads one contiguous record per candidate.
+__device__ __forceinline__ void put_sorted(const NeighborBufs &nb, int q, int j) {
+  nb.bin_atoms[q] = j;
+  double2 *sp = (double2 *)(nb.spos + 4 * (size_t)q);
+  sp[0] = make_double2(nb.wpos[3 * j], nb.wpos[3 * j + 1]);
+  sp[1] = make_double2(nb.wpos[3 * j + 2], 0.0);
+  ((int4 *)nb.sinfo)[q] = make_int4(j, nb.shift[3 * j], nb.shift[3 * j + 1], nb.shift[3 * j + 2]);
+}
+
 // Order each bin's atoms by index: one wavefront per bin, every element's
 // rank is the number of smaller indices in the bin (bins hold tens of atoms).
 __global__ __launch_bounds__(64) void k_rank_bins(int nbins, NeighborBufs nb) {
@@ -100,19 +123,16 @@ __global__ __launch_bounds__(64) void k_rank_bins(int nbins, NeighborBufs nb) {
     const int v = (lane < n) ? nb.bin_atoms[lo + lane] : 0x7fffffff;
     int rank = 0;
     for (int k = 0; k < n; ++k) rank += (__shfl(v, k, 64) < v) ? 1 : 0;
-    __syncthreads();
-    if (lane < n) nb.bin_cursor[lo + rank] = v;  // bin_cursor doubles as scratch
-    __syncthreads();
-    if (lane < n) nb.bin_atoms[lo + lane] = nb.bin_cursor[lo + lane];
+    if (lane < n) put_sorted(nb, lo + rank, v);
   } else {
     for (int e = lane; e < n; e += 64) {
       const int v = nb.bin_atoms[lo + e];
       int rank = 0;
       for (int k = 0; k < n; ++k) rank += (nb.bin_atoms[lo + k] < v) ? 1 : 0;
-      nb.bin_cursor[lo + rank] = v;
+      nb.bin_cursor[lo + rank] = v;  // bin_cursor doubles as scratch
     }
     __syncthreads();
-    for (int e = lane; e < n; e += 64) nb.bin_atoms[lo + e] = nb.bin_cursor[lo + e];
+    for (int e = lane; e < n; e += 64) put_sorted(nb, lo + e, nb.bin_cursor[lo + e]);
   }
 }
 
@@ -121,10 +141,13 @@ __device__ __forceinline__ unsigned long long nb_key(int j, int n0, int n1, int 
          ((unsigned long long)((n1 + 128) & 255) << 8) | (unsigned long long)((n2 + 128) & 255);
 }
 
-// One wavefront per centre atom: visit the (2S+1)^3 bin images in a fixed
-// order, test the bin's atoms one per lane and append the ones within rcut
-// (ballot + prefix count), so the list order is bin order, then index order.
-// Slots live at i * kcap + a; counts above kcap are reported, not written.
+// One wavefront per centre atom.  The (2S+1)^3 bin images are taken 64 at a
+// time, one per lane (bin range and image shift); a wave prefix sum over the
+// bin sizes flattens their atoms into one candidate sequence, which the lanes
+// then test 64 at a time, each finding its image by a binary search over the
+// prefix sums.  Accepted pairs are appended with ballot + prefix count, so the
+// list order is image order, then index order inside the bin (deterministic).
+// Slots live at i * kcap + a; counts above kcap are recorded, not written.
 __global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ atom_struct,
                                               const StructParams *__restrict__ sp, double rcut, NeighborBufs nb) {
   const int i = blockIdx.x, lane = threadIdx.x;
@@ -134,63 +157,83 @@ __global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ ato
   const int bi2 = b % P.nb[2], bi1 = (b / P.nb[2]) % P.nb[1], bi0 = b / (P.nb[1] * P.nb[2]);
   const double xi = nb.wpos[3 * i], yi = nb.wpos[3 * i + 1], zi = nb.wpos[3 * i + 2];
   const int si0 = nb.shift[3 * i], si1 = nb.shift[3 * i + 1], si2 = nb.shift[3 * i + 2];
+  const int m1 = 2 * P.S[1] + 1, m2 = 2 * P.S[2] + 1, nimg = (2 * P.S[0] + 1) * m1 * m2;
   const size_t base = (size_t)i * nb.kcap;
+  const double2 *sp2 = (const double2 *)nb.spos;
+  const int4 *sinfo = (const int4 *)nb.sinfo;
   int count = 0;
-  for (int o0 = -P.S[0]; o0 <= P.S[0]; ++o0) {
-    const int bb0 = bi0 + o0, t0 = floordiv(bb0, P.nb[0]), w0 = bb0 - t0 * P.nb[0];
-    for (int o1 = -P.S[1]; o1 <= P.S[1]; ++o1) {
-      const int bb1 = bi1 + o1, t1 = floordiv(bb1, P.nb[1]), w1 = bb1 - t1 * P.nb[1];
-      for (int o2 = -P.S[2]; o2 <= P.S[2]; ++o2) {
-        const int bb2 = bi2 + o2, t2 = floordiv(bb2, P.nb[2]), w2 = bb2 - t2 * P.nb[2];
-        const double sx = t0 * P.L[0] + t1 * P.L[3] + t2 * P.L[6];
-        const double sy = t0 * P.L[1] + t1 * P.L[4] + t2 * P.L[7];
-        const double sz = t0 * P.L[2] + t1 * P.L[5] + t2 * P.L[8];
-        const bool zero = (t0 == 0) && (t1 == 0) && (t2 == 0);
-        const int bin = P.bin_base + (w0 * P.nb[1] + w1) * P.nb[2] + w2;
-        const int lo = nb.bin_start[bin], hi = nb.bin_start[bin + 1];
-        for (int q0 = lo; q0 < hi; q0 += 64) {
-          const int q = q0 + lane;
-          bool ok = false;
-          int j = 0;
-          double dx = 0, dy = 0, dz = 0, r = 0;
-          if (q < hi) {
-            j = nb.bin_atoms[q];
-            if (!(zero && j == i)) {
-              dx = nb.wpos[3 * j] + sx - xi;
-              dy = nb.wpos[3 * j + 1] + sy - yi;
-              dz = nb.wpos[3 * j + 2] + sz - zi;
-              r = sqrt(dx * dx + dy * dy + dz * dz);
-              ok = !(r > rcut);
-            }
-          }
-          const unsigned long long m = __ballot(ok);
-          const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
-          if (ok && pos < nb.kcap) {
-            const size_t sl = base + pos;
-            nb.nb_j[sl] = j;
-            nb.nb_key[sl] = nb_key(j, t0 - nb.shift[3 * j] + si0, t1 - nb.shift[3 * j + 1] + si1,
-                                   t2 - nb.shift[3 * j + 2] + si2);
-            nb.nb_d[4 * sl] = dx;
-            nb.nb_d[4 * sl + 1] = dy;
-            nb.nb_d[4 * sl + 2] = dz;
-            nb.nb_d[4 * sl + 3] = r;
-          }
-          count += __popcll(m);
+  for (int g0 = 0; g0 < nimg; g0 += 64) {
+    const int img = g0 + lane;
+    int lo = 0, n = 0, t0 = 0, t1 = 0, t2 = 0;
+    if (img < nimg) {
+      const int o2 = img % m2 - P.S[2], o1 = (img / m2) % m1 - P.S[1], o0 = img / (m1 * m2) - P.S[0];
+      const int bb0 = bi0 + o0, bb1 = bi1 + o1, bb2 = bi2 + o2;
+      t0 = floordiv(bb0, P.nb[0]);
+      t1 = floordiv(bb1, P.nb[1]);
+      t2 = floordiv(bb2, P.nb[2]);
+      const int w0 = bb0 - t0 * P.nb[0], w1 = bb1 - t1 * P.nb[1], w2 = bb2 - t2 * P.nb[2];
+      const int bin = P.bin_base + (w0 * P.nb[1] + w1) * P.nb[2] + w2;
+      lo = nb.bin_start[bin];
+      n = nb.bin_start[bin + 1] - lo;
+    }
+    int incl = n;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
+    }
+    const int total = __shfl(incl, 63, 64);
+    const int first = lo - (incl - n);  // candidate c of this image sits at slot first + c
+    for (int c0 = 0; c0 < total; c0 += 64) {
+      const int c = c0 + lane;
+      int src = 0;  // number of images whose prefix ends at or before c
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (__shfl(incl, src + step - 1, 64) <= c) src += step;
+      const int q = __shfl(first, src, 64) + c;
+      const int T0 = __shfl(t0, src, 64), T1 = __shfl(t1, src, 64), T2 = __shfl(t2, src, 64);
+      bool ok = false;
+      int4 inf = make_int4(0, 0, 0, 0);
+      double dx = 0, dy = 0, dz = 0, r = 0;
+      if (c < total) {
+        inf = sinfo[q];
+        const double2 pxy = sp2[2 * (size_t)q], pz = sp2[2 * (size_t)q + 1];
+        if (!(T0 == 0 && T1 == 0 && T2 == 0 && inf.x == i)) {
+          const double sx = T0 * P.L[0] + T1 * P.L[3] + T2 * P.L[6];
+          const double sy = T0 * P.L[1] + T1 * P.L[4] + T2 * P.L[7];
+          const double sz = T0 * P.L[2] + T1 * P.L[5] + T2 * P.L[8];
+          dx = pxy.x + sx - xi;
+          dy = pxy.y + sy - yi;
+          dz = pz.x + sz - zi;
+          r = sqrt(dx * dx + dy * dy + dz * dz);
+          ok = !(r > rcut);
         }
       }
+      const unsigned long long m = __ballot(ok);
+      const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+      if (ok && pos < nb.kcap) {
+        const size_t sl = base + pos;
+        nb.nb_j[sl] = inf.x;
+        nb.nb_key[sl] = nb_key(inf.x, T0 - inf.y + si0, T1 - inf.z + si1, T2 - inf.w + si2);
+        double2 *d2 = (double2 *)(nb.nb_d + 4 * sl);
+        d2[0] = make_double2(dx, dy);
+        d2[1] = make_double2(dz, r);
+      }
+      count += __popcll(m);
     }
   }
   if (lane == 0) {
     nb.nb_count[i] = min(count, nb.kcap);
-    atomicMax(&nb.total[1], count);
-    atomicAdd(&nb.total[0], min(count, nb.kcap));
-    if (count > nb.kcap) atomicAdd(nb.overflow, 1);
+    nb.nb_raw[i] = count;
   }
 }
 
 // For slot (i -> j, n) find the slot (j -> i, -n); -1 if absent (a pair within
 // one rounding step of rcut can be seen from one side only).  One wavefront
-// per centre atom, one slot per lane, a linear scan of j's packed keys.
+// per centre atom, one slot per lane; j's packed keys are read 16 at a time
+// (8 independent 16-byte loads) and the first match wins.  nb_key carries 16
+// keys of padding past the last atom's slots, so the batched reads stay in
+// bounds; entries at or beyond j's count are never compared.
 __global__ __launch_bounds__(64) void k_reverse(int N, NeighborBufs nb) {
   const int i = blockIdx.x;
   if (i >= N) return;
@@ -202,14 +245,67 @@ __global__ __launch_bounds__(64) void k_reverse(int N, NeighborBufs nb) {
     const int n0 = (int)((key >> 16) & 255) - 128, n1 = (int)((key >> 8) & 255) - 128, n2 = (int)(key & 255) - 128;
     const unsigned long long want = nb_key(i, -n0, -n1, -n2);
     const size_t jb = (size_t)j * nb.kcap;
+    const ulonglong2 *kj = (const ulonglong2 *)(nb.nb_key + jb);
     const int Kj = nb.nb_count[j];
     int found = -1;
-    for (int q = 0; q < Kj; ++q)
-      if (nb.nb_key[jb + q] == want) {
-        found = (int)(jb + q);
-        break;
+    for (int q0 = 0; q0 < Kj && found < 0; q0 += 16) {
+      unsigned long long k[16];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const ulonglong2 v = kj[q0 / 2 + u];
+        k[2 * u] = v.x;
+        k[2 * u + 1] = v.y;
       }
-    nb.nb_rev[base + a] = found;
+#pragma unroll
+      for (int u = 15; u >= 0; --u)
+        if (q0 + u < Kj && k[u] == want) found = q0 + u;
+    }
+    nb.nb_rev[base + a] = (found >= 0) ? (int)(jb + found) : -1;
+  }
+}
+
+// Fold the per-atom counts into the run's statistics (one workgroup).
+__global__ __launch_bounds__(1024) void k_list_stats(int N, int kcap, const int *__restrict__ nb_count,
+                                                     const int *__restrict__ nb_raw, int tcap,
+                                                     const int *__restrict__ t_count, const int *__restrict__ t_raw,
+                                                     const int *__restrict__ guard, int *__restrict__ out) {
+  __shared__ long long red[6][1024];
+  const int t = threadIdx.x;
+  long long v[6] = {0, 0, 0, 0, 0, 0};
+  for (int a = t; a < N; a += 1024) {
+    const int r = nb_raw[a];
+    v[0] += nb_count[a];
+    v[1] = max(v[1], (long long)r);
+    v[2] += (r > kcap) ? 1 : 0;
+    if (t_raw) {
+      const int tr = t_raw[a];
+      v[3] += t_count[a];
+      v[4] = max(v[4], (long long)tr);
+      v[5] += (tr > tcap) ? 1 : 0;
+    }
+  }
+  for (int k = 0; k < 6; ++k) red[k][t] = v[k];
+  __syncthreads();
+  for (int off = 512; off > 0; off >>= 1) {
+    if (t < off) {
+      red[0][t] += red[0][t + off];
+      red[1][t] = max(red[1][t], red[1][t + off]);
+      red[2][t] += red[2][t + off];
+      red[3][t] += red[3][t + off];
+      red[4][t] = max(red[4][t], red[4][t + off]);
+      red[5][t] += red[5][t + off];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = (int)red[0][0];
+    out[1] = (int)red[1][0];
+    out[2] = (int)red[2][0];
+    out[3] = guard[0];
+    out[4] = (int)red[3][0];
+    out[5] = (int)red[4][0];
+    out[6] = (int)red[5][0];
+    out[7] = 0;
   }
 }
 
@@ -285,6 +381,19 @@ __global__ __launch_bounds__(256) void k_struct_reduce(const StructParams *__res
 hipError_t launch_fill_int(hipStream_t s, int *p, int n, int v) {
   if (n <= 0) return hipSuccess;
   k_fill_int<<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_ints(hipStream_t s, int *a, int na, int *b, int nb, int *c, int nc) {
+  const int n = std::max(na, std::max(nb, nc));
+  if (n <= 0) return hipSuccess;
+  k_zero3<<<(n + 255) / 256, 256, 0, s>>>(a, na, b, nb, c, nc);
+  return hipGetLastError();
+}
+
+hipError_t launch_list_stats(hipStream_t s, int N, const NeighborBufs &nb, const TripletBufs &tb, int *out) {
+  k_list_stats<<<1, 1024, 0, s>>>(N, nb.kcap, nb.nb_count, nb.nb_raw, tb.tcap, tb.raw ? tb.count : nullptr, tb.raw,
+                                  nb.total + 3, out);
   return hipGetLastError();
 }
 
