@@ -128,6 +128,7 @@ static void detect_grid(const SFTables &t, SFDev &sd) {
       }
   for (int k = 0; k < nz; ++k) {
     const int z = t.ang_zeta[k];
+    if (k > 0 && z < t.ang_zeta[k - 1]) return;  // the kernels advance one power chain
     if (z != 1 && z != 2 && z != 4 && z != 8 && z != 16) return;
     sd.zlog[k] = (z == 1) ? 0 : (z == 2) ? 1 : (z == 4) ? 2 : (z == 8) ? 3 : 4;
     sd.zeta_k[k] = z;

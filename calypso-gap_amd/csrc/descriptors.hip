@@ -46,6 +46,7 @@ struct Math<double> {
   static __device__ __forceinline__ double sn(double x) { return sin(x); }
   static __device__ __forceinline__ double sq(double x) { return sqrt(x); }
   static __device__ __forceinline__ double pw(double b, double e) { return pow(b, e); }
+  static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
 };
 
 template <>
@@ -55,6 +56,8 @@ struct Math<float> {
   static __device__ __forceinline__ float sn(float x) { return __sinf(x); }
   static __device__ __forceinline__ float sq(float x) { return sqrtf(x); }
   static __device__ __forceinline__ float pw(float b, float e) { return powf(b, e); }
+  // hardware approximations (about 1 ulp) for the mixed path's inner loops
+  static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 };
 
 template <typename T>
@@ -361,44 +364,32 @@ __global__ __launch_bounds__(256) void k_desc_fwd(int N, const int *__restrict__
 // squaring chain per lambda, (lambda, zeta) products are unrolled, and only
 // the eta loop is a (uniform) runtime loop.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T pick5(const T (&v)[5], int j) {
-  T r = v[0];
-  r = (j == 1) ? v[1] : r;
-  r = (j == 2) ? v[2] : r;
-  r = (j == 3) ? v[3] : r;
-  r = (j == 4) ? v[4] : r;
-  return r;
-}
-
 // P_k = pref_k b^zeta_k and dP_k/dcos = pref_k zeta_k lam b^(zeta_k - 1) for
-// b = 1 + lam cos, lam = +1 (p) and -1 (m).
+// b = 1 + lam cos, lam = +1 (p) and -1 (m).  zeta_k = 2^zlog[k] with zlog
+// ascending: one squaring chain, advanced by uniform trip counts (registers
+// only; an indexed table of powers would go to scratch memory).
 template <typename T, int NZ, bool DERIV>
 __device__ __forceinline__ void grid_powers(T cth, const SFDev &sf, T (&Pp)[NZ], T (&Pm)[NZ], T (&dPp)[NZ],
                                             T (&dPm)[NZ]) {
-  const T bp = T(1) + cth, bm = T(1) - cth;
-  T pp[5], pm[5], qp[5], qm[5];  // b^(2^j) and b^(2^j - 1)
-  pp[0] = bp;
-  pm[0] = bm;
-  qp[0] = T(1);
-  qm[0] = T(1);
-#pragma unroll
-  for (int j = 1; j < 5; ++j) {
-    qp[j] = pp[j - 1] * qp[j - 1];
-    qm[j] = pm[j - 1] * qm[j - 1];
-    pp[j] = pp[j - 1] * pp[j - 1];
-    pm[j] = pm[j - 1] * pm[j - 1];
-  }
+  T p = T(1) + cth, m = T(1) - cth;  // b^(2^t)
+  T qp = T(1), qm = T(1);            // b^(2^t - 1)
+  int t = 0;
 #pragma unroll
   for (int k = 0; k < NZ; ++k) {
     const int j = sf.zlog[k];
+    for (; t < j; ++t) {
+      qp *= p;
+      qm *= m;
+      p *= p;
+      m *= m;
+    }
     const T pref = (T)sf.pref_k[k];
-    Pp[k] = pref * pick5(pp, j);
-    Pm[k] = pref * pick5(pm, j);
+    Pp[k] = pref * p;
+    Pm[k] = pref * m;
     if (DERIV) {
       const T pz = pref * (T)sf.zeta_k[k];
-      dPp[k] = pz * pick5(qp, j);
-      dPm[k] = -pz * pick5(qm, j);
+      dPp[k] = pz * qp;
+      dPm[k] = -pz * qm;
     }
   }
 }
@@ -443,7 +434,8 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
   double *s_xn = (double *)smem;  // [W]
   T *s_d = (T *)(s_xn + W);
   T *s_w = s_d + 4 * KP;
-  T *s_fc = s_w + KP;
+  T *s_ri = s_w + KP;           // 1 / r
+  T *s_fc = s_ri + KP;
   T *s_feat = s_fc + L.G * KP;
   T *s_part = s_feat + Dp;      // [W * NV]
   T *s_sum = s_part + W * NV;   // [NV]
@@ -461,6 +453,7 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
     s_d[4 * a + 1] = (T)dd[1];
     s_d[4 * a + 2] = (T)dd[2];
     s_d[4 * a + 3] = (T)r;
+    s_ri[a] = (T)(1.0 / r);
     s_w[a] = (T)wz[species[nb.nb_j[start + a]]];
     for (int g = 0; g < sf.n_groups; ++g) {
       const double rc = sf.group_rc[g];
@@ -496,6 +489,7 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
   const int nt = tb.count[i];
   const int g = sf.g_ang;
   const T rc = (T)sf.group_rc[g];
+  const T pirc = (T)(kPi / sf.group_rc[g]);
   for (int h0 = 0; h0 < sf.n_eta; h0 += NE) {
     T acc[NV];
 #pragma unroll
@@ -509,10 +503,10 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
       const T ex = bx - ax, ey = by - ay, ez = bz - az;
       const T e2 = ex * ex + ey * ey + ez * ez;
       const T e = Math<T>::sq(e2);
-      const T cth = (ra * ra + rb * rb - e2) / (T(2) * ra * rb);
+      const T cth = (ra * ra + rb * rb - e2) * (T(0.5) * s_ri[a] * s_ri[b]);
       const T S2 = ra * ra + rb * rb + e2;
       const T wab = s_w[a] * s_w[b];
-      const T fe = (e <= rc) ? T(0.5) * (Math<T>::cs((T)kPi * e / rc) + T(1)) : T(0);
+      const T fe = (e <= rc) ? T(0.5) * (Math<T>::cs(pirc * e) + T(1)) : T(0);
       const T F3 = act ? s_fc[g * KP + a] * s_fc[g * KP + b] * fe : T(0);
       T Pp[NZ], Pm[NZ], dPp[NZ], dPm[NZ];
       grid_powers<T, NZ, false>(cth, sf, Pp, Pm, dPp, dPm);
@@ -576,27 +570,28 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
 }
 
 template <typename T, int NZ>
-__device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDev &sf, const T *s_d, const T *s_w,
-                                             const T *s_fc, const T *s_dfc, const T *s_gA, const T *s_gW, int KP,
-                                             T *s_T, T *s_F, T (&vir)[6]) {
+__device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDev &sf, const T *s_d, const T *s_ri,
+                                             const T *s_w, const T *s_fc, const T *s_dfc, const T *s_gA,
+                                             const T *s_gW, int KP, T rc, T pirc, double *s_T, double *s_F,
+                                             T (&vir)[6]) {
   if (!active) return;
   const int a = q & 0xffff, b = q >> 16;
   const T ax = s_d[4 * a], ay = s_d[4 * a + 1], az = s_d[4 * a + 2], ra = s_d[4 * a + 3];
   const T bx = s_d[4 * b], by = s_d[4 * b + 1], bz = s_d[4 * b + 2], rb = s_d[4 * b + 3];
+  const T ria = s_ri[a], rib = s_ri[b];
   const T ex = bx - ax, ey = by - ay, ez = bz - az;
   const T e2 = ex * ex + ey * ey + ez * ez;
   const T e = Math<T>::sq(e2);
-  const T inv2ab = T(1) / (T(2) * ra * rb);
+  const T inv2ab = T(0.5) * ria * rib;
   const T cth = (ra * ra + rb * rb - e2) * inv2ab;
   const T S2 = ra * ra + rb * rb + e2;
-  const T dca = T(1) / rb - cth / ra, dcb = T(1) / ra - cth / rb, dce = -e * (T(2) * inv2ab);
+  const T dca = rib - cth * ria, dcb = ria - cth * rib, dce = -e * (T(2) * inv2ab);
   const T wab = s_w[a] * s_w[b];
   const int g = sf.g_ang;
-  const T rc = (T)sf.group_rc[g];
   const T fa = s_fc[g * KP + a], dfa = s_dfc[g * KP + a], fb = s_fc[g * KP + b], dfb = s_dfc[g * KP + b];
   const bool in = e <= rc;
-  const T fe = in ? T(0.5) * (Math<T>::cs((T)kPi * e / rc) + T(1)) : T(0);
-  const T dfe = in ? T(0.5) * (-Math<T>::sn((T)kPi * e / rc)) * ((T)kPi / rc) : T(0);
+  const T fe = in ? T(0.5) * (Math<T>::cs(pirc * e) + T(1)) : T(0);
+  const T dfe = in ? T(-0.5) * Math<T>::sn(pirc * e) * pirc : T(0);
   T Pp[NZ], Pm[NZ], dPp[NZ], dPm[NZ];
   grid_powers<T, NZ, true>(cth, sf, Pp, Pm, dPp, dPm);
   T SA = 0, SB = 0, SC = 0;
@@ -620,17 +615,17 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
   const T Ta = F * SA * dca - T(2) * ra * F * SC + SB * dfa * fb * fe;
   const T Tb = F * SA * dcb - T(2) * rb * F * SC + SB * fa * dfb * fe;
   const T Te = F * SA * dce - T(2) * e * F * SC + SB * fa * fb * dfe;
-  atomicAdd(&s_T[a], Ta);
-  atomicAdd(&s_T[b], Tb);
+  atomicAdd(&s_T[a], (double)Ta);
+  atomicAdd(&s_T[b], (double)Tb);
   if (e > T(0)) {
-    const T sc = Te / e;
+    const T sc = Te * Math<T>::rcp(e);
     const T ux = sc * ex, uy = sc * ey, uz = sc * ez;
-    atomicAdd(&s_F[3 * b], ux);
-    atomicAdd(&s_F[3 * b + 1], uy);
-    atomicAdd(&s_F[3 * b + 2], uz);
-    atomicAdd(&s_F[3 * a], -ux);
-    atomicAdd(&s_F[3 * a + 1], -uy);
-    atomicAdd(&s_F[3 * a + 2], -uz);
+    atomicAdd(&s_F[3 * b], (double)ux);
+    atomicAdd(&s_F[3 * b + 1], (double)uy);
+    atomicAdd(&s_F[3 * b + 2], (double)uz);
+    atomicAdd(&s_F[3 * a], (double)-ux);
+    atomicAdd(&s_F[3 * a + 1], (double)-uy);
+    atomicAdd(&s_F[3 * a + 2], (double)-uz);
     vir[0] += ux * ex;
     vir[1] += uy * ey;
     vir[2] += uz * ez;
@@ -646,7 +641,7 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
 template <typename T>
 __device__ __forceinline__ void ang_bwd_term(uint32_t q, bool active, const SFDev &sf, const T *s_d,
                                              const T *s_w, const T *s_fc, const T *s_dfc, const T *s_gA,
-                                             const T *s_gW, int KP, T *s_T, T *s_F, T (&vir)[6]) {
+                                             const T *s_gW, int KP, double *s_T, double *s_F, T (&vir)[6]) {
   if (!active) return;
   const int a = q & 0xffff, b = q >> 16;
   const T ax = s_d[4 * a], ay = s_d[4 * a + 1], az = s_d[4 * a + 2], ra = s_d[4 * a + 3];
@@ -700,17 +695,17 @@ __device__ __forceinline__ void ang_bwd_term(uint32_t q, bool active, const SFDe
     SC += eta * B;
   }
   if (cur_g >= 0) flush();
-  atomicAdd(&s_T[a], Ta);
-  atomicAdd(&s_T[b], Tb);
+  atomicAdd(&s_T[a], (double)Ta);
+  atomicAdd(&s_T[b], (double)Tb);
   if (e > T(0)) {
     const T s = Te / e;
     const T ux = s * ex, uy = s * ey, uz = s * ez;
-    atomicAdd(&s_F[3 * b], ux);
-    atomicAdd(&s_F[3 * b + 1], uy);
-    atomicAdd(&s_F[3 * b + 2], uz);
-    atomicAdd(&s_F[3 * a], -ux);
-    atomicAdd(&s_F[3 * a + 1], -uy);
-    atomicAdd(&s_F[3 * a + 2], -uz);
+    atomicAdd(&s_F[3 * b], (double)ux);
+    atomicAdd(&s_F[3 * b + 1], (double)uy);
+    atomicAdd(&s_F[3 * b + 2], (double)uz);
+    atomicAdd(&s_F[3 * a], (double)-ux);
+    atomicAdd(&s_F[3 * a + 1], (double)-uy);
+    atomicAdd(&s_F[3 * a + 2], (double)-uz);
     vir[0] += ux * ex;
     vir[1] += uy * ey;
     vir[2] += uz * ez;
@@ -730,17 +725,19 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   const int i = blockIdx.x;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int KP = L.KP, Dp = L.Dp, G = L.G;
+  // per-wave accumulators in FP64 for both precisions (LDS FP64 atomics are
+  // also the faster ones here), summed in wave order at the end: deterministic
   double *s_vir = (double *)smem;  // [W * 6]
-  T *s_d = (T *)(s_vir + 6 * W);
+  double *s_T = s_vir + 6 * W;     // [W][KP]
+  double *s_F = s_T + W * KP;      // [W][3 KP]
+  T *s_d = (T *)(s_F + 3 * W * KP);
   T *s_w = s_d + 4 * KP;
-  T *s_fc = s_w + KP;
+  T *s_ri = s_w + KP;  // 1 / r
+  T *s_fc = s_ri + KP;
   T *s_dfc = s_fc + G * KP;
   T *s_g = s_dfc + G * KP;
   T *s_gA = s_g + Dp;  // per angular line: g of the plain and of the weighted feature
   T *s_gW = s_gA + sf.n_ang;
-  // per-wave accumulators (summed in wave order at the end: deterministic)
-  T *s_T = s_gA + Dp;     // [W][KP]
-  T *s_F = s_T + W * KP;  // [W][3 KP]
 
   const int start = i * nb.kcap;
   const int K = nb.nb_count[i];
@@ -755,6 +752,7 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
     s_d[4 * a + 1] = (T)dd[1];
     s_d[4 * a + 2] = (T)dd[2];
     s_d[4 * a + 3] = (T)r;
+    s_ri[a] = (T)(1.0 / r);
     s_w[a] = (T)wz[species[nb.nb_j[start + a]]];
     for (int g = 0; g < sf.n_groups; ++g) {
       const double rc = sf.group_rc[g];
@@ -763,7 +761,7 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
       s_dfc[g * KP + a] = in ? (T)(0.5 * (-sin(kPi * r / rc)) * (kPi / rc)) : T(0);
     }
   }
-  for (int k = tid; k < 4 * W * KP; k += 64 * W) s_T[k] = T(0);  // s_T and s_F
+  for (int k = tid; k < 4 * W * KP; k += 64 * W) s_T[k] = 0.0;  // s_T and s_F
   const size_t row = (size_t)i * Dp;
   for (int d = tid; d < Dp; d += 64 * W) s_g[d] = g64 ? (T)g64[row + d] : (T)g32[row + d];
   for (int s = tid; s < sf.n_ang; s += 64 * W) {
@@ -785,7 +783,7 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
       const T geff = s_g[f] + w * s_g[f + sf.nsf];
       acc += geff * Math<T>::ex(-A * dr * dr) * (T(-2) * A * dr * s_fc[g * KP + a] + s_dfc[g * KP + a]);
     }
-    s_T[a] = acc;  // wave 0's row
+    s_T[a] = (double)acc;  // wave 0's row
   }
   __syncthreads();
 
@@ -793,11 +791,13 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   if (sf.n_ang > 0) {
     const uint32_t *tl = tb.list + (size_t)i * tb.tcap;
     const int nt = tb.count[i];
+    const T rcA = (T)sf.group_rc[sf.g_ang], pirc = (T)(kPi / sf.group_rc[sf.g_ang]);
     for (int t0 = 64 * wv; t0 < nt; t0 += 64 * W) {
       const bool act = t0 + lane < nt;
       const uint32_t q = act ? tl[t0 + lane] : (1u << 16);
       if constexpr (GNZ > 0)
-        ang_bwd_grid<T, GNZ>(q, act, sf, s_d, s_w, s_fc, s_dfc, s_gA, s_gW, KP, s_T + wv * KP, s_F + 3 * wv * KP, vir);
+        ang_bwd_grid<T, GNZ>(q, act, sf, s_d, s_ri, s_w, s_fc, s_dfc, s_gA, s_gW, KP, rcA, pirc, s_T + wv * KP,
+                             s_F + 3 * wv * KP, vir);
       else
         ang_bwd_term<T>(q, act, sf, s_d, s_w, s_fc, s_dfc, s_gA, s_gW, KP, s_T + wv * KP, s_F + 3 * wv * KP, vir);
     }
@@ -807,7 +807,7 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   // per-slot force contributions dE_i/dR(slot atom) and the radial virial
   for (int a = tid; a < K; a += 64 * W) {
     const T r = s_d[4 * a + 3];
-    T Ts = T(0), Fx = T(0), Fy = T(0), Fz = T(0);
+    double Ts = 0.0, Fx = 0.0, Fy = 0.0, Fz = 0.0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       Ts += s_T[w * KP + a];
@@ -815,19 +815,18 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
       Fy += s_F[3 * (w * KP + a) + 1];
       Fz += s_F[3 * (w * KP + a) + 2];
     }
-    const T s = Ts / r;
-    const T dx = s_d[4 * a], dy = s_d[4 * a + 1], dz = s_d[4 * a + 2];
-    const T fx = s * dx + Fx, fy = s * dy + Fy, fz = s * dz + Fz;
+    const double sc = Ts / (double)r;
+    const double dx = s_d[4 * a], dy = s_d[4 * a + 1], dz = s_d[4 * a + 2];
     double *o = slot_f + 3 * (size_t)(start + a);
-    o[0] = (double)fx;
-    o[1] = (double)fy;
-    o[2] = (double)fz;
-    vir[0] += s * dx * dx;
-    vir[1] += s * dy * dy;
-    vir[2] += s * dz * dz;
-    vir[3] += s * dx * dy;
-    vir[4] += s * dy * dz;
-    vir[5] += s * dx * dz;
+    o[0] = sc * dx + Fx;
+    o[1] = sc * dy + Fy;
+    o[2] = sc * dz + Fz;
+    vir[0] += (T)(sc * dx * dx);
+    vir[1] += (T)(sc * dy * dy);
+    vir[2] += (T)(sc * dz * dz);
+    vir[3] += (T)(sc * dx * dy);
+    vir[4] += (T)(sc * dy * dz);
+    vir[5] += (T)(sc * dx * dz);
   }
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
@@ -850,7 +849,7 @@ size_t fwd_lds(const Lds &L) {
 
 template <typename T, int W>
 size_t bwd_lds(const Lds &L) {
-  return 6 * W * sizeof(double) + (size_t)(5 * L.KP + 2 * L.G * L.KP + 2 * L.Dp + 4 * W * L.KP) * sizeof(T);
+  return (6 * W + 4 * W * (size_t)L.KP) * sizeof(double) + (size_t)(6 * L.KP + 2 * L.G * L.KP + 2 * L.Dp) * sizeof(T);
 }
 
 constexpr size_t kLdsBudget = 64 * 1024;
@@ -899,7 +898,7 @@ hipError_t bwd_grid(hipStream_t s, int N, const int *species, const double *wz, 
 template <typename T, int NZ, int W>
 size_t fwd_grid_lds(const Lds &L) {
   constexpr int NV = GridShape<T, NZ>::NV;
-  return W * sizeof(double) + (size_t)(4 * L.KP + L.KP + L.G * L.KP + L.Dp + W * NV + NV) * sizeof(T);
+  return W * sizeof(double) + (size_t)(6 * L.KP + L.G * L.KP + L.Dp + W * NV + NV) * sizeof(T);
 }
 
 template <typename T, int NZ, int W>
@@ -934,8 +933,9 @@ hipError_t fwd_grid(int waves, hipStream_t s, int N, const int *species, const d
 int max_neighbors_for_lds(bool fp64, int G, int Dp) {
   const size_t budget = kLdsBudget - 64;
   const size_t bytes = fp64 ? 8 : 4;
-  // the backward with one wave is the largest single-wave layout: (9 + 2G) KP + 2 Dp values
-  long kp = (long)(budget / bytes - 2 * Dp) / (9 + 2 * G);
+  // the backward with one wave is the largest single-wave layout: (6 + 2G) KP + 2 Dp
+  // values of the compute type and 4 KP FP64 accumulators
+  long kp = (long)(budget - 2 * Dp * bytes) / (long)((6 + 2 * G) * bytes + 4 * sizeof(double));
   return (int)(kp / 8 * 8);
 }
 
