@@ -201,9 +201,14 @@ static int upload_model(Model &m) {
   m.w_scale = std::ldexp(1.0f, e2);
   std::vector<uint16_t> hi((size_t)Mp * Dp), lo((size_t)Mp * Dp), bhi((size_t)Mp * Dp), blo((size_t)Mp * Dp);
   std::vector<uint16_t> thi((size_t)Mp * Dp), tlo((size_t)Mp * Dp), tbhi((size_t)Mp * Dp), tblo((size_t)Mp * Dp);
+  // Feature-major copies: within every group of 32 points, point p sits at
+  // position 8 ((p % 16) / 4) + p % 4 + 4 (p / 16), the K order in which the
+  // mixed contraction holds W (see gpr_mixed.hip).
   for (int k = 0; k < Mp; ++k)
     for (int d = 0; d < Dp; ++d) {
-      const size_t a = (size_t)k * Dp + d, b = (size_t)d * Mp + k;
+      const int p = k & 31;
+      const int kq = (k & ~31) + 8 * ((p & 15) >> 2) + (p & 3) + 4 * (p >> 4);
+      const size_t a = (size_t)k * Dp + d, b = (size_t)d * Mp + kq;
       const double v = mt[a];
       hi[a] = thi[b] = f16_bits(v);
       lo[a] = tlo[b] = f16_bits(v - f16_val(hi[a]));
@@ -437,7 +442,7 @@ static int ctx_load(Ctx &c, int nstruct, const int *natoms, const int *species, 
     c.offsets[s + 1] = c.offsets[s] + natoms[s];
   }
   c.N = c.offsets[nstruct];
-  c.Np = round_up(c.N, 64);
+  c.Np = round_up(c.N, 128);  // row tiles of the contraction kernels
   // species must be listed in the model header (the reference leaves their
   // weight undefined, gap_calc.f90:73-83)
   std::vector<int> listed(kMaxSpeciesZ, 0);
@@ -569,20 +574,20 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b, bool grad) {
   return GAPX_OK;
 }
 
-static int gpr_layout(Ctx &c, GprArgs &g) {
+static int gpr_layout(Ctx &c, GprArgs &g, bool fp64) {
   const Model &m = *c.m;
   g.N = c.N;
   g.Np = c.Np;
   g.Dp = m.Dp;
   g.Mp = m.Mp;
-  const int tiles = c.Np / 64;
-  int splits = std::max(1, (1024 + tiles - 1) / tiles);
-  splits = std::min(splits, m.Mp / 32);
-  // 64 points per split piece: a multiple of both kernels' LDS chunks
-  splits = std::min(splits, m.Mp / 64);
-  int per = round_up((m.Mp + splits - 1) / splits, 64);
-  splits = (m.Mp + per - 1) / per;
-  g.splits = splits;
+  // Split the sparse points over grid.y until there are about two row tiles
+  // per CU (enough resident waves; more splits only add partial traffic).
+  const int tiles = c.Np / gpr_rows_per_block(fp64, m.Dp);
+  const int chunk = gpr_points_per_chunk(fp64, m.Dp);
+  int splits = std::max(1, (512 + tiles - 1) / tiles);
+  splits = std::min(splits, m.Mp / chunk);
+  const int per = round_up((m.Mp + splits - 1) / splits, chunk);
+  g.splits = (m.Mp + per - 1) / per;
   g.pts_per_split = per;
   return GAPX_OK;
 }
@@ -634,7 +639,7 @@ static int ctx_run(Ctx &c, int flags) {
 
   // contraction
   GprArgs &g = b.g;
-  gpr_layout(c, g);
+  gpr_layout(c, g, fp64);
   g.xt64 = o.xt64;
   g.xn = o.xn;
   g.xt_hi = o.xt_hi;
@@ -654,7 +659,11 @@ static int ctx_run(Ctx &c, int flags) {
   g.wscale = m.w_scale;
   if ((rc = c.get("Epart", (size_t)g.splits * c.Np, &g.Epart)) || (rc = c.get("eatom", c.Np, &g.eatom))) return rc;
   if (grad) {
-    if ((rc = c.get("Gpart", (size_t)g.splits * ND, &g.Gpart))) return rc;
+    if (fp64) {
+      if ((rc = c.get("Gpart", (size_t)g.splits * ND, &g.Gpart))) return rc;
+    } else {
+      if ((rc = c.get("Gpart32", (size_t)g.splits * ND, &g.Gpart32))) return rc;
+    }
     if (fp64) {
       if ((rc = c.get("g64", ND, &g.g64))) return rc;
     } else {

@@ -185,6 +185,7 @@ struct GprArgs {
   const uint16_t *mt_hi, *mt_lo, *mtT_hi, *mtT_lo;
   const double *theta_inv;
   double *Gpart; double *Epart;   // [splits][Np][Dp], [splits][Np]
+  float *Gpart32;                 // [splits][Np][Dp]: the mixed path's partials
   double *eatom;                  // [N]
   double *g64; float *g32;        // [N][Dp]
   int grad;
@@ -193,6 +194,8 @@ struct GprArgs {
 };
 hipError_t launch_gpr_fp64(hipStream_t s, const GprArgs &a);
 hipError_t launch_gpr_mixed(hipStream_t s, const GprArgs &a);
+int gpr_rows_per_block(bool fp64, int Dp);   // row tile of the contraction kernels
+int gpr_points_per_chunk(bool fp64, int Dp); // sparse points per LDS chunk
 hipError_t launch_gpr_finish(hipStream_t s, const GprArgs &a, bool fp64);
 
 }  // namespace gapx

@@ -10,11 +10,15 @@
 // a dense N x D . D x M product, and with W = K o c
 //   E_i = sum_k W_ik,   g_id = -(xt_id E_i - (W mt)_id) / theta_d,
 // a second dense N x M . M x D product.  One kernel computes a 64-atom row
-// tile against a range of sparse points: S = xt mt^T on MFMA, the exp and the
-// alpha weights in registers, W staged through LDS, G += W mt on MFMA.  The
-// N x M kernel matrix never reaches HBM.  The sparse points are split over
-// grid.y to fill the GPU; partial E and G are summed by gpr_finish in a fixed
-// order (deterministic).
+// tile against a range of sparse points: S^T = mt xt^T on MFMA (sparse points
+// from LDS as the A operand, the wave's 16 descriptor rows in registers as B),
+// the exp and the alpha weights in registers, then G^T += mt^T W^T on MFMA
+// with W taken straight from the S^T accumulators: in the f64 16x16x4 layout
+// the result rows lq + 4 r of a 16-point block are the K index lq of the
+// r-th 4-point step, so no transpose through LDS is needed.  The N x M
+// kernel matrix never reaches HBM.  The sparse points are split over grid.y
+// to fill the GPU; partial E and G are summed by gpr_finish in a fixed order
+// (deterministic).
 #include "gapx_internal.h"
 
 namespace gapx {
@@ -30,9 +34,9 @@ constexpr int kPC = 32;  // sparse points per LDS chunk
 // (rows strided by 4, unlike the FP16/BF16 forms; measured on gfx950).
 template <int DP>
 __global__ __launch_bounds__(256) void k_gpr_f64(GprArgs a) {
-  __shared__ double s_m[kPC][DP + 4];
+  constexpr int SM = DP + 1;  // odd row stride (doubles): conflict-free column reads
+  __shared__ double s_m[kPC * SM];
   __shared__ double s_mn[kPC], s_c[kPC];
-  __shared__ double s_w[4][16][kPC + 1];
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int l16 = lane & 15, lq = lane >> 4;
   const int row0 = blockIdx.x * 64 + wv * 16;
@@ -40,72 +44,65 @@ __global__ __launch_bounds__(256) void k_gpr_f64(GprArgs a) {
   const int pb = split * a.pts_per_split;
   const int pe = min(a.Mp, pb + a.pts_per_split);
 
+  // B operand of S^T: B[k = lq][row l16] = xt[row][4 t + lq]
   double xa[DP / 4];
 #pragma unroll
   for (int t = 0; t < DP / 4; ++t) xa[t] = a.xt64[(size_t)(row0 + l16) * DP + 4 * t + lq];
-  double xnr[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) xnr[r] = a.xn[row0 + lq + 4 * r];
+  const double xnr = a.xn[row0 + l16];
 
-  v4d G[DP / 16];
+  v4d G[DP / 16];  // G^T: features 16 b + lq + 4 r of row l16
 #pragma unroll
   for (int b = 0; b < DP / 16; ++b) G[b] = v4d{0, 0, 0, 0};
-  double er[4] = {0, 0, 0, 0};
+  double er = 0.0;
 
   for (int p0 = pb; p0 < pe; p0 += kPC) {
     __syncthreads();
     for (int e = tid; e < kPC * DP; e += 256) {
       const int pc = e / DP, d = e - pc * DP;
-      s_m[pc][d] = a.mt[(size_t)(p0 + pc) * DP + d];
+      s_m[pc * SM + d] = a.mt[(size_t)(p0 + pc) * DP + d];
     }
     if (tid < kPC) {
       s_mn[tid] = a.mn[p0 + tid];
       s_c[tid] = a.coeff[p0 + tid];
     }
     __syncthreads();
-    v4d S0 = {0, 0, 0, 0}, S1 = {0, 0, 0, 0};
 #pragma unroll
-    for (int t = 0; t < DP / 4; ++t) {
-      S0 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[t], s_m[l16][4 * t + lq], S0, 0, 0, 0);
-      S1 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[t], s_m[16 + l16][4 * t + lq], S1, 0, 0, 0);
-    }
+    for (int h = 0; h < kPC / 16; ++h) {
+      // S^T[p = lq + 4 r][row l16] for the points 16 h + (0..15)
+      v4d S = {0, 0, 0, 0};
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int rr = lq + 4 * r;
-      const double w0 = exp(-0.5 * (xnr[r] + s_mn[l16] - 2.0 * S0[r])) * s_c[l16];
-      const double w1 = exp(-0.5 * (xnr[r] + s_mn[16 + l16] - 2.0 * S1[r])) * s_c[16 + l16];
-      er[r] += w0 + w1;
-      s_w[wv][rr][l16] = w0;
-      s_w[wv][rr][16 + l16] = w1;
-    }
-    __syncthreads();
-    if (a.grad) {
+      for (int t = 0; t < DP / 4; ++t)
+        S = __builtin_amdgcn_mfma_f64_16x16x4f64(s_m[(16 * h + l16) * SM + 4 * t + lq], xa[t], S, 0, 0, 0);
+      double w[4];
 #pragma unroll
-      for (int k4 = 0; k4 < kPC / 4; ++k4) {
-        const double aw = s_w[wv][l16][4 * k4 + lq];
+      for (int r = 0; r < 4; ++r) {
+        const int p = 16 * h + lq + 4 * r;
+        w[r] = exp(-0.5 * (xnr + s_mn[p] - 2.0 * S[r])) * s_c[p];
+        er += w[r];
+      }
+      if (a.grad) {
+        // K step r: points 16 h + 4 r + lq; W is lane (row, lq)'s w[r]
 #pragma unroll
-        for (int b = 0; b < DP / 16; ++b)
-          G[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(aw, s_m[4 * k4 + lq][16 * b + l16], G[b], 0, 0, 0);
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int b = 0; b < DP / 16; ++b)
+            G[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(s_m[(16 * h + 4 * r + lq) * SM + 16 * b + l16], w[r], G[b],
+                                                        0, 0, 0);
       }
     }
   }
 
   const size_t base = (size_t)split * a.Np;
+  const size_t row = (size_t)(row0 + l16);
   if (a.grad) {
 #pragma unroll
     for (int b = 0; b < DP / 16; ++b)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) a.Gpart[(base + row0 + lq + 4 * r) * DP + 16 * b + l16] = G[b][r];
+      for (int r = 0; r < 4; ++r) a.Gpart[(base + row) * DP + 16 * b + lq + 4 * r] = G[b][r];
   }
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    double v = er[r];
-    v += __shfl_xor(v, 1, 64);
-    v += __shfl_xor(v, 2, 64);
-    v += __shfl_xor(v, 4, 64);
-    v += __shfl_xor(v, 8, 64);
-    if (l16 == 0) a.Epart[base + row0 + lq + 4 * r] = v;
-  }
+  er += __shfl_xor(er, 16, 64);
+  er += __shfl_xor(er, 32, 64);
+  if (lq == 0) a.Epart[base + row] = er;
 }
 
 // E_i = sum of split partials; g_id = -(xt_id E_i - G_id) theta_inv_d.
@@ -118,7 +115,10 @@ __global__ __launch_bounds__(256) void k_gpr_finish(GprArgs a, int fp64) {
   if (!a.grad) return;
   for (int d = threadIdx.x; d < a.Dp; d += blockDim.x) {
     double G = 0.0;
-    for (int s = 0; s < a.splits; ++s) G += a.Gpart[((size_t)s * a.Np + i) * a.Dp + d];
+    if (fp64)
+      for (int s = 0; s < a.splits; ++s) G += a.Gpart[((size_t)s * a.Np + i) * a.Dp + d];
+    else
+      for (int s = 0; s < a.splits; ++s) G += (double)a.Gpart32[((size_t)s * a.Np + i) * a.Dp + d];
     const size_t idx = (size_t)i * a.Dp + d;
     const double xt = fp64 ? a.xt64[idx] : (double)a.xt32[idx];
     const double g = -(xt * E - G) * a.theta_inv[d];
@@ -132,6 +132,7 @@ __global__ __launch_bounds__(256) void k_gpr_finish(GprArgs a, int fp64) {
 }  // namespace
 
 hipError_t launch_gpr_fp64(hipStream_t s, const GprArgs &a) {
+  if (a.pts_per_split % kPC != 0 || a.Np % 64 != 0) return hipErrorInvalidValue;
   dim3 grid(a.Np / 64, a.splits);
   switch (a.Dp) {
     case 32: k_gpr_f64<32><<<grid, 256, 0, s>>>(a); break;

@@ -2,8 +2,8 @@
 // v_mfma_f32_16x16x32_{f16,bf16} with FP32 accumulation, FP64 norms and FP64
 // energy partial sums.
 //
-// Same mathematics as k_gpr_f64 (gpr.hip): for a 64-atom row tile against a
-// range of sparse points,
+// Same mathematics as k_gpr_f64 (gpr.hip): for a row tile against a range of
+// sparse points,
 //   S = xt mt^T,  d2 = |xt|^2 + |mt|^2 - 2 S  (norms in FP64),
 //   W = exp(-d2 / 2) c,  E_i += sum_k W_ik,  G += W mt.
 // Every operand x is carried as hi = round(x), lo = round(x - hi) in the
@@ -15,6 +15,16 @@
 // Operand layout of the 16x16x32 forms per lane l (l16 = l % 16, lq = l / 16):
 // A[l16][8 lq + e], B[8 lq + e][l16] (e = 0..7, one 16-byte register group),
 // result D[4 lq + r][l16], r = 0..3.
+//
+// The kernel computes S^T = mt xt^T (A = sparse points from LDS, B = the
+// wave's descriptor rows, held in registers for the whole kernel).  Lane
+// (l16, lq) then holds S^T for row l16 and points 4 lq + r of each 16-point
+// block, which is, for a 32-point step, exactly the B operand of
+// G^T = mt^T W^T when the K index runs over the points in the order
+// {4 lq .. 4 lq + 3, 16 + 4 lq .. 16 + 4 lq + 3} for lane group lq.  The
+// feature-major copy of mt is stored in that order (api.hip), so W goes from
+// the exp straight into MFMA operands without touching LDS.  Each wave owns R
+// 16-row blocks, so every LDS fragment feeds 3 R MFMAs.
 #include "gapx_internal.h"
 
 namespace gapx {
@@ -46,41 +56,48 @@ struct Half<true> {
   }
 };
 
-template <bool BF, int DP, int PC>
+template <bool BF, int DP, int PC, int R>
 __global__ __launch_bounds__(256) void k_gpr_mixed(GprArgs a) {
   typedef typename Half<BF>::vec vec;
   typedef typename Half<BF>::scalar hs;
-  constexpr int SM = DP + 8;   // row stride (halves) of the point-major tile
-  constexpr int ST = PC + 8;   // row stride (halves) of the feature-major tile and of W
+  constexpr int SM = DP + 8;  // row stride (halves) of the point-major tile
+  constexpr int ST = PC + 8;  // row stride (halves) of the feature-major tile
   __shared__ __align__(16) uint16_t s_mh[PC * SM], s_ml[PC * SM];
   __shared__ __align__(16) uint16_t s_th[DP * ST], s_tl[DP * ST];
-  __shared__ __align__(16) uint16_t s_wh[4][16 * ST], s_wl[4][16 * ST];
   __shared__ double s_mn[PC];
   __shared__ float s_c[PC];
 
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int l16 = lane & 15, lq = lane >> 4;
-  const int row0 = blockIdx.x * 64 + wv * 16;
+  const int row0 = blockIdx.x * (64 * R) + wv * (16 * R);
   const int split = blockIdx.y;
   const int pb = split * a.pts_per_split;
   const int pe = min(a.Mp, pb + a.pts_per_split);
 
-  // A operands (this wave's 16 rows), 8 features per lane per 32-wide step
-  vec xh[DP / 32], xl[DP / 32];
+  // B operands of S^T: row l16 of each row block, 8 features per lane per
+  // 32-wide step
+  vec xh[R][DP / 32], xl[R][DP / 32];
+  double xnr[R];
 #pragma unroll
-  for (int t = 0; t < DP / 32; ++t) {
-    const size_t off = (size_t)(row0 + l16) * DP + 32 * t + 8 * lq;
-    xh[t] = *(const vec *)(a.xt_hi + off);
-    xl[t] = *(const vec *)(a.xt_lo + off);
+  for (int rb = 0; rb < R; ++rb) {
+    const size_t row = (size_t)(row0 + 16 * rb + l16);
+#pragma unroll
+    for (int t = 0; t < DP / 32; ++t) {
+      const size_t off = row * DP + 32 * t + 8 * lq;
+      xh[rb][t] = *(const vec *)(a.xt_hi + off);
+      xl[rb][t] = *(const vec *)(a.xt_lo + off);
+    }
+    xnr[rb] = a.xn[row];
   }
-  double xnr[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) xnr[r] = a.xn[row0 + 4 * lq + r];
 
-  v4f G[DP / 16];
+  v4f G[R][DP / 16];  // G^T: features 16 b + 4 lq + r of row l16
 #pragma unroll
-  for (int b = 0; b < DP / 16; ++b) G[b] = v4f{0.f, 0.f, 0.f, 0.f};
-  double er[4] = {0, 0, 0, 0};
+  for (int rb = 0; rb < R; ++rb)
+#pragma unroll
+    for (int b = 0; b < DP / 16; ++b) G[rb][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  double er[R];
+#pragma unroll
+  for (int rb = 0; rb < R; ++rb) er[rb] = 0.0;
 
   for (int p0 = pb; p0 < pe; p0 += PC) {
     __syncthreads();
@@ -91,7 +108,7 @@ __global__ __launch_bounds__(256) void k_gpr_mixed(GprArgs a) {
       *(uint4 *)(s_mh + pc * SM + 8 * q) = *(const uint4 *)(a.mt_hi + g);
       *(uint4 *)(s_ml + pc * SM + 8 * q) = *(const uint4 *)(a.mt_lo + g);
     }
-    // feature-major tile: DP rows of PC halves
+    // feature-major tile (points in the permuted K order): DP rows of PC halves
     for (int e = tid; e < DP * PC / 8; e += 256) {
       const int d = e / (PC / 8), q = e - d * (PC / 8);
       const size_t g = (size_t)d * a.Mp + p0 + 8 * q;
@@ -104,85 +121,97 @@ __global__ __launch_bounds__(256) void k_gpr_mixed(GprArgs a) {
     }
     __syncthreads();
 
-    // S = xt mt^T for PC/16 column blocks, then W into LDS (hi/lo)
 #pragma unroll
-    for (int cb = 0; cb < PC / 16; ++cb) {
-      v4f S = {0.f, 0.f, 0.f, 0.f};
-      const int pt = 16 * cb + l16;
+    for (int sc = 0; sc < PC / 32; ++sc) {
+      // S^T for two 16-point blocks x R row blocks
+      v4f S[2][R];
 #pragma unroll
-      for (int t = 0; t < DP / 32; ++t) {
-        const vec mh = *(const vec *)(s_mh + pt * SM + 32 * t + 8 * lq);
-        const vec ml = *(const vec *)(s_ml + pt * SM + 32 * t + 8 * lq);
-        S = Half<BF>::mfma(xh[t], mh, S);
-        S = Half<BF>::mfma(xh[t], ml, S);
-        S = Half<BF>::mfma(xl[t], mh, S);
-      }
-      const double mnp = s_mn[pt];
-      const float cp = s_c[pt];
+      for (int h = 0; h < 2; ++h)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const float d2 = (float)(xnr[r] + mnp - 2.0 * (double)S[r]);
-        const float w = __expf(-0.5f * d2) * cp;
-        er[r] += (double)w;
-        const float ws = w * a.wscale;
-        const hs hi = (hs)ws;
-        const hs lo = (hs)(ws - (float)hi);
-        const int o = (4 * lq + r) * ST + pt;
-        s_wh[wv][o] = __builtin_bit_cast(uint16_t, hi);
-        s_wl[wv][o] = __builtin_bit_cast(uint16_t, lo);
-      }
-    }
-    __syncthreads();
-    if (a.grad) {
+        for (int rb = 0; rb < R; ++rb) S[h][rb] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int s = 0; s < PC / 32; ++s) {
-        const vec wh = *(const vec *)(&s_wh[wv][l16 * ST + 32 * s + 8 * lq]);
-        const vec wl = *(const vec *)(&s_wl[wv][l16 * ST + 32 * s + 8 * lq]);
+      for (int t = 0; t < DP / 32; ++t)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int pt = 32 * sc + 16 * h + l16;
+          const vec mh = *(const vec *)(s_mh + pt * SM + 32 * t + 8 * lq);
+          const vec ml = *(const vec *)(s_ml + pt * SM + 32 * t + 8 * lq);
+#pragma unroll
+          for (int rb = 0; rb < R; ++rb) {
+            S[h][rb] = Half<BF>::mfma(mh, xh[rb][t], S[h][rb]);
+            S[h][rb] = Half<BF>::mfma(ml, xh[rb][t], S[h][rb]);
+            S[h][rb] = Half<BF>::mfma(mh, xl[rb][t], S[h][rb]);
+          }
+        }
+      // W in registers, already in the B layout of the G step
+      vec wh[R], wl[R];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int pl = 32 * sc + 16 * h + 4 * lq + r;
+          const double mnp = s_mn[pl];
+          const float cp = s_c[pl];
+#pragma unroll
+          for (int rb = 0; rb < R; ++rb) {
+            const float d2 = (float)(xnr[rb] + mnp - 2.0 * (double)S[h][rb][r]);
+            const float w = __expf(-0.5f * d2) * cp;
+            er[rb] += (double)w;
+            const float ws = w * a.wscale;
+            const hs hi = (hs)ws;
+            wh[rb][4 * h + r] = hi;
+            wl[rb][4 * h + r] = (hs)(ws - (float)hi);
+          }
+        }
+      if (a.grad) {
 #pragma unroll
         for (int b = 0; b < DP / 16; ++b) {
           const int d = 16 * b + l16;
-          const vec th = *(const vec *)(s_th + d * ST + 32 * s + 8 * lq);
-          const vec tl = *(const vec *)(s_tl + d * ST + 32 * s + 8 * lq);
-          G[b] = Half<BF>::mfma(wh, th, G[b]);
-          G[b] = Half<BF>::mfma(wh, tl, G[b]);
-          G[b] = Half<BF>::mfma(wl, th, G[b]);
+          const vec th = *(const vec *)(s_th + d * ST + 32 * sc + 8 * lq);
+          const vec tl = *(const vec *)(s_tl + d * ST + 32 * sc + 8 * lq);
+#pragma unroll
+          for (int rb = 0; rb < R; ++rb) {
+            G[rb][b] = Half<BF>::mfma(th, wh[rb], G[rb][b]);
+            G[rb][b] = Half<BF>::mfma(tl, wh[rb], G[rb][b]);
+            G[rb][b] = Half<BF>::mfma(th, wl[rb], G[rb][b]);
+          }
         }
       }
     }
   }
 
   const size_t base = (size_t)split * a.Np;
-  if (a.grad) {
-    const double inv = 1.0 / (double)a.wscale;
 #pragma unroll
-    for (int b = 0; b < DP / 16; ++b)
+  for (int rb = 0; rb < R; ++rb) {
+    const size_t row = (size_t)(row0 + 16 * rb + l16);
+    if (a.grad) {
+      const float inv = 1.0f / a.wscale;  // a power of two: exact
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        a.Gpart[(base + row0 + 4 * lq + r) * DP + 16 * b + l16] = (double)G[b][r] * inv;
-  }
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    double v = er[r];
-    v += __shfl_xor(v, 1, 64);
-    v += __shfl_xor(v, 2, 64);
-    v += __shfl_xor(v, 4, 64);
-    v += __shfl_xor(v, 8, 64);
-    if (l16 == 0) a.Epart[base + row0 + 4 * lq + r] = v;
+      for (int b = 0; b < DP / 16; ++b) {
+        const float4 v = make_float4(G[rb][b][0] * inv, G[rb][b][1] * inv, G[rb][b][2] * inv, G[rb][b][3] * inv);
+        *(float4 *)(a.Gpart32 + (base + row) * DP + 16 * b + 4 * lq) = v;
+      }
+    }
+    double v = er[rb];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (lq == 0) a.Epart[base + row] = v;
   }
 }
 
 template <bool BF>
 hipError_t launch_typed(hipStream_t s, const GprArgs &a) {
-  dim3 grid(a.Np / 64, a.splits);
+  const int rows = gpr_rows_per_block(false, a.Dp);
+  dim3 grid(a.Np / rows, a.splits);
   switch (a.Dp) {
-    case 32: k_gpr_mixed<BF, 32, 64><<<grid, 256, 0, s>>>(a); break;
-    case 64: k_gpr_mixed<BF, 64, 64><<<grid, 256, 0, s>>>(a); break;
-    case 96: k_gpr_mixed<BF, 96, 64><<<grid, 256, 0, s>>>(a); break;
-    case 128: k_gpr_mixed<BF, 128, 64><<<grid, 256, 0, s>>>(a); break;
-    case 160: k_gpr_mixed<BF, 160, 32><<<grid, 256, 0, s>>>(a); break;
-    case 192: k_gpr_mixed<BF, 192, 32><<<grid, 256, 0, s>>>(a); break;
-    case 224: k_gpr_mixed<BF, 224, 32><<<grid, 256, 0, s>>>(a); break;
-    case 256: k_gpr_mixed<BF, 256, 32><<<grid, 256, 0, s>>>(a); break;
+    case 32: k_gpr_mixed<BF, 32, 64, 2><<<grid, 256, 0, s>>>(a); break;
+    case 64: k_gpr_mixed<BF, 64, 64, 2><<<grid, 256, 0, s>>>(a); break;
+    case 96: k_gpr_mixed<BF, 96, 64, 2><<<grid, 256, 0, s>>>(a); break;
+    case 128: k_gpr_mixed<BF, 128, 64, 2><<<grid, 256, 0, s>>>(a); break;
+    case 160: k_gpr_mixed<BF, 160, 32, 1><<<grid, 256, 0, s>>>(a); break;
+    case 192: k_gpr_mixed<BF, 192, 32, 1><<<grid, 256, 0, s>>>(a); break;
+    case 224: k_gpr_mixed<BF, 224, 32, 1><<<grid, 256, 0, s>>>(a); break;
+    case 256: k_gpr_mixed<BF, 256, 32, 1><<<grid, 256, 0, s>>>(a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -190,11 +219,13 @@ hipError_t launch_typed(hipStream_t s, const GprArgs &a) {
 
 }  // namespace
 
-// Points per LDS chunk of the mixed kernel for a padded feature count.
-int gpr_mixed_chunk(int Dp) { return Dp <= 128 ? 64 : 32; }
+int gpr_rows_per_block(bool fp64, int Dp) { return (fp64 || Dp > 128) ? 64 : 128; }
+
+int gpr_points_per_chunk(bool fp64, int Dp) { return fp64 ? 32 : (Dp <= 128 ? 64 : 32); }
 
 hipError_t launch_gpr_mixed(hipStream_t s, const GprArgs &a) {
-  if (a.pts_per_split % gpr_mixed_chunk(a.Dp) != 0) return hipErrorInvalidValue;
+  if (a.pts_per_split % gpr_points_per_chunk(false, a.Dp) != 0 || a.Np % gpr_rows_per_block(false, a.Dp) != 0)
+    return hipErrorInvalidValue;
   return a.bf16 ? launch_typed<true>(s, a) : launch_typed<false>(s, a);
 }
 
