@@ -10,6 +10,11 @@ in HBM: neighbour list, descriptors, contraction, backward pass, forces and
 stress.  Prints one JSON line (see the fields below).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload ternary4k]
+
+The JSON line also carries "population": the batched form (512 uneven
+structures, 8-256 atoms, D = 128, M = 4000), autotuned and replayed from a
+HIP graph, in mixed precision and in FP64, with the per-structure mixed
+error against FP64 (skip with --no-population).
 """
 import argparse
 import json
@@ -92,6 +97,40 @@ def stage_times(ctx, flags, steps):
     return {k: (v[0] / max(v[1], 1)) for k, v in st.items()}
 
 
+def population_bench(steps, warmup):
+    """Batched population in both precisions: autotuned, graph replay."""
+    t0 = time.perf_counter()
+    structs = synth.population(synth.POPULATION["n_struct"])
+    model = synth.population_model(gpu_descriptor_fn)
+    print("bench: population ready in %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
+    m = gapx.Model.from_model(model)
+    ctx = gapx.Context(m)
+    ctx.load(structs, synth.POPULATION["rc"])
+    natoms = np.array([len(s["species"]) for s in structs])
+    out, res = {}, {}
+    for name, flags in (("fp64", gapx.GRAD), ("mixed", gapx.GRAD | gapx.MIXED)):
+        tune = ctx.autotune(flags)
+        ctx.use_graph(True)
+        sec = time_runs(ctx, flags, steps, warmup)
+        ctx.use_graph(False)
+        st = stage_times(ctx, flags, max(3, steps // 4))
+        ctx.use_graph(True)
+        ctx.run(flags)
+        res[name] = ctx.results()
+        out[name] = dict(value=round(float(natoms.sum()) / sec, 1), ms_per_step=round(sec * 1e3, 4), tuning=tune,
+                         stages_ms={k: round(v, 4) for k, v in st.items()})
+        print("bench: population %s %.3f ms/step tuning %s" % (name, sec * 1e3, tune), file=sys.stderr, flush=True)
+    ctx.close()
+    m.close()
+    r64, rmx = res["fp64"], res["mixed"]
+    err = dict(energy_meV_per_atom=float(np.abs((rmx["energy"] - r64["energy"]) / natoms).max()) * 1e3,
+               force_meV_per_A=float(np.abs(rmx["forces"] - r64["forces"]).max()) * 1e3,
+               stress_GPa=float(np.abs(rmx["stress"] - r64["stress"]).max()))
+    return dict(structures=len(structs), atoms=int(natoms.sum()), unit="atom E+F+S evals/s", **out,
+                mixed_over_fp64=round(out["mixed"]["value"] / out["fp64"]["value"], 3),
+                mixed_error_vs_fp64_max={k: float("%.4g" % v) for k, v in err.items()})
+
+
 def cpu_baseline(model, s, rc, seconds):
     """The CPU oracle (sparse FP64 path, OpenMP over centre atoms) on a
     bounded sample: the first n centre atoms of the same structure."""
@@ -123,6 +162,7 @@ def main():
     ap.add_argument("--workload", default="ternary4k")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-population", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this evaluator runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -146,7 +186,9 @@ def main():
 
     runs = {}
     for name, flags in (("fp64", gapx.GRAD), ("mixed", gapx.GRAD | gapx.MIXED)):
+        ctx.use_graph(True)  # replay the captured launch sequence
         sec = time_runs(ctx, flags, args.steps, args.warmup)
+        ctx.use_graph(False)  # per-stage events need eager launches
         st = stage_times(ctx, flags, max(3, args.steps // 4))
         ctx.run(flags)
         res = ctx.results()
@@ -168,6 +210,11 @@ def main():
     roof["kernel"] = dom
     roof["stages"] = {k: dict(ms=round(st[k], 4), **{kk: vv for kk, vv in rates[k].items()
                                                      if kk in ("achieved", "frac", "bound")}) for k in st}
+    ctx.close()
+    m.close()
+    popr = None
+    if not args.no_population:
+        popr = population_bench(args.steps, args.warmup)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(model, s, rc, args.cpu_seconds)
@@ -192,6 +239,7 @@ def main():
         "mixed_error_vs_fp64": {k: float("%.4g" % v) for k, v in err.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "population": popr,
     }
     print(json.dumps(out), flush=True)
 

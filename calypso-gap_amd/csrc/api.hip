@@ -10,6 +10,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -300,6 +301,14 @@ enum Stage { kNeighbors, kDescFwd, kContraction, kFinish, kDescBwd, kForces, kNu
 static const char *kStageNames[kNumStages] = {"neighbor_list", "descriptors_fwd", "gpr_contraction",
                                               "gpr_finish", "descriptors_bwd", "forces_stress"};
 
+// Launch shapes chosen per context and precision (defaults, environment
+// overrides, or gapx_ctx_autotune).
+struct Tuning {
+  int waves_fwd = 4, waves_bwd = 4;  // wavefronts per atom in the descriptor kernels
+  int gpr_rows = 64;                 // rows per workgroup of the contraction (64 or 128)
+  int gpr_blocks = 512;              // target workgroups of the contraction (sets the splits)
+};
+
 struct Ctx {
   Model *m = nullptr;
   hipStream_t stream = nullptr;
@@ -316,8 +325,8 @@ struct Ctx {
   int *host_stats_dev = nullptr;  // its device-side address
   int last_flags = -1;
   bool ran = false;
-  int chunk = 0;  // lines per forward pass (0: 16)
-  int waves_fwd = 4, waves_bwd = 4;  // wavefronts per atom in the descriptor kernels
+  int chunk = 0;  // lines per forward pass of the generic kernels (0: 16)
+  Tuning tune[2];  // launch shapes for FP64 [0] and mixed [1] runs
   // timing
   bool timing = false;
   hipEvent_t ev[2 * kNumStages] = {};
@@ -328,7 +337,7 @@ struct Ctx {
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
   int graph_flags = -1;
-  int graph_nb_cap = -1, graph_KP = -1, graph_N = -1;
+  int graph_kcap = -1, graph_tcap = -1, graph_N = -1;
 
   template <typename T>
   int get(const char *name, size_t count, T **out) {
@@ -576,15 +585,17 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b, bool grad) {
 
 static int gpr_layout(Ctx &c, GprArgs &g, bool fp64) {
   const Model &m = *c.m;
+  const Tuning &t = c.tune[fp64 ? 0 : 1];
   g.N = c.N;
   g.Np = c.Np;
   g.Dp = m.Dp;
   g.Mp = m.Mp;
-  // Split the sparse points over grid.y until there are about two row tiles
-  // per CU (enough resident waves; more splits only add partial traffic).
-  const int tiles = c.Np / gpr_rows_per_block(fp64, m.Dp);
+  g.rows = (!fp64 && m.Dp <= 128 && t.gpr_rows == 128) ? 128 : 64;
+  // Split the sparse points over grid.y until there are about gpr_blocks
+  // workgroups (resident waves; more splits only add partial-sum traffic).
+  const int tiles = c.Np / g.rows;
   const int chunk = gpr_points_per_chunk(fp64, m.Dp);
-  int splits = std::max(1, (512 + tiles - 1) / tiles);
+  int splits = std::max(1, (t.gpr_blocks + tiles - 1) / tiles);
   splits = std::min(splits, m.Mp / chunk);
   const int per = round_up((m.Mp + splits - 1) / splits, chunk);
   g.splits = (m.Mp + per - 1) / per;
@@ -592,9 +603,9 @@ static int gpr_layout(Ctx &c, GprArgs &g, bool fp64) {
   return GAPX_OK;
 }
 
-static int ctx_run(Ctx &c, int flags) {
-  if (c.N <= 0) return fail(GAPX_ERR_ARG, "no structures loaded");
-  GAPX_HIP(hipSetDevice(c.device));
+// Enqueue one evaluation on the context's stream; nothing here waits for the
+// device, so the sequence can be captured into a graph.
+static int enqueue_run(Ctx &c, int flags) {
   Model &m = *c.m;
   const bool fp64 = !(flags & GAPX_MIXED);
   const bool grad = (flags & GAPX_GRAD) != 0;
@@ -633,8 +644,9 @@ static int ctx_run(Ctx &c, int flags) {
   const int split_mode = fp64 ? 0 : ((flags & GAPX_BF16) ? 2 : 1);
   stage_begin(c, kDescFwd);
   const int chunk = c.chunk > 0 ? c.chunk : 16;
+  const Tuning &tu = c.tune[fp64 ? 0 : 1];
   GAPX_HIP(launch_descriptors_fwd(c.stream, fp64, c.N, d_species, m.d_wz, nb, b.tb, m.sfd, m.d_theta_inv, m.d_mu,
-                                  split_mode, o, chunk, c.waves_fwd, c.KP));
+                                  split_mode, o, chunk, tu.waves_fwd, c.KP));
   stage_end(c, kDescFwd);
 
   // contraction
@@ -687,7 +699,7 @@ static int ctx_run(Ctx &c, int flags) {
     if ((rc = c.get("slot_f", 3 * (size_t)c.N * c.kcap, &b.slot_f))) return rc;
     stage_begin(c, kDescBwd);
     GAPX_HIP(launch_descriptors_bwd(c.stream, fp64, c.N, d_species, m.d_wz, nb, b.tb, m.sfd, Dp, g.g64, g.g32,
-                                    b.slot_f, b.atom_vir, c.waves_bwd, c.KP));
+                                    b.slot_f, b.atom_vir, tu.waves_bwd, c.KP));
     stage_end(c, kDescBwd);
   }
   stage_begin(c, kForces);
@@ -699,6 +711,47 @@ static int ctx_run(Ctx &c, int flags) {
   stage_end(c, kForces);
   // list statistics straight into pinned host memory (read by ctx_settle)
   GAPX_HIP(launch_list_stats(c.stream, c.N, nb, b.tb, c.host_stats_dev));
+  return GAPX_OK;
+}
+
+// One evaluation.  With graphs enabled (and timing off) the enqueued sequence
+// is captured once per (flags, atom count, list capacities) and replayed: an
+// eager run first brings every workspace buffer to its final size, so the
+// capture itself allocates nothing.
+static int ctx_run(Ctx &c, int flags) {
+  if (c.N <= 0) return fail(GAPX_ERR_ARG, "no structures loaded");
+  GAPX_HIP(hipSetDevice(c.device));
+  int rc;
+  if (c.use_graph && !c.timing) {
+    const bool valid = c.gexec && c.graph_flags == flags && c.graph_kcap == c.kcap && c.graph_tcap == c.tcap &&
+                       c.graph_N == c.N;
+    if (!valid) {
+      c.invalidate_graph();
+      if ((rc = enqueue_run(c, flags))) return rc;
+      hipGraph_t graph = nullptr;
+      GAPX_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_run(c, flags);
+      const hipError_t e = hipStreamEndCapture(c.stream, &graph);
+      if (rc || e != hipSuccess || !graph) {
+        if (graph) hipGraphDestroy(graph);
+        c.invalidate_graph();
+        return rc ? rc : fail(GAPX_ERR_HIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+      }
+      const hipError_t ei = hipGraphInstantiate(&c.gexec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        c.gexec = nullptr;
+        return fail(GAPX_ERR_HIP, std::string("graph instantiation failed: ") + hipGetErrorString(ei));
+      }
+      c.graph_flags = flags;
+      c.graph_kcap = c.kcap;
+      c.graph_tcap = c.tcap;
+      c.graph_N = c.N;
+    }
+    GAPX_HIP(hipGraphLaunch(c.gexec, c.stream));
+  } else {
+    if ((rc = enqueue_run(c, flags))) return rc;
+  }
   c.last_flags = flags;
   c.ran = true;
   if (c.timing) {  // one event pair per stage: read them before the next run
@@ -732,6 +785,104 @@ static int ctx_settle(Ctx &c) {
     collect_timing(c);
   }
   return fail(GAPX_ERR_NEIGHBORS, "neighbour capacity did not converge");
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: coordinate descent over the launch shapes, each knob timed on the
+// stage it changes (HIP events), on the loaded batch itself.  Results are
+// cached per process for the shape class (padded D, M, table form, precision,
+// atom and structure counts rounded up to powers of two).
+// ---------------------------------------------------------------------------
+static std::mutex g_tune_mu;
+static std::map<std::string, Tuning> g_tune_cache;
+
+static int pow2_bucket(int n) {
+  int b = 1;
+  while (b < n && b < (1 << 30)) b <<= 1;
+  return b;
+}
+
+static std::string tune_key(const Ctx &c, int flags) {
+  char buf[200];
+  std::snprintf(buf, sizeof buf, "D%d M%d G%d grid%d mixed%d bf%d N%d S%d", c.m->Dp, c.m->Mp, c.m->sf.n_groups,
+                c.m->sfd.grid, (flags & GAPX_MIXED) ? 1 : 0, (flags & GAPX_BF16) ? 1 : 0, pow2_bucket(c.N),
+                pow2_bucket(c.nstruct));
+  return buf;
+}
+
+static void reset_timing(Ctx &c) {
+  for (int s = 0; s < kNumStages; ++s) {
+    c.stage_ms[s] = 0;
+    c.stage_launches[s] = 0;
+    c.stage_pending[s] = false;
+  }
+}
+
+static int ctx_autotune(Ctx &c, int flags, bool force) {
+  if (c.N <= 0) return fail(GAPX_ERR_ARG, "no structures loaded");
+  const int p = (flags & GAPX_MIXED) ? 1 : 0;
+  const std::string key = tune_key(c, flags);
+  if (!force) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      c.tune[p] = it->second;
+      c.invalidate_graph();
+      return GAPX_OK;
+    }
+  }
+  const bool was_timing = c.timing, was_graph = c.use_graph;
+  c.use_graph = false;
+  c.invalidate_graph();
+  int rc = ctx_run(c, flags);
+  if (!rc) rc = ctx_settle(c);  // capacities first
+  c.timing = true;
+  reset_timing(c);
+  const int reps = 5;
+  auto measure = [&](std::initializer_list<int> stages, double &out) -> int {
+    int r = ctx_run(c, flags);  // warm-up (buffers, caches)
+    if (r) return r;
+    reset_timing(c);
+    for (int k = 0; k < reps; ++k)
+      if ((r = ctx_run(c, flags))) return r;
+    out = 0.0;
+    for (int st : stages) out += c.stage_ms[st] / std::max(1, c.stage_launches[st]);
+    return GAPX_OK;
+  };
+  struct Knob {
+    int *field;
+    std::vector<int> values;
+    std::initializer_list<int> stages;
+  };
+  Tuning &t = c.tune[p];
+  std::vector<Knob> knobs;
+  knobs.push_back({&t.waves_fwd, {1, 2, 4}, {kDescFwd}});
+  if (flags & GAPX_GRAD) knobs.push_back({&t.waves_bwd, {1, 2, 4}, {kDescBwd}});
+  if (p == 1 && c.m->Dp <= 128) knobs.push_back({&t.gpr_rows, {64, 128}, {kContraction, kFinish}});
+  knobs.push_back({&t.gpr_blocks, {256, 512, 1024, 2048}, {kContraction, kFinish}});
+  for (Knob &k : knobs) {
+    if (rc) break;
+    int best = *k.field;
+    double best_ms = 1e300;
+    for (int v : k.values) {
+      *k.field = v;
+      double ms = 0;
+      if ((rc = measure(k.stages, ms))) break;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = v;
+      }
+    }
+    *k.field = best;
+  }
+  c.timing = was_timing;
+  reset_timing(c);
+  c.use_graph = was_graph;
+  c.invalidate_graph();
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = t;
+  return GAPX_OK;
 }
 
 static int ctx_results(Ctx &c, double *ene, double *force, double *stress6, double *eatom, bool fortran) {
@@ -773,11 +924,16 @@ static int ctx_create(Model *m, Ctx **out) {
   }
   std::memset(c->host_stats, 0, 8 * sizeof(int));
   // Grid-form tables run fastest with one wavefront per atom (measured on the
-  // 4096-atom ternary cell, D = 128); the generic kernels with four.
-  if (c->m->sfd.grid) c->waves_fwd = c->waves_bwd = 1;
+  // 4096-atom ternary cell, D = 128); the generic kernels with four.  The
+  // mixed contraction takes 128-row tiles where its registers allow.
+  for (int p = 0; p < 2; ++p) {
+    Tuning &t = c->tune[p];
+    if (c->m->sfd.grid) t.waves_fwd = t.waves_bwd = 1;
+    t.gpr_rows = (p == 1 && c->m->Dp <= 128) ? 128 : 64;
+    if (const char *w = std::getenv("GAPX_DESC_WAVES_FWD")) t.waves_fwd = std::atoi(w);
+    if (const char *w = std::getenv("GAPX_DESC_WAVES_BWD")) t.waves_bwd = std::atoi(w);
+  }
   if (const char *ch = std::getenv("GAPX_DESC_CHUNK")) c->chunk = std::atoi(ch);
-  if (const char *w = std::getenv("GAPX_DESC_WAVES_FWD")) c->waves_fwd = std::atoi(w);
-  if (const char *w = std::getenv("GAPX_DESC_WAVES_BWD")) c->waves_bwd = std::atoi(w);
   *out = c;
   return GAPX_OK;
 }
@@ -1200,6 +1356,37 @@ int gapx_ctx_stage_info(gapx_ctx *c, int k, const char **name, double *ms, int *
   if (name) *name = kStageNames[k];
   if (ms) *ms = c->stage_ms[k];
   if (launches) *launches = c->stage_launches[k];
+  return GAPX_OK;
+}
+
+int gapx_ctx_autotune(gapx_ctx *c, int flags, int force) {
+  if (!c) return fail(GAPX_ERR_ARG, "null context");
+  GAPX_HIP(hipSetDevice(c->device));
+  return ctx_autotune(*c, flags & ~GAPX_LAYOUT_F, force != 0);
+}
+
+int gapx_ctx_get_tuning(gapx_ctx *c, int flags, int *waves_fwd, int *waves_bwd, int *gpr_rows, int *gpr_blocks) {
+  if (!c) return fail(GAPX_ERR_ARG, "null context");
+  const Tuning &t = c->tune[(flags & GAPX_MIXED) ? 1 : 0];
+  if (waves_fwd) *waves_fwd = t.waves_fwd;
+  if (waves_bwd) *waves_bwd = t.waves_bwd;
+  if (gpr_rows) *gpr_rows = t.gpr_rows;
+  if (gpr_blocks) *gpr_blocks = t.gpr_blocks;
+  return GAPX_OK;
+}
+
+int gapx_ctx_set_tuning(gapx_ctx *c, int flags, int waves_fwd, int waves_bwd, int gpr_rows, int gpr_blocks) {
+  if (!c) return fail(GAPX_ERR_ARG, "null context");
+  auto ok_w = [](int w) { return w == 1 || w == 2 || w == 4; };
+  if (!ok_w(waves_fwd) || !ok_w(waves_bwd) || (gpr_rows != 64 && gpr_rows != 128) || gpr_blocks < 1 ||
+      gpr_blocks > (1 << 20))
+    return fail(GAPX_ERR_ARG, "gapx_ctx_set_tuning: waves must be 1, 2 or 4, rows 64 or 128, blocks in [1, 2^20]");
+  Tuning &t = c->tune[(flags & GAPX_MIXED) ? 1 : 0];
+  t.waves_fwd = waves_fwd;
+  t.waves_bwd = waves_bwd;
+  t.gpr_rows = gpr_rows;
+  t.gpr_blocks = gpr_blocks;
+  c->invalidate_graph();
   return GAPX_OK;
 }
 

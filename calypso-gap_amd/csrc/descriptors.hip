@@ -130,12 +130,14 @@ struct Lds {
 // One wavefront per centre atom, neighbours staged in LDS.  Pairs (a, b),
 // a < b, with r_a, r_b, r_ab all within the largest angular cutoff are
 // appended (ballot + prefix count) to the atom's list at i * tcap, packed as
-// a | b << 16, in diagonal order: d = b - a outer, a inner.  Consecutive
-// entries then have distinct a and distinct b, so the backward pass's LDS
-// atomics on per-neighbour accumulators rarely collide within one
-// instruction.  The cutoff test is a squared comparison with a relative
-// margin of 1e-6; a triplet admitted by the margin contributes zero, since the
-// descriptor kernels apply each line's cutoff to every distance.
+// a | b << 16, in diagonal order: the pairs with b - a = d form a run of
+// K - d entries.  Runs d and K - d together hold K entries, so one wave step
+// takes both (full lanes for K <= 64).  Consecutive entries of a run have
+// distinct a and distinct b, so the backward pass's LDS atomics on
+// per-neighbour accumulators rarely collide within one instruction.  The
+// cutoff test is a squared comparison with a relative margin of 1e-6; a
+// triplet admitted by the margin contributes zero, since the descriptor
+// kernels apply each line's cutoff to every distance.
 __global__ __launch_bounds__(64) void k_triplets(int N, NeighborBufs nb, double rcA, TripletBufs tb) {
   extern __shared__ __align__(16) unsigned char smem[];
   float *s_d = (float *)smem;  // [4 * kcap]
@@ -155,11 +157,16 @@ __global__ __launch_bounds__(64) void k_triplets(int N, NeighborBufs nb, double 
   const float rc2 = rc * rc;
   const size_t out = (size_t)i * tb.tcap;
   int count = 0;
-  for (int d = 1; d < K; ++d) {
-    for (int a0 = 0; a0 + d < K; a0 += 64) {
-      const int a = a0 + lane, b = a + d;
+  for (int d1 = 1; 2 * d1 <= K; ++d1) {
+    // run d1 (K - d1 pairs), then run d2 = K - d1 (d1 pairs) unless it is d1
+    const int d2 = K - d1;
+    const int n1 = K - d1, n = (d2 != d1) ? K : n1;
+    for (int c0 = 0; c0 < n; c0 += 64) {
+      const int c = c0 + lane;
+      const bool first = c < n1;
+      const int a = first ? c : c - n1, b = a + (first ? d1 : d2);
       bool ok = false;
-      if (b < K && s_d[4 * a + 3] <= rc && s_d[4 * b + 3] <= rc) {
+      if (c < n && s_d[4 * a + 3] <= rc && s_d[4 * b + 3] <= rc) {
         const float ex = s_d[4 * b] - s_d[4 * a], ey = s_d[4 * b + 1] - s_d[4 * a + 1],
                     ez = s_d[4 * b + 2] - s_d[4 * a + 2];
         ok = ex * ex + ey * ey + ez * ez <= rc2;

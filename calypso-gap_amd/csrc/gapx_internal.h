@@ -178,6 +178,7 @@ hipError_t launch_struct_reduce(hipStream_t s, int nstruct, const StructParams *
 // g_i = -(xt_i E_i - sum_k W_ik mt_k) * theta_inv.
 struct GprArgs {
   int N, Np, Dp, Mp, splits, pts_per_split;
+  int rows;                       // rows per workgroup (64; the mixed kernel also 128)
   const double *xt64; const double *xn;
   const uint16_t *xt_hi, *xt_lo;
   const float *xt32;
@@ -194,7 +195,6 @@ struct GprArgs {
 };
 hipError_t launch_gpr_fp64(hipStream_t s, const GprArgs &a);
 hipError_t launch_gpr_mixed(hipStream_t s, const GprArgs &a);
-int gpr_rows_per_block(bool fp64, int Dp);   // row tile of the contraction kernels
 int gpr_points_per_chunk(bool fp64, int Dp); // sparse points per LDS chunk
 hipError_t launch_gpr_finish(hipStream_t s, const GprArgs &a, bool fp64);
 

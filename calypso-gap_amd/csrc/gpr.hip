@@ -66,18 +66,26 @@ __global__ __launch_bounds__(256) void k_gpr_f64(GprArgs a) {
       s_c[tid] = a.coeff[p0 + tid];
     }
     __syncthreads();
+    // S^T[p = lq + 4 r][row l16] for both 16-point blocks: four independent
+    // accumulator chains (even and odd feature steps) keep the MFMA pipe full
+    v4d Sa[kPC / 16], Sb[kPC / 16];
+#pragma unroll
+    for (int h = 0; h < kPC / 16; ++h) Sa[h] = Sb[h] = v4d{0, 0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < DP / 4; t += 2)
+#pragma unroll
+      for (int h = 0; h < kPC / 16; ++h) {
+        Sa[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(s_m[(16 * h + l16) * SM + 4 * t + lq], xa[t], Sa[h], 0, 0, 0);
+        Sb[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(s_m[(16 * h + l16) * SM + 4 * t + 4 + lq], xa[t + 1], Sb[h], 0,
+                                                     0, 0);
+      }
 #pragma unroll
     for (int h = 0; h < kPC / 16; ++h) {
-      // S^T[p = lq + 4 r][row l16] for the points 16 h + (0..15)
-      v4d S = {0, 0, 0, 0};
-#pragma unroll
-      for (int t = 0; t < DP / 4; ++t)
-        S = __builtin_amdgcn_mfma_f64_16x16x4f64(s_m[(16 * h + l16) * SM + 4 * t + lq], xa[t], S, 0, 0, 0);
       double w[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int p = 16 * h + lq + 4 * r;
-        w[r] = exp(-0.5 * (xnr + s_mn[p] - 2.0 * S[r])) * s_c[p];
+        w[r] = exp(-0.5 * (xnr + s_mn[p] - 2.0 * (Sa[h][r] + Sb[h][r]))) * s_c[p];
         er += w[r];
       }
       if (a.grad) {

@@ -199,32 +199,38 @@ __global__ __launch_bounds__(256) void k_gpr_mixed(GprArgs a) {
   }
 }
 
-template <bool BF>
-hipError_t launch_typed(hipStream_t s, const GprArgs &a) {
-  const int rows = gpr_rows_per_block(false, a.Dp);
-  dim3 grid(a.Np / rows, a.splits);
-  switch (a.Dp) {
-    case 32: k_gpr_mixed<BF, 32, 64, 2><<<grid, 256, 0, s>>>(a); break;
-    case 64: k_gpr_mixed<BF, 64, 64, 2><<<grid, 256, 0, s>>>(a); break;
-    case 96: k_gpr_mixed<BF, 96, 64, 2><<<grid, 256, 0, s>>>(a); break;
-    case 128: k_gpr_mixed<BF, 128, 64, 2><<<grid, 256, 0, s>>>(a); break;
-    case 160: k_gpr_mixed<BF, 160, 32, 1><<<grid, 256, 0, s>>>(a); break;
-    case 192: k_gpr_mixed<BF, 192, 32, 1><<<grid, 256, 0, s>>>(a); break;
-    case 224: k_gpr_mixed<BF, 224, 32, 1><<<grid, 256, 0, s>>>(a); break;
-    case 256: k_gpr_mixed<BF, 256, 32, 1><<<grid, 256, 0, s>>>(a); break;
-    default: return hipErrorInvalidValue;
+template <bool BF, int DP, int PC>
+hipError_t launch_rows(hipStream_t s, const GprArgs &a) {
+  if (a.rows == 128 && DP <= 128) {
+    k_gpr_mixed<BF, DP, PC, 2><<<dim3(a.Np / 128, a.splits), 256, 0, s>>>(a);
+  } else {
+    k_gpr_mixed<BF, DP, PC, 1><<<dim3(a.Np / 64, a.splits), 256, 0, s>>>(a);
   }
   return hipGetLastError();
 }
 
-}  // namespace
+template <bool BF>
+hipError_t launch_typed(hipStream_t s, const GprArgs &a) {
+  switch (a.Dp) {
+    case 32: return launch_rows<BF, 32, 64>(s, a);
+    case 64: return launch_rows<BF, 64, 64>(s, a);
+    case 96: return launch_rows<BF, 96, 64>(s, a);
+    case 128: return launch_rows<BF, 128, 64>(s, a);
+    case 160: return launch_rows<BF, 160, 32>(s, a);
+    case 192: return launch_rows<BF, 192, 32>(s, a);
+    case 224: return launch_rows<BF, 224, 32>(s, a);
+    case 256: return launch_rows<BF, 256, 32>(s, a);
+    default: return hipErrorInvalidValue;
+  }
+}
 
-int gpr_rows_per_block(bool fp64, int Dp) { return (fp64 || Dp > 128) ? 64 : 128; }
+}  // namespace
 
 int gpr_points_per_chunk(bool fp64, int Dp) { return fp64 ? 32 : (Dp <= 128 ? 64 : 32); }
 
 hipError_t launch_gpr_mixed(hipStream_t s, const GprArgs &a) {
-  if (a.pts_per_split % gpr_points_per_chunk(false, a.Dp) != 0 || a.Np % gpr_rows_per_block(false, a.Dp) != 0)
+  if (a.pts_per_split % gpr_points_per_chunk(false, a.Dp) != 0 || (a.rows != 64 && a.rows != 128) ||
+      a.Np % a.rows != 0)
     return hipErrorInvalidValue;
   return a.bf16 ? launch_typed<true>(s, a) : launch_typed<false>(s, a);
 }

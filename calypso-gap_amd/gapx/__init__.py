@@ -38,6 +38,7 @@ EXPORTS = [
     "gapx_ctx_update", "gapx_ctx_run", "gapx_ctx_sync", "gapx_ctx_results", "gapx_ctx_features",
     "gapx_ctx_timing", "gapx_ctx_stage_count", "gapx_ctx_stage_info", "gapx_ctx_use_graph",
     "gapx_ctx_device_bytes", "gapx_ctx_free", "gapx_ctx_counts", "gapx_min_distance",
+    "gapx_ctx_autotune", "gapx_ctx_get_tuning", "gapx_ctx_set_tuning",
 ]
 
 
@@ -77,6 +78,9 @@ def lib():
         L.gapx_ctx_results.argtypes = [vp, vp, vp, vp, vp]
         L.gapx_ctx_features.argtypes = [vp, vp, vp]
         L.gapx_ctx_timing.argtypes = [vp, ci]
+        L.gapx_ctx_autotune.argtypes = [vp, ci, ci]
+        L.gapx_ctx_get_tuning.argtypes = [vp, ci, vp, vp, vp, vp]
+        L.gapx_ctx_set_tuning.argtypes = [vp, ci, ci, ci, ci, ci]
         L.gapx_ctx_stage_count.argtypes = [vp, vp]
         L.gapx_ctx_stage_info.argtypes = [vp, ci, vp, vp, vp]
         L.gapx_ctx_use_graph.argtypes = [vp, ci]
@@ -281,6 +285,20 @@ class Context:
 
     def use_graph(self, on=True):
         _check(lib().gapx_ctx_use_graph(self._h, int(on)))
+
+    def autotune(self, flags=GRAD, force=False):
+        """Time launch shapes on the loaded batch and keep the fastest."""
+        _check(lib().gapx_ctx_autotune(self._h, int(flags), int(force)))
+        return self.tuning(flags)
+
+    def tuning(self, flags=GRAD):
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(lib().gapx_ctx_get_tuning(self._h, int(flags), *[ctypes.byref(x) for x in v]))
+        return dict(zip(("waves_fwd", "waves_bwd", "gpr_rows", "gpr_blocks"), (x.value for x in v)))
+
+    def set_tuning(self, flags=GRAD, waves_fwd=1, waves_bwd=1, gpr_rows=64, gpr_blocks=512):
+        _check(lib().gapx_ctx_set_tuning(self._h, int(flags), int(waves_fwd), int(waves_bwd), int(gpr_rows),
+                                         int(gpr_blocks)))
 
     def counts(self):
         """(pairs, triplets) of the last run."""

@@ -250,3 +250,28 @@ def population(n_struct=512, nmin=8, nmax=256, seed=1004, species_set=(1, 6, 8, 
         sp = rng.choice(np.asarray(species_set), size=ns, replace=False)
         out.append(structure(n, n * vpa, sp, rng, cell="triclinic", min_dist=min_dist))
     return out
+
+
+# Batched form: a population of uneven periodic structures, and the D = 128,
+# M = 4000 model it is evaluated with (species 1, 6, 8, 22, weights = Z; the
+# SF table is the ternary4k one; the sparse set is drawn from the descriptors
+# of a separately seeded population).
+POPULATION = dict(n_struct=512, nmin=8, nmax=256, seed=1004, species=(1, 6, 8, 22), rc=6.0, n_radial=32,
+                  etas=logspace(0.005, 0.5, 4), zetas=(1, 2, 4, 16), M=4000, sigma=0.05, pool_structs=48,
+                  pool_seed=2004)
+
+
+def population_sf():
+    p = POPULATION
+    return sf_table(p["n_radial"], p["etas"], p["zetas"], p["rc"])
+
+
+def population_model(descriptor_fn, M=None):
+    """GAP model for the population workload; descriptor_fn as for workload_model."""
+    p = POPULATION
+    sf = population_sf()
+    header = dict(z=np.array(p["species"], np.int32), w=np.array(p["species"], float), sf=sf)
+    pool = population(p["pool_structs"], seed=p["pool_seed"], species_set=p["species"])
+    xs = [descriptor_fn(header, s["species"], s["lat"], s["pos"], p["rc"]) for s in pool]
+    rng = np.random.default_rng(p["seed"] + 100000)
+    return make_model(p["species"], p["species"], sf, np.concatenate(xs), M or p["M"], p["sigma"], rng)

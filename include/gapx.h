@@ -137,8 +137,18 @@ int gapx_ctx_timing(gapx_ctx *c, int enable);
 int gapx_ctx_stage_count(gapx_ctx *c, int *n);
 int gapx_ctx_stage_info(gapx_ctx *c, int k, const char **name, double *ms, int *launches);
 /* Capture gapx_ctx_run(flags) into a HIP graph and replay it on later runs
- * with the same flags (on = 1), or go back to eager launches (on = 0). */
+ * with the same flags, atom count and list capacities (on = 1), or go back to
+ * eager launches (on = 0).  The first run after a change is captured after
+ * one eager run; runs with timing enabled are always eager.  No reference
+ * counterpart (the reference is a CPU library). */
 int gapx_ctx_use_graph(gapx_ctx *c, int on);
+/* Pick launch shapes for runs with these flags (wavefronts per atom in the
+ * descriptor kernels, contraction row tile and workgroup count) by timing the
+ * loaded batch; results are cached per process for the shape class unless
+ * force != 0.  No reference counterpart. */
+int gapx_ctx_autotune(gapx_ctx *c, int flags, int force);
+int gapx_ctx_get_tuning(gapx_ctx *c, int flags, int *waves_fwd, int *waves_bwd, int *gpr_rows, int *gpr_blocks);
+int gapx_ctx_set_tuning(gapx_ctx *c, int flags, int waves_fwd, int waves_bwd, int gpr_rows, int gpr_blocks);
 /* Work counts of the last run: (centre, neighbour) pairs within rcut and
  * neighbour pairs j < k of a centre within the largest angular cutoff. */
 int gapx_ctx_counts(gapx_ctx *c, long long *pairs, long long *triplets);
