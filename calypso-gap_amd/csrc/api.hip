@@ -327,6 +327,7 @@ struct Ctx {
   bool ran = false;
   int chunk = 0;  // lines per forward pass of the generic kernels (0: 16)
   Tuning tune[2];  // launch shapes for FP64 [0] and mixed [1] runs
+  Mlp *mlp = nullptr;  // set while the last run used the surrogate
   // timing
   bool timing = false;
   hipEvent_t ev[2 * kNumStages] = {};
@@ -603,6 +604,37 @@ static int gpr_layout(Ctx &c, GprArgs &g, bool fp64) {
   return GAPX_OK;
 }
 
+// From dE/dx to forces and stress: descriptor backward pass, force gather,
+// per-structure energy and virial, list statistics.
+static int enqueue_forces(Ctx &c, Bufs &b, bool fp64, bool grad, const double *eatom, const double *g64,
+                          const float *g32, int waves_bwd) {
+  Model &m = *c.m;
+  NeighborBufs &nb = b.nb;
+  const int *d_species = c.ptr<int>("species");
+  const StructParams *d_sp = c.ptr<StructParams>("sp");
+  int rc;
+  if ((rc = c.get("force", 3 * (size_t)c.N, &b.force)) || (rc = c.get("ene", c.nstruct, &b.ene)) ||
+      (rc = c.get("stress", 6 * (size_t)c.nstruct, &b.stress)) || (rc = c.get("atom_vir", 6 * (size_t)c.N, &b.atom_vir)))
+    return rc;
+  if (grad) {
+    if ((rc = c.get("slot_f", 3 * (size_t)c.N * c.kcap, &b.slot_f))) return rc;
+    stage_begin(c, kDescBwd);
+    GAPX_HIP(launch_descriptors_bwd(c.stream, fp64, c.N, d_species, m.d_wz, nb, b.tb, m.sfd, m.Dp, g64, g32,
+                                    b.slot_f, b.atom_vir, waves_bwd, c.KP));
+    stage_end(c, kDescBwd);
+  }
+  stage_begin(c, kForces);
+  if (grad)
+    GAPX_HIP(launch_gather_forces(c.stream, c.N, nb, b.slot_f, b.force));
+  else
+    GAPX_HIP(launch_fill_double(c.stream, b.force, 3 * (size_t)c.N, 0.0));
+  GAPX_HIP(launch_struct_reduce(c.stream, c.nstruct, d_sp, eatom, b.atom_vir, grad ? 1 : 0, b.ene, b.stress));
+  stage_end(c, kForces);
+  // list statistics straight into pinned host memory (read by ctx_settle)
+  GAPX_HIP(launch_list_stats(c.stream, c.N, nb, b.tb, c.host_stats_dev));
+  return GAPX_OK;
+}
+
 // Enqueue one evaluation on the context's stream; nothing here waits for the
 // device, so the sequence can be captured into a graph.
 static int enqueue_run(Ctx &c, int flags) {
@@ -692,37 +724,74 @@ static int enqueue_run(Ctx &c, int flags) {
   GAPX_HIP(launch_gpr_finish(c.stream, g, fp64));
   stage_end(c, kFinish);
 
-  if ((rc = c.get("force", 3 * (size_t)c.N, &b.force)) || (rc = c.get("ene", c.nstruct, &b.ene)) ||
-      (rc = c.get("stress", 6 * (size_t)c.nstruct, &b.stress)) || (rc = c.get("atom_vir", 6 * (size_t)c.N, &b.atom_vir)))
-    return rc;
-  if (grad) {
-    if ((rc = c.get("slot_f", 3 * (size_t)c.N * c.kcap, &b.slot_f))) return rc;
-    stage_begin(c, kDescBwd);
-    GAPX_HIP(launch_descriptors_bwd(c.stream, fp64, c.N, d_species, m.d_wz, nb, b.tb, m.sfd, Dp, g.g64, g.g32,
-                                    b.slot_f, b.atom_vir, tu.waves_bwd, c.KP));
-    stage_end(c, kDescBwd);
+  return enqueue_forces(c, b, fp64, grad, g.eatom, g.g64, g.g32, tu.waves_bwd);
+}
+
+// The surrogate in place of the contraction: same neighbour lists and
+// descriptors, MLP energies and dE/dx, same backward pass.  The MLP time is
+// reported as the contraction stage.
+static int enqueue_surrogate(Ctx &c, Mlp &mlp, int flags) {
+  Model &m = *c.m;
+  if (m.D != 128 || m.Dp != 128) return fail(GAPX_ERR_MODEL, "the surrogate takes models with 128 descriptor columns");
+  const bool fp64 = !(flags & GAPX_MIXED);
+  const bool grad = (flags & GAPX_GRAD) != 0;
+  const int Dp = m.Dp;
+  Bufs b;
+  int rc;
+  stage_begin(c, kNeighbors);
+  if ((rc = run_neighbors(c, flags, b, grad))) return rc;
+  stage_end(c, kNeighbors);
+  DescOut &o = b.out;
+  o.N = c.N;
+  o.Dp = Dp;
+  const size_t ND = (size_t)c.Np * Dp;
+  if ((rc = c.get("xn", c.Np, &o.xn))) return rc;
+  if (fp64) {
+    if ((rc = c.get("x64", ND, &o.x64)) || (rc = c.get("xt64", ND, &o.xt64))) return rc;
+  } else {
+    if ((rc = c.get("x32", ND, &o.x32)) || (rc = c.get("xt32", ND, &o.xt32))) return rc;
   }
-  stage_begin(c, kForces);
-  if (grad)
-    GAPX_HIP(launch_gather_forces(c.stream, c.N, nb, b.slot_f, b.force));
-  else
-    GAPX_HIP(launch_fill_double(c.stream, b.force, 3 * (size_t)c.N, 0.0));
-  GAPX_HIP(launch_struct_reduce(c.stream, c.nstruct, d_sp, g.eatom, b.atom_vir, grad ? 1 : 0, b.ene, b.stress));
-  stage_end(c, kForces);
-  // list statistics straight into pinned host memory (read by ctx_settle)
-  GAPX_HIP(launch_list_stats(c.stream, c.N, nb, b.tb, c.host_stats_dev));
-  return GAPX_OK;
+  const Tuning &tu = c.tune[fp64 ? 0 : 1];
+  stage_begin(c, kDescFwd);
+  GAPX_HIP(launch_descriptors_fwd(c.stream, fp64, c.N, c.ptr<int>("species"), m.d_wz, b.nb, b.tb, m.sfd,
+                                  m.d_theta_inv, m.d_mu, 0, o, c.chunk > 0 ? c.chunk : 16, tu.waves_fwd, c.KP));
+  stage_end(c, kDescFwd);
+  float *xin = o.xt32;
+  if (fp64) {
+    if ((rc = c.get("xtf", ND, &xin))) return rc;
+    GAPX_HIP(launch_f64_to_f32(c.stream, (size_t)c.N * Dp, o.xt64, xin));
+  }
+  if (c.Np > c.N) GAPX_HIP(launch_fill_int(c.stream, (int *)(xin + (size_t)c.N * Dp), (c.Np - c.N) * Dp, 0));
+  double *eatom = nullptr, *g64 = nullptr;
+  float *g32 = nullptr;
+  if ((rc = c.get("eatom", c.Np, &eatom))) return rc;
+  if (grad) {
+    if (fp64) {
+      if ((rc = c.get("g64", ND, &g64))) return rc;
+    } else {
+      if ((rc = c.get("g32", ND, &g32))) return rc;
+    }
+  }
+  stage_begin(c, kContraction);
+  const bool exact = (flags & GAPX_MLP_EXACT) != 0;
+  if ((rc = mlp_eval_into_gap(mlp, c.stream, exact, c.N, c.Np, xin, grad, Dp, m.d_theta_inv, eatom, g32, g64)))
+    return rc;
+  stage_end(c, kContraction);
+  return enqueue_forces(c, b, fp64, grad, eatom, g64, g32, tu.waves_bwd);
 }
 
 // One evaluation.  With graphs enabled (and timing off) the enqueued sequence
 // is captured once per (flags, atom count, list capacities) and replayed: an
 // eager run first brings every workspace buffer to its final size, so the
 // capture itself allocates nothing.
-static int ctx_run(Ctx &c, int flags) {
+static int ctx_run(Ctx &c, int flags, Mlp *mlp = nullptr) {
   if (c.N <= 0) return fail(GAPX_ERR_ARG, "no structures loaded");
   GAPX_HIP(hipSetDevice(c.device));
   int rc;
-  if (c.use_graph && !c.timing) {
+  c.mlp = mlp;
+  if (c.mlp) {  // surrogate runs are eager
+    if ((rc = enqueue_surrogate(c, *c.mlp, flags))) return rc;
+  } else if (c.use_graph && !c.timing) {
     const bool valid = c.gexec && c.graph_flags == flags && c.graph_kcap == c.kcap && c.graph_tcap == c.tcap &&
                        c.graph_N == c.N;
     if (!valid) {
@@ -779,7 +848,7 @@ static int ctx_settle(Ctx &c) {
     if (h[2]) c.kcap = round_up(h[1] + 8, 8);
     if (h[6]) c.tcap = round_up(h[5] + 64, 64);
     c.invalidate_graph();
-    int rc = ctx_run(c, c.last_flags);
+    int rc = ctx_run(c, c.last_flags, c.mlp);
     if (rc) return rc;
     GAPX_HIP(hipStreamSynchronize(c.stream));
     collect_timing(c);
@@ -1357,6 +1426,25 @@ int gapx_ctx_stage_info(gapx_ctx *c, int k, const char **name, double *ms, int *
   if (ms) *ms = c->stage_ms[k];
   if (launches) *launches = c->stage_launches[k];
   return GAPX_OK;
+}
+
+int gapx_ctx_run_surrogate(gapx_ctx *c, gapx_mlp *m, int flags) {
+  if (!c || !m) return fail(GAPX_ERR_ARG, "null argument");
+  collect_timing(*c);
+  return ctx_run(*c, flags & ~GAPX_LAYOUT_F, mlp_of(m));
+}
+
+int gapx_ctx_export_labels(gapx_ctx *c, gapx_mlp *m) {
+  if (!c || !m) return fail(GAPX_ERR_ARG, "null argument");
+  if (!c->ran || c->mlp || !(c->last_flags & GAPX_GRAD))
+    return fail(GAPX_ERR_ARG, "gapx_ctx_export_labels: needs a GAP run with GAPX_GRAD first");
+  GAPX_HIP(hipSetDevice(c->device));
+  int rc;
+  if ((rc = ctx_settle(*c))) return rc;
+  const bool fp64 = !(c->last_flags & GAPX_MIXED);
+  return mlp_add_labels(*mlp_of(m), c->stream, c->N, fp64 ? nullptr : c->ptr<float>("xt32"),
+                        fp64 ? c->ptr<double>("xt64") : nullptr, fp64 ? nullptr : c->ptr<float>("g32"),
+                        fp64 ? c->ptr<double>("g64") : nullptr, c->ptr<double>("eatom"), c->m->d_theta_inv, c->m->Dp);
 }
 
 int gapx_ctx_autotune(gapx_ctx *c, int flags, int force) {

@@ -7,6 +7,8 @@
 #include <string>
 #include <vector>
 
+struct gapx_mlp;
+
 namespace gapx {
 
 constexpr double kPi = 3.141592654;       // truncated pi of wacsf.f90:31
@@ -197,5 +199,18 @@ hipError_t launch_gpr_fp64(hipStream_t s, const GprArgs &a);
 hipError_t launch_gpr_mixed(hipStream_t s, const GprArgs &a);
 int gpr_points_per_chunk(bool fp64, int Dp); // sparse points per LDS chunk
 hipError_t launch_gpr_finish(hipStream_t s, const GprArgs &a, bool fp64);
+
+// MLP surrogate (mlp.hip), as used by the context.
+struct Mlp;
+Mlp *mlp_of(gapx_mlp *h);
+// Surrogate energies (eatom, FP64) and, with grad, the descriptor-space
+// gradient g = dE/dx (g32 or g64) for rows of standardised features xt
+// (rows a multiple of 64, rows >= n zero).
+int mlp_eval_into_gap(Mlp &m, hipStream_t s, bool split, int n, int rows, const float *xt, bool grad, int Dp,
+                      const double *theta_inv, double *eatom, float *g32, double *g64);
+// Append (xt, eatom, g / theta_inv) of n atoms to the surrogate's training set.
+int mlp_add_labels(Mlp &m, hipStream_t s, int n, const float *xt32, const double *xt64, const float *g32,
+                   const double *g64, const double *eatom, const double *theta_inv, int Dp);
+hipError_t launch_f64_to_f32(hipStream_t s, size_t n, const double *a, float *b);
 
 }  // namespace gapx

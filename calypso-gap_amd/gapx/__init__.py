@@ -39,6 +39,9 @@ EXPORTS = [
     "gapx_ctx_timing", "gapx_ctx_stage_count", "gapx_ctx_stage_info", "gapx_ctx_use_graph",
     "gapx_ctx_device_bytes", "gapx_ctx_free", "gapx_ctx_counts", "gapx_min_distance",
     "gapx_ctx_autotune", "gapx_ctx_get_tuning", "gapx_ctx_set_tuning",
+    "gapx_mlp_param_count", "gapx_mlp_create", "gapx_mlp_free", "gapx_mlp_get_params", "gapx_mlp_set_params",
+    "gapx_mlp_forward", "gapx_mlp_loss_grad", "gapx_mlp_data_add", "gapx_mlp_data_clear", "gapx_mlp_data_count",
+    "gapx_mlp_train", "gapx_ctx_export_labels", "gapx_ctx_run_surrogate",
 ]
 
 
@@ -79,6 +82,22 @@ def lib():
         L.gapx_ctx_features.argtypes = [vp, vp, vp]
         L.gapx_ctx_timing.argtypes = [vp, ci]
         L.gapx_ctx_autotune.argtypes = [vp, ci, ci]
+        cu = ctypes.c_ulonglong
+        cf = ctypes.c_float
+        L.gapx_mlp_param_count.argtypes = [vp]
+        L.gapx_mlp_create.argtypes = [cu, vp]
+        L.gapx_mlp_free.argtypes = [vp]
+        L.gapx_mlp_free.restype = None
+        L.gapx_mlp_get_params.argtypes = [vp, vp]
+        L.gapx_mlp_set_params.argtypes = [vp, vp]
+        L.gapx_mlp_forward.argtypes = [vp, ci, vp, ci, vp, vp]
+        L.gapx_mlp_loss_grad.argtypes = [vp, ci, vp, vp, vp, cf, cf, ci, vp, vp]
+        L.gapx_mlp_data_add.argtypes = [vp, ci, vp, vp, vp]
+        L.gapx_mlp_data_clear.argtypes = [vp]
+        L.gapx_mlp_data_count.argtypes = [vp, vp]
+        L.gapx_mlp_train.argtypes = [vp, ci, ci, cf, cf, cf, cu, ci, vp]
+        L.gapx_ctx_export_labels.argtypes = [vp, vp]
+        L.gapx_ctx_run_surrogate.argtypes = [vp, vp, ci]
         L.gapx_ctx_get_tuning.argtypes = [vp, ci, vp, vp, vp, vp]
         L.gapx_ctx_set_tuning.argtypes = [vp, ci, ci, ci, ci, ci]
         L.gapx_ctx_stage_count.argtypes = [vp, vp]
@@ -296,6 +315,14 @@ class Context:
         _check(lib().gapx_ctx_get_tuning(self._h, int(flags), *[ctypes.byref(x) for x in v]))
         return dict(zip(("waves_fwd", "waves_bwd", "gpr_rows", "gpr_blocks"), (x.value for x in v)))
 
+    def export_labels(self, mlp):
+        """Append the last GAP run's (xt, per-atom E, dE/dxt) to mlp's training set."""
+        _check(lib().gapx_ctx_export_labels(self._h, mlp._h))
+
+    def run_surrogate(self, mlp, flags=GRAD):
+        """Evaluate the loaded batch with the MLP in place of the GAP contraction."""
+        _check(lib().gapx_ctx_run_surrogate(self._h, mlp._h, int(flags)))
+
     def set_tuning(self, flags=GRAD, waves_fwd=1, waves_bwd=1, gpr_rows=64, gpr_blocks=512):
         _check(lib().gapx_ctx_set_tuning(self._h, int(flags), int(waves_fwd), int(waves_bwd), int(gpr_rows),
                                          int(gpr_blocks)))
@@ -315,6 +342,83 @@ class Context:
     def close(self):
         if self._h:
             lib().gapx_ctx_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+MLP_SPLIT = 1
+MLP_EXACT = 32   # run flag: hi + lo products in the surrogate path
+
+
+class MLP:
+    """Per-atom surrogate 128 -> 128 -> 128 -> 1 (SiLU) on the GPU."""
+
+    def __init__(self, seed=0):
+        h = ctypes.c_void_p()
+        _check(lib().gapx_mlp_create(ctypes.c_ulonglong(seed), ctypes.byref(h)))
+        self._h = h
+        n = ctypes.c_int()
+        _check(lib().gapx_mlp_param_count(ctypes.byref(n)))
+        self.n_params = n.value
+
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        _check(lib().gapx_mlp_get_params(self._h, _p(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        _check(lib().gapx_mlp_set_params(self._h, _p(p)))
+
+    def forward(self, xt, grad=True, split=False):
+        xt = np.ascontiguousarray(xt, np.float32)
+        n = len(xt)
+        e = np.zeros(n, np.float32)
+        G = np.zeros((n, 128), np.float32) if grad else None
+        _check(lib().gapx_mlp_forward(self._h, n, _p(xt), MLP_SPLIT if split else 0, _p(e), _p(G)))
+        return (e, G) if grad else e
+
+    def loss_grad(self, xt, et, Gt, cE=1.0, cG=1.0, split=False):
+        xt = np.ascontiguousarray(xt, np.float32)
+        et = np.ascontiguousarray(et, np.float32)
+        Gt = np.ascontiguousarray(Gt, np.float32)
+        sums = np.zeros(2)
+        g = np.zeros(self.n_params, np.float32)
+        _check(lib().gapx_mlp_loss_grad(self._h, len(xt), _p(xt), _p(et), _p(Gt), float(cE), float(cG),
+                                        MLP_SPLIT if split else 0, _p(sums), _p(g)))
+        n = len(xt)
+        return 0.5 * cE / n * sums[0] + 0.5 * cG / n * sums[1], g
+
+    def add_data(self, xt, et, Gt):
+        xt = np.ascontiguousarray(xt, np.float32)
+        _check(lib().gapx_mlp_data_add(self._h, len(xt), _p(xt), _p(np.ascontiguousarray(et, np.float32)),
+                                       _p(np.ascontiguousarray(Gt, np.float32))))
+
+    def clear_data(self):
+        _check(lib().gapx_mlp_data_clear(self._h))
+
+    def data_count(self):
+        n = ctypes.c_longlong()
+        _check(lib().gapx_mlp_data_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def train(self, steps, batch=4096, lr=1e-3, cE=1.0, cG=1.0, seed=0, split=False):
+        """Adam steps on the device-resident training set; returns the last
+        batch's (mean squared energy error, mean squared |G| error)."""
+        sums = np.zeros(2)
+        _check(lib().gapx_mlp_train(self._h, int(steps), int(batch), float(lr), float(cE), float(cG),
+                                    ctypes.c_ulonglong(seed), MLP_SPLIT if split else 0, _p(sums)))
+        return sums[0] / batch, sums[1] / batch
+
+    def close(self):
+        if self._h:
+            lib().gapx_mlp_free(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

@@ -275,3 +275,85 @@ def population_model(descriptor_fn, M=None):
     xs = [descriptor_fn(header, s["species"], s["lat"], s["pos"], p["rc"]) for s in pool]
     rng = np.random.default_rng(p["seed"] + 100000)
     return make_model(p["species"], p["species"], sf, np.concatenate(xs), M or p["M"], p["sigma"], rng)
+
+
+# Surrogate form: many small ternary structures drawn like the model's own
+# descriptor pool (random sequential insertion with a 1.8 A exclusion at
+# 0.07 atoms/A^3).  rsa_structure draws 4 n candidates at once, finds every
+# close pair among them with one vectorised minimum-image test (exact for a
+# threshold below half the smallest interplanar spacing, which holds for the
+# cells used here), and accepts candidates in draw order unless they clash
+# with one accepted before: sequential insertion, about 5x faster than
+# random_positions.  Structure k of a set uses seed (seed, k), so a set is the
+# same for any number of worker processes.
+SURROGATE = dict(n_train=200000, nmin=16, nmax=128, density=0.07, species=(1, 8, 22), seed=1005,
+                 candidates=10000, top_fraction=0.05, min_dist=1.8)
+
+
+def rsa_structure(n, volume, species, rng, min_dist=MIN_DIST):
+    lat = triclinic_cell(volume, rng)
+    sp = np.asarray(species, np.int32)
+    inv = np.linalg.inv(lat)
+    half_spacing = 0.5 / np.linalg.norm(inv, axis=0).max()
+    if min_dist >= half_spacing:
+        return dict(species=rng.choice(sp, size=n).astype(np.int32), lat=lat,
+                    pos=random_positions(lat, n, rng, min_dist=min_dist))
+    md2 = min_dist * min_dist
+    frac = np.zeros((0, 3))
+    while len(frac) < n:
+        cand = rng.random((4 * n, 3))
+        allf = np.concatenate([frac, cand])
+        df = cand[:, None, :] - allf[None, :, :]
+        df -= np.round(df)
+        d2 = ((df @ lat) ** 2).sum(-1)
+        clash = d2 < md2                       # [candidate, accepted + candidate]
+        ok_old = ~clash[:, :len(frac)].any(axis=1)
+        cc = clash[:, len(frac):]
+        blocked = np.zeros(len(cand), bool)
+        take = []
+        for j in np.flatnonzero(ok_old):
+            if blocked[j]:
+                continue
+            take.append(j)
+            blocked |= cc[j]
+            if len(frac) + len(take) == n:
+                break
+        frac = np.concatenate([frac, cand[take]])
+    return dict(species=rng.choice(sp, size=n).astype(np.int32), lat=lat, pos=frac[:n] @ lat)
+
+
+def _rsa_one(args):
+    k, seed, nmin, nmax, density, species, min_dist = args
+    rng = np.random.default_rng([seed, k])
+    n = int(rng.integers(nmin, nmax + 1))
+    return rsa_structure(n, n / density, species, rng, min_dist=min_dist)
+
+
+def random_ternary(n_struct, nmin=16, nmax=128, density=0.07, species=(1, 8, 22), seed=1005, min_dist=MIN_DIST,
+                   workers=1, start=0):
+    """Structures start .. start + n_struct - 1 of the seeded set."""
+    args = [(k, seed, nmin, nmax, density, tuple(species), min_dist) for k in range(start, start + n_struct)]
+    if workers > 1 and n_struct >= 4 * workers:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(workers) as pool:
+            return pool.map(_rsa_one, args, chunksize=max(1, n_struct // (8 * workers)))
+    return [_rsa_one(a) for a in args]
+
+
+def jittered_structures(n_struct, nmin, nmax, density, species, seed, jitter=0.2):
+    """Jittered lattice sites (no rejection; about 0.1 ms per structure):
+    with jitter 0.2 of the site spacing the closest pair is at least 0.6
+    spacings.  Fast, but its environments are far from a random-insertion
+    pool's."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for n in rng.integers(nmin, nmax + 1, size=n_struct):
+        n = int(n)
+        m = int(math.ceil(n ** (1.0 / 3.0) - 1e-9))
+        lat = triclinic_cell(n / density, rng)
+        idx = rng.choice(m ** 3, size=n, replace=False)
+        g = np.stack(np.unravel_index(idx, (m, m, m)), axis=1).astype(float)
+        frac = (g + 0.5 + rng.uniform(-jitter, jitter, size=(n, 3))) / m
+        out.append(dict(species=rng.choice(np.asarray(species, np.int32), size=n).astype(np.int32), lat=lat,
+                        pos=frac @ lat))
+    return out

@@ -46,6 +46,7 @@ extern "C" {
 
 typedef struct gapx_model gapx_model;
 typedef struct gapx_ctx gapx_ctx;
+typedef struct gapx_mlp gapx_mlp;
 
 int gapx_version(void);
 const char *gapx_last_error(void);
@@ -155,6 +156,47 @@ int gapx_ctx_counts(gapx_ctx *c, long long *pairs, long long *triplets);
 /* Bytes of device memory held by the context. */
 int gapx_ctx_device_bytes(gapx_ctx *c, long long *bytes);
 void gapx_ctx_free(gapx_ctx *c);
+
+/* ---------------------------------------------------------------------------
+ * Per-atom MLP surrogate (no reference counterpart): 128 -> 128 -> 128 -> 1,
+ * SiLU, on the standardised descriptors xt = x theta_inv - mu of a model with
+ * 128 descriptor columns.  FP32 weights, BF16 MFMA products (FP32
+ * accumulation); GAPX_MLP_SPLIT uses hi + lo BF16 operands (FP32-level
+ * products).  Trained on GAP per-atom energies e_i and per-atom gradients
+ * G_i = dE_i/dxt_i, which fix the forces through the descriptor Jacobian:
+ *   L = cE/(2n) sum_i (e_i - et_i)^2 + cG/(2n) sum_i |G_i - Gt_i|^2.
+ * A handle is not thread-safe. */
+#define GAPX_MLP_SPLIT 1
+/* run flag of gapx_ctx_run_surrogate: hi + lo products in the MLP */
+#define GAPX_MLP_EXACT 32
+int gapx_mlp_param_count(int *n);
+int gapx_mlp_create(unsigned long long seed, gapx_mlp **out);
+void gapx_mlp_free(gapx_mlp *m);
+/* Flat FP32 parameters: W1[128][128] (row = input), b1[128], W2[128][128],
+ * b2[128], w3[128], b3.  Setting them resets the optimiser state. */
+int gapx_mlp_get_params(gapx_mlp *m, float *params);
+int gapx_mlp_set_params(gapx_mlp *m, const float *params);
+/* e[n] and, if G is not NULL, G[n][128] = de/dxt for xt[n][128] (host). */
+int gapx_mlp_forward(gapx_mlp *m, int n, const float *xt, int flags, float *e, float *G);
+/* Loss sums (sum (e - et)^2, sum |G - Gt|^2) and dL/dparams for one batch. */
+int gapx_mlp_loss_grad(gapx_mlp *m, int n, const float *xt, const float *e_t, const float *G_t, float cE, float cG,
+                       int flags, double *sums, float *grad);
+/* Device-resident training set: append rows (host arrays), clear, count. */
+int gapx_mlp_data_add(gapx_mlp *m, int n, const float *xt, const float *e_t, const float *G_t);
+int gapx_mlp_data_clear(gapx_mlp *m);
+int gapx_mlp_data_count(gapx_mlp *m, long long *n);
+/* `steps` Adam steps on uniformly sampled batches of `batch` rows; sums gets
+ * the last batch's two sums of squares. */
+int gapx_mlp_train(gapx_mlp *m, int steps, int batch, float lr, float cE, float cG, unsigned long long seed,
+                   int flags, double *sums);
+/* Append the context's last GAP run (GAPX_GRAD) to the training set: xt, the
+ * per-atom energies and dE_i/dxt_i, device to device. */
+int gapx_ctx_export_labels(gapx_ctx *c, gapx_mlp *m);
+/* Evaluate the loaded batch with the surrogate in place of the GAP
+ * contraction: neighbours, descriptors (FP64, or FP32 with GAPX_MIXED), MLP
+ * energies and, with GAPX_GRAD, forces and stress through the descriptor
+ * backward pass.  Results through gapx_ctx_results. */
+int gapx_ctx_run_surrogate(gapx_ctx *c, gapx_mlp *m, int flags);
 
 #ifdef __cplusplus
 }
