@@ -42,13 +42,27 @@ def gpu_descriptor_fn(header, species, lat, pos, rc):
     return x
 
 
-# Algorithmic work per stage (documented in DESIGN.md).  Descriptor flops are
-# counted per (pair, radial line) and (triplet, angular line) from the
-# arithmetic each term needs, a transcendental counted as one flop.
-FLOPS_RAD_FWD, FLOPS_RAD_BWD = 8, 12          # per pair and radial line
-FLOPS_ANG_FWD, FLOPS_ANG_BWD = 10, 17         # per triplet and angular line
-FLOPS_TRIP_FWD, FLOPS_TRIP_BWD = 30, 60       # per triplet (geometry, cutoff products)
-BYTES_NEIGHBOR = 4 + 12 + 32 + 4              # per pair: index, image, (dx,dy,dz,r), reverse slot
+# Algorithmic work per unit, following the per-unit figures of the reference
+# path's cost model: a (pair, radial line) costs about 30 flops and a
+# (triplet, angular line) about 80 flops for forward plus backward (geometry,
+# exponential, cutoff products and the three distance derivatives, counted per
+# line as the reference computes them); split 1/3 forward, 2/3 backward.  The
+# contraction is 4 D M flops per atom (S = xt mt^T and G = W mt); the other
+# kernels are byte-bound and counted in bytes.
+FLOPS_RAD_FWD, FLOPS_RAD_BWD = 10, 20          # per (pair, radial line)
+FLOPS_ANG_FWD, FLOPS_ANG_BWD = 27, 53          # per (triplet, angular line)
+BYTES_NEIGHBOR = 4 + 8 + 32 + 4                # per pair: index, key, (dx,dy,dz,r), reverse slot
+SINGLE_KERNEL_STAGES = ("descriptors_fwd", "gpr_contraction", "gpr_finish", "descriptors_bwd")
+STAGE_KERNEL = {  # the one kernel of each single-kernel stage, as rocprofv3 names it
+    ("descriptors_fwd", True): "k_desc_fwd_grid<float, 4, 1>",
+    ("descriptors_fwd", False): "k_desc_fwd_grid<double, 4, 1>",
+    ("descriptors_bwd", True): "k_desc_bwd<float, 1, 4>",
+    ("descriptors_bwd", False): "k_desc_bwd<double, 1, 4>",
+    ("gpr_contraction", True): "k_gpr_mixed<false, 128, 64, 2>",
+    ("gpr_contraction", False): "k_gpr_f64<128>",
+    ("gpr_finish", True): "k_gpr_finish",
+    ("gpr_finish", False): "k_gpr_finish",
+}
 
 
 def stage_rates(st, N, D, M, pairs, triplets, m, mixed):
@@ -59,12 +73,12 @@ def stage_rates(st, N, D, M, pairs, triplets, m, mixed):
     mpeak = PEAK["fp16"] if mixed else PEAK["fp64"]
     work = {
         "neighbor_list": ("hbm", pairs * BYTES_NEIGHBOR * 2 + N * 64, 8000.0, "GB/s", 1e9),
-        "descriptors_fwd": ("valu", pairs * n_rad * FLOPS_RAD_FWD + triplets * (n_ang * FLOPS_ANG_FWD + FLOPS_TRIP_FWD),
-                            vpeak, "TFLOP/s", 1e12),
+        "descriptors_fwd": ("valu", pairs * n_rad * FLOPS_RAD_FWD + triplets * n_ang * FLOPS_ANG_FWD, vpeak,
+                            "TFLOP/s", 1e12),
         "gpr_contraction": ("mfma", 4.0 * D * M * N, mpeak, "TFLOP/s", 1e12),
         "gpr_finish": ("hbm", N * D * 8 * 3.0, 8000.0, "GB/s", 1e9),
-        "descriptors_bwd": ("valu", pairs * n_rad * FLOPS_RAD_BWD + triplets * (n_ang * FLOPS_ANG_BWD + FLOPS_TRIP_BWD),
-                            vpeak, "TFLOP/s", 1e12),
+        "descriptors_bwd": ("valu", pairs * n_rad * FLOPS_RAD_BWD + triplets * n_ang * FLOPS_ANG_BWD, vpeak,
+                            "TFLOP/s", 1e12),
         "forces_stress": ("hbm", pairs * 24 * 2 + N * 80, 8000.0, "GB/s", 1e9),
     }
     out = {}
@@ -72,8 +86,26 @@ def stage_rates(st, N, D, M, pairs, triplets, m, mixed):
         bound, amount, peak, unit, scale = work[k]
         ach = amount / (ms * 1e-3) / scale if ms > 0 else 0.0
         out[k] = dict(bound=bound, achieved=round(ach, 3), peak=peak, unit=unit, frac=round(ach / peak, 4),
-                      traffic=None, work=float(amount))
+                      work=float(amount))
     return out
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (profiles/traffic.json, two rocprofv3 passes scaled by a copy of known
+    size), or None when that summary does not cover this workload/kernel."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload:
+        return None
+    for v in t["kernels"].values():
+        if v["kernel"] == kernel:
+            return v["traffic_bytes"]
+    return None
 
 
 def time_runs(ctx, flags, steps, warmup):
@@ -131,6 +163,82 @@ def population_bench(steps, warmup):
                 mixed_error_vs_fp64_max={k: float("%.4g" % v) for k, v in err.items()})
 
 
+def surrogate_bench(m, n_train, n_cand, steps, batch=4096):
+    """Surrogate form: label random ternary structures with the GPU GAP (mixed
+    precision), train the MLP on the GPU, then pre-screen candidates with it
+    and judge its top 5% with the GAP."""
+    w = synth.SURROGATE
+    workers = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1))
+    t0 = time.perf_counter()
+    train = synth.random_ternary(n_train, w["nmin"], w["nmax"], w["density"], w["species"], w["seed"],
+                                 workers=workers)
+    t_gen = time.perf_counter() - t0
+    net = gapx.MLP(seed=0)
+    ctx = gapx.Context(m)
+    t0 = time.perf_counter()
+    s_e = s_e2 = s_g2 = 0.0
+    rows = 0
+    theta = m.arrays()[0]
+    for k in range(0, len(train), 2000):
+        ctx.load(train[k:k + 2000], 6.0)
+        ctx.run(gapx.GRAD | gapx.MIXED)
+        ctx.export_labels(net)
+        e = ctx.results()["eatom"]
+        g = ctx.features(dedx=True)[1] * theta
+        s_e += e.sum()
+        s_e2 += (e * e).sum()
+        s_g2 += (g * g).sum()
+        rows += len(e)
+    t_label = time.perf_counter() - t0
+    var_e = s_e2 / rows - (s_e / rows) ** 2
+    cE, cG = 1.0 / var_e, rows / s_g2
+    net.train(1, batch=batch, lr=0.0, cE=cE, cG=cG)  # allocate the work buffers
+    t0 = time.perf_counter()
+    last = None
+    for k in range(0, steps, 500):
+        last = net.train(min(500, steps - k), batch=batch, lr=1e-3, cE=cE, cG=cG, seed=k)
+    t_train = time.perf_counter() - t0
+    # per row: 2 forward + 2 input-gradient + 3 second-order + 4 weight-gradient products of 128 x 128
+    train_flops = 11 * 2 * 128 * 128 * batch * steps
+    cands = synth.random_ternary(n_cand, w["nmin"], w["nmax"], w["density"], w["species"], w["seed"] + 1,
+                                 workers=workers)
+    n = np.array([len(c["species"]) for c in cands])
+    ctx.load(cands, 6.0)
+
+    def timed(fn, reps=3):
+        fn()
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.sync()
+        return (time.perf_counter() - t) / reps
+
+    sec_sur = timed(lambda: ctx.run_surrogate(net, gapx.MIXED))
+    e_sur = ctx.results()["energy"] / n
+    sec_gap = timed(lambda: ctx.run(gapx.MIXED))
+    e_gap = ctx.results()["energy"] / n
+    sec_gap64 = timed(lambda: ctx.run(gapx.GRAD), reps=1)
+    ctx.close()
+    net.close()
+    k = max(1, int(w["top_fraction"] * len(cands)))
+    pick = np.argsort(e_sur)[:k]
+    recall = len(set(pick) & set(np.argsort(e_gap)[:k])) / k
+    rank = lambda v: np.argsort(np.argsort(v))
+    return dict(
+        train_structures=n_train, train_atoms=int(rows), generate_s=round(t_gen, 2), label_s=round(t_label, 2),
+        train_steps=steps, batch=batch, train_s=round(t_train, 3),
+        train_rows_per_s=round(steps * batch / t_train, 1), train_tflops=round(train_flops / t_train / 1e12, 2),
+        last_batch_mse_E=float("%.4g" % last[0]), label_var_E=float("%.4g" % var_e),
+        candidates=n_cand, candidate_atoms=int(n.sum()),
+        surrogate_cands_per_s=round(n_cand / sec_sur, 1), gap_mixed_energy_cands_per_s=round(n_cand / sec_gap, 1),
+        gap_fp64_efs_cands_per_s=round(n_cand / sec_gap64, 1),
+        speedup_vs_gap_mixed_energy=round(sec_gap / sec_sur, 2), speedup_vs_gap_fp64_efs=round(sec_gap64 / sec_sur, 2),
+        judged=k, recall_of_gap_top=round(recall, 3),
+        spearman=round(float(np.corrcoef(rank(e_gap), rank(e_sur))[0, 1]), 3),
+        judged_mean_abs_err_eV_per_atom=float("%.4g" % np.abs(e_sur[pick] - e_gap[pick]).mean()))
+
+
 def cpu_baseline(model, s, rc, seconds):
     """The CPU oracle (sparse FP64 path, OpenMP over centre atoms) on a
     bounded sample: the first n centre atoms of the same structure."""
@@ -154,6 +262,43 @@ def cpu_baseline(model, s, rc, seconds):
                 sample="first %d of %d centre atoms of the same structure, %.1f s" % (n, len(s["species"]), t))
 
 
+def device_copy(nbytes):
+    """One device-to-device copy of nbytes through the HIP runtime (the
+    runtime's plain copy kernel), for calibrating HBM byte counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:  # 3: device to device
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+
+
+def probe(args):
+    """Copy of known size, then evaluations (for rocprofv3 counter passes)."""
+    device_copy(1 << 30)
+    print("probe: copied %d bytes" % (1 << 30), flush=True)
+    model = synth.workload_model(args.workload, gpu_descriptor_fn)
+    s = synth.workload_structure(args.workload)
+    m = gapx.Model.from_model(model)
+    ctx = gapx.Context(m)
+    ctx.load([s], synth.WORKLOADS[args.workload]["rc"])
+    for flags in (gapx.GRAD, gapx.GRAD | gapx.MIXED):
+        for _ in range(args.probe):
+            ctx.run(flags)
+        ctx.results()
+    ctx.close()
+    m.close()
+    print("probe: done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +308,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-population", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--surrogate-train", type=int, default=20000,
+                    help="training structures (the surrogate form names 200000)")
+    ap.add_argument("--surrogate-steps", type=int, default=3000)
+    ap.add_argument("--probe", type=int, default=0,
+                    help="profiling probe: a 1 GiB device copy (calibrates HBM byte counters), then this many "
+                         "FP64 and mixed evaluations of the workload, no timing, no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this evaluator runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -173,6 +325,8 @@ def main():
 
     w = synth.WORKLOADS[args.workload]
     rc = w["rc"]
+    if args.probe:
+        return probe(args)
     t0 = time.perf_counter()
     model = synth.workload_model(args.workload, gpu_descriptor_fn)
     s = synth.workload_structure(args.workload)
@@ -205,12 +359,22 @@ def main():
     value = N / sec
     st = runs["mixed"]["stages"]
     rates = stage_rates(st, N, D, M, pairs, triplets, m, mixed=True)
-    dom = max(st, key=lambda k: st[k])
-    roof = dict(rates[dom])
-    roof["kernel"] = dom
+    # the dominant kernel: the longest of the stages that are one kernel each
+    dom = max(SINGLE_KERNEL_STAGES, key=lambda k: st[k])
+    kname = STAGE_KERNEL[(dom, True)]
+    roof = {kk: rates[dom][kk] for kk in ("bound", "achieved", "peak", "unit", "frac")}
+    roof["traffic"] = measured_traffic(args.workload, kname)
+    roof["kernel"] = kname
+    roof["stage"] = dom
+    roof["ms_per_launch"] = round(st[dom], 4)
+    roof["work_per_launch"] = rates[dom]["work"]
     roof["stages"] = {k: dict(ms=round(st[k], 4), **{kk: vv for kk, vv in rates[k].items()
-                                                     if kk in ("achieved", "frac", "bound")}) for k in st}
+                                                     if kk in ("achieved", "frac", "bound", "unit")}) for k in st}
     ctx.close()
+    surr = None
+    if not args.no_surrogate and args.workload == "ternary4k":
+        surr = surrogate_bench(m, args.surrogate_train, synth.SURROGATE["candidates"], args.surrogate_steps)
+        print("bench: surrogate %s" % surr, file=sys.stderr, flush=True)
     m.close()
     popr = None
     if not args.no_population:
@@ -240,6 +404,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "population": popr,
+        "surrogate": surr,
     }
     print(json.dumps(out), flush=True)
 

@@ -140,17 +140,14 @@ struct Lds {
 // kernels apply each line's cutoff to every distance.
 __global__ __launch_bounds__(64) void k_triplets(int N, NeighborBufs nb, double rcA, TripletBufs tb) {
   extern __shared__ __align__(16) unsigned char smem[];
-  float *s_d = (float *)smem;  // [4 * kcap]
+  float4 *s_p = (float4 *)smem;  // [kcap]: x y z r, one 16-byte record per neighbour
   const int i = blockIdx.x, lane = threadIdx.x;
   if (i >= N) return;
   const int K = nb.nb_count[i];
   const size_t start = (size_t)i * nb.kcap;
   for (int a = lane; a < K; a += 64) {
     const double *dd = nb.nb_d + 4 * (start + a);
-    s_d[4 * a] = (float)dd[0];
-    s_d[4 * a + 1] = (float)dd[1];
-    s_d[4 * a + 2] = (float)dd[2];
-    s_d[4 * a + 3] = (float)dd[3];
+    s_p[a] = make_float4((float)dd[0], (float)dd[1], (float)dd[2], (float)dd[3]);
   }
   __syncthreads();
   const float rc = (float)rcA * (1.0f + 1e-6f);
@@ -166,10 +163,10 @@ __global__ __launch_bounds__(64) void k_triplets(int N, NeighborBufs nb, double 
       const bool first = c < n1;
       const int a = first ? c : c - n1, b = a + (first ? d1 : d2);
       bool ok = false;
-      if (c < n && s_d[4 * a + 3] <= rc && s_d[4 * b + 3] <= rc) {
-        const float ex = s_d[4 * b] - s_d[4 * a], ey = s_d[4 * b + 1] - s_d[4 * a + 1],
-                    ez = s_d[4 * b + 2] - s_d[4 * a + 2];
-        ok = ex * ex + ey * ey + ez * ez <= rc2;
+      if (c < n) {
+        const float4 pa = s_p[a], pb = s_p[b];
+        const float ex = pb.x - pa.x, ey = pb.y - pa.y, ez = pb.z - pa.z;
+        ok = pa.w <= rc && pb.w <= rc && ex * ex + ey * ey + ez * ez <= rc2;
       }
       const unsigned long long m = __ballot(ok);
       const int pos = count + __popcll(m & ((1ull << lane) - 1ull));

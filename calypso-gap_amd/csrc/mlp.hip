@@ -487,7 +487,8 @@ int enqueue_forward(Mlp &m, hipStream_t s, bool split, int n, int rows, const fl
 
 // Reduce the rows-over partial sums of a TN product into grad[off ...].
 int tn_product(Mlp &m, bool split, int rows, const float *A, const float *B, float *dst, bool accumulate) {
-  const int rpb = 1024;
+  // rows per workgroup: enough workgroups to cover the GPU, at most 64 partials
+  const int rpb = std::max(64, (rows + 63) / 64 / 32 * 32);
   const int nb = (rows + rpb - 1) / rpb;
   float *part;
   int rc;
@@ -503,7 +504,7 @@ int tn_product(Mlp &m, bool split, int rows, const float *A, const float *B, flo
 }
 
 int colsum(Mlp &m, int rows, const float *M, float *dst) {
-  const int rpb = 512;
+  const int rpb = 64;
   const int nb = (rows + rpb - 1) / rpb;
   float *part;
   int rc;

@@ -264,13 +264,14 @@ __global__ __launch_bounds__(64) void k_reverse(int N, NeighborBufs nb) {
   }
 }
 
-// Fold the per-atom counts into the run's statistics (one workgroup).
+// Fold the per-atom counts into the run's statistics (one workgroup):
+// per-thread values, wavefront shuffles, then the 16 wavefront results.
 __global__ __launch_bounds__(1024) void k_list_stats(int N, int kcap, const int *__restrict__ nb_count,
                                                      const int *__restrict__ nb_raw, int tcap,
                                                      const int *__restrict__ t_count, const int *__restrict__ t_raw,
                                                      const int *__restrict__ guard, int *__restrict__ out) {
-  __shared__ long long red[6][1024];
-  const int t = threadIdx.x;
+  __shared__ long long red[16][6];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   long long v[6] = {0, 0, 0, 0, 0, 0};
   for (int a = t; a < N; a += 1024) {
     const int r = nb_raw[a];
@@ -284,29 +285,23 @@ __global__ __launch_bounds__(1024) void k_list_stats(int N, int kcap, const int 
       v[5] += (tr > tcap) ? 1 : 0;
     }
   }
-  for (int k = 0; k < 6; ++k) red[k][t] = v[k];
-  __syncthreads();
-  for (int off = 512; off > 0; off >>= 1) {
-    if (t < off) {
-      red[0][t] += red[0][t + off];
-      red[1][t] = max(red[1][t], red[1][t + off]);
-      red[2][t] += red[2][t + off];
-      red[3][t] += red[3][t + off];
-      red[4][t] = max(red[4][t], red[4][t + off]);
-      red[5][t] += red[5][t + off];
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const long long w = __shfl_xor(v[k], o, 64);
+      v[k] = (k == 1 || k == 4) ? max(v[k], w) : v[k] + w;
     }
-    __syncthreads();
   }
-  if (t == 0) {
-    out[0] = (int)red[0][0];
-    out[1] = (int)red[1][0];
-    out[2] = (int)red[2][0];
-    out[3] = guard[0];
-    out[4] = (int)red[3][0];
-    out[5] = (int)red[4][0];
-    out[6] = (int)red[5][0];
-    out[7] = 0;
+  if (lane == 0)
+    for (int k = 0; k < 6; ++k) red[wv][k] = v[k];
+  __syncthreads();
+  if (t < 6) {
+    long long a = 0;
+    for (int w = 0; w < 16; ++w) a = (t == 1 || t == 4) ? max(a, red[w][t]) : a + red[w][t];
+    out[t < 3 ? t : t + 1] = (int)a;
   }
+  if (t == 6) out[3] = guard[0];
+  if (t == 7) out[7] = 0;
 }
 
 // F_m = sum_s f^(m)_s - sum_s f^(j_s)_{rev(s)}: slot_f[s] holds dE_i/dR of the
@@ -345,34 +340,44 @@ __global__ __launch_bounds__(64) void k_gather(int N, NeighborBufs nb, const dou
   }
 }
 
-// Per-structure energy and stress: deterministic tree sums over the atoms.
-__global__ __launch_bounds__(256) void k_struct_reduce(const StructParams *__restrict__ sp,
-                                                       const double *__restrict__ eatom,
-                                                       const double *__restrict__ vir, int grad,
-                                                       double *__restrict__ ene, double *__restrict__ stress6) {
-  const int s = blockIdx.x, t = threadIdx.x;
+template <typename V>
+__device__ __forceinline__ V wave_reduce_sum(V v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Per-structure energy and stress: per-thread sums, wavefront shuffles, then
+// the 16 wavefront totals in order (deterministic).
+__global__ __launch_bounds__(1024) void k_struct_reduce(const StructParams *__restrict__ sp,
+                                                        const double *__restrict__ eatom,
+                                                        const double *__restrict__ vir, int grad,
+                                                        double *__restrict__ ene, double *__restrict__ stress6) {
+  const int s = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const StructParams &P = sp[s];
-  __shared__ double red[7][256];
+  __shared__ double red[16][7];
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int a = t; a < P.natoms; a += 256) {
-    int i = P.atom_base + a;
+  for (int a = t; a < P.natoms; a += 1024) {
+    const int i = P.atom_base + a;
     acc[0] += eatom[i];
     if (grad)
       for (int c = 0; c < 6; ++c) acc[1 + c] += vir[6 * i + c];
   }
-  for (int c = 0; c < 7; ++c) red[c][t] = acc[c];
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (t < off)
-      for (int c = 0; c < 7; ++c) red[c][t] += red[c][t + off];
-    __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 7; ++c) {
+    const double v = wave_reduce_sum(acc[c]);
+    if (lane == 0) red[wv][c] = v;
   }
-  if (t == 0) {
-    ene[s] = red[0][0];
-    // stress_cal = -virial / V converted to GPa (gap_calc.f90:189-203);
-    // virial order xx yy zz xy yz xz is already libgap's output order.
-    double f = -(1.0 / kGPa) / P.vol;
-    for (int c = 0; c < 6; ++c) stress6[6 * s + c] = grad ? red[1 + c][0] * f : 0.0;
+  __syncthreads();
+  if (t < 7) {
+    double v = 0.0;
+    for (int w = 0; w < 16; ++w) v += red[w][t];
+    if (t == 0) {
+      ene[s] = v;
+    } else {
+      // stress_cal = -virial / V converted to GPa (gap_calc.f90:189-203);
+      // virial order xx yy zz xy yz xz is already libgap's output order.
+      stress6[6 * s + t - 1] = grad ? v * (-(1.0 / kGPa) / P.vol) : 0.0;
+    }
   }
 }
 
@@ -441,7 +446,7 @@ hipError_t launch_gather_forces(hipStream_t s, int N, const NeighborBufs &nb, co
 
 hipError_t launch_struct_reduce(hipStream_t s, int nstruct, const StructParams *sp, const double *eatom,
                                 const double *atom_vir, int grad, double *ene, double *stress6) {
-  k_struct_reduce<<<nstruct, 256, 0, s>>>(sp, eatom, atom_vir, grad, ene, stress6);
+  k_struct_reduce<<<nstruct, 1024, 0, s>>>(sp, eatom, atom_vir, grad, ene, stress6);
   return hipGetLastError();
 }
 
