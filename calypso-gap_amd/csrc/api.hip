@@ -606,8 +606,8 @@ static int gpr_layout(Ctx &c, GprArgs &g, bool fp64) {
 
 // From dE/dx to forces and stress: descriptor backward pass, force gather,
 // per-structure energy and virial, list statistics.
-static int enqueue_forces(Ctx &c, Bufs &b, bool fp64, bool grad, const double *eatom, const double *g64,
-                          const float *g32, int waves_bwd) {
+static int enqueue_forces(Ctx &c, Bufs &b, bool fp64, bool grad, const double *eatom, const GradSrc &gs,
+                          int waves_bwd) {
   Model &m = *c.m;
   NeighborBufs &nb = b.nb;
   const int *d_species = c.ptr<int>("species");
@@ -619,8 +619,8 @@ static int enqueue_forces(Ctx &c, Bufs &b, bool fp64, bool grad, const double *e
   if (grad) {
     if ((rc = c.get("slot_f", 3 * (size_t)c.N * c.kcap, &b.slot_f))) return rc;
     stage_begin(c, kDescBwd);
-    GAPX_HIP(launch_descriptors_bwd(c.stream, fp64, c.N, d_species, m.d_wz, nb, b.tb, m.sfd, m.Dp, g64, g32,
-                                    b.slot_f, b.atom_vir, waves_bwd, c.KP));
+    GAPX_HIP(launch_descriptors_bwd(c.stream, fp64, c.N, d_species, m.d_wz, nb, b.tb, m.sfd, m.Dp, gs, b.slot_f,
+                                    b.atom_vir, waves_bwd, c.KP));
     stage_end(c, kDescBwd);
   }
   stage_begin(c, kForces);
@@ -628,10 +628,11 @@ static int enqueue_forces(Ctx &c, Bufs &b, bool fp64, bool grad, const double *e
     GAPX_HIP(launch_gather_forces(c.stream, c.N, nb, b.slot_f, b.force));
   else
     GAPX_HIP(launch_fill_double(c.stream, b.force, 3 * (size_t)c.N, 0.0));
-  GAPX_HIP(launch_struct_reduce(c.stream, c.nstruct, d_sp, eatom, b.atom_vir, grad ? 1 : 0, b.ene, b.stress));
+  // energies, stress, and the list statistics straight into pinned host
+  // memory (read by ctx_settle)
+  GAPX_HIP(launch_struct_reduce(c.stream, c.nstruct, d_sp, eatom, b.atom_vir, grad ? 1 : 0, b.ene, b.stress, c.N,
+                                nb, b.tb, c.host_stats_dev));
   stage_end(c, kForces);
-  // list statistics straight into pinned host memory (read by ctx_settle)
-  GAPX_HIP(launch_list_stats(c.stream, c.N, nb, b.tb, c.host_stats_dev));
   return GAPX_OK;
 }
 
@@ -720,11 +721,26 @@ static int enqueue_run(Ctx &c, int flags) {
   else
     GAPX_HIP(launch_gpr_mixed(c.stream, g));
   stage_end(c, kContraction);
-  stage_begin(c, kFinish);
-  GAPX_HIP(launch_gpr_finish(c.stream, g, fp64));
-  stage_end(c, kFinish);
-
-  return enqueue_forces(c, b, fp64, grad, g.eatom, g.g64, g.g32, tu.waves_bwd);
+  GradSrc gs;
+  if (grad) {
+    // the finish (sum of split partials, dE/dx) runs inside the backward kernel
+    gs.splits = g.splits;
+    gs.Np = g.Np;
+    gs.Epart = g.Epart;
+    gs.Gpart = fp64 ? g.Gpart : nullptr;
+    gs.Gpart32 = fp64 ? nullptr : g.Gpart32;
+    gs.xt64 = o.xt64;
+    gs.xt32 = o.xt32;
+    gs.theta_inv = m.d_theta_inv;
+    gs.eatom = g.eatom;
+    gs.g64_out = g.g64;
+    gs.g32_out = g.g32;
+  } else {
+    stage_begin(c, kFinish);
+    GAPX_HIP(launch_gpr_finish(c.stream, g, fp64));
+    stage_end(c, kFinish);
+  }
+  return enqueue_forces(c, b, fp64, grad, g.eatom, gs, tu.waves_bwd);
 }
 
 // The surrogate in place of the contraction: same neighbour lists and
@@ -777,7 +793,10 @@ static int enqueue_surrogate(Ctx &c, Mlp &mlp, int flags) {
   if ((rc = mlp_eval_into_gap(mlp, c.stream, exact, c.N, c.Np, xin, grad, Dp, m.d_theta_inv, eatom, g32, g64)))
     return rc;
   stage_end(c, kContraction);
-  return enqueue_forces(c, b, fp64, grad, eatom, g64, g32, tu.waves_bwd);
+  GradSrc gs;
+  gs.g64 = g64;
+  gs.g32 = g32;
+  return enqueue_forces(c, b, fp64, grad, eatom, gs, tu.waves_bwd);
 }
 
 // One evaluation.  With graphs enabled (and timing off) the enqueued sequence
@@ -927,8 +946,9 @@ static int ctx_autotune(Ctx &c, int flags, bool force) {
   std::vector<Knob> knobs;
   knobs.push_back({&t.waves_fwd, {1, 2, 4}, {kDescFwd}});
   if (flags & GAPX_GRAD) knobs.push_back({&t.waves_bwd, {1, 2, 4}, {kDescBwd}});
-  if (p == 1 && c.m->Dp <= 128) knobs.push_back({&t.gpr_rows, {64, 128}, {kContraction, kFinish}});
-  knobs.push_back({&t.gpr_blocks, {256, 512, 1024, 2048}, {kContraction, kFinish}});
+  // the split count also changes the finish, which runs in the backward pass
+  if (p == 1 && c.m->Dp <= 128) knobs.push_back({&t.gpr_rows, {64, 128}, {kContraction, kFinish, kDescBwd}});
+  knobs.push_back({&t.gpr_blocks, {256, 512, 1024, 2048}, {kContraction, kFinish, kDescBwd}});
   for (Knob &k : knobs) {
     if (rc) break;
     int best = *k.field;

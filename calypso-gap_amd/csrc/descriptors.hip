@@ -398,14 +398,40 @@ __device__ __forceinline__ void grid_powers(T cth, const SFDev &sf, T (&Pp)[NZ],
   }
 }
 
-// Sum NV per-lane values over the workgroup: s_out[j] = total of v[j].
+// Sum NV per-lane values over the workgroup: s_out[j] = total of v[j].  Within
+// a wavefront a butterfly reduce-scatter: at offset m (32, 16, .., 1) every
+// lane keeps the half of its live values selected by its own bit m and adds
+// the partner's copy of that half, so lane l ends with the total of value l
+// (NV padded to 32 or 64; for 32 the two half-waves are folded first).  63
+// exchanges instead of NV full wave reductions.  The waves' totals are then
+// added in wave order (deterministic).
 template <typename T, int NV, int W>
 __device__ __forceinline__ void group_sum(T (&v)[NV], int wv, int lane, int tid, T *s_part, T *s_out) {
+  constexpr int NVP = NV <= 32 ? 32 : 64;
+  static_assert(NV <= 64, "at most 64 values");
+  T x[NVP];
 #pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    const T t = wave_sum(v[j]);
-    if (lane == 0) s_part[wv * NV + j] = t;
+  for (int j = 0; j < NVP; ++j) x[j] = (j < NV) ? v[j] : T(0);
+  if (NVP == 32) {
+#pragma unroll
+    for (int j = 0; j < 32; ++j) x[j] += __shfl_xor(x[j], 32, 64);
   }
+#pragma unroll
+  for (int m = NVP / 2; m >= 1; m >>= 1) {
+    const bool upper = (lane & m) != 0;
+#pragma unroll
+    for (int j = 0; j < m; ++j) {
+      const T send = upper ? x[j] : x[j + m];
+      const T keep = upper ? x[j + m] : x[j];
+      x[j] = keep + __shfl_xor(send, m, 64);
+    }
+  }
+  if (W == 1) {
+    if (lane < NV) s_out[lane] = x[0];
+    __syncthreads();
+    return;
+  }
+  if (lane < NV) s_part[wv * NV + lane] = x[0];
   __syncthreads();
   for (int j = tid; j < NV; j += 64 * W) {
     T sum = T(0);
@@ -722,8 +748,7 @@ __device__ __forceinline__ void ang_bwd_term(uint32_t q, bool active, const SFDe
 template <typename T, int W, int GNZ>
 __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__ species,
                                                   const double *__restrict__ wz, NeighborBufs nb, TripletBufs tb,
-                                                  SFDev sf, Lds L, const double *__restrict__ g64,
-                                                  const float *__restrict__ g32, double *__restrict__ slot_f,
+                                                  SFDev sf, Lds L, GradSrc gs, double *__restrict__ slot_f,
                                                   double *__restrict__ atom_vir) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int i = blockIdx.x;
@@ -767,11 +792,36 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   }
   for (int k = tid; k < 4 * W * KP; k += 64 * W) s_T[k] = 0.0;  // s_T and s_F
   const size_t row = (size_t)i * Dp;
-  for (int d = tid; d < Dp; d += 64 * W) s_g[d] = g64 ? (T)g64[row + d] : (T)g32[row + d];
+  if (gs.splits > 0) {
+    // the contraction's finish for this atom (same order as k_gpr_finish)
+    double E = 0.0;
+    for (int sp = 0; sp < gs.splits; ++sp) E += gs.Epart[(size_t)sp * gs.Np + i];
+    if (tid == 0) gs.eatom[i] = E;
+    for (int d = tid; d < Dp; d += 64 * W) {
+      double G = 0.0;
+      if (gs.Gpart)
+        for (int sp = 0; sp < gs.splits; ++sp) G += gs.Gpart[((size_t)sp * gs.Np + i) * Dp + d];
+      else
+        for (int sp = 0; sp < gs.splits; ++sp) G += (double)gs.Gpart32[((size_t)sp * gs.Np + i) * Dp + d];
+      const double xt = gs.xt64 ? gs.xt64[row + d] : (double)gs.xt32[row + d];
+      const double g = -(xt * E - G) * gs.theta_inv[d];
+      if (gs.g64_out) {
+        gs.g64_out[row + d] = g;
+        s_g[d] = (T)g;
+      } else {
+        const float gf = (float)g;
+        gs.g32_out[row + d] = gf;
+        s_g[d] = (T)gf;
+      }
+    }
+  } else {
+    for (int d = tid; d < Dp; d += 64 * W) s_g[d] = gs.g64 ? (T)gs.g64[row + d] : (T)gs.g32[row + d];
+  }
+  __syncthreads();
   for (int s = tid; s < sf.n_ang; s += 64 * W) {
     const int f = sf.ang_feat[s];
-    s_gA[s] = g64 ? (T)g64[row + f] : (T)g32[row + f];
-    s_gW[s] = g64 ? (T)g64[row + f + sf.nsf] : (T)g32[row + f + sf.nsf];
+    s_gA[s] = s_g[f];
+    s_gW[s] = s_g[f + sf.nsf];
   }
   __syncthreads();
 
@@ -878,24 +928,23 @@ hipError_t fwd_waves(int waves, hipStream_t s, int N, const int *species, const 
 
 template <typename T, int W, int GNZ>
 hipError_t bwd_launch(hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
-                      const TripletBufs &tb, const SFDev &sf, Lds L, const double *g64, const float *g32,
-                      double *slot_f, double *atom_vir) {
-  k_desc_bwd<T, W, GNZ><<<N, 64 * W, bwd_lds<T, W>(L), s>>>(N, species, wz, nb, tb, sf, L, g64, g32, slot_f,
-                                                            atom_vir);
+                      const TripletBufs &tb, const SFDev &sf, Lds L, const GradSrc &gs, double *slot_f,
+                      double *atom_vir) {
+  k_desc_bwd<T, W, GNZ><<<N, 64 * W, bwd_lds<T, W>(L), s>>>(N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
   return hipGetLastError();
 }
 
 template <typename T, int W>
 hipError_t bwd_grid(hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
-                    const TripletBufs &tb, const SFDev &sf, Lds L, const double *g64, const float *g32,
-                    double *slot_f, double *atom_vir) {
-  if (!sf.grid) return bwd_launch<T, W, 0>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+                    const TripletBufs &tb, const SFDev &sf, Lds L, const GradSrc &gs, double *slot_f,
+                    double *atom_vir) {
+  if (!sf.grid) return bwd_launch<T, W, 0>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
   switch (sf.nz) {
-    case 1: return bwd_launch<T, W, 1>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
-    case 2: return bwd_launch<T, W, 2>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
-    case 3: return bwd_launch<T, W, 3>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
-    case 4: return bwd_launch<T, W, 4>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
-    default: return bwd_launch<T, W, 5>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+    case 1: return bwd_launch<T, W, 1>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+    case 2: return bwd_launch<T, W, 2>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+    case 3: return bwd_launch<T, W, 3>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+    case 4: return bwd_launch<T, W, 4>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+    default: return bwd_launch<T, W, 5>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
   }
 }
 
@@ -969,23 +1018,22 @@ hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *sp
 
 hipError_t launch_descriptors_bwd(hipStream_t s, bool fp64, int N, const int *species, const double *wz,
                                   const NeighborBufs &nb, const TripletBufs &tb, const SFDev &sf, int Dp,
-                                  const double *g64, const float *g32, double *slot_f, double *atom_vir, int waves,
-                                  int KP) {
+                                  const GradSrc &gs, double *slot_f, double *atom_vir, int waves, int KP) {
   Lds L{KP, sf.n_groups > 0 ? sf.n_groups : 1, Dp};
   if (N <= 0) return hipSuccess;
   // fewer waves when the per-wave accumulators would not fit in LDS
   if (fp64) {
     if (waves >= 4 && bwd_lds<double, 4>(L) <= kLdsBudget)
-      return bwd_grid<double, 4>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+      return bwd_grid<double, 4>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
     if (waves >= 2 && bwd_lds<double, 2>(L) <= kLdsBudget)
-      return bwd_grid<double, 2>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
-    return bwd_grid<double, 1>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+      return bwd_grid<double, 2>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+    return bwd_grid<double, 1>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
   }
   if (waves >= 4 && bwd_lds<float, 4>(L) <= kLdsBudget)
-    return bwd_grid<float, 4>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+    return bwd_grid<float, 4>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
   if (waves >= 2 && bwd_lds<float, 2>(L) <= kLdsBudget)
-    return bwd_grid<float, 2>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
-  return bwd_grid<float, 1>(s, N, species, wz, nb, tb, sf, L, g64, g32, slot_f, atom_vir);
+    return bwd_grid<float, 2>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+  return bwd_grid<float, 1>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
 }
 
 }  // namespace gapx

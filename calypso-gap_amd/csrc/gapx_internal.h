@@ -122,7 +122,7 @@ struct NeighborBufs {
   double *spos;         // [N*4] wrapped positions in bin order (x y z, pad)
   int *sinfo;           // [N*4] in bin order: atom index, its three cell shifts
   int *overflow;        // = total + 2
-  int *total;           // [8] see launch_list_stats; [3] is the descriptor kernels' guard
+  int *total;           // [8] see launch_struct_reduce; [3] is the descriptor kernels' guard
 };
 
 hipError_t launch_wrap_bin(hipStream_t s, int N, const double *pos, const int *atom_struct,
@@ -156,25 +156,43 @@ struct TripletBufs {
 };
 
 hipError_t launch_triplets(hipStream_t s, int N, const NeighborBufs &nb, double rcA, TripletBufs tb);
-// out[0..7] = kept pairs, largest raw pair count, atoms above kcap, descriptor
-// guard (nb.total[3]), kept triplets, largest raw triplet count, atoms above
-// tcap, 0.  `out` may be pinned host memory (device-visible).
-hipError_t launch_list_stats(hipStream_t s, int N, const NeighborBufs &nb, const TripletBufs &tb, int *out);
+
 hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *species,
                                   const double *wz, const NeighborBufs &nb, const TripletBufs &tb,
                                   const SFDev &sf, const double *theta_inv, const double *mu,
                                   int split_mode, DescOut out, int chunk, int waves, int KP);
+// dE/dx for the backward pass: either given (g64 or g32, splits = 0), or
+// finished from the contraction's split partials inside the backward kernel
+// (splits > 0): E_i = sum_s Epart[s][i] and g = -(xt E - sum_s G[s][i]) theta_inv,
+// summed in split order; E_i goes to eatom and g to g64/g32 as well.
+struct GradSrc {
+  const double *g64 = nullptr;
+  const float *g32 = nullptr;
+  int splits = 0, Np = 0;
+  const double *Epart = nullptr, *Gpart = nullptr;
+  const float *Gpart32 = nullptr;
+  const double *xt64 = nullptr;
+  const float *xt32 = nullptr;
+  const double *theta_inv = nullptr;
+  double *eatom = nullptr;
+  double *g64_out = nullptr;
+  float *g32_out = nullptr;
+};
 hipError_t launch_descriptors_bwd(hipStream_t s, bool fp64, int N, const int *species,
                                   const double *wz, const NeighborBufs &nb, const TripletBufs &tb,
-                                  const SFDev &sf, int Dp, const double *g64, const float *g32,
+                                  const SFDev &sf, int Dp, const GradSrc &gs,
                                   double *slot_f, double *atom_vir, int waves, int KP);
 // Largest per-atom neighbour count the descriptor kernels' LDS layout holds.
 int max_neighbors_for_lds(bool fp64, int G, int Dp);
 hipError_t launch_gather_forces(hipStream_t s, int N, const NeighborBufs &nb, const double *slot_f,
                                 double *force);
-hipError_t launch_struct_reduce(hipStream_t s, int nstruct, const StructParams *sp,
-                                const double *eatom, const double *atom_vir, int grad,
-                                double *ene, double *stress6);
+// Per-structure energy and stress, plus (one more workgroup) the list
+// statistics: out[0..7] = kept pairs, largest raw pair count, atoms above
+// kcap, descriptor guard (nb.total[3]), kept triplets, largest raw triplet
+// count, atoms above tcap, 0.  stats_out may be pinned host memory.
+hipError_t launch_struct_reduce(hipStream_t s, int nstruct, const StructParams *sp, const double *eatom,
+                                const double *atom_vir, int grad, double *ene, double *stress6, int N,
+                                const NeighborBufs &nb, const TripletBufs &tb, int *stats_out);
 
 // Contraction: E_i = sum_k exp(-|xt_i - mt_k|^2 / 2) c_k and
 // g_i = -(xt_i E_i - sum_k W_ik mt_k) * theta_inv.

@@ -266,10 +266,9 @@ __global__ __launch_bounds__(64) void k_reverse(int N, NeighborBufs nb) {
 
 // Fold the per-atom counts into the run's statistics (one workgroup):
 // per-thread values, wavefront shuffles, then the 16 wavefront results.
-__global__ __launch_bounds__(1024) void k_list_stats(int N, int kcap, const int *__restrict__ nb_count,
-                                                     const int *__restrict__ nb_raw, int tcap,
-                                                     const int *__restrict__ t_count, const int *__restrict__ t_raw,
-                                                     const int *__restrict__ guard, int *__restrict__ out) {
+__device__ void list_stats(int N, int kcap, const int *__restrict__ nb_count, const int *__restrict__ nb_raw,
+                           int tcap, const int *__restrict__ t_count, const int *__restrict__ t_raw,
+                           const int *__restrict__ guard, int *__restrict__ out) {
   __shared__ long long red[16][6];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   long long v[6] = {0, 0, 0, 0, 0, 0};
@@ -348,11 +347,19 @@ __device__ __forceinline__ V wave_reduce_sum(V v) {
 
 // Per-structure energy and stress: per-thread sums, wavefront shuffles, then
 // the 16 wavefront totals in order (deterministic).
-__global__ __launch_bounds__(1024) void k_struct_reduce(const StructParams *__restrict__ sp,
+// Workgroups 0 .. nstruct-1: one structure each; workgroup nstruct: the list
+// statistics.
+__global__ __launch_bounds__(1024) void k_struct_reduce(int nstruct, const StructParams *__restrict__ sp,
                                                         const double *__restrict__ eatom,
                                                         const double *__restrict__ vir, int grad,
-                                                        double *__restrict__ ene, double *__restrict__ stress6) {
+                                                        double *__restrict__ ene, double *__restrict__ stress6,
+                                                        int N, NeighborBufs nb, TripletBufs tb, int *stats_out) {
   const int s = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (s == nstruct) {
+    list_stats(N, nb.kcap, nb.nb_count, nb.nb_raw, tb.tcap, tb.raw ? tb.count : nullptr, tb.raw, nb.total + 3,
+               stats_out);
+    return;
+  }
   const StructParams &P = sp[s];
   __shared__ double red[16][7];
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -396,11 +403,6 @@ hipError_t launch_zero_ints(hipStream_t s, int *a, int na, int *b, int nb, int *
   return hipGetLastError();
 }
 
-hipError_t launch_list_stats(hipStream_t s, int N, const NeighborBufs &nb, const TripletBufs &tb, int *out) {
-  k_list_stats<<<1, 1024, 0, s>>>(N, nb.kcap, nb.nb_count, nb.nb_raw, tb.tcap, tb.raw ? tb.count : nullptr, tb.raw,
-                                  nb.total + 3, out);
-  return hipGetLastError();
-}
 
 hipError_t launch_fill_double(hipStream_t s, double *p, size_t n, double v) {
   if (n == 0) return hipSuccess;
@@ -445,8 +447,10 @@ hipError_t launch_gather_forces(hipStream_t s, int N, const NeighborBufs &nb, co
 }
 
 hipError_t launch_struct_reduce(hipStream_t s, int nstruct, const StructParams *sp, const double *eatom,
-                                const double *atom_vir, int grad, double *ene, double *stress6) {
-  k_struct_reduce<<<nstruct, 1024, 0, s>>>(sp, eatom, atom_vir, grad, ene, stress6);
+                                const double *atom_vir, int grad, double *ene, double *stress6, int N,
+                                const NeighborBufs &nb, const TripletBufs &tb, int *stats_out) {
+  k_struct_reduce<<<nstruct + 1, 1024, 0, s>>>(nstruct, sp, eatom, atom_vir, grad, ene, stress6, N, nb, tb,
+                                               stats_out);
   return hipGetLastError();
 }
 
