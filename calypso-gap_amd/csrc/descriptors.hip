@@ -54,7 +54,7 @@ struct Math<float> {
   static __device__ __forceinline__ float ex(float x) { return __expf(x); }
   static __device__ __forceinline__ float cs(float x) { return __cosf(x); }
   static __device__ __forceinline__ float sn(float x) { return __sinf(x); }
-  static __device__ __forceinline__ float sq(float x) { return sqrtf(x); }
+  static __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float pw(float b, float e) { return powf(b, e); }
   // hardware approximations (about 1 ulp) for the mixed path's inner loops
   static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -124,6 +124,25 @@ struct Lds {
   int KP, G, Dp;
 };
 
+// Triplet indices of wave steps t, t + stride, .. (4 steps) for one lane; the
+// loads are issued together, one iteration ahead of their use, so a wave
+// waits for global memory once per four steps at most.  Inactive entries
+// point at neighbours 0 and 1 (valid whenever the list is not empty).
+__device__ __forceinline__ void load_q4(const uint32_t *tl, int nt, int t, int stride, uint32_t (&q)[4]) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) q[u] = (t + u * stride < nt) ? tl[t + u * stride] : (1u << 16);
+}
+
+// Front of the queue, the rest moved up (static register indices: the step's
+// body stays a single copy, so the kernel keeps its register budget).
+__device__ __forceinline__ uint32_t pop_q4(uint32_t (&q)[4]) {
+  const uint32_t v = q[0];
+  q[0] = q[1];
+  q[1] = q[2];
+  q[2] = q[3];
+  return v;
+}
+
 // ---------------------------------------------------------------------------
 // triplet lists
 // ---------------------------------------------------------------------------
@@ -162,12 +181,12 @@ __global__ __launch_bounds__(64) void k_triplets(int N, NeighborBufs nb, double 
       const int c = c0 + lane;
       const bool first = c < n1;
       const int a = first ? c : c - n1, b = a + (first ? d1 : d2);
-      bool ok = false;
-      if (c < n) {
-        const float4 pa = s_p[a], pb = s_p[b];
-        const float ex = pb.x - pa.x, ey = pb.y - pa.y, ez = pb.z - pa.z;
-        ok = pa.w <= rc && pb.w <= rc && ex * ex + ey * ey + ez * ez <= rc2;
-      }
+      // both records read unconditionally (idle lanes read record 0) and
+      // the tests combined without short-circuits: one LDS round trip
+      const bool live = c < n;
+      const float4 pa = s_p[live ? a : 0], pb = s_p[live ? b : 0];
+      const float ex = pb.x - pa.x, ey = pb.y - pa.y, ez = pb.z - pa.z;
+      const bool ok = live & (pa.w <= rc) & (pb.w <= rc) & (ex * ex + ey * ey + ez * ez <= rc2);
       const unsigned long long m = __ballot(ok);
       const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
       if (ok && pos < tb.tcap) tb.list[out + pos] = (uint32_t)a | ((uint32_t)b << 16);
@@ -320,10 +339,18 @@ __global__ __launch_bounds__(256) void k_desc_fwd(int N, const int *__restrict__
     T acc[NCH], accw[NCH];
 #pragma unroll
     for (int l = 0; l < NCH; ++l) acc[l] = accw[l] = T(0);
-    for (int t0 = 64 * wv; t0 < nt; t0 += 64 * W) {
-      const bool act = t0 + lane < nt;
-      const uint32_t q = act ? tl[t0 + lane] : (1u << 16);
-      ang_fwd_term<T, NCH>(q, act, c0, sf, s_d, s_w, s_fc, KP, acc, accw);
+    constexpr int ST = 64 * W;
+    uint32_t qc[4], qn[4];
+    load_q4(tl, nt, 64 * wv + lane, ST, qc);
+    for (int t0 = 64 * wv; t0 < nt; t0 += 4 * ST) {
+      load_q4(tl, nt, t0 + 4 * ST + lane, ST, qn);
+#pragma unroll 1
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t q = pop_q4(qc);
+        ang_fwd_term<T, NCH>(q, t0 + u * ST + lane < nt, c0, sf, s_d, s_w, s_fc, KP, acc, accw);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) qc[u] = qn[u];
     }
     chunk_reduce<T, NCH, W>(acc, accw, wv, lane, tid, c0, sf.n_ang, sf.ang_feat, sf.nsf, s_part, s_feat);
   }
@@ -524,37 +551,49 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
     T acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) acc[j] = T(0);
-    for (int t0 = 64 * wv; t0 < nt; t0 += 64 * W) {
-      const bool act = t0 + lane < nt;
-      const uint32_t q = act ? tl[t0 + lane] : (1u << 16);
-      const int a = q & 0xffff, b = q >> 16;
-      const T ax = s_d[4 * a], ay = s_d[4 * a + 1], az = s_d[4 * a + 2], ra = s_d[4 * a + 3];
-      const T bx = s_d[4 * b], by = s_d[4 * b + 1], bz = s_d[4 * b + 2], rb = s_d[4 * b + 3];
-      const T ex = bx - ax, ey = by - ay, ez = bz - az;
-      const T e2 = ex * ex + ey * ey + ez * ez;
-      const T e = Math<T>::sq(e2);
-      const T cth = (ra * ra + rb * rb - e2) * (T(0.5) * s_ri[a] * s_ri[b]);
-      const T S2 = ra * ra + rb * rb + e2;
-      const T wab = s_w[a] * s_w[b];
-      const T fe = (e <= rc) ? T(0.5) * (Math<T>::cs(pirc * e) + T(1)) : T(0);
-      const T F3 = act ? s_fc[g * KP + a] * s_fc[g * KP + b] * fe : T(0);
-      T Pp[NZ], Pm[NZ], dPp[NZ], dPm[NZ];
-      grid_powers<T, NZ, false>(cth, sf, Pp, Pm, dPp, dPm);
+    T eta_h[NE];  // this pass's etas (uniform), loaded once
 #pragma unroll
-      for (int hh = 0; hh < NE; ++hh) {
-        const int h = h0 + hh;
-        if (h < sf.n_eta) {
-          const T E = Math<T>::ex(-(T)sf.ang_eta[h * 2 * NZ] * S2) * F3;
+    for (int hh = 0; hh < NE; ++hh) eta_h[hh] = (h0 + hh < sf.n_eta) ? (T)sf.ang_eta[(h0 + hh) * 2 * NZ] : T(0);
+    constexpr int ST = 64 * W;
+    uint32_t qc[4], qn[4];
+    load_q4(tl, nt, 64 * wv + lane, ST, qc);
+    for (int t00 = 64 * wv; t00 < nt; t00 += 4 * ST) {
+      load_q4(tl, nt, t00 + 4 * ST + lane, ST, qn);
+#pragma unroll 1
+      for (int u = 0; u < 4; ++u) {
+        const bool act = t00 + u * ST + lane < nt;
+        const uint32_t q = pop_q4(qc);
+        const int a = q & 0xffff, b = q >> 16;
+        const T ax = s_d[4 * a], ay = s_d[4 * a + 1], az = s_d[4 * a + 2], ra = s_d[4 * a + 3];
+        const T bx = s_d[4 * b], by = s_d[4 * b + 1], bz = s_d[4 * b + 2], rb = s_d[4 * b + 3];
+        const T ex = bx - ax, ey = by - ay, ez = bz - az;
+        const T e2 = ex * ex + ey * ey + ez * ez;
+        const T e = Math<T>::sq(e2);
+        const T cth = (ra * ra + rb * rb - e2) * (T(0.5) * s_ri[a] * s_ri[b]);
+        const T S2 = ra * ra + rb * rb + e2;
+        const T wab = s_w[a] * s_w[b];
+        const T fe = (e <= rc) ? T(0.5) * (Math<T>::cs(pirc * e) + T(1)) : T(0);
+        const T F3 = act ? s_fc[g * KP + a] * s_fc[g * KP + b] * fe : T(0);
+        T Pp[NZ], Pm[NZ], dPp[NZ], dPm[NZ];
+        grid_powers<T, NZ, false>(cth, sf, Pp, Pm, dPp, dPm);
 #pragma unroll
-          for (int k = 0; k < NZ; ++k) {
-            const T tp = E * Pp[k], tm = E * Pm[k];
-            acc[(hh * 2 + 0) * NZ + k] += tp;
-            acc[(hh * 2 + 1) * NZ + k] += tm;
-            acc[NV / 2 + (hh * 2 + 0) * NZ + k] += tp * wab;
-            acc[NV / 2 + (hh * 2 + 1) * NZ + k] += tm * wab;
+        for (int hh = 0; hh < NE; ++hh) {
+          const int h = h0 + hh;
+          if (h < sf.n_eta) {
+            const T E = Math<T>::ex(-eta_h[hh] * S2) * F3;
+#pragma unroll
+            for (int k = 0; k < NZ; ++k) {
+              const T tp = E * Pp[k], tm = E * Pm[k];
+              acc[(hh * 2 + 0) * NZ + k] += tp;
+              acc[(hh * 2 + 1) * NZ + k] += tm;
+              acc[NV / 2 + (hh * 2 + 0) * NZ + k] += tp * wab;
+              acc[NV / 2 + (hh * 2 + 1) * NZ + k] += tm * wab;
+            }
           }
         }
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) qc[u] = qn[u];
     }
     group_sum<T, NV, W>(acc, wv, lane, tid, s_part, s_sum);
     for (int j = tid; j < NV; j += 64 * W) {
@@ -602,8 +641,8 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
 template <typename T, int NZ>
 __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDev &sf, const T *s_d, const T *s_ri,
                                              const T *s_w, const T *s_fc, const T *s_dfc, const T *s_gA,
-                                             const T *s_gW, int KP, T rc, T pirc, double *s_T, double *s_F,
-                                             T (&vir)[6]) {
+                                             const T *s_gW, const T *s_eta, int KP, T rc, T pirc, double *s_T,
+                                             double *s_F, T (&vir)[6]) {
   if (!active) return;
   const int a = q & 0xffff, b = q >> 16;
   const T ax = s_d[4 * a], ay = s_d[4 * a + 1], az = s_d[4 * a + 2], ra = s_d[4 * a + 3];
@@ -626,7 +665,7 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
   grid_powers<T, NZ, true>(cth, sf, Pp, Pm, dPp, dPm);
   T SA = 0, SB = 0, SC = 0;
   for (int h = 0; h < sf.n_eta; ++h) {
-    const T eta = (T)sf.ang_eta[h * 2 * NZ];
+    const T eta = s_eta[h];
     const T E = Math<T>::ex(-eta * S2);
     const int s0 = h * 2 * NZ;
     T sa = 0, sb = 0;
@@ -767,6 +806,7 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   T *s_g = s_dfc + G * KP;
   T *s_gA = s_g + Dp;  // per angular line: g of the plain and of the weighted feature
   T *s_gW = s_gA + sf.n_ang;
+  T *s_eta = s_g + 2 * Dp;                        // [Dp / 2] grid form: eta of each pass
 
   const int start = i * nb.kcap;
   const int K = nb.nb_count[i];
@@ -791,6 +831,8 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
     }
   }
   for (int k = tid; k < 4 * W * KP; k += 64 * W) s_T[k] = 0.0;  // s_T and s_F
+  if (GNZ > 0)
+    for (int h = tid; h < sf.n_eta; h += 64 * W) s_eta[h] = (T)sf.ang_eta[h * 2 * GNZ];
   const size_t row = (size_t)i * Dp;
   if (gs.splits > 0) {
     // the contraction's finish for this atom (same order as k_gpr_finish)
@@ -846,14 +888,24 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
     const uint32_t *tl = tb.list + (size_t)i * tb.tcap;
     const int nt = tb.count[i];
     const T rcA = (T)sf.group_rc[sf.g_ang], pirc = (T)(kPi / sf.group_rc[sf.g_ang]);
-    for (int t0 = 64 * wv; t0 < nt; t0 += 64 * W) {
-      const bool act = t0 + lane < nt;
-      const uint32_t q = act ? tl[t0 + lane] : (1u << 16);
-      if constexpr (GNZ > 0)
-        ang_bwd_grid<T, GNZ>(q, act, sf, s_d, s_ri, s_w, s_fc, s_dfc, s_gA, s_gW, KP, rcA, pirc, s_T + wv * KP,
-                             s_F + 3 * wv * KP, vir);
-      else
-        ang_bwd_term<T>(q, act, sf, s_d, s_w, s_fc, s_dfc, s_gA, s_gW, KP, s_T + wv * KP, s_F + 3 * wv * KP, vir);
+    constexpr int ST = 64 * W;
+    uint32_t qc[4], qn[4];
+    load_q4(tl, nt, 64 * wv + lane, ST, qc);
+    for (int t0 = 64 * wv; t0 < nt; t0 += 4 * ST) {
+      load_q4(tl, nt, t0 + 4 * ST + lane, ST, qn);
+#pragma unroll 1
+      for (int u = 0; u < 4; ++u) {
+        const bool act = t0 + u * ST + lane < nt;
+        const uint32_t q = pop_q4(qc);
+        if constexpr (GNZ > 0)
+          ang_bwd_grid<T, GNZ>(q, act, sf, s_d, s_ri, s_w, s_fc, s_dfc, s_gA, s_gW, s_eta, KP, rcA, pirc,
+                               s_T + wv * KP, s_F + 3 * wv * KP, vir);
+        else
+          ang_bwd_term<T>(q, act, sf, s_d, s_w, s_fc, s_dfc, s_gA, s_gW, KP, s_T + wv * KP, s_F + 3 * wv * KP,
+                          vir);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) qc[u] = qn[u];
     }
   }
   __syncthreads();
@@ -903,7 +955,8 @@ size_t fwd_lds(const Lds &L) {
 
 template <typename T, int W>
 size_t bwd_lds(const Lds &L) {
-  return (6 * W + 4 * W * (size_t)L.KP) * sizeof(double) + (size_t)(6 * L.KP + 2 * L.G * L.KP + 2 * L.Dp) * sizeof(T);
+  return (6 * W + 4 * W * (size_t)L.KP) * sizeof(double) +
+         (size_t)(6 * L.KP + 2 * L.G * L.KP + 2 * L.Dp + L.Dp / 2) * sizeof(T);
 }
 
 constexpr size_t kLdsBudget = 64 * 1024;
@@ -986,9 +1039,9 @@ hipError_t fwd_grid(int waves, hipStream_t s, int N, const int *species, const d
 int max_neighbors_for_lds(bool fp64, int G, int Dp) {
   const size_t budget = kLdsBudget - 64;
   const size_t bytes = fp64 ? 8 : 4;
-  // the backward with one wave is the largest single-wave layout: (6 + 2G) KP + 2 Dp
+  // the backward with one wave is the largest single-wave layout: (6 + 2G) KP + 2.5 Dp
   // values of the compute type and 4 KP FP64 accumulators
-  long kp = (long)(budget - 2 * Dp * bytes) / (long)((6 + 2 * G) * bytes + 4 * sizeof(double));
+  long kp = (long)(budget - (2 * Dp + Dp / 2) * bytes) / (long)((6 + 2 * G) * bytes + 4 * sizeof(double));
   return (int)(kp / 8 * 8);
 }
 
@@ -1016,24 +1069,25 @@ hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *sp
   return fwd_waves<float, 16>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
 }
 
+// The most waves per atom (up to `waves`) whose layout fits the LDS budget.
+template <typename T>
+hipError_t bwd_fit(int waves, hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
+                   const TripletBufs &tb, const SFDev &sf, Lds L, const GradSrc &gs, double *slot_f,
+                   double *atom_vir) {
+  if (waves >= 4 && bwd_lds<T, 4>(L) <= kLdsBudget)
+    return bwd_grid<T, 4>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+  if (waves >= 2 && bwd_lds<T, 2>(L) <= kLdsBudget)
+    return bwd_grid<T, 2>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+  return bwd_grid<T, 1>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+}
+
 hipError_t launch_descriptors_bwd(hipStream_t s, bool fp64, int N, const int *species, const double *wz,
                                   const NeighborBufs &nb, const TripletBufs &tb, const SFDev &sf, int Dp,
                                   const GradSrc &gs, double *slot_f, double *atom_vir, int waves, int KP) {
   Lds L{KP, sf.n_groups > 0 ? sf.n_groups : 1, Dp};
   if (N <= 0) return hipSuccess;
-  // fewer waves when the per-wave accumulators would not fit in LDS
-  if (fp64) {
-    if (waves >= 4 && bwd_lds<double, 4>(L) <= kLdsBudget)
-      return bwd_grid<double, 4>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
-    if (waves >= 2 && bwd_lds<double, 2>(L) <= kLdsBudget)
-      return bwd_grid<double, 2>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
-    return bwd_grid<double, 1>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
-  }
-  if (waves >= 4 && bwd_lds<float, 4>(L) <= kLdsBudget)
-    return bwd_grid<float, 4>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
-  if (waves >= 2 && bwd_lds<float, 2>(L) <= kLdsBudget)
-    return bwd_grid<float, 2>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
-  return bwd_grid<float, 1>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+  if (fp64) return bwd_fit<double>(waves, s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+  return bwd_fit<float>(waves, s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
 }
 
 }  // namespace gapx
