@@ -339,7 +339,9 @@ def main():
     D, M = m.des_len, m.nsparse
 
     runs = {}
+    tuning = {}
     for name, flags in (("fp64", gapx.GRAD), ("mixed", gapx.GRAD | gapx.MIXED)):
+        tuning[name] = ctx.autotune(flags)  # launch shapes for this workload (cached per shape)
         ctx.use_graph(True)  # replay the captured launch sequence
         sec = time_runs(ctx, flags, args.steps, args.warmup)
         ctx.use_graph(False)  # per-stage events need eager launches
@@ -396,7 +398,7 @@ def main():
         "dtype": "f32 descriptors + f16 hi/lo MFMA (f32 accumulate) + f64 norms/reductions",
         "data": "synthetic (seeded structures and model)",
         "config": {"workload": args.workload, "atoms": N, "des_len": D, "nsparse": M, "rcut": rc,
-                   "precision": "mixed", "pairs": pairs, "triplets": triplets},
+                   "precision": "mixed", "pairs": pairs, "triplets": triplets, "tuning": tuning},
         "fp64": {"value": round(N / runs["fp64"]["sec"], 1), "ms_per_step": round(runs["fp64"]["sec"] * 1e3, 4),
                  "stages_ms": {k: round(v, 4) for k, v in runs["fp64"]["stages"].items()}},
         "mixed_over_fp64": round(runs["fp64"]["sec"] / sec, 3),

@@ -876,8 +876,8 @@ static int ctx_settle(Ctx &c) {
 }
 
 // ---------------------------------------------------------------------------
-// autotuner: coordinate descent over the launch shapes, each knob timed on the
-// stage it changes (HIP events), on the loaded batch itself.  Results are
+// autotuner: coordinate descent over the launch shapes, each candidate timed
+// as whole evaluations of the loaded batch, back to back (HIP events).  Results are
 // cached per process for the shape class (padded D, M, table form, precision,
 // atom and structure counts rounded up to powers of two).
 // ---------------------------------------------------------------------------
@@ -921,34 +921,42 @@ static int ctx_autotune(Ctx &c, int flags, bool force) {
   }
   const bool was_timing = c.timing, was_graph = c.use_graph;
   c.use_graph = false;
+  c.timing = false;
   c.invalidate_graph();
   int rc = ctx_run(c, flags);
   if (!rc) rc = ctx_settle(c);  // capacities first
-  c.timing = true;
-  reset_timing(c);
-  const int reps = 5;
-  auto measure = [&](std::initializer_list<int> stages, double &out) -> int {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(GAPX_ERR_HIP, "autotune: could not create events");
+  // whole evaluations, back to back on the stream (as an application runs
+  // them): the mean of `reps` runs between two events, best of two trials
+  const int reps = 10;
+  auto measure = [&](double &out) -> int {
     int r = ctx_run(c, flags);  // warm-up (buffers, caches)
     if (r) return r;
-    reset_timing(c);
-    for (int k = 0; k < reps; ++k)
-      if ((r = ctx_run(c, flags))) return r;
-    out = 0.0;
-    for (int st : stages) out += c.stage_ms[st] / std::max(1, c.stage_launches[st]);
+    out = 1e300;
+    for (int trial = 0; trial < 2; ++trial) {
+      GAPX_HIP(hipEventRecord(e0, c.stream));
+      for (int k = 0; k < reps; ++k)
+        if ((r = ctx_run(c, flags))) return r;
+      GAPX_HIP(hipEventRecord(e1, c.stream));
+      GAPX_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      GAPX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      out = std::min(out, (double)ms / reps);
+    }
     return GAPX_OK;
   };
   struct Knob {
     int *field;
     std::vector<int> values;
-    std::initializer_list<int> stages;
   };
   Tuning &t = c.tune[p];
   std::vector<Knob> knobs;
-  knobs.push_back({&t.waves_fwd, {1, 2, 4}, {kDescFwd}});
-  if (flags & GAPX_GRAD) knobs.push_back({&t.waves_bwd, {1, 2, 4}, {kDescBwd}});
-  // the split count also changes the finish, which runs in the backward pass
-  if (p == 1 && c.m->Dp <= 128) knobs.push_back({&t.gpr_rows, {64, 128}, {kContraction, kFinish, kDescBwd}});
-  knobs.push_back({&t.gpr_blocks, {256, 512, 1024, 2048}, {kContraction, kFinish, kDescBwd}});
+  knobs.push_back({&t.waves_fwd, {1, 2, 4}});
+  if (flags & GAPX_GRAD) knobs.push_back({&t.waves_bwd, {1, 2, 4}});
+  if (p == 1 && c.m->Dp <= 128) knobs.push_back({&t.gpr_rows, {64, 128}});
+  knobs.push_back({&t.gpr_blocks, {256, 512, 1024, 2048}});
   for (Knob &k : knobs) {
     if (rc) break;
     int best = *k.field;
@@ -956,7 +964,7 @@ static int ctx_autotune(Ctx &c, int flags, bool force) {
     for (int v : k.values) {
       *k.field = v;
       double ms = 0;
-      if ((rc = measure(k.stages, ms))) break;
+      if ((rc = measure(ms))) break;
       if (ms < best_ms) {
         best_ms = ms;
         best = v;
@@ -964,6 +972,8 @@ static int ctx_autotune(Ctx &c, int flags, bool force) {
     }
     *k.field = best;
   }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
   c.timing = was_timing;
   reset_timing(c);
   c.use_graph = was_graph;
