@@ -314,6 +314,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   int device = 0;
   int nstruct = 0, N = 0, Np = 0, nbins = 0;
+  bool fused_bins = false;  // every structure small enough for k_bin_struct
   double rcut = 6.0;
   std::vector<int> natoms, offsets;
   std::vector<double> lat;
@@ -417,6 +418,12 @@ static int struct_params(Ctx &c) {
     P.pad = 0;
   }
   c.nbins = bin_base;
+  // one-launch binning when every structure fits its LDS layout
+  c.fused_bins = std::getenv("GAPX_NO_FUSED_BINS") == nullptr;
+  for (int s = 0; s < c.nstruct; ++s) {
+    const StructParams &P = c.sp[s];
+    if (P.natoms > kFusedMaxAtoms || P.nb[0] * P.nb[1] * P.nb[2] > kFusedBins) c.fused_bins = false;
+  }
   return GAPX_OK;
 }
 
@@ -562,10 +569,14 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b, bool grad) {
   const int *d_astruct = c.ptr<int>("astruct");
   const double *d_pos = c.ptr<double>("pos");
   const StructParams *d_sp = c.ptr<StructParams>("sp");
-  GAPX_HIP(launch_zero_ints(c.stream, nb.bin_count, c.nbins, nb.bin_cursor, c.nbins, stats, 8));
-  GAPX_HIP(launch_wrap_bin(c.stream, c.N, d_pos, d_astruct, d_sp, nb));
-  GAPX_HIP(launch_scan(c.stream, nb.bin_count, nb.bin_start, c.nbins, nullptr));
-  GAPX_HIP(launch_scatter_bins(c.stream, c.N, c.nbins, nb));
+  if (c.fused_bins) {
+    GAPX_HIP(launch_bin_structures(c.stream, c.nstruct, d_pos, d_sp, nb, c.nbins, stats));
+  } else {
+    GAPX_HIP(launch_zero_ints(c.stream, nb.bin_count, c.nbins, nb.bin_cursor, c.nbins, stats, 8));
+    GAPX_HIP(launch_wrap_bin(c.stream, c.N, d_pos, d_astruct, d_sp, nb));
+    GAPX_HIP(launch_scan(c.stream, nb.bin_count, nb.bin_start, c.nbins, nullptr));
+    GAPX_HIP(launch_scatter_bins(c.stream, c.N, c.nbins, nb));
+  }
   GAPX_HIP(launch_build_neighbors(c.stream, c.N, d_astruct, d_sp, c.rcut, nb));
   if (grad) GAPX_HIP(launch_reverse_slots(c.stream, c.N, nb));
   c.KP = c.kcap;

@@ -129,6 +129,13 @@ hipError_t launch_wrap_bin(hipStream_t s, int N, const double *pos, const int *a
                            const StructParams *sp, NeighborBufs nb);
 hipError_t launch_scan(hipStream_t s, const int *in, int *out, int n, int *total);
 hipError_t launch_scatter_bins(hipStream_t s, int N, int nbins, NeighborBufs nb);
+// The whole binning of structures with at most kFusedAtoms atoms and
+// kFusedBins bins each, one workgroup per structure (also clears stats[0..7]).
+// The host uses it for batches of small structures only (kFusedMaxAtoms):
+// one large structure would leave all but one CU idle.
+constexpr int kFusedAtoms = 8192, kFusedBins = 2048, kFusedMaxAtoms = 1024;
+hipError_t launch_bin_structures(hipStream_t s, int nstruct, const double *pos, const StructParams *sp,
+                                 NeighborBufs nb, int nbins_total, int *stats);
 hipError_t launch_build_neighbors(hipStream_t s, int N, const int *atom_struct,
                                   const StructParams *sp, double rcut, NeighborBufs nb);
 hipError_t launch_reverse_slots(hipStream_t s, int N, NeighborBufs nb);

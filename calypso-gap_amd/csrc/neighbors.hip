@@ -136,6 +136,85 @@ __global__ __launch_bounds__(64) void k_rank_bins(int nbins, NeighborBufs nb) {
   }
 }
 
+// One workgroup per structure does the whole binning in LDS: wrap and bin
+// every atom, count per bin, exclusive scan, scatter, then order each bin's
+// atoms by index and write the bin-sorted records; workgroup 0 also clears
+// the run's statistics.  Same output as k_zero3 + k_wrap_bin + k_scan +
+// k_scatter + k_rank_bins, in one launch, for structures of at most
+// kFusedAtoms atoms and kFusedBins bins.
+__global__ __launch_bounds__(1024) void k_bin_struct(const double *__restrict__ pos,
+                                                     const StructParams *__restrict__ sp, NeighborBufs nb,
+                                                     int nbins_total, int *__restrict__ stats) {
+  __shared__ int s_cnt[kFusedBins + 1];  // counts, then local bin starts
+  __shared__ int s_cur[kFusedBins];      // scatter cursors
+  __shared__ int s_tmp[kFusedAtoms];     // local atom index per slot, unsorted inside each bin
+  __shared__ int s_part[1024];
+  const int s = blockIdx.x, t = threadIdx.x;
+  const StructParams &P = sp[s];
+  const int n = P.natoms, base = P.atom_base, nbs = P.nb[0] * P.nb[1] * P.nb[2];
+  if (s == 0 && t < 8) stats[t] = 0;
+  for (int b = t; b < nbs; b += 1024) {
+    s_cnt[b] = 0;
+    s_cur[b] = 0;
+  }
+  __syncthreads();
+  for (int a = t; a < n; a += 1024) {
+    const int i = base + a;
+    const double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    int bk3[3], sh[3];
+    for (int k = 0; k < 3; ++k) {
+      const double f = r[0] * P.Linv[k] + r[1] * P.Linv[3 + k] + r[2] * P.Linv[6 + k];
+      const double fl = floor(f);
+      sh[k] = (int)fl;
+      const int bk = (int)floor((f - fl) * P.nb[k]);
+      bk3[k] = bk < 0 ? 0 : (bk >= P.nb[k] ? P.nb[k] - 1 : bk);
+    }
+    for (int c = 0; c < 3; ++c) {
+      nb.wpos[3 * i + c] = r[c] - sh[0] * P.L[c] - sh[1] * P.L[3 + c] - sh[2] * P.L[6 + c];
+      nb.shift[3 * i + c] = sh[c];
+    }
+    const int lb = (bk3[0] * P.nb[1] + bk3[1]) * P.nb[2] + bk3[2];
+    nb.atom_bin[i] = P.bin_base + lb;
+    atomicAdd(&s_cnt[lb], 1);
+  }
+  __syncthreads();
+  // exclusive scan of the counts
+  const int per = (nbs + 1023) / 1024, lo = t * per, hi = min(nbs, lo + per);
+  int sum = 0;
+  for (int b = lo; b < hi; ++b) sum += s_cnt[b];
+  s_part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = (t >= off) ? s_part[t - off] : 0;
+    __syncthreads();
+    s_part[t] += v;
+    __syncthreads();
+  }
+  int run = s_part[t] - sum;
+  for (int b = lo; b < hi; ++b) {
+    const int v = s_cnt[b];
+    s_cnt[b] = run;
+    run += v;
+  }
+  if (t == 0) s_cnt[nbs] = n;
+  __syncthreads();
+  for (int b = t; b < nbs; b += 1024) nb.bin_start[P.bin_base + b] = base + s_cnt[b];
+  if (t == 0 && s == (int)gridDim.x - 1) nb.bin_start[nbins_total] = base + n;
+  for (int a = t; a < n; a += 1024) {
+    const int lb = nb.atom_bin[base + a] - P.bin_base;
+    s_tmp[s_cnt[lb] + atomicAdd(&s_cur[lb], 1)] = a;
+  }
+  __syncthreads();
+  for (int q = t; q < n; q += 1024) {
+    const int a = s_tmp[q];
+    const int lb = nb.atom_bin[base + a] - P.bin_base;
+    const int b0 = s_cnt[lb], b1 = s_cnt[lb + 1];
+    int rank = 0;
+    for (int k = b0; k < b1; ++k) rank += (s_tmp[k] < a) ? 1 : 0;
+    put_sorted(nb, base + b0 + rank, base + a);
+  }
+}
+
 __device__ __forceinline__ unsigned long long nb_key(int j, int n0, int n1, int n2) {
   return ((unsigned long long)(unsigned)j << 24) | ((unsigned long long)((n0 + 128) & 255) << 16) |
          ((unsigned long long)((n1 + 128) & 255) << 8) | (unsigned long long)((n2 + 128) & 255);
@@ -426,6 +505,12 @@ hipError_t launch_scatter_bins(hipStream_t s, int N, int nbins, NeighborBufs nb)
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   k_rank_bins<<<nbins, 64, 0, s>>>(nbins, nb);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin_structures(hipStream_t s, int nstruct, const double *pos, const StructParams *sp,
+                                 NeighborBufs nb, int nbins_total, int *stats) {
+  k_bin_struct<<<nstruct, 1024, 0, s>>>(pos, sp, nb, nbins_total, stats);
   return hipGetLastError();
 }
 
