@@ -152,13 +152,21 @@ def population_bench(steps, warmup):
         out[name] = dict(value=round(float(natoms.sum()) / sec, 1), ms_per_step=round(sec * 1e3, 4), tuning=tune,
                          stages_ms={k: round(v, 4) for k, v in st.items()})
         print("bench: population %s %.3f ms/step tuning %s" % (name, sec * 1e3, tune), file=sys.stderr, flush=True)
+    # batched relaxation of the whole population on the device (mixed, graph replay)
+    ctx.use_graph(True)
+    t0 = time.perf_counter()
+    rel = ctx.relax(gapx.GRAD | gapx.MIXED, max_steps=200, fmax=0.05, maxstep=0.2)
+    t_rel = time.perf_counter() - t0
+    relax = dict(steps=rel["steps"], converged=rel["converged"], seconds=round(t_rel, 3),
+                 structure_steps_per_s=round(len(structs) * rel["steps"] / t_rel, 1), fmax=0.05, max_steps=200)
+    print("bench: population relax %s" % relax, file=sys.stderr, flush=True)
     ctx.close()
     m.close()
     r64, rmx = res["fp64"], res["mixed"]
     err = dict(energy_meV_per_atom=float(np.abs((rmx["energy"] - r64["energy"]) / natoms).max()) * 1e3,
                force_meV_per_A=float(np.abs(rmx["forces"] - r64["forces"]).max()) * 1e3,
                stress_GPa=float(np.abs(rmx["stress"] - r64["stress"]).max()))
-    return dict(structures=len(structs), atoms=int(natoms.sum()), unit="atom E+F+S evals/s", **out,
+    return dict(structures=len(structs), atoms=int(natoms.sum()), unit="atom E+F+S evals/s", **out, relax=relax,
                 mixed_over_fp64=round(out["mixed"]["value"] / out["fp64"]["value"], 3),
                 mixed_error_vs_fp64_max={k: float("%.4g" % v) for k, v in err.items()})
 

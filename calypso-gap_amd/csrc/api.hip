@@ -995,6 +995,50 @@ static int ctx_autotune(Ctx &c, int flags, bool force) {
   return GAPX_OK;
 }
 
+// Relax every structure of the batch at fixed cell (FIRE, relax.hip): each
+// step is one E+F evaluation (graph replay when enabled) and two small
+// kernels that update velocities and positions in place; the step waits for
+// the device once to check list capacities and convergence.
+static int ctx_relax(Ctx &c, int flags, int max_steps, double fmax, double maxstep, int *steps_done,
+                     int *n_converged) {
+  if (c.N <= 0) return fail(GAPX_ERR_ARG, "no structures loaded");
+  if (!(fmax > 0) || !(maxstep > 0) || max_steps < 0) return fail(GAPX_ERR_ARG, "gapx_ctx_relax: invalid argument");
+  GAPX_HIP(hipSetDevice(c.device));
+  flags = (flags | GAPX_GRAD) & ~GAPX_LAYOUT_F;
+  double *vel;
+  FireState *st;
+  int rc;
+  if ((rc = c.get("fire_vel", 3 * (size_t)c.N, &vel)) || (rc = c.get("fire_state", c.nstruct, &st))) return rc;
+  GAPX_HIP(launch_fill_double(c.stream, vel, 3 * (size_t)c.N, 0.0));
+  FireParams fp{0.1, 1.0, 1.1, 0.5, 0.1, 0.99, maxstep, fmax, 5, 0};
+  std::vector<FireState> hs(c.nstruct);
+  for (FireState &S : hs) {
+    S = FireState{};
+    S.dt = fp.dt0;
+    S.a = fp.astart;
+  }
+  GAPX_HIP(hipMemcpyAsync(st, hs.data(), sizeof(FireState) * c.nstruct, hipMemcpyHostToDevice, c.stream));
+  int step = 0, conv = 0;
+  for (; step < max_steps; ++step) {
+    if ((rc = ctx_run(c, flags)) || (rc = ctx_settle(c))) return rc;
+    GAPX_HIP(launch_fire_step(c.stream, c.nstruct, c.N, c.ptr<StructParams>("sp"), c.ptr<int>("astruct"),
+                              c.ptr<double>("force"), vel, c.ptr<double>("pos"), st, fp));
+    GAPX_HIP(hipMemcpyAsync(hs.data(), st, sizeof(FireState) * c.nstruct, hipMemcpyDeviceToHost, c.stream));
+    GAPX_HIP(hipStreamSynchronize(c.stream));
+    conv = 0;
+    for (const FireState &S : hs) conv += S.converged;
+    if (conv == c.nstruct) {
+      ++step;
+      break;
+    }
+  }
+  // results at the final positions
+  if ((rc = ctx_run(c, flags)) || (rc = ctx_settle(c))) return rc;
+  if (steps_done) *steps_done = step;
+  if (n_converged) *n_converged = conv;
+  return GAPX_OK;
+}
+
 static int ctx_results(Ctx &c, double *ene, double *force, double *stress6, double *eatom, bool fortran) {
   if (!c.ran) return fail(GAPX_ERR_ARG, "no evaluation has run on this context");
   int rc = ctx_settle(c);
@@ -1486,6 +1530,21 @@ int gapx_ctx_export_labels(gapx_ctx *c, gapx_mlp *m) {
   return mlp_add_labels(*mlp_of(m), c->stream, c->N, fp64 ? nullptr : c->ptr<float>("xt32"),
                         fp64 ? c->ptr<double>("xt64") : nullptr, fp64 ? nullptr : c->ptr<float>("g32"),
                         fp64 ? c->ptr<double>("g64") : nullptr, c->ptr<double>("eatom"), c->m->d_theta_inv, c->m->Dp);
+}
+
+int gapx_ctx_relax(gapx_ctx *c, int flags, int max_steps, double fmax, double maxstep, int *steps_done,
+                   int *n_converged) {
+  if (!c) return fail(GAPX_ERR_ARG, "null context");
+  collect_timing(*c);
+  return ctx_relax(*c, flags, max_steps, fmax, maxstep, steps_done, n_converged);
+}
+
+int gapx_ctx_positions(gapx_ctx *c, double *pos) {
+  if (!c || !pos || c->N <= 0) return fail(GAPX_ERR_ARG, "gapx_ctx_positions: nothing loaded");
+  GAPX_HIP(hipSetDevice(c->device));
+  GAPX_HIP(hipStreamSynchronize(c->stream));
+  GAPX_HIP(hipMemcpy(pos, c->ptr<double>("pos"), sizeof(double) * 3 * c->N, hipMemcpyDeviceToHost));
+  return GAPX_OK;
 }
 
 int gapx_ctx_autotune(gapx_ctx *c, int flags, int force) {

@@ -238,4 +238,19 @@ int mlp_add_labels(Mlp &m, hipStream_t s, int n, const float *xt32, const double
                    const double *g64, const double *eatom, const double *theta_inv, int Dp);
 hipError_t launch_f64_to_f32(hipStream_t s, size_t n, const double *a, float *b);
 
+// Batched FIRE relaxation (relax.hip): per-structure state and parameters.
+struct FireState {
+  double dt, a;        // time step, mixing parameter
+  double cv, cf, step; // this step: v' = cv v + cf F, x += step v'
+  double fmax;         // largest |F_i| at the last evaluation
+  int nsteps;          // downhill steps since the last restart
+  int started, converged, pad;
+};
+struct FireParams {
+  double dt0, dtmax, finc, fdec, astart, fa, maxstep, fmax;
+  int nmin, pad;
+};
+hipError_t launch_fire_step(hipStream_t s, int nstruct, int N, const StructParams *sp, const int *atom_struct,
+                            const double *force, double *vel, double *pos, FireState *st, const FireParams &fp);
+
 }  // namespace gapx

@@ -41,7 +41,8 @@ EXPORTS = [
     "gapx_ctx_autotune", "gapx_ctx_get_tuning", "gapx_ctx_set_tuning",
     "gapx_mlp_param_count", "gapx_mlp_create", "gapx_mlp_free", "gapx_mlp_get_params", "gapx_mlp_set_params",
     "gapx_mlp_forward", "gapx_mlp_loss_grad", "gapx_mlp_data_add", "gapx_mlp_data_clear", "gapx_mlp_data_count",
-    "gapx_mlp_train", "gapx_ctx_export_labels", "gapx_ctx_run_surrogate",
+    "gapx_mlp_train", "gapx_ctx_export_labels", "gapx_ctx_run_surrogate", "gapx_ctx_relax",
+    "gapx_ctx_positions",
 ]
 
 
@@ -98,6 +99,8 @@ def lib():
         L.gapx_mlp_train.argtypes = [vp, ci, ci, cf, cf, cf, cu, ci, vp]
         L.gapx_ctx_export_labels.argtypes = [vp, vp]
         L.gapx_ctx_run_surrogate.argtypes = [vp, vp, ci]
+        L.gapx_ctx_relax.argtypes = [vp, ci, ci, cd, cd, vp, vp]
+        L.gapx_ctx_positions.argtypes = [vp, vp]
         L.gapx_ctx_get_tuning.argtypes = [vp, ci, vp, vp, vp, vp]
         L.gapx_ctx_set_tuning.argtypes = [vp, ci, ci, ci, ci, ci]
         L.gapx_ctx_stage_count.argtypes = [vp, vp]
@@ -314,6 +317,19 @@ class Context:
         v = [ctypes.c_int() for _ in range(4)]
         _check(lib().gapx_ctx_get_tuning(self._h, int(flags), *[ctypes.byref(x) for x in v]))
         return dict(zip(("waves_fwd", "waves_bwd", "gpr_rows", "gpr_blocks"), (x.value for x in v)))
+
+    def relax(self, flags=GRAD, max_steps=500, fmax=0.05, maxstep=0.2):
+        """FIRE relaxation of every loaded structure at fixed cell, on the device."""
+        steps = ctypes.c_int()
+        conv = ctypes.c_int()
+        _check(lib().gapx_ctx_relax(self._h, int(flags), int(max_steps), float(fmax), float(maxstep),
+                                    ctypes.byref(steps), ctypes.byref(conv)))
+        return dict(steps=steps.value, converged=conv.value)
+
+    def positions(self):
+        pos = np.zeros((int(self.natoms.sum()), 3))
+        _check(lib().gapx_ctx_positions(self._h, _p(pos)))
+        return pos
 
     def export_labels(self, mlp):
         """Append the last GAP run's (xt, per-atom E, dE/dxt) to mlp's training set."""

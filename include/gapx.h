@@ -143,6 +143,18 @@ int gapx_ctx_stage_info(gapx_ctx *c, int k, const char **name, double *ms, int *
  * one eager run; runs with timing enabled are always eager.  No reference
  * counterpart (the reference is a CPU library). */
 int gapx_ctx_use_graph(gapx_ctx *c, int on);
+/* Relax every loaded structure at fixed cell with FIRE (ASE's FIRE defaults:
+ * dt 0.1, dtmax 1.0, Nmin 5, finc 1.1, fdec 0.5, astart 0.1, fa 0.99) until
+ * max |F_i| <= fmax (eV/A) for every structure or max_steps steps; a step
+ * moves each atom by at most maxstep (A, the norm over the structure).
+ * Positions are updated on the device; gapx_ctx_results then gives the
+ * energies and forces at the final positions, gapx_ctx_positions the
+ * positions (C layout).  Replaces the reference's per-structure relaxation
+ * processes (gappy/example/ASE-GAPPY/run_opt.py:108-109 with LBFGS, driven
+ * by tools/cgg2.py:100-137); the optimiser differs (FIRE, not LBFGS). */
+int gapx_ctx_relax(gapx_ctx *c, int flags, int max_steps, double fmax, double maxstep, int *steps_done,
+                   int *n_converged);
+int gapx_ctx_positions(gapx_ctx *c, double *pos);
 /* Pick launch shapes for runs with these flags (wavefronts per atom in the
  * descriptor kernels, contraction row tile and workgroup count) by timing the
  * loaded batch; results are cached per process for the shape class unless
