@@ -14,7 +14,10 @@ stress.  Prints one JSON line (see the fields below).
 The JSON line also carries "population": the batched form (512 uneven
 structures, 8-256 atoms, D = 128, M = 4000), autotuned and replayed from a
 HIP graph, in mixed precision and in FP64, with the per-structure mixed
-error against FP64 (skip with --no-population).
+error against FP64 (skip with --no-population), and "variance": the
+predictive variance of the workload structure after an FP64 evaluation
+(rocBLAS FP64 GEMMs S = xt mt^T and U = K Q plus the row kernels), timed
+apart from the headline step.
 """
 import argparse
 import json
@@ -247,6 +250,29 @@ def surrogate_bench(m, n_train, n_cand, steps, batch=4096):
         judged_mean_abs_err_eV_per_atom=float("%.4g" % np.abs(e_sur[pick] - e_gap[pick]).mean()))
 
 
+def variance_bench(m, ctx, N, reps=10):
+    """Predictive variance after an FP64 run of the loaded structure: Q is a
+    synthetic symmetric M x M matrix (timing does not depend on its values)."""
+    M, D = m.nsparse, m.des_len
+    rng = np.random.default_rng(7)
+    A = rng.normal(0, 1e-3, (M, M))
+    m.set_qmm(0.5 * (A + A.T))
+    ctx.run(gapx.GRAD)
+    ctx.results()
+    ctx.variance()  # rocBLAS handle and workspace
+    t = time.perf_counter()
+    for _ in range(reps):
+        ctx.variance()
+    sec = (time.perf_counter() - t) / reps
+    m.set_qmm(None)
+    Mp, Dp = -(-M // 64) * 64, -(-D // 32) * 32
+    flops = 2.0 * N * Mp * Dp + 2.0 * N * Mp * Mp
+    return dict(ms=round(sec * 1e3, 4), atoms_per_s=round(N / sec, 1), gemm_tflops=round(flops / sec / 1e12, 2),
+                frac_fp64_peak=round(flops / sec / 1e12 / PEAK["fp64"], 3),
+                note="wall clock per gapx_ctx_variance call (enqueue, GEMMs and row kernels, sync, copy-back); "
+                     "flops = 2 N Mp Dp + 2 N Mp^2")
+
+
 def cpu_baseline(model, s, rc, seconds):
     """The CPU oracle (sparse FP64 path, OpenMP over centre atoms) on a
     bounded sample: the first n centre atoms of the same structure."""
@@ -380,6 +406,8 @@ def main():
     roof["work_per_launch"] = rates[dom]["work"]
     roof["stages"] = {k: dict(ms=round(st[k], 4), **{kk: vv for kk, vv in rates[k].items()
                                                      if kk in ("achieved", "frac", "bound", "unit")}) for k in st}
+    var = variance_bench(m, ctx, N)
+    print("bench: variance %s" % var, file=sys.stderr, flush=True)
     ctx.close()
     surr = None
     if not args.no_surrogate and args.workload == "ternary4k":
@@ -415,6 +443,7 @@ def main():
         "cpu_baseline": cpu,
         "population": popr,
         "surrogate": surr,
+        "variance": var,
     }
     print(json.dumps(out), flush=True)
 

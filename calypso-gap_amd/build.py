@@ -24,7 +24,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["api.hip", "neighbors.hip", "descriptors.hip", "gpr.hip", "gpr_mixed.hip", "bond.hip", "mlp.hip", "relax.hip"]
+HIP_SOURCES = ["api.hip", "neighbors.hip", "descriptors.hip", "gpr.hip", "gpr_mixed.hip", "bond.hip", "mlp.hip", "relax.hip",
+               "variance.hip"]
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wno-unused-result", "-Wno-unused-value",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -57,7 +58,8 @@ def build_hip(jobs=8):
     if failed:
         raise SystemExit("HIP compilation failed")
     out = os.path.join(LIB, "libgapx.so")
-    run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", out] + objs)
+    run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", out] + objs +
+        ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     return out
 
 

@@ -329,6 +329,7 @@ struct Ctx {
   int chunk = 0;  // lines per forward pass of the generic kernels (0: 16)
   Tuning tune[2];  // launch shapes for FP64 [0] and mixed [1] runs
   Mlp *mlp = nullptr;  // set while the last run used the surrogate
+  void *blas = nullptr;  // rocBLAS handle on the stream (variance only, made on first use)
   // timing
   bool timing = false;
   hipEvent_t ev[2 * kNumStages] = {};
@@ -1059,6 +1060,43 @@ static int ctx_results(Ctx &c, double *ene, double *force, double *stress6, doub
   return GAPX_OK;
 }
 
+// Predictive variance over the last run's features (after it settled).
+// Eager: the GEMMs are library calls, kept out of captured graphs.
+static int ctx_variance(Ctx &c, double *var, double *sigma2) {
+  if (!c.ran) return fail(GAPX_ERR_ARG, "no evaluation has run on this context");
+  int rc = ctx_settle(c);
+  if (rc) return rc;
+  const Model &m = *c.m;
+  const bool fp64 = !(c.last_flags & GAPX_MIXED);
+  VarianceArgs v;
+  v.N = c.N;
+  v.Dp = m.Dp;
+  v.Mp = m.Mp;
+  v.nstruct = c.nstruct;
+  // K and U blocks of at most 512 MiB each
+  v.rows = (int)std::max<size_t>(64, std::min<size_t>(c.N, ((size_t)1 << 26) / m.Mp));
+  v.xt64 = fp64 ? c.ptr<double>("xt64") : nullptr;
+  v.xt32 = fp64 ? nullptr : c.ptr<float>("xt32");
+  if (!v.xt64 && !v.xt32) return fail(GAPX_ERR_ARG, "the last run kept no features");
+  v.mt = m.d_mt;
+  v.mn = m.d_mn;
+  v.qmm = m.d_qmm;
+  v.sp = c.ptr<StructParams>("sp");
+  if ((rc = c.get("var_sigma2", c.N, &v.sigma2)) || (rc = c.get("var_struct", c.nstruct, &v.var))) return rc;
+  if (v.qmm) {
+    if ((rc = c.get("var_xn", c.N, &v.xn)) || (rc = c.get("var_K", (size_t)v.rows * m.Mp, &v.K)) ||
+        (rc = c.get("var_U", (size_t)v.rows * m.Mp, &v.U)))
+      return rc;
+    if (!fp64 && (rc = c.get("var_xw", (size_t)c.N * m.Dp, &v.xw))) return rc;
+    if (!c.blas && (rc = variance_create_handle(&c.blas, c.stream))) return rc;
+  }
+  if ((rc = enqueue_variance(c.blas, c.stream, v))) return rc;
+  if (var) GAPX_HIP(hipMemcpyAsync(var, v.var, sizeof(double) * c.nstruct, hipMemcpyDeviceToHost, c.stream));
+  if (sigma2) GAPX_HIP(hipMemcpyAsync(sigma2, v.sigma2, sizeof(double) * c.N, hipMemcpyDeviceToHost, c.stream));
+  GAPX_HIP(hipStreamSynchronize(c.stream));
+  return GAPX_OK;
+}
+
 static int ctx_create(Model *m, Ctx **out) {
   Ctx *c = new Ctx();
   c->m = m;
@@ -1097,6 +1135,7 @@ static void ctx_free(Ctx *c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   c->invalidate_graph();
+  variance_destroy_handle(c->blas);
   for (auto &kv : c->mem)
     if (kv.second.p) hipFree(kv.second.p);
   for (auto &e : c->ev)
@@ -1332,11 +1371,38 @@ int gapx_model_header(const gapx_model *m, int *z, double *w, int *sf_type, doub
 void gapx_model_free(gapx_model *m) {
   if (!m) return;
   if (m->cached_ctx) ctx_free(m->cached_ctx);
-  if (m->dev_block) {
+  if (m->dev_block || m->d_qmm) {
     hipSetDevice(m->device);
-    hipFree(m->dev_block);
+    if (m->dev_block) hipFree(m->dev_block);
+    if (m->d_qmm) hipFree(m->d_qmm);
   }
   delete m;
+}
+
+int gapx_model_set_qmm(gapx_model *m, const double *qmm, int layout, int ld) {
+  if (!m) return fail(GAPX_ERR_ARG, "null model");
+  GAPX_HIP(hipSetDevice(m->device));
+  if (!qmm) {
+    if (m->d_qmm) GAPX_HIP(hipFree(m->d_qmm));
+    m->d_qmm = nullptr;
+    return GAPX_OK;
+  }
+  const int M = m->M, Mp = m->Mp;
+  if (layout != 0 && layout != 1) return fail(GAPX_ERR_ARG, "gapx_model_set_qmm: layout must be 0 or 1");
+  const size_t lda = (layout == 1) ? (size_t)std::max(ld, M) : (size_t)M;
+  std::vector<double> q((size_t)Mp * Mp, 0.0);
+  for (int k = 0; k < M; ++k)
+    for (int l = 0; l < M; ++l) {
+      const double v = (layout == 1) ? qmm[k + lda * l] : qmm[(size_t)k * M + l];
+      if (!std::isfinite(v)) return fail(GAPX_ERR_ARG, "gapx_model_set_qmm: non-finite entry");
+      q[(size_t)k * Mp + l] = v;
+    }
+  if (!m->d_qmm && hipMalloc((void **)&m->d_qmm, q.size() * sizeof(double)) != hipSuccess) {
+    m->d_qmm = nullptr;
+    return fail(GAPX_ERR_NOMEM, "hipMalloc of Q failed");
+  }
+  GAPX_HIP(hipMemcpy(m->d_qmm, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
+  return GAPX_OK;
 }
 
 int gapx_file_stamp(const char *path, long long *size, long long *mtime_ns) {
@@ -1365,9 +1431,12 @@ int gapx_calc(gapx_model *m, int na, const int *species, const double *lat, cons
   if (na <= 0) return fail(GAPX_ERR_ARG, "na must be positive");
   const bool fortran = (flags & GAPX_LAYOUT_F) != 0;
   if ((rc = ctx_load(*c, 1, &na, species, lat, pos, rcut, fortran))) return rc;
-  if ((rc = ctx_run(*c, flags))) return rc;
+  if ((rc = ctx_run(*c, flags & ~GAPX_VARIANCE))) return rc;
   if ((rc = ctx_results(*c, ene, force, stress6, nullptr, fortran))) return rc;
-  if (variance) *variance = 0.0;  // gap_calc.f90:206
+  if (variance) {
+    if (flags & GAPX_VARIANCE) return ctx_variance(*c, variance, nullptr);
+    *variance = 0.0;  // gap_calc.f90:206
+  }
   return GAPX_OK;
 }
 
@@ -1482,6 +1551,11 @@ int gapx_ctx_features(gapx_ctx *c, double *x, double *dedx) {
     out(dedx);
   }
   return GAPX_OK;
+}
+
+int gapx_ctx_variance(gapx_ctx *c, double *var, double *sigma2) {
+  if (!c) return fail(GAPX_ERR_ARG, "null context");
+  return ctx_variance(*c, var, sigma2);
 }
 
 int gapx_ctx_timing(gapx_ctx *c, int enable) {

@@ -77,6 +77,7 @@ struct Model {
   uint16_t *d_mt_bhi = nullptr, *d_mt_blo = nullptr;   // BF16 split [Mp][Dp]
   uint16_t *d_mtT_hi = nullptr, *d_mtT_lo = nullptr;   // FP16 split, transposed [Dp][Mp]
   uint16_t *d_mtT_bhi = nullptr, *d_mtT_blo = nullptr; // BF16 split, transposed
+  double *d_qmm = nullptr;               // [Mp][Mp] variance matrix Q, zero padded (own allocation)
   float w_scale = 1024.f;                // power of 2 applied to W before its FP16 split
   size_t dev_bytes = 0;
   struct Ctx *cached_ctx = nullptr;      // workspace for gapx_calc
@@ -252,5 +253,25 @@ struct FireParams {
 };
 hipError_t launch_fire_step(hipStream_t s, int nstruct, int N, const StructParams *sp, const int *atom_struct,
                             const double *force, double *vel, double *pos, FireState *st, const FireParams &fp);
+
+// Predictive variance (variance.hip): sigma2_i = 1 - k_i Q k_i^T over the
+// last run's standardised features, in blocks of `rows` atoms.
+struct VarianceArgs {
+  int N = 0, Dp = 0, Mp = 0, nstruct = 0;
+  int rows = 0;                   // atoms per block of K (K and U hold rows x Mp)
+  const double *xt64 = nullptr;   // [N][Dp] the run's features: FP64 run ...
+  const float *xt32 = nullptr;    // ... or mixed run
+  const double *mt = nullptr, *mn = nullptr;
+  const double *qmm = nullptr;    // [Mp][Mp] row-major, zero padded; null: Q = 0
+  const StructParams *sp = nullptr;
+  double *xw = nullptr;           // [N][Dp] FP64 copy of xt32
+  double *xn = nullptr;           // [N]
+  double *K = nullptr, *U = nullptr;
+  double *sigma2 = nullptr;       // [N]
+  double *var = nullptr;          // [nstruct]
+};
+int variance_create_handle(void **handle, hipStream_t s);
+void variance_destroy_handle(void *handle);
+int enqueue_variance(void *handle, hipStream_t s, const VarianceArgs &v);
 
 }  // namespace gapx

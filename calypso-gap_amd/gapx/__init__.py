@@ -26,6 +26,7 @@ MIXED = 2
 LAYOUT_F = 4
 REF_NEIGHBOR_CAP = 8
 BF16 = 16
+VARIANCE = 64   # gapx_calc: compute VARIANCE instead of the reference's 0
 
 ERR_NAMES = {1: "GAPX_ERR_ARG", 2: "GAPX_ERR_FILE", 3: "GAPX_ERR_MODEL", 4: "GAPX_ERR_SPECIES",
              5: "GAPX_ERR_NEIGHBORS", 6: "GAPX_ERR_HIP", 7: "GAPX_ERR_NOMEM", 8: "GAPX_ERR_CAP"}
@@ -42,7 +43,7 @@ EXPORTS = [
     "gapx_mlp_param_count", "gapx_mlp_create", "gapx_mlp_free", "gapx_mlp_get_params", "gapx_mlp_set_params",
     "gapx_mlp_forward", "gapx_mlp_loss_grad", "gapx_mlp_data_add", "gapx_mlp_data_clear", "gapx_mlp_data_count",
     "gapx_mlp_train", "gapx_ctx_export_labels", "gapx_ctx_run_surrogate", "gapx_ctx_relax",
-    "gapx_ctx_positions",
+    "gapx_ctx_positions", "gapx_model_set_qmm", "gapx_ctx_variance",
 ]
 
 
@@ -101,6 +102,8 @@ def lib():
         L.gapx_ctx_run_surrogate.argtypes = [vp, vp, ci]
         L.gapx_ctx_relax.argtypes = [vp, ci, ci, cd, cd, vp, vp]
         L.gapx_ctx_positions.argtypes = [vp, vp]
+        L.gapx_model_set_qmm.argtypes = [vp, vp, ci, ci]
+        L.gapx_ctx_variance.argtypes = [vp, vp, vp]
         L.gapx_ctx_get_tuning.argtypes = [vp, ci, vp, vp, vp, vp]
         L.gapx_ctx_set_tuning.argtypes = [vp, ci, ci, ci, ci, ci]
         L.gapx_ctx_stage_count.argtypes = [vp, vp]
@@ -224,6 +227,16 @@ class Model:
         offs = np.concatenate([[0], np.cumsum(natoms)])
         return ene, [force[offs[k]:offs[k + 1]] for k in range(len(structures))], stress
 
+    def set_qmm(self, qmm):
+        """Q of the predictive variance (nsparse x nsparse), or None for Q = 0."""
+        if qmm is None:
+            _check(lib().gapx_model_set_qmm(self._h, None, 0, 0))
+            return
+        q = np.ascontiguousarray(qmm, np.float64)
+        if q.shape != (self.nsparse, self.nsparse):
+            raise GapxError(1, "qmm must be %d x %d" % (self.nsparse, self.nsparse))
+        _check(lib().gapx_model_set_qmm(self._h, _p(q), 0, self.nsparse))
+
     def descriptors(self, species, lat, pos, rcut=6.0):
         species = np.ascontiguousarray(species, np.int32)
         x = np.zeros((len(species), self.des_len))
@@ -289,6 +302,13 @@ class Context:
         g = np.zeros((n, self.model.des_len)) if dedx else None
         _check(lib().gapx_ctx_features(self._h, _p(x), _p(g)))
         return x, g
+
+    def variance(self):
+        """Predictive variance of the last run: (per-structure mean, per-atom sigma^2)."""
+        var = np.zeros(self.nstruct)
+        s2 = np.zeros(int(self.natoms.sum()))
+        _check(lib().gapx_ctx_variance(self._h, _p(var), _p(s2)))
+        return var, s2
 
     def timing(self, enable=True):
         _check(lib().gapx_ctx_timing(self._h, int(enable)))

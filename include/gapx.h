@@ -43,6 +43,7 @@ extern "C" {
 #define GAPX_LAYOUT_F 4       /* Fortran column-major lat/pos/force             */
 #define GAPX_REF_NEIGHBOR_CAP 8 /* fail above 1000 neighbours like the reference */
 #define GAPX_BF16 16          /* with GAPX_MIXED: BF16 hi/lo instead of FP16     */
+#define GAPX_VARIANCE 64      /* gapx_calc: compute VARIANCE (else 0 as FGAP_CALC) */
 
 typedef struct gapx_model gapx_model;
 typedef struct gapx_ctx gapx_ctx;
@@ -88,11 +89,20 @@ void gapx_model_free(gapx_model *m);
  * on the file (the reference re-reads ./gap_parameters on every call). */
 int gapx_file_stamp(const char *path, long long *size, long long *mtime_ns);
 
+/* Q of the predictive variance (FGAP_CALC's QMM argument, gap_calc.f90:38,
+ * which FGAP_READ leaves zero, gap_calc.f90:355-358): nsparse x nsparse,
+ * layout 0 = row-major, 1 = Fortran QMM(nsparse,nsparse) with leading
+ * dimension ld (>= nsparse).  NULL clears it (Q = 0).  Kept on the device. */
+int gapx_model_set_qmm(gapx_model *m, const double *qmm, int layout, int ld);
+
 /* ---- one-shot evaluation ---------------------------------------------------
- * FGAP_CALC (gap_calc.f90:1-301): energy, forces, stress and variance (always
- * 0, gap_calc.f90:206) of one periodic structure.  rcut is the neighbour
- * cutoff (per-line cutoffs above it are clipped by the neighbour list, as in
- * the reference).  Uses a workspace cached inside the model handle. */
+ * FGAP_CALC (gap_calc.f90:1-301): energy, forces, stress and variance of one
+ * periodic structure.  The variance is 0 as in the reference
+ * (gap_calc.f90:206) unless flags has GAPX_VARIANCE: then it is the mean
+ * over atoms of the predictive variance (see gapx_ctx_variance).  rcut is the
+ * neighbour cutoff (per-line cutoffs above it are clipped by the neighbour
+ * list, as in the reference).  Uses a workspace cached inside the model
+ * handle. */
 int gapx_calc(gapx_model *m, int na, const int *species, const double *lat, const double *pos,
               double rcut, int flags, double *ene, double *force, double *stress6,
               double *variance);
@@ -132,6 +142,14 @@ int gapx_ctx_sync(gapx_ctx *c);
 int gapx_ctx_results(gapx_ctx *c, double *ene, double *force, double *stress6, double *eatom);
 /* Per-atom descriptors (raw, FP64 or FP32 run) and dE/dx of the last run. */
 int gapx_ctx_features(gapx_ctx *c, double *x, double *dedx);
+/* Predictive variance of the last run's structures, the formula FGAP_CALC
+ * keeps commented out (gap_calc.f90:205-210): per atom
+ *   sigma2_i = 1 - k_i Q k_i^T,  k_ik = exp(-sum_d ((x_id - MM_kd)/theta_d)^2 / 2)
+ * (the covariance of gap_calc.f90:270-287, delta = 1), Q from
+ * gapx_model_set_qmm (sigma2_i = 1 while none is set), and var[s] the mean
+ * over structure s's atoms.  FP64 (a mixed run's FP32 features are widened).
+ * var[nstruct], sigma2[sum natoms]; either may be NULL.  Synchronises. */
+int gapx_ctx_variance(gapx_ctx *c, double *var, double *sigma2);
 /* Stage timing with HIP events on the context's stream: enable, then read
  * the summed milliseconds and launch counts per stage since the last reset. */
 int gapx_ctx_timing(gapx_ctx *c, int enable);

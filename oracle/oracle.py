@@ -286,6 +286,42 @@ def neighbor_set(lat, pos, rcut):
     return rows, dist[:total]
 
 
+def covariance(model, x, block=64):
+    """k_ik = delta exp(-1/2 sum_d ((x_id - MM_kd) / theta_d)^2), delta = 1:
+    FGAP_CALC's covariance (gap_calc.f90:270-287, delta from gap_calc.f90:8),
+    from direct differences, a block of rows at a time."""
+    x = np.asarray(x, dtype=np.float64)
+    theta = np.asarray(model.theta, dtype=np.float64)
+    mt = np.asarray(model.mm, dtype=np.float64) / theta
+    out = np.empty((len(x), len(mt)))
+    for r0 in range(0, len(x), block):
+        xt = x[r0:r0 + block] / theta
+        d2 = ((xt[:, None, :] - mt[None, :, :]) ** 2).sum(axis=2)
+        out[r0:r0 + block] = np.exp(-0.5 * d2)
+    return out
+
+
+def gpr_variance(model, x, qmm, natoms=None):
+    """Predictive variance: sigma2_i = delta - k_i Q k_i^T per atom, and the
+    mean over each structure's atoms (natoms per structure; one structure if
+    None).  This is the formula FGAP_CALC keeps commented out
+    (gap_calc.f90:205-210: covf(i) = delta - dot_product(ckm(i,:),
+    matmul(qmm, ckm(i,:))), VARIANCE = sum covf / na); the reference itself
+    returns VARIANCE = 0.  qmm None means Q = 0 (FGAP_READ's INVCMM,
+    gap_calc.f90:355-358)."""
+    x = np.asarray(x, dtype=np.float64)
+    if qmm is None:
+        s2 = np.ones(len(x))
+    else:
+        K = covariance(model, x)
+        s2 = 1.0 - np.einsum("ik,ik->i", K @ np.asarray(qmm, dtype=np.float64).T, K)
+    if natoms is None:
+        natoms = [len(x)]
+    offs = np.concatenate([[0], np.cumsum(natoms)]).astype(int)
+    var = np.array([s2[offs[k]:offs[k + 1]].mean() for k in range(len(natoms))])
+    return var, s2
+
+
 # --------------------------------------------------------------------------
 # the reference itself (oracle/_ref), for writing golden vectors
 # --------------------------------------------------------------------------
