@@ -56,16 +56,19 @@ FLOPS_RAD_FWD, FLOPS_RAD_BWD = 10, 20          # per (pair, radial line)
 FLOPS_ANG_FWD, FLOPS_ANG_BWD = 27, 53          # per (triplet, angular line)
 BYTES_NEIGHBOR = 4 + 8 + 32 + 4                # per pair: index, key, (dx,dy,dz,r), reverse slot
 SINGLE_KERNEL_STAGES = ("descriptors_fwd", "gpr_contraction", "gpr_finish", "descriptors_bwd")
-STAGE_KERNEL = {  # the one kernel of each single-kernel stage, as rocprofv3 names it
-    ("descriptors_fwd", True): "k_desc_fwd_grid<float, 4, 1>",
-    ("descriptors_fwd", False): "k_desc_fwd_grid<double, 4, 1>",
-    ("descriptors_bwd", True): "k_desc_bwd<float, 1, 4>",
-    ("descriptors_bwd", False): "k_desc_bwd<double, 1, 4>",
-    ("gpr_contraction", True): "k_gpr_mixed<false, 128, 64, 2>",
-    ("gpr_contraction", False): "k_gpr_f64<128>",
-    ("gpr_finish", True): "k_gpr_finish",
-    ("gpr_finish", False): "k_gpr_finish",
-}
+
+
+def stage_kernel(stage, mixed, tune):
+    """The one kernel of a single-kernel stage under the given launch shapes,
+    as rocprofv3 names it (grid-form tables with 4 zetas, D = 128)."""
+    t = "float" if mixed else "double"
+    if stage == "descriptors_fwd":
+        return "k_desc_fwd_grid<%s, 4, %d>" % (t, tune["waves_fwd"])
+    if stage == "descriptors_bwd":
+        return "k_desc_bwd<%s, %d, 4>" % (t, tune["waves_bwd"])
+    if stage == "gpr_contraction":
+        return "k_gpr_mixed<false, 128, 64, %d>" % (tune["gpr_rows"] // 64) if mixed else "k_gpr_f64<128>"
+    return "k_gpr_finish"
 
 
 def stage_rates(st, N, D, M, pairs, triplets, m, mixed):
@@ -325,6 +328,8 @@ def probe(args):
     ctx = gapx.Context(m)
     ctx.load([s], synth.WORKLOADS[args.workload]["rc"])
     for flags in (gapx.GRAD, gapx.GRAD | gapx.MIXED):
+        # every launch shape the autotuner tries, then the evaluations at its choice
+        print("probe: tuning %s" % ctx.autotune(flags), flush=True)
         for _ in range(args.probe):
             ctx.run(flags)
         ctx.results()
@@ -347,8 +352,9 @@ def main():
                     help="training structures (the surrogate form names 200000)")
     ap.add_argument("--surrogate-steps", type=int, default=3000)
     ap.add_argument("--probe", type=int, default=0,
-                    help="profiling probe: a 1 GiB device copy (calibrates HBM byte counters), then this many "
-                         "FP64 and mixed evaluations of the workload, no timing, no JSON line")
+                    help="profiling probe: a 1 GiB device copy (calibrates HBM byte counters), then per precision "
+                         "the autotuner's candidate launch shapes and this many evaluations at its choice; "
+                         "no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this evaluator runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -397,7 +403,7 @@ def main():
     rates = stage_rates(st, N, D, M, pairs, triplets, m, mixed=True)
     # the dominant kernel: the longest of the stages that are one kernel each
     dom = max(SINGLE_KERNEL_STAGES, key=lambda k: st[k])
-    kname = STAGE_KERNEL[(dom, True)]
+    kname = stage_kernel(dom, True, tuning["mixed"])
     roof = {kk: rates[dom][kk] for kk in ("bound", "achieved", "peak", "unit", "frac")}
     roof["traffic"] = measured_traffic(args.workload, kname)
     roof["kernel"] = kname
