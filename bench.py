@@ -166,13 +166,29 @@ def population_bench(steps, warmup):
     relax = dict(steps=rel["steps"], converged=rel["converged"], seconds=round(t_rel, 3),
                  structure_steps_per_s=round(len(structs) * rel["steps"] / t_rel, 1), fmax=0.05, max_steps=200)
     print("bench: population relax %s" % relax, file=sys.stderr, flush=True)
+    # batched NVT MD of the whole population (NVTBerendsen as the reference's
+    # MD driver configures it: 1 fs, 300 K, taut 500 fs), from the starting
+    # positions with Maxwell-Boltzmann momenta; mixed precision, graph replay
+    ctx.load(structs, synth.POPULATION["rc"])
+    mass_of = {1: 1.008, 6: 12.011, 8: 15.999, 22: 47.867}
+    masses = np.concatenate([[mass_of[int(z)] for z in s["species"]] for s in structs])
+    p0 = np.random.default_rng(3).normal(size=(len(masses), 3)) * np.sqrt(masses * 300.0 * gapx.KB)[:, None]
+    md_steps = 100
+    t0 = time.perf_counter()
+    log = ctx.md(masses, md_steps, dt_fs=1.0, temperature=300.0, taut_fs=500.0, momenta=p0,
+                 flags=gapx.GRAD | gapx.MIXED)
+    t_md = time.perf_counter() - t0
+    md = dict(steps=md_steps, seconds=round(t_md, 3), atom_steps_per_s=round(natoms.sum() * md_steps / t_md, 1),
+              ms_per_step=round(t_md / md_steps * 1e3, 4), mean_T_end=round(float(log[-1, :, 2].mean()), 1),
+              note="includes the initial force evaluation and one device sync per step (list capacity check)")
+    print("bench: population md %s" % md, file=sys.stderr, flush=True)
     ctx.close()
     m.close()
     r64, rmx = res["fp64"], res["mixed"]
     err = dict(energy_meV_per_atom=float(np.abs((rmx["energy"] - r64["energy"]) / natoms).max()) * 1e3,
                force_meV_per_A=float(np.abs(rmx["forces"] - r64["forces"]).max()) * 1e3,
                stress_GPa=float(np.abs(rmx["stress"] - r64["stress"]).max()))
-    return dict(structures=len(structs), atoms=int(natoms.sum()), unit="atom E+F+S evals/s", **out, relax=relax,
+    return dict(structures=len(structs), atoms=int(natoms.sum()), unit="atom E+F+S evals/s", **out, relax=relax, md=md,
                 mixed_over_fp64=round(out["mixed"]["value"] / out["fp64"]["value"], 3),
                 mixed_error_vs_fp64_max={k: float("%.4g" % v) for k, v in err.items()})
 

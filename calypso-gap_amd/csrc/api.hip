@@ -330,6 +330,7 @@ struct Ctx {
   Tuning tune[2];  // launch shapes for FP64 [0] and mixed [1] runs
   Mlp *mlp = nullptr;  // set while the last run used the surrogate
   void *blas = nullptr;  // rocBLAS handle on the stream (variance only, made on first use)
+  bool md_ready = false;  // md_p holds the momenta of the last MD run
   // timing
   bool timing = false;
   hipEvent_t ev[2 * kNumStages] = {};
@@ -1040,6 +1041,58 @@ static int ctx_relax(Ctx &c, int flags, int max_steps, double fmax, double maxst
   return GAPX_OK;
 }
 
+// Batched NVT MD: nsteps of ASE's NVTBerendsen over the loaded batch (md.hip
+// has the step).  The forces of the starting positions come from one
+// evaluation; each step is the velocity update and drift (two kernels), an
+// E+F evaluation (graph replay when enabled), and the second half-kick.
+// Optional log rows: [nsteps + 1][nstruct][E_pot, E_kin, T].
+static constexpr double kBoltzmannEv = 8.6173303e-5;  // eV/K, CODATA 2014 (ASE 3.18.1 units)
+
+static int ctx_md(Ctx &c, int flags, int nsteps, double dt, double temperature, double taut, int fixcm,
+                  const double *masses, const double *momenta, double *log) {
+  if (c.N <= 0) return fail(GAPX_ERR_ARG, "no structures loaded");
+  if (!masses || nsteps < 0 || !(dt > 0) || !(temperature >= 0) || !std::isfinite(taut))
+    return fail(GAPX_ERR_ARG, "gapx_ctx_md: invalid argument");
+  for (int i = 0; i < c.N; ++i)
+    if (!(masses[i] > 0)) return fail(GAPX_ERR_ARG, "gapx_ctx_md: masses must be positive");
+  GAPX_HIP(hipSetDevice(c.device));
+  flags = (flags | GAPX_GRAD) & ~GAPX_LAYOUT_F;
+  double *mass, *p, *coef, *dlog = nullptr;
+  int rc;
+  if ((rc = c.get("md_mass", c.N, &mass)) || (rc = c.get("md_p", 3 * (size_t)c.N, &p)) ||
+      (rc = c.get("md_coef", 4 * (size_t)c.nstruct, &coef)))
+    return rc;
+  if (log && (rc = c.get("md_log", 3 * (size_t)c.nstruct * (nsteps + 1), &dlog))) return rc;
+  GAPX_HIP(hipMemcpyAsync(mass, masses, sizeof(double) * c.N, hipMemcpyHostToDevice, c.stream));
+  if (momenta)
+    GAPX_HIP(hipMemcpyAsync(p, momenta, sizeof(double) * 3 * c.N, hipMemcpyHostToDevice, c.stream));
+  else
+    GAPX_HIP(launch_fill_double(c.stream, p, 3 * (size_t)c.N, 0.0));
+  const MdParams mp{dt, temperature, taut, kBoltzmannEv, fixcm ? 1 : 0, 0};
+  const StructParams *d_sp = c.ptr<StructParams>("sp");
+  const int *d_as = c.ptr<int>("astruct");
+  if ((rc = ctx_run(c, flags)) || (rc = ctx_settle(c))) return rc;
+  if (dlog) GAPX_HIP(launch_md_log(c.stream, c.nstruct, d_sp, mass, p, c.ptr<double>("force"), c.ptr<double>("ene"),
+                                   mp.kB, dlog));
+  for (int step = 0; step < nsteps; ++step) {
+    double *x = c.ptr<double>("pos"), *f = c.ptr<double>("force");
+    GAPX_HIP(launch_md_prepare(c.stream, c.nstruct, d_sp, mass, p, f, mp, coef));
+    GAPX_HIP(launch_md_drift(c.stream, c.N, d_as, mass, f, coef, dt, p, x));
+    if ((rc = ctx_run(c, flags)) || (rc = ctx_settle(c))) return rc;
+    f = c.ptr<double>("force");
+    GAPX_HIP(launch_md_kick(c.stream, c.N, f, 0.5 * dt, p));
+    if (dlog)
+      GAPX_HIP(launch_md_log(c.stream, c.nstruct, d_sp, mass, p, f, c.ptr<double>("ene"), mp.kB,
+                             dlog + 3 * (size_t)c.nstruct * (step + 1)));
+  }
+  if (log)
+    GAPX_HIP(hipMemcpyAsync(log, dlog, sizeof(double) * 3 * c.nstruct * (nsteps + 1), hipMemcpyDeviceToHost,
+                            c.stream));
+  GAPX_HIP(hipStreamSynchronize(c.stream));
+  c.md_ready = true;
+  return GAPX_OK;
+}
+
 static int ctx_results(Ctx &c, double *ene, double *force, double *stress6, double *eatom, bool fortran) {
   if (!c.ran) return fail(GAPX_ERR_ARG, "no evaluation has run on this context");
   int rc = ctx_settle(c);
@@ -1550,6 +1603,20 @@ int gapx_ctx_features(gapx_ctx *c, double *x, double *dedx) {
     }
     out(dedx);
   }
+  return GAPX_OK;
+}
+
+int gapx_ctx_md(gapx_ctx *c, int flags, int nsteps, double dt, double temperature, double taut, int fixcm,
+                const double *masses, const double *momenta, double *log) {
+  if (!c) return fail(GAPX_ERR_ARG, "null context");
+  return ctx_md(*c, flags, nsteps, dt, temperature, taut, fixcm, masses, momenta, log);
+}
+
+int gapx_ctx_momenta(gapx_ctx *c, double *p) {
+  if (!c || !p || !c->md_ready) return fail(GAPX_ERR_ARG, "gapx_ctx_momenta: no MD run on this context");
+  GAPX_HIP(hipSetDevice(c->device));
+  GAPX_HIP(hipMemcpyAsync(p, c->ptr<double>("md_p"), sizeof(double) * 3 * c->N, hipMemcpyDeviceToHost, c->stream));
+  GAPX_HIP(hipStreamSynchronize(c->stream));
   return GAPX_OK;
 }
 

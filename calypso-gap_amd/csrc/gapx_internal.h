@@ -254,6 +254,20 @@ struct FireParams {
 hipError_t launch_fire_step(hipStream_t s, int nstruct, int N, const StructParams *sp, const int *atom_struct,
                             const double *force, double *vel, double *pos, FireState *st, const FireParams &fp);
 
+// Batched NVT MD (md.hip): ASE NVTBerendsen steps in ASE units.
+struct MdParams {
+  double dt, temperature, taut, kB;
+  int fixcm, pad;
+};
+// coef[4 s + 0] = velocity scale of structure s, [1..3] = momentum shift per atom
+hipError_t launch_md_prepare(hipStream_t s, int nstruct, const StructParams *sp, const double *mass, const double *p,
+                             const double *f, const MdParams &mp, double *coef);
+hipError_t launch_md_drift(hipStream_t s, int N, const int *atom_struct, const double *mass, const double *f,
+                           const double *coef, double dt, double *p, double *x);
+hipError_t launch_md_kick(hipStream_t s, int N, const double *f, double half_dt, double *p);
+hipError_t launch_md_log(hipStream_t s, int nstruct, const StructParams *sp, const double *mass, const double *p,
+                         const double *f, const double *ene, double kB, double *log);
+
 // Predictive variance (variance.hip): sigma2_i = 1 - k_i Q k_i^T over the
 // last run's standardised features, in blocks of `rows` atoms.
 struct VarianceArgs {

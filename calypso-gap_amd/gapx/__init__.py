@@ -27,6 +27,8 @@ LAYOUT_F = 4
 REF_NEIGHBOR_CAP = 8
 BF16 = 16
 VARIANCE = 64   # gapx_calc: compute VARIANCE instead of the reference's 0
+FS = 0.09822694788464063   # 1 fs in ASE time units (CODATA 2014, ASE 3.18.1)
+KB = 8.6173303e-5          # eV/K, the same units
 
 ERR_NAMES = {1: "GAPX_ERR_ARG", 2: "GAPX_ERR_FILE", 3: "GAPX_ERR_MODEL", 4: "GAPX_ERR_SPECIES",
              5: "GAPX_ERR_NEIGHBORS", 6: "GAPX_ERR_HIP", 7: "GAPX_ERR_NOMEM", 8: "GAPX_ERR_CAP"}
@@ -44,6 +46,7 @@ EXPORTS = [
     "gapx_mlp_forward", "gapx_mlp_loss_grad", "gapx_mlp_data_add", "gapx_mlp_data_clear", "gapx_mlp_data_count",
     "gapx_mlp_train", "gapx_ctx_export_labels", "gapx_ctx_run_surrogate", "gapx_ctx_relax",
     "gapx_ctx_positions", "gapx_model_set_qmm", "gapx_ctx_variance",
+    "gapx_ctx_md", "gapx_ctx_momenta",
 ]
 
 
@@ -104,6 +107,8 @@ def lib():
         L.gapx_ctx_positions.argtypes = [vp, vp]
         L.gapx_model_set_qmm.argtypes = [vp, vp, ci, ci]
         L.gapx_ctx_variance.argtypes = [vp, vp, vp]
+        L.gapx_ctx_md.argtypes = [vp, ci, ci, cd, cd, cd, ci, vp, vp, vp]
+        L.gapx_ctx_momenta.argtypes = [vp, vp]
         L.gapx_ctx_get_tuning.argtypes = [vp, ci, vp, vp, vp, vp]
         L.gapx_ctx_set_tuning.argtypes = [vp, ci, ci, ci, ci, ci]
         L.gapx_ctx_stage_count.argtypes = [vp, vp]
@@ -302,6 +307,26 @@ class Context:
         g = np.zeros((n, self.model.des_len)) if dedx else None
         _check(lib().gapx_ctx_features(self._h, _p(x), _p(g)))
         return x, g
+
+    def md(self, masses, steps, dt_fs=1.0, temperature=300.0, taut_fs=500.0, fixcm=True, momenta=None,
+           flags=GRAD, log=True):
+        """Batched NVT MD (ASE NVTBerendsen steps) of the loaded batch on the device.
+        masses per atom in amu; returns the log array [steps+1, nstruct, (E_pot, E_kin, T)] or None."""
+        n = int(self.natoms.sum())
+        m = np.ascontiguousarray(masses, np.float64)
+        if m.shape != (n,):
+            raise GapxError(1, "masses must have one entry per atom")
+        p = None if momenta is None else np.ascontiguousarray(momenta, np.float64).reshape(n, 3)
+        out = np.zeros((steps + 1, self.nstruct, 3)) if log else None
+        taut = taut_fs * FS if taut_fs and taut_fs > 0 else 0.0
+        _check(lib().gapx_ctx_md(self._h, int(flags), int(steps), float(dt_fs * FS), float(temperature), float(taut),
+                                 int(bool(fixcm)), _p(m), _p(p), _p(out)))
+        return out
+
+    def momenta(self):
+        p = np.zeros((int(self.natoms.sum()), 3))
+        _check(lib().gapx_ctx_momenta(self._h, _p(p)))
+        return p
 
     def variance(self):
         """Predictive variance of the last run: (per-structure mean, per-atom sigma^2)."""

@@ -173,6 +173,23 @@ int gapx_ctx_use_graph(gapx_ctx *c, int on);
 int gapx_ctx_relax(gapx_ctx *c, int flags, int max_steps, double fmax, double maxstep, int *steps_done,
                    int *n_converged);
 int gapx_ctx_positions(gapx_ctx *c, double *pos);
+
+/* Batched NVT molecular dynamics on the device: nsteps of ASE's NVTBerendsen
+ * step (ASE 3.18.1), which the reference's MD driver runs for one structure
+ * per process (gappy/example/ASE-GAPPY/run_md.py:10-21, timestep POTIM fs,
+ * temperature TEBEG K, taut 500 fs), for every loaded structure at once:
+ * Berendsen scale s = sqrt(1 + (T0/T - 1) dt/taut) clipped to [0.9, 1.1]
+ * (taut <= 0: no scaling), p <- s p + dt/2 F, total momentum removed when
+ * fixcm, x <- x + dt p/m, forces at x, p <- p + dt/2 F.  ASE units: dt in
+ * ASE time units (1 fs = 0.09822694788464063), masses[N] in amu, momenta[3N]
+ * in amu A per time unit (NULL: start at rest), temperature in K with
+ * kB = 8.6173303e-5 eV/K.  log (may be NULL): [nsteps+1][nstruct][3] rows of
+ * (E_pot eV, E_kin eV, T K), row 0 at the start.  Positions and momenta stay
+ * on the device (gapx_ctx_positions, gapx_ctx_momenta); gapx_ctx_results
+ * gives energies and forces at the final positions. */
+int gapx_ctx_md(gapx_ctx *c, int flags, int nsteps, double dt, double temperature, double taut, int fixcm,
+                const double *masses, const double *momenta, double *log);
+int gapx_ctx_momenta(gapx_ctx *c, double *p);
 /* Pick launch shapes for runs with these flags (wavefronts per atom in the
  * descriptor kernels, contraction row tile and workgroup count) by timing the
  * loaded batch; results are cached per process for the shape class unless
