@@ -1,6 +1,7 @@
 // gapx host code: model upload, device workspaces, the evaluation pipeline
 // and the C ABI declared in include/gapx.h.
 #include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <cerrno>
@@ -9,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <map>
 #include <mutex>
 #include <sstream>
@@ -1306,6 +1308,158 @@ static int finish_model(Model *m) {
   return upload_model(*m);
 }
 
+// ---------------------------------------------------------------------------
+// model writer (gap_parameters text) and binary model files
+// ---------------------------------------------------------------------------
+
+// Text in the layout the reference reads (gap_calc.f90:328-362,
+// wacsf.f90:46-55): species table, SF lines (a 4th column zeta only when some
+// zeta differs from 1; the reference's list-directed read ignores it), sizes,
+// three blank records, theta, one record per sparse point, coefficients.
+// Numbers carry 17 significant digits, so a read gives the same doubles back.
+static int write_model_text(const Model &m, const char *path) {
+  std::FILE *f = std::fopen(path, "w");
+  if (!f) return fail(GAPX_ERR_FILE, std::string("cannot write ") + path);
+  bool any_zeta = false;
+  for (double z : m.sf_zeta) any_zeta |= (z != 1.0);
+  std::fprintf(f, "%d\n", m.nspecies);
+  for (int k = 0; k < m.nspecies; ++k) std::fprintf(f, "%d %.17g\n", m.z[k], m.w[k]);
+  std::fprintf(f, "%d\n", m.nsf);
+  for (int s = 0; s < m.nsf; ++s) {
+    std::fprintf(f, "%d %.17g %.17g", m.sf_type[s], m.sf_alpha[s], m.sf_cut[s]);
+    if (any_zeta) std::fprintf(f, " %.17g", m.sf_zeta[s]);
+    std::fprintf(f, "\n");
+  }
+  std::fprintf(f, "%d %d\n\n\n\n", m.M, m.D);
+  auto row = [&](const double *a, int n) {
+    for (int d = 0; d < n; ++d) std::fprintf(f, d ? " %.17g" : "%.17g", a[d]);
+    std::fprintf(f, "\n");
+  };
+  row(m.theta.data(), m.D);
+  for (int k = 0; k < m.M; ++k) row(m.mm.data() + (size_t)k * m.D, m.D);
+  row(m.coeff.data(), m.M);
+  const bool ok = std::ferror(f) == 0;
+  if (std::fclose(f) != 0 || !ok) return fail(GAPX_ERR_FILE, std::string("write error on ") + path);
+  return GAPX_OK;
+}
+
+// Binary model file: "GAPXMDL1", u32 version 1, u32 0x01020304 (byte order),
+// i64 size and mtime (ns) of the text file it was made from (0 if none),
+// i32 nspecies nsf D M, then z[nspecies] (i32), w, sf_type (i32), sf_alpha,
+// sf_cut, sf_zeta, theta[D], mm[M][D] row-major, coeff[M] (f64), and a 64-bit
+// FNV-1a of every byte after the magic.
+static const char kModelMagic[8] = {'G', 'A', 'P', 'X', 'M', 'D', 'L', '1'};
+
+struct ByteSink {
+  std::vector<unsigned char> b;
+  template <typename T>
+  void put(const T *p, size_t n) {
+    const unsigned char *c = (const unsigned char *)p;
+    b.insert(b.end(), c, c + n * sizeof(T));
+  }
+  template <typename T>
+  void put1(T v) { put(&v, 1); }
+};
+
+struct ByteSource {
+  const unsigned char *p, *end;
+  template <typename T>
+  bool get(T *out, size_t n) {
+    const size_t bytes = n * sizeof(T);
+    if ((size_t)(end - p) < bytes) return false;
+    std::memcpy(out, p, bytes);
+    p += bytes;
+    return true;
+  }
+};
+
+static uint64_t fnv1a(const unsigned char *p, size_t n) {
+  uint64_t h = 14695981039346656037ULL;
+  for (size_t k = 0; k < n; ++k) {
+    h ^= p[k];
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
+static int save_model_binary(const Model &m, const char *path, long long src_size, long long src_mtime) {
+  ByteSink s;
+  s.put1<uint32_t>(1);
+  s.put1<uint32_t>(0x01020304u);
+  s.put1<int64_t>(src_size);
+  s.put1<int64_t>(src_mtime);
+  s.put1<int32_t>(m.nspecies);
+  s.put1<int32_t>(m.nsf);
+  s.put1<int32_t>(m.D);
+  s.put1<int32_t>(m.M);
+  s.put(m.z.data(), m.z.size());
+  s.put(m.w.data(), m.w.size());
+  s.put(m.sf_type.data(), m.sf_type.size());
+  s.put(m.sf_alpha.data(), m.sf_alpha.size());
+  s.put(m.sf_cut.data(), m.sf_cut.size());
+  s.put(m.sf_zeta.data(), m.sf_zeta.size());
+  s.put(m.theta.data(), m.theta.size());
+  s.put(m.mm.data(), m.mm.size());
+  s.put(m.coeff.data(), m.coeff.size());
+  const uint64_t h = fnv1a(s.b.data(), s.b.size());
+  // written to a temporary name first, then renamed over the target, so a
+  // reader never sees a partial file
+  const std::string tmp = std::string(path) + ".tmp" + std::to_string((long long)getpid());
+  std::FILE *f = std::fopen(tmp.c_str(), "wb");
+  if (!f) return fail(GAPX_ERR_FILE, std::string("cannot write ") + tmp);
+  bool ok = std::fwrite(kModelMagic, 1, 8, f) == 8 && std::fwrite(s.b.data(), 1, s.b.size(), f) == s.b.size() &&
+            std::fwrite(&h, sizeof h, 1, f) == 1;
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok || std::rename(tmp.c_str(), path) != 0) {
+    std::remove(tmp.c_str());
+    return fail(GAPX_ERR_FILE, std::string("write error on ") + path);
+  }
+  return GAPX_OK;
+}
+
+// Reads a binary model into m (host arrays only).  src_size/src_mtime get the
+// stamp of the text file it records.
+static int load_model_binary(const char *path, Model &m, long long *src_size, long long *src_mtime) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return fail(GAPX_ERR_FILE, std::string(path) + " does not exist");
+  std::vector<unsigned char> all((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  if (all.size() < 8 + 8 || std::memcmp(all.data(), kModelMagic, 8) != 0)
+    return fail(GAPX_ERR_FILE, std::string(path) + ": not a gapx model file");
+  const size_t body = all.size() - 8 - 8;
+  uint64_t h;
+  std::memcpy(&h, all.data() + 8 + body, 8);
+  if (fnv1a(all.data() + 8, body) != h) return fail(GAPX_ERR_FILE, std::string(path) + ": checksum mismatch");
+  ByteSource r{all.data() + 8, all.data() + 8 + body};
+  uint32_t ver = 0, order = 0;
+  int64_t sz = 0, mt = 0;
+  int32_t ns = 0, nsf = 0, D = 0, M = 0;
+  if (!r.get(&ver, 1) || !r.get(&order, 1) || ver != 1 || order != 0x01020304u)
+    return fail(GAPX_ERR_FILE, std::string(path) + ": unsupported version or byte order");
+  if (!r.get(&sz, 1) || !r.get(&mt, 1) || !r.get(&ns, 1) || !r.get(&nsf, 1) || !r.get(&D, 1) || !r.get(&M, 1) ||
+      ns <= 0 || ns > kMaxSpeciesZ || nsf <= 0 || D <= 0 || M <= 0 || D > 4096 || M > (1 << 24))
+    return fail(GAPX_ERR_FILE, std::string(path) + ": bad header");
+  m.nspecies = ns;
+  m.nsf = nsf;
+  m.D = D;
+  m.M = M;
+  m.z.resize(ns);
+  m.w.resize(ns);
+  m.sf_type.resize(nsf);
+  m.sf_alpha.resize(nsf);
+  m.sf_cut.resize(nsf);
+  m.sf_zeta.resize(nsf);
+  m.theta.resize(D);
+  m.mm.resize((size_t)M * D);
+  m.coeff.resize(M);
+  if (!r.get(m.z.data(), ns) || !r.get(m.w.data(), ns) || !r.get(m.sf_type.data(), nsf) ||
+      !r.get(m.sf_alpha.data(), nsf) || !r.get(m.sf_cut.data(), nsf) || !r.get(m.sf_zeta.data(), nsf) ||
+      !r.get(m.theta.data(), D) || !r.get(m.mm.data(), m.mm.size()) || !r.get(m.coeff.data(), M) || r.p != r.end)
+    return fail(GAPX_ERR_FILE, std::string(path) + ": truncated or oversized");
+  if (src_size) *src_size = sz;
+  if (src_mtime) *src_mtime = mt;
+  return GAPX_OK;
+}
+
 }  // namespace gapx
 
 // ===========================================================================
@@ -1419,6 +1573,61 @@ int gapx_model_header(const gapx_model *m, int *z, double *w, int *sf_type, doub
   if (sf_cut) std::copy(m->sf_cut.begin(), m->sf_cut.end(), sf_cut);
   if (sf_zeta) std::copy(m->sf_zeta.begin(), m->sf_zeta.end(), sf_zeta);
   return GAPX_OK;
+}
+
+int gapx_model_write(const gapx_model *m, const char *path) {
+  if (!m || !path) return fail(GAPX_ERR_ARG, "gapx_model_write: null argument");
+  return write_model_text(*m, path);
+}
+
+int gapx_model_save(const gapx_model *m, const char *path) {
+  if (!m || !path) return fail(GAPX_ERR_ARG, "gapx_model_save: null argument");
+  return save_model_binary(*m, path, 0, 0);
+}
+
+static int model_from_host(gapx_model *m, gapx_model **out) {
+  int rc = finish_model(m);
+  if (rc) {
+    if (m->dev_block) hipFree(m->dev_block);
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return GAPX_OK;
+}
+
+int gapx_model_load(const char *path, gapx_model **out) {
+  if (!path || !out) return fail(GAPX_ERR_ARG, "gapx_model_load: null argument");
+  gapx_model *m = new gapx_model();
+  int rc = load_model_binary(path, *m, nullptr, nullptr);
+  if (rc) {
+    delete m;
+    return rc;
+  }
+  return model_from_host(m, out);
+}
+
+int gapx_model_read_cached(const char *path, const char *cache_path, gapx_model **out) {
+  if (!path || !out) return fail(GAPX_ERR_ARG, "gapx_model_read_cached: null argument");
+  long long size = 0, mtime = 0;
+  int rc = gapx_file_stamp(path, &size, &mtime);
+  if (rc) return rc;
+  const std::string cache = cache_path ? std::string(cache_path) : std::string(path) + ".gapxbin";
+  gapx_model *m = new gapx_model();
+  long long cs = -1, cm = -1;
+  if (load_model_binary(cache.c_str(), *m, &cs, &cm) == GAPX_OK && cs == size && cm == mtime)
+    return model_from_host(m, out);
+  // stale, missing or damaged cache: parse the text and refresh the cache
+  // (a cache that cannot be written is not an error)
+  delete m;
+  m = new gapx_model();
+  if ((rc = read_model_file(path, *m))) {
+    delete m;
+    return rc;
+  }
+  const std::string keep = g_err;
+  if (save_model_binary(*m, cache.c_str(), size, mtime) != GAPX_OK) g_err = keep;
+  return model_from_host(m, out);
 }
 
 void gapx_model_free(gapx_model *m) {

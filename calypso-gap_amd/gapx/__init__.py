@@ -46,7 +46,8 @@ EXPORTS = [
     "gapx_mlp_forward", "gapx_mlp_loss_grad", "gapx_mlp_data_add", "gapx_mlp_data_clear", "gapx_mlp_data_count",
     "gapx_mlp_train", "gapx_ctx_export_labels", "gapx_ctx_run_surrogate", "gapx_ctx_relax",
     "gapx_ctx_positions", "gapx_model_set_qmm", "gapx_ctx_variance",
-    "gapx_ctx_md", "gapx_ctx_momenta",
+    "gapx_ctx_md", "gapx_ctx_momenta", "gapx_model_write", "gapx_model_save", "gapx_model_load",
+    "gapx_model_read_cached",
 ]
 
 
@@ -107,6 +108,10 @@ def lib():
         L.gapx_ctx_positions.argtypes = [vp, vp]
         L.gapx_model_set_qmm.argtypes = [vp, vp, ci, ci]
         L.gapx_ctx_variance.argtypes = [vp, vp, vp]
+        L.gapx_model_write.argtypes = [vp, ctypes.c_char_p]
+        L.gapx_model_save.argtypes = [vp, ctypes.c_char_p]
+        L.gapx_model_load.argtypes = [ctypes.c_char_p, vp]
+        L.gapx_model_read_cached.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp]
         L.gapx_ctx_md.argtypes = [vp, ci, ci, cd, cd, cd, ci, vp, vp, vp]
         L.gapx_ctx_momenta.argtypes = [vp, vp]
         L.gapx_ctx_get_tuning.argtypes = [vp, ci, vp, vp, vp, vp]
@@ -158,10 +163,30 @@ class Model:
         self.nspecies, self.nsf, self.des_len, self.nsparse = ns.value, nsf.value, D.value, M.value
 
     @classmethod
-    def read(cls, path):
+    def read(cls, path, cache=False):
+        """Parse a gap_parameters file; cache=True (or a path) goes through a
+        binary copy keyed on the file's size and mtime (gapx_model_read_cached)."""
         h = ctypes.c_void_p()
-        _check(lib().gapx_model_read(os.fsencode(path), ctypes.byref(h)))
+        if cache:
+            cp = None if cache is True else os.fsencode(cache)
+            _check(lib().gapx_model_read_cached(os.fsencode(path), cp, ctypes.byref(h)))
+        else:
+            _check(lib().gapx_model_read(os.fsencode(path), ctypes.byref(h)))
         return cls(h.value)
+
+    @classmethod
+    def load(cls, path):
+        """A binary model file written by save()."""
+        h = ctypes.c_void_p()
+        _check(lib().gapx_model_load(os.fsencode(path), ctypes.byref(h)))
+        return cls(h.value)
+
+    def save(self, path):
+        _check(lib().gapx_model_save(self._h, os.fsencode(path)))
+
+    def write(self, path):
+        """The model as a gap_parameters text file."""
+        _check(lib().gapx_model_write(self._h, os.fsencode(path)))
 
     @classmethod
     def from_arrays(cls, z, w, sf_type, sf_alpha, sf_cut, theta, mm, coeff, sf_zeta=None):

@@ -85,6 +85,23 @@ int gapx_model_header(const gapx_model *m, int *z, double *w, int *sf_type, doub
 
 void gapx_model_free(gapx_model *m);
 
+/* Model I/O beyond FGAP_READ (gap_calc.f90:303-364 reads the text once per
+ * call and keeps no cache).
+ * gapx_model_write: the model as a gap_parameters text file in the layout
+ * FGAP_READ and CAR2ACSF read (an SF line gets a 4th column zeta only when
+ * some zeta differs from 1), 17 significant digits, so reading it back gives
+ * the same doubles.
+ * gapx_model_save / gapx_model_load: the same content as one binary file with
+ * a checksum; no text parsing on load.
+ * gapx_model_read_cached: gapx_model_read of a text file through a binary
+ * copy (cache_path, or path + ".gapxbin" when NULL) that records the text
+ * file's size and mtime; a missing, stale or damaged copy is rebuilt from the
+ * text (failure to write it is not an error). */
+int gapx_model_write(const gapx_model *m, const char *path);
+int gapx_model_save(const gapx_model *m, const char *path);
+int gapx_model_load(const char *path, gapx_model **out);
+int gapx_model_read_cached(const char *path, const char *cache_path, gapx_model **out);
+
 /* Size and modification time of a file, for callers that cache models keyed
  * on the file (the reference re-reads ./gap_parameters on every call). */
 int gapx_file_stamp(const char *path, long long *size, long long *mtime_ns);
