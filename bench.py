@@ -311,8 +311,15 @@ def cpu_baseline(model, s, rc, seconds):
         if t > seconds / 4 or n >= len(s["species"]):
             break
         n = min(len(s["species"]), int(n * max(2.0, seconds / 4 / max(t, 1e-3))))
+    # the same oracle on one thread, over the first 256 centre atoms
+    n1 = min(256, len(s["species"]))
+    t0 = time.perf_counter()
+    oracle.gap_calc_range(om, s["species"], s["lat"], s["pos"], rc, 0, n1, nthreads=1)
+    t1 = time.perf_counter() - t0
     return dict(value=n / t, unit="atom E+F+S evals/s", cores=cores, kind="port",
-                sample="first %d of %d centre atoms of the same structure, %.1f s" % (n, len(s["species"]), t))
+                sample="first %d of %d centre atoms of the same structure, %.1f s" % (n, len(s["species"]), t),
+                value_1_thread=round(n1 / t1, 1),
+                sample_1_thread="first %d centre atoms on one thread, %.2f s" % (n1, t1))
 
 
 def device_copy(nbytes):
