@@ -641,15 +641,11 @@ static int enqueue_forces(Ctx &c, Bufs &b, bool fp64, bool grad, const double *e
                                     b.atom_vir, waves_bwd, c.KP));
     stage_end(c, kDescBwd);
   }
+  // forces, energies, stress, and the list statistics straight into pinned
+  // host memory (read by ctx_settle), in one launch
   stage_begin(c, kForces);
-  if (grad)
-    GAPX_HIP(launch_gather_forces(c.stream, c.N, nb, b.slot_f, b.force));
-  else
-    GAPX_HIP(launch_fill_double(c.stream, b.force, 3 * (size_t)c.N, 0.0));
-  // energies, stress, and the list statistics straight into pinned host
-  // memory (read by ctx_settle)
-  GAPX_HIP(launch_struct_reduce(c.stream, c.nstruct, d_sp, eatom, b.atom_vir, grad ? 1 : 0, b.ene, b.stress, c.N,
-                                nb, b.tb, c.host_stats_dev));
+  GAPX_HIP(launch_forces_reduce(c.stream, c.N, grad ? 1 : 0, nb, b.slot_f, b.force, c.nstruct, d_sp, eatom,
+                                b.atom_vir, b.ene, b.stress, b.tb, c.host_stats_dev));
   stage_end(c, kForces);
   return GAPX_OK;
 }

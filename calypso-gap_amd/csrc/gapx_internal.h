@@ -123,7 +123,7 @@ struct NeighborBufs {
   double *spos;         // [N*4] wrapped positions in bin order (x y z, pad)
   int *sinfo;           // [N*4] in bin order: atom index, its three cell shifts
   int *overflow;        // = total + 2
-  int *total;           // [8] see launch_struct_reduce; [3] is the descriptor kernels' guard
+  int *total;           // [8] see launch_forces_reduce; [3] is the descriptor kernels' guard
 };
 
 hipError_t launch_wrap_bin(hipStream_t s, int N, const double *pos, const int *atom_struct,
@@ -192,15 +192,16 @@ hipError_t launch_descriptors_bwd(hipStream_t s, bool fp64, int N, const int *sp
                                   double *slot_f, double *atom_vir, int waves, int KP);
 // Largest per-atom neighbour count the descriptor kernels' LDS layout holds.
 int max_neighbors_for_lds(bool fp64, int G, int Dp);
-hipError_t launch_gather_forces(hipStream_t s, int N, const NeighborBufs &nb, const double *slot_f,
-                                double *force);
-// Per-structure energy and stress, plus (one more workgroup) the list
+
+// After the backward pass, one launch: per-atom forces (zeros without grad),
+// per-structure energy and stress, and (one more workgroup) the list
 // statistics: out[0..7] = kept pairs, largest raw pair count, atoms above
 // kcap, descriptor guard (nb.total[3]), kept triplets, largest raw triplet
 // count, atoms above tcap, 0.  stats_out may be pinned host memory.
-hipError_t launch_struct_reduce(hipStream_t s, int nstruct, const StructParams *sp, const double *eatom,
-                                const double *atom_vir, int grad, double *ene, double *stress6, int N,
-                                const NeighborBufs &nb, const TripletBufs &tb, int *stats_out);
+hipError_t launch_forces_reduce(hipStream_t s, int N, int grad, const NeighborBufs &nb, const double *slot_f,
+                                double *force, int nstruct, const StructParams *sp, const double *eatom,
+                                const double *atom_vir, double *ene, double *stress6, const TripletBufs &tb,
+                                int *stats_out);
 
 // Contraction: E_i = sum_k exp(-|xt_i - mt_k|^2 / 2) c_k and
 // g_i = -(xt_i E_i - sum_k W_ik mt_k) * theta_inv.

@@ -345,13 +345,15 @@ __global__ __launch_bounds__(64) void k_reverse(int N, NeighborBufs nb) {
 
 // Fold the per-atom counts into the run's statistics (one workgroup):
 // per-thread values, wavefront shuffles, then the 16 wavefront results.
+template <int NT>
 __device__ void list_stats(int N, int kcap, const int *__restrict__ nb_count, const int *__restrict__ nb_raw,
                            int tcap, const int *__restrict__ t_count, const int *__restrict__ t_raw,
                            const int *__restrict__ guard, int *__restrict__ out) {
-  __shared__ long long red[16][6];
+  constexpr int NW = NT / 64;
+  __shared__ long long red[NW][6];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   long long v[6] = {0, 0, 0, 0, 0, 0};
-  for (int a = t; a < N; a += 1024) {
+  for (int a = t; a < N; a += NT) {
     const int r = nb_raw[a];
     v[0] += nb_count[a];
     v[1] = max(v[1], (long long)r);
@@ -375,47 +377,11 @@ __device__ void list_stats(int N, int kcap, const int *__restrict__ nb_count, co
   __syncthreads();
   if (t < 6) {
     long long a = 0;
-    for (int w = 0; w < 16; ++w) a = (t == 1 || t == 4) ? max(a, red[w][t]) : a + red[w][t];
+    for (int w = 0; w < NW; ++w) a = (t == 1 || t == 4) ? max(a, red[w][t]) : a + red[w][t];
     out[t < 3 ? t : t + 1] = (int)a;
   }
   if (t == 6) out[3] = guard[0];
   if (t == 7) out[7] = 0;
-}
-
-// F_m = sum_s f^(m)_s - sum_s f^(j_s)_{rev(s)}: slot_f[s] holds dE_i/dR of the
-// atom in slot s of centre i, so the force on m collects minus its own
-// centre's derivative (the -sum over its slots) and minus what every centre
-// that sees m puts on it (the reverse slots).  One wavefront per atom; the
-// lane sums are combined in a fixed order (deterministic).
-__global__ __launch_bounds__(64) void k_gather(int N, NeighborBufs nb, const double *__restrict__ slot_f,
-                                               double *__restrict__ force) {
-  const int m = blockIdx.x, lane = threadIdx.x;
-  if (m >= N) return;
-  const int K = nb.nb_count[m];
-  const size_t base = (size_t)m * nb.kcap;
-  double f0 = 0, f1 = 0, f2 = 0;
-  for (int a = lane; a < K; a += 64) {
-    const size_t s = base + a;
-    f0 += slot_f[3 * s];
-    f1 += slot_f[3 * s + 1];
-    f2 += slot_f[3 * s + 2];
-    const int r = nb.nb_rev[s];
-    if (r >= 0) {
-      f0 -= slot_f[3 * (size_t)r];
-      f1 -= slot_f[3 * (size_t)r + 1];
-      f2 -= slot_f[3 * (size_t)r + 2];
-    }
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    f0 += __shfl_xor(f0, o, 64);
-    f1 += __shfl_xor(f1, o, 64);
-    f2 += __shfl_xor(f2, o, 64);
-  }
-  if (lane == 0) {
-    force[3 * m] = f0;
-    force[3 * m + 1] = f1;
-    force[3 * m + 2] = f2;
-  }
 }
 
 template <typename V>
@@ -424,25 +390,63 @@ __device__ __forceinline__ V wave_reduce_sum(V v) {
   return v;
 }
 
-// Per-structure energy and stress: per-thread sums, wavefront shuffles, then
-// the 16 wavefront totals in order (deterministic).
-// Workgroups 0 .. nstruct-1: one structure each; workgroup nstruct: the list
-// statistics.
-__global__ __launch_bounds__(1024) void k_struct_reduce(int nstruct, const StructParams *__restrict__ sp,
-                                                        const double *__restrict__ eatom,
-                                                        const double *__restrict__ vir, int grad,
-                                                        double *__restrict__ ene, double *__restrict__ stress6,
-                                                        int N, NeighborBufs nb, TripletBufs tb, int *stats_out) {
-  const int s = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+// Everything after the backward pass in one launch of 256-thread blocks:
+//   blocks [0, G)            forces, one wavefront per atom (four atoms per
+//                            block): F_m = sum of own slots - sum of reverse
+//                            slots in slot order (zeros without grad);
+//   blocks [G, G + nstruct)  energy and virial of one structure, stress in GPa
+//                            (gap_calc.f90:152-154, 189-203);
+//   block G + nstruct        list statistics (see launch_forces_reduce).
+__global__ __launch_bounds__(256) void k_forces_reduce(int N, int G, int nstruct, NeighborBufs nb,
+                                                       const double *__restrict__ slot_f, double *__restrict__ force,
+                                                       const StructParams *__restrict__ sp,
+                                                       const double *__restrict__ eatom,
+                                                       const double *__restrict__ vir, int grad,
+                                                       double *__restrict__ ene, double *__restrict__ stress6,
+                                                       TripletBufs tb, int *stats_out) {
+  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (b < G) {
+    const int m = 4 * b + wv;
+    if (m >= N) return;
+    double f0 = 0, f1 = 0, f2 = 0;
+    if (grad) {
+      const int K = nb.nb_count[m];
+      const size_t base = (size_t)m * nb.kcap;
+      for (int a = lane; a < K; a += 64) {
+        const size_t sl = base + a;
+        f0 += slot_f[3 * sl];
+        f1 += slot_f[3 * sl + 1];
+        f2 += slot_f[3 * sl + 2];
+        const int r = nb.nb_rev[sl];
+        if (r >= 0) {
+          f0 -= slot_f[3 * (size_t)r];
+          f1 -= slot_f[3 * (size_t)r + 1];
+          f2 -= slot_f[3 * (size_t)r + 2];
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        f0 += __shfl_xor(f0, o, 64);
+        f1 += __shfl_xor(f1, o, 64);
+        f2 += __shfl_xor(f2, o, 64);
+      }
+    }
+    if (lane == 0) {
+      force[3 * m] = f0;
+      force[3 * m + 1] = f1;
+      force[3 * m + 2] = f2;
+    }
+    return;
+  }
+  const int s = b - G;
   if (s == nstruct) {
-    list_stats(N, nb.kcap, nb.nb_count, nb.nb_raw, tb.tcap, tb.raw ? tb.count : nullptr, tb.raw, nb.total + 3,
-               stats_out);
+    list_stats<256>(N, nb.kcap, nb.nb_count, nb.nb_raw, tb.tcap, tb.raw ? tb.count : nullptr, tb.raw, nb.total + 3,
+                    stats_out);
     return;
   }
   const StructParams &P = sp[s];
-  __shared__ double red[16][7];
+  __shared__ double red[4][7];
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int a = t; a < P.natoms; a += 1024) {
+  for (int a = t; a < P.natoms; a += 256) {
     const int i = P.atom_base + a;
     acc[0] += eatom[i];
     if (grad)
@@ -455,15 +459,11 @@ __global__ __launch_bounds__(1024) void k_struct_reduce(int nstruct, const Struc
   }
   __syncthreads();
   if (t < 7) {
-    double v = 0.0;
-    for (int w = 0; w < 16; ++w) v += red[w][t];
-    if (t == 0) {
+    const double v = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+    if (t == 0)
       ene[s] = v;
-    } else {
-      // stress_cal = -virial / V converted to GPa (gap_calc.f90:189-203);
-      // virial order xx yy zz xy yz xz is already libgap's output order.
+    else
       stress6[6 * s + t - 1] = grad ? v * (-(1.0 / kGPa) / P.vol) : 0.0;
-    }
   }
 }
 
@@ -525,17 +525,13 @@ hipError_t launch_reverse_slots(hipStream_t s, int N, NeighborBufs nb) {
   return hipGetLastError();
 }
 
-hipError_t launch_gather_forces(hipStream_t s, int N, const NeighborBufs &nb, const double *slot_f,
-                                double *force) {
-  k_gather<<<N, 64, 0, s>>>(N, nb, slot_f, force);
-  return hipGetLastError();
-}
-
-hipError_t launch_struct_reduce(hipStream_t s, int nstruct, const StructParams *sp, const double *eatom,
-                                const double *atom_vir, int grad, double *ene, double *stress6, int N,
-                                const NeighborBufs &nb, const TripletBufs &tb, int *stats_out) {
-  k_struct_reduce<<<nstruct + 1, 1024, 0, s>>>(nstruct, sp, eatom, atom_vir, grad, ene, stress6, N, nb, tb,
-                                               stats_out);
+hipError_t launch_forces_reduce(hipStream_t s, int N, int grad, const NeighborBufs &nb, const double *slot_f,
+                                double *force, int nstruct, const StructParams *sp, const double *eatom,
+                                const double *atom_vir, double *ene, double *stress6, const TripletBufs &tb,
+                                int *stats_out) {
+  const int G = (N + 3) / 4;
+  k_forces_reduce<<<G + nstruct + 1, 256, 0, s>>>(N, G, nstruct, nb, slot_f, force, sp, eatom, atom_vir, grad, ene,
+                                                  stress6, tb, stats_out);
   return hipGetLastError();
 }
 
