@@ -297,6 +297,7 @@ static int upload_model(Model &m) {
 struct DevMem {
   void *p = nullptr;
   size_t cap = 0;
+  unsigned gen = 0;  // bumped on every (re)allocation
 };
 
 enum Stage { kNeighbors, kDescFwd, kContraction, kFinish, kDescBwd, kForces, kNumStages };
@@ -314,6 +315,11 @@ struct Tuning {
 struct Ctx {
   Model *m = nullptr;
   hipStream_t stream = nullptr;
+  // second stream for the reverse-slot map, which only the final force gather
+  // reads: it runs beside the contraction (fork/join events, captured as
+  // parallel graph branches)
+  hipStream_t aux = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   int device = 0;
   int nstruct = 0, N = 0, Np = 0, nbins = 0;
   bool fused_bins = false;  // every structure small enough for k_bin_struct
@@ -333,6 +339,7 @@ struct Ctx {
   Mlp *mlp = nullptr;  // set while the last run used the surrogate
   void *blas = nullptr;  // rocBLAS handle on the stream (variance only, made on first use)
   bool md_ready = false;  // md_p holds the momenta of the last MD run
+  unsigned long long bins_clean = ~0ULL;  // allocation generations of bin_count/bin_ticket last cleared
   // timing
   bool timing = false;
   hipEvent_t ev[2 * kNumStages] = {};
@@ -361,6 +368,7 @@ struct Ctx {
         return fail(GAPX_ERR_NOMEM, std::string("hipMalloc failed for ") + name);
       }
       d.cap = alloc;
+      ++d.gen;
       bytes += alloc;
       invalidate_graph();
     }
@@ -579,26 +587,45 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b, bool grad) {
   if (c.fused_bins) {
     GAPX_HIP(launch_bin_structures(c.stream, c.nstruct, d_pos, d_sp, nb, c.nbins, stats));
   } else {
-    GAPX_HIP(launch_zero_ints(c.stream, nb.bin_count, c.nbins, nb.bin_cursor, c.nbins, stats, 8));
-    GAPX_HIP(launch_wrap_bin(c.stream, c.N, d_pos, d_astruct, d_sp, nb));
-    GAPX_HIP(launch_scan(c.stream, nb.bin_count, nb.bin_start, c.nbins, nullptr));
+    // k_wrap_scan keeps bin_count and its ticket at zero between runs; clear
+    // them (whole capacity) once after they are (re)allocated
+    int *ticket;
+    if ((rc = c.get("bin_ticket", 1, &ticket))) return rc;
+    const DevMem &dc = c.mem["bin_count"], &dt = c.mem["bin_ticket"];
+    const unsigned long long g = ((unsigned long long)dc.gen << 32) | dt.gen;
+    if (g != c.bins_clean) {
+      GAPX_HIP(launch_zero_ints(c.stream, nb.bin_count, (int)(dc.cap / sizeof(int)), ticket, 1, nullptr, 0));
+      c.bins_clean = g;
+    }
+    GAPX_HIP(launch_wrap_scan(c.stream, c.N, c.nbins, d_pos, d_astruct, d_sp, nb, ticket, stats));
     GAPX_HIP(launch_scatter_bins(c.stream, c.N, c.nbins, nb));
   }
-  GAPX_HIP(launch_build_neighbors(c.stream, c.N, d_astruct, d_sp, c.rcut, nb));
-  if (grad) GAPX_HIP(launch_reverse_slots(c.stream, c.N, nb));
-  c.KP = c.kcap;
-  // triplet lists for the angular lines (forward and backward share them)
+  // triplet lists for the angular lines (forward and backward share them),
+  // built by the same wavefronts as the neighbour lists
   TripletBufs &tb = b.tb;
   tb.tcap = c.tcap;
   if ((rc = c.get("trip_list", (size_t)c.N * c.tcap, &tb.list)) || (rc = c.get("trip_count", c.N, &tb.count)) ||
       (rc = c.get("trip_raw", c.N, &tb.raw)))
     return rc;
-  if (c.m->sf.n_ang > 0) {
-    GAPX_HIP(launch_triplets(c.stream, c.N, nb, c.m->sf.rc_max_ang, tb));
-  } else {
+  const bool ang = c.m->sf.n_ang > 0;
+  GAPX_HIP(launch_build_neighbors(c.stream, c.N, d_astruct, d_sp, c.rcut, nb, c.m->sf.rc_max_ang, ang ? &tb : nullptr));
+  if (!ang) {
     GAPX_HIP(launch_fill_int(c.stream, tb.count, c.N, 0));
     tb.raw = nullptr;  // no triplet statistics
   }
+  c.KP = c.kcap;
+  return GAPX_OK;
+}
+
+// The reverse-slot map on the second stream, forked after the descriptor
+// forward was enqueued: it then runs beside the contraction, which leaves the
+// vector units mostly idle (running it beside the descriptor kernels slowed
+// them by as much as it saved).  enqueue_forces joins it before the gather.
+static int fork_reverse(Ctx &c, Bufs &b) {
+  GAPX_HIP(hipEventRecord(c.fork_ev, c.stream));
+  GAPX_HIP(hipStreamWaitEvent(c.aux, c.fork_ev, 0));
+  GAPX_HIP(launch_reverse_slots(c.aux, c.N, b.nb));
+  GAPX_HIP(hipEventRecord(c.join_ev, c.aux));
   return GAPX_OK;
 }
 
@@ -642,7 +669,9 @@ static int enqueue_forces(Ctx &c, Bufs &b, bool fp64, bool grad, const double *e
     stage_end(c, kDescBwd);
   }
   // forces, energies, stress, and the list statistics straight into pinned
-  // host memory (read by ctx_settle), in one launch
+  // host memory (read by ctx_settle), in one launch; the force gather needs
+  // the reverse-slot map from the second stream
+  if (grad) GAPX_HIP(hipStreamWaitEvent(c.stream, c.join_ev, 0));
   stage_begin(c, kForces);
   GAPX_HIP(launch_forces_reduce(c.stream, c.N, grad ? 1 : 0, nb, b.slot_f, b.force, c.nstruct, d_sp, eatom,
                                 b.atom_vir, b.ene, b.stress, b.tb, c.host_stats_dev));
@@ -695,6 +724,7 @@ static int enqueue_run(Ctx &c, int flags) {
   GAPX_HIP(launch_descriptors_fwd(c.stream, fp64, c.N, d_species, m.d_wz, nb, b.tb, m.sfd, m.d_theta_inv, m.d_mu,
                                   split_mode, o, chunk, tu.waves_fwd, c.KP));
   stage_end(c, kDescFwd);
+  if (grad && (rc = fork_reverse(c, b))) return rc;
 
   // contraction
   GprArgs &g = b.g;
@@ -786,6 +816,7 @@ static int enqueue_surrogate(Ctx &c, Mlp &mlp, int flags) {
   GAPX_HIP(launch_descriptors_fwd(c.stream, fp64, c.N, c.ptr<int>("species"), m.d_wz, b.nb, b.tb, m.sfd,
                                   m.d_theta_inv, m.d_mu, 0, o, c.chunk > 0 ? c.chunk : 16, tu.waves_fwd, c.KP));
   stage_end(c, kDescFwd);
+  if (grad && (rc = fork_reverse(c, b))) return rc;
   float *xin = o.xt32;
   if (fp64) {
     if ((rc = c.get("xtf", ND, &xin))) return rc;
@@ -1151,6 +1182,8 @@ static int ctx_variance(Ctx &c, double *var, double *sigma2) {
   return GAPX_OK;
 }
 
+static void ctx_free(Ctx *c);
+
 static int ctx_create(Model *m, Ctx **out) {
   Ctx *c = new Ctx();
   c->m = m;
@@ -1160,12 +1193,16 @@ static int ctx_create(Model *m, Ctx **out) {
     return fail(GAPX_ERR_HIP, "could not create a HIP stream");
   }
   for (auto &e : c->ev) hipEventCreate(&e);
+  if (hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess) {
+    ctx_free(c);
+    return fail(GAPX_ERR_HIP, "could not create the second stream or its events");
+  }
   if (hipHostMalloc((void **)&c->host_stats, 8 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) !=
           hipSuccess ||
       hipHostGetDevicePointer((void **)&c->host_stats_dev, c->host_stats, 0) != hipSuccess) {
-    if (c->host_stats) hipHostFree(c->host_stats);
-    hipStreamDestroy(c->stream);
-    delete c;
+    ctx_free(c);
     return fail(GAPX_ERR_NOMEM, "pinned host allocation failed");
   }
   std::memset(c->host_stats, 0, 8 * sizeof(int));
@@ -1188,6 +1225,7 @@ static void ctx_free(Ctx *c) {
   if (!c) return;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->aux) hipStreamSynchronize(c->aux);
   c->invalidate_graph();
   variance_destroy_handle(c->blas);
   for (auto &kv : c->mem)
@@ -1195,6 +1233,9 @@ static void ctx_free(Ctx *c) {
   for (auto &e : c->ev)
     if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
+  if (c->aux) hipStreamDestroy(c->aux);
+  if (c->fork_ev) hipEventDestroy(c->fork_ev);
+  if (c->join_ev) hipEventDestroy(c->join_ev);
   if (c->host_stats) hipHostFree(c->host_stats);
   delete c;
 }

@@ -22,7 +22,7 @@
 // Work layout: the atom's neighbours live in LDS (positions relative to the
 // centre, distance, weight, cutoff function per cutoff group).  Radial sums
 // put one neighbour on each lane.  Angular sums read the atom's triplet list,
-// built once per evaluation by k_triplets (neighbour pairs j < k with all
+// built once per evaluation by k_build<true> (neighbour pairs j < k with all
 // three distances within the largest angular cutoff, compacted with a
 // ballot), 64 triplets per step, so every lane holds a live triplet and
 // forward and backward share one enumeration.  The forward keeps NCH lines'
@@ -145,60 +145,6 @@ __device__ __forceinline__ uint32_t pop_q4(uint32_t (&q)[4]) {
 
 // ---------------------------------------------------------------------------
 // triplet lists
-// ---------------------------------------------------------------------------
-// One wavefront per centre atom, neighbours staged in LDS.  Pairs (a, b),
-// a < b, with r_a, r_b, r_ab all within the largest angular cutoff are
-// appended (ballot + prefix count) to the atom's list at i * tcap, packed as
-// a | b << 16, in diagonal order: the pairs with b - a = d form a run of
-// K - d entries.  Runs d and K - d together hold K entries, so one wave step
-// takes both (full lanes for K <= 64).  Consecutive entries of a run have
-// distinct a and distinct b, so the backward pass's LDS atomics on
-// per-neighbour accumulators rarely collide within one instruction.  The
-// cutoff test is a squared comparison with a relative margin of 1e-6; a
-// triplet admitted by the margin contributes zero, since the descriptor
-// kernels apply each line's cutoff to every distance.
-__global__ __launch_bounds__(64) void k_triplets(int N, NeighborBufs nb, double rcA, TripletBufs tb) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  float4 *s_p = (float4 *)smem;  // [kcap]: x y z r, one 16-byte record per neighbour
-  const int i = blockIdx.x, lane = threadIdx.x;
-  if (i >= N) return;
-  const int K = nb.nb_count[i];
-  const size_t start = (size_t)i * nb.kcap;
-  for (int a = lane; a < K; a += 64) {
-    const double *dd = nb.nb_d + 4 * (start + a);
-    s_p[a] = make_float4((float)dd[0], (float)dd[1], (float)dd[2], (float)dd[3]);
-  }
-  __syncthreads();
-  const float rc = (float)rcA * (1.0f + 1e-6f);
-  const float rc2 = rc * rc;
-  const size_t out = (size_t)i * tb.tcap;
-  int count = 0;
-  for (int d1 = 1; 2 * d1 <= K; ++d1) {
-    // run d1 (K - d1 pairs), then run d2 = K - d1 (d1 pairs) unless it is d1
-    const int d2 = K - d1;
-    const int n1 = K - d1, n = (d2 != d1) ? K : n1;
-    for (int c0 = 0; c0 < n; c0 += 64) {
-      const int c = c0 + lane;
-      const bool first = c < n1;
-      const int a = first ? c : c - n1, b = a + (first ? d1 : d2);
-      // both records read unconditionally (idle lanes read record 0) and
-      // the tests combined without short-circuits: one LDS round trip
-      const bool live = c < n;
-      const float4 pa = s_p[live ? a : 0], pb = s_p[live ? b : 0];
-      const float ex = pb.x - pa.x, ey = pb.y - pa.y, ez = pb.z - pa.z;
-      const bool ok = live & (pa.w <= rc) & (pb.w <= rc) & (ex * ex + ey * ey + ez * ez <= rc2);
-      const unsigned long long m = __ballot(ok);
-      const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
-      if (ok && pos < tb.tcap) tb.list[out + pos] = (uint32_t)a | ((uint32_t)b << 16);
-      count += __popcll(m);
-    }
-  }
-  if (lane == 0) {
-    tb.count[i] = min(count, tb.tcap);
-    tb.raw[i] = count;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
@@ -1043,12 +989,6 @@ int max_neighbors_for_lds(bool fp64, int G, int Dp) {
   // values of the compute type and 4 KP FP64 accumulators
   long kp = (long)(budget - (2 * Dp + Dp / 2) * bytes) / (long)((6 + 2 * G) * bytes + 4 * sizeof(double));
   return (int)(kp / 8 * 8);
-}
-
-hipError_t launch_triplets(hipStream_t s, int N, const NeighborBufs &nb, double rcA, TripletBufs tb) {
-  if (N <= 0) return hipSuccess;
-  k_triplets<<<N, 64, 4 * sizeof(float) * (size_t)nb.kcap, s>>>(N, nb, rcA, tb);
-  return hipGetLastError();
 }
 
 hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *species, const double *wz,

@@ -13,9 +13,7 @@ namespace gapx {
 
 constexpr double kPi = 3.141592654;       // truncated pi of wacsf.f90:31
 constexpr double kGPa = 6.24219e-3;       // eV/A^3 per GPa, gap_calc.f90:9
-constexpr int kMaxNeighbors = 256;        // per-atom list held in LDS
 constexpr int kRefNeighborCap = 1000;     // gap_calc.f90:68
-constexpr int kMaxGroups = 8;             // distinct per-line cutoffs
 constexpr int kMaxSpeciesZ = 128;
 
 // Symmetry-function tables as the kernels read them.  Radial lines (types 1
@@ -126,9 +124,11 @@ struct NeighborBufs {
   int *total;           // [8] see launch_forces_reduce; [3] is the descriptor kernels' guard
 };
 
-hipError_t launch_wrap_bin(hipStream_t s, int N, const double *pos, const int *atom_struct,
-                           const StructParams *sp, NeighborBufs nb);
-hipError_t launch_scan(hipStream_t s, const int *in, int *out, int n, int *total);
+// Wrap positions into their cells, bin and count the atoms, and scan the
+// counts into bin_start, in one launch; needs bin_count and *ticket zero on
+// entry and leaves them zero (with bin_cursor[0, nbins) and stats[0..7]).
+hipError_t launch_wrap_scan(hipStream_t s, int N, int nbins, const double *pos, const int *atom_struct,
+                            const StructParams *sp, NeighborBufs nb, int *ticket, int *stats);
 hipError_t launch_scatter_bins(hipStream_t s, int N, int nbins, NeighborBufs nb);
 // The whole binning of structures with at most kFusedAtoms atoms and
 // kFusedBins bins each, one workgroup per structure (also clears stats[0..7]).
@@ -137,8 +137,6 @@ hipError_t launch_scatter_bins(hipStream_t s, int N, int nbins, NeighborBufs nb)
 constexpr int kFusedAtoms = 8192, kFusedBins = 2048, kFusedMaxAtoms = 1024;
 hipError_t launch_bin_structures(hipStream_t s, int nstruct, const double *pos, const StructParams *sp,
                                  NeighborBufs nb, int nbins_total, int *stats);
-hipError_t launch_build_neighbors(hipStream_t s, int N, const int *atom_struct,
-                                  const StructParams *sp, double rcut, NeighborBufs nb);
 hipError_t launch_reverse_slots(hipStream_t s, int N, NeighborBufs nb);
 hipError_t launch_fill_int(hipStream_t s, int *p, int n, int v);
 hipError_t launch_zero_ints(hipStream_t s, int *a, int na, int *b, int nb, int *c, int nc);
@@ -163,7 +161,10 @@ struct TripletBufs {
   int *raw;             // [N] triplet count before the tcap clamp
 };
 
-hipError_t launch_triplets(hipStream_t s, int N, const NeighborBufs &nb, double rcA, TripletBufs tb);
+// Neighbour lists (one wavefront per atom); with tb also the triplet lists
+// for angular cutoff rcA, from the same wave.
+hipError_t launch_build_neighbors(hipStream_t s, int N, const int *atom_struct, const StructParams *sp,
+                                  double rcut, NeighborBufs nb, double rcA, const TripletBufs *tb);
 
 hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *species,
                                   const double *wz, const NeighborBufs &nb, const TripletBufs &tb,

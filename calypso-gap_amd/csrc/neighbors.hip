@@ -11,9 +11,9 @@
 // by one small kernel at the end of a run: same-address atomics from every
 // wavefront serialise in the memory system and cost more than the search.
 //
-// Bins: nb_k = max(1, floor(d_k / rcut)) along each lattice direction (d_k the
-// interplanar spacing); a neighbour can be at most S_k = ceil(rcut nb_k / d_k)
-// bins away, and offsets beyond the cell wrap with an image shift, so cells
+// Bins: nb_k = max(1, floor(2 d_k / rcut)) along each lattice direction (d_k
+// the interplanar spacing); a neighbour can be at most S_k = ceil(rcut nb_k /
+// d_k) bins away (2 where the cell holds two bins per cutoff), and offsets beyond the cell wrap with an image shift, so cells
 // smaller than rcut need no special case.  Atoms inside a bin are kept in
 // ascending index order so the lists (and every sum over them) are
 // deterministic.
@@ -44,55 +44,68 @@ __global__ void k_fill_double(double *p, size_t n, double v) {
   if (t < n) p[t] = v;
 }
 
-__global__ void k_wrap_bin(int N, const double *__restrict__ pos, const int *__restrict__ atom_struct,
-                           const StructParams *__restrict__ sp, NeighborBufs nb) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const StructParams &P = sp[atom_struct[i]];
-  double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-  int b[3], sh[3];
-  for (int k = 0; k < 3; ++k) {
-    double f = r[0] * P.Linv[k] + r[1] * P.Linv[3 + k] + r[2] * P.Linv[6 + k];
-    double fl = floor(f);
-    sh[k] = (int)fl;
-    int bk = (int)floor((f - fl) * P.nb[k]);
-    b[k] = bk < 0 ? 0 : (bk >= P.nb[k] ? P.nb[k] - 1 : bk);
+// Wrap, bin and count, then scan, in one launch of 256-thread blocks.  Every block
+// wraps and bins its atoms (device-scope atomics on bin_count); the last
+// block to finish (a ticket counter, with device fences on both sides) then
+// scans the counts into bin_start and leaves the buffers ready for the next
+// run: bin_count zero (so no clearing launch is needed), the scatter cursors
+// zero, the run's statistics zero and the ticket back at zero.  The host
+// clears bin_count and the ticket once whenever they are newly allocated.
+__global__ __launch_bounds__(256) void k_wrap_scan(int N, int nbins, const double *__restrict__ pos,
+                                                   const int *__restrict__ atom_struct,
+                                                   const StructParams *__restrict__ sp, NeighborBufs nb,
+                                                   int *__restrict__ ticket, int *__restrict__ stats) {
+  const int i = blockIdx.x * 256 + threadIdx.x, t = threadIdx.x;
+  if (i < N) {
+    const StructParams &P = sp[atom_struct[i]];
+    const double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    int bk3[3], sh[3];
+    for (int k = 0; k < 3; ++k) {
+      const double f = r[0] * P.Linv[k] + r[1] * P.Linv[3 + k] + r[2] * P.Linv[6 + k];
+      const double fl = floor(f);
+      sh[k] = (int)fl;
+      const int bk = (int)floor((f - fl) * P.nb[k]);
+      bk3[k] = bk < 0 ? 0 : (bk >= P.nb[k] ? P.nb[k] - 1 : bk);
+    }
+    for (int c = 0; c < 3; ++c) {
+      nb.wpos[3 * i + c] = r[c] - sh[0] * P.L[c] - sh[1] * P.L[3 + c] - sh[2] * P.L[6 + c];
+      nb.shift[3 * i + c] = sh[c];
+    }
+    const int bin = P.bin_base + (bk3[0] * P.nb[1] + bk3[1]) * P.nb[2] + bk3[2];
+    nb.atom_bin[i] = bin;
+    atomicAdd(&nb.bin_count[bin], 1);
   }
-  for (int c = 0; c < 3; ++c) {
-    nb.wpos[3 * i + c] = r[c] - sh[0] * P.L[c] - sh[1] * P.L[3 + c] - sh[2] * P.L[6 + c];
-    nb.shift[3 * i + c] = sh[c];
-  }
-  int bin = P.bin_base + (b[0] * P.nb[1] + b[1]) * P.nb[2] + b[2];
-  nb.atom_bin[i] = bin;
-  atomicAdd(&nb.bin_count[bin], 1);
-}
-
-// Exclusive scan of n ints by one workgroup of 1024 threads; out[n] = total.
-__global__ __launch_bounds__(1024) void k_scan(const int *__restrict__ in, int *__restrict__ out, int n,
-                                               int *total) {
-  __shared__ long long part[1024];
-  const int t = threadIdx.x;
-  const int per = (n + 1023) / 1024;
-  const int lo = t * per, hi = min(n, lo + per);
-  long long s = 0;
-  for (int k = lo; k < hi; ++k) s += in[k];
-  part[t] = s;
+  __shared__ int s_last;
+  __shared__ long long part[256];
+  __threadfence();
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    long long v = (t >= off) ? part[t - off] : 0;
+  if (t == 0) s_last = (atomicAdd(ticket, 1) == (int)gridDim.x - 1);
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  // exclusive scan of bin_count, 256 segments
+  const int per = (nbins + 255) / 256, lo = t * per, hi = min(nbins, lo + per);
+  long long sum = 0;
+  for (int k = lo; k < hi; ++k) sum += __hip_atomic_load(&nb.bin_count[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const long long v = (t >= off) ? part[t - off] : 0;
     __syncthreads();
     part[t] += v;
     __syncthreads();
   }
-  long long run = part[t] - s;  // exclusive prefix of this thread's segment
+  long long run = part[t] - sum;
   for (int k = lo; k < hi; ++k) {
-    out[k] = (int)run;
-    run += in[k];
+    const int c = __hip_atomic_load(&nb.bin_count[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    nb.bin_start[k] = (int)run;
+    run += c;
+    nb.bin_count[k] = 0;
+    nb.bin_cursor[k] = 0;
   }
-  if (t == 1023) {
-    out[n] = (int)part[1023];
-    if (total) total[0] = (int)part[1023];
-  }
+  if (t == 255) nb.bin_start[nbins] = (int)part[255];
+  if (t < 8) stats[t] = 0;
+  if (t == 0) *ticket = 0;
 }
 
 __global__ void k_scatter(int N, NeighborBufs nb) {
@@ -139,7 +152,7 @@ __global__ __launch_bounds__(64) void k_rank_bins(int nbins, NeighborBufs nb) {
 // One workgroup per structure does the whole binning in LDS: wrap and bin
 // every atom, count per bin, exclusive scan, scatter, then order each bin's
 // atoms by index and write the bin-sorted records; workgroup 0 also clears
-// the run's statistics.  Same output as k_zero3 + k_wrap_bin + k_scan +
+// the run's statistics.  Same output as k_wrap_scan +
 // k_scatter + k_rank_bins, in one launch, for structures of at most
 // kFusedAtoms atoms and kFusedBins bins.
 __global__ __launch_bounds__(1024) void k_bin_struct(const double *__restrict__ pos,
@@ -220,6 +233,49 @@ __device__ __forceinline__ unsigned long long nb_key(int j, int n0, int n1, int 
          ((unsigned long long)((n1 + 128) & 255) << 8) | (unsigned long long)((n2 + 128) & 255);
 }
 
+// Triplet lists, from the neighbour records the building wave just staged in
+// LDS.  Pairs (a, b), a < b, with r_a, r_b, r_ab all within the largest
+// angular cutoff are appended (ballot + prefix count) to the atom's list at
+// i * tcap, packed as a | b << 16, in diagonal order: the pairs with b - a = d
+// form a run of K - d entries.  Runs d and K - d together hold K entries, so
+// one wave step takes both (full lanes for K <= 64).  Consecutive entries of
+// a run have distinct a and distinct b, so the backward pass's LDS atomics on
+// per-neighbour accumulators rarely collide within one instruction.  The
+// cutoff test is a squared comparison with a relative margin of 1e-6; a
+// triplet admitted by the margin contributes zero, since the descriptor
+// kernels apply each line's cutoff to every distance.
+__device__ __forceinline__ void triplets_from_lds(int i, int K, const float4 *s_p, double rcA, const TripletBufs &tb,
+                                                  int lane) {
+  const float rc = (float)rcA * (1.0f + 1e-6f);
+  const float rc2 = rc * rc;
+  const size_t out = (size_t)i * tb.tcap;
+  int count = 0;
+  for (int d1 = 1; 2 * d1 <= K; ++d1) {
+    // run d1 (K - d1 pairs), then run d2 = K - d1 (d1 pairs) unless it is d1
+    const int d2 = K - d1;
+    const int n1 = K - d1, n = (d2 != d1) ? K : n1;
+    for (int c0 = 0; c0 < n; c0 += 64) {
+      const int c = c0 + lane;
+      const bool first = c < n1;
+      const int a = first ? c : c - n1, b = a + (first ? d1 : d2);
+      // both records read unconditionally (idle lanes read record 0) and
+      // the tests combined without short-circuits: one LDS round trip
+      const bool live = c < n;
+      const float4 pa = s_p[live ? a : 0], pb = s_p[live ? b : 0];
+      const float ex = pb.x - pa.x, ey = pb.y - pa.y, ez = pb.z - pa.z;
+      const bool ok = live & (pa.w <= rc) & (pb.w <= rc) & (ex * ex + ey * ey + ez * ez <= rc2);
+      const unsigned long long m = __ballot(ok);
+      const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+      if (ok && pos < tb.tcap) tb.list[out + pos] = (uint32_t)a | ((uint32_t)b << 16);
+      count += __popcll(m);
+    }
+  }
+  if (lane == 0) {
+    tb.count[i] = min(count, tb.tcap);
+    tb.raw[i] = count;
+  }
+}
+
 // One wavefront per centre atom.  The (2S+1)^3 bin images are taken 64 at a
 // time, one per lane (bin range and image shift); a wave prefix sum over the
 // bin sizes flattens their atoms into one candidate sequence, which the lanes
@@ -227,8 +283,14 @@ __device__ __forceinline__ unsigned long long nb_key(int j, int n0, int n1, int 
 // prefix sums.  Accepted pairs are appended with ballot + prefix count, so the
 // list order is image order, then index order inside the bin (deterministic).
 // Slots live at i * kcap + a; counts above kcap are recorded, not written.
+// With TRIP the wave also keeps each kept neighbour's (dx, dy, dz, r) in LDS
+// (FP32) and builds the atom's triplet list from there (triplets_from_lds).
+template <bool TRIP>
 __global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ atom_struct,
-                                              const StructParams *__restrict__ sp, double rcut, NeighborBufs nb) {
+                                              const StructParams *__restrict__ sp, double rcut, NeighborBufs nb,
+                                              double rcA, TripletBufs tb) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float4 *s_p = (float4 *)smem;  // [kcap] with TRIP
   const int i = blockIdx.x, lane = threadIdx.x;
   if (i >= N) return;
   const StructParams &P = sp[atom_struct[i]];
@@ -297,6 +359,7 @@ __global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ ato
         double2 *d2 = (double2 *)(nb.nb_d + 4 * sl);
         d2[0] = make_double2(dx, dy);
         d2[1] = make_double2(dz, r);
+        if (TRIP) s_p[pos] = make_float4((float)dx, (float)dy, (float)dz, (float)r);
       }
       count += __popcll(m);
     }
@@ -304,6 +367,10 @@ __global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ ato
   if (lane == 0) {
     nb.nb_count[i] = min(count, nb.kcap);
     nb.nb_raw[i] = count;
+  }
+  if (TRIP) {
+    __syncthreads();
+    triplets_from_lds(i, min(count, nb.kcap), s_p, rcA, tb, lane);
   }
 }
 
@@ -489,14 +556,9 @@ hipError_t launch_fill_double(hipStream_t s, double *p, size_t n, double v) {
   return hipGetLastError();
 }
 
-hipError_t launch_wrap_bin(hipStream_t s, int N, const double *pos, const int *atom_struct,
-                           const StructParams *sp, NeighborBufs nb) {
-  k_wrap_bin<<<(N + 255) / 256, 256, 0, s>>>(N, pos, atom_struct, sp, nb);
-  return hipGetLastError();
-}
-
-hipError_t launch_scan(hipStream_t s, const int *in, int *out, int n, int *total) {
-  k_scan<<<1, 1024, 0, s>>>(in, out, n, total);
+hipError_t launch_wrap_scan(hipStream_t s, int N, int nbins, const double *pos, const int *atom_struct,
+                            const StructParams *sp, NeighborBufs nb, int *ticket, int *stats) {
+  k_wrap_scan<<<(N + 255) / 256, 256, 0, s>>>(N, nbins, pos, atom_struct, sp, nb, ticket, stats);
   return hipGetLastError();
 }
 
@@ -515,8 +577,11 @@ hipError_t launch_bin_structures(hipStream_t s, int nstruct, const double *pos, 
 }
 
 hipError_t launch_build_neighbors(hipStream_t s, int N, const int *atom_struct, const StructParams *sp,
-                                  double rcut, NeighborBufs nb) {
-  k_build<<<N, 64, 0, s>>>(N, atom_struct, sp, rcut, nb);
+                                  double rcut, NeighborBufs nb, double rcA, const TripletBufs *tb) {
+  if (tb)
+    k_build<true><<<N, 64, 4 * sizeof(float) * (size_t)nb.kcap, s>>>(N, atom_struct, sp, rcut, nb, rcA, *tb);
+  else
+    k_build<false><<<N, 64, 0, s>>>(N, atom_struct, sp, rcut, nb, 0.0, TripletBufs{});
   return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@ import pytest
 
 import gapx
 from gapx import synth
+from helpers import ternary_structure
 
 pytestmark = pytest.mark.gpu
 
@@ -137,3 +138,23 @@ def test_population_mixed_vs_fp64(pop, bf16):
     print("population mixed(%s) vs fp64: dE/atom %.3e eV  dF %.3e eV/A  dS %.3e GPa"
           % ("bf16" if bf16 else "fp16", de, df, ds))
     assert de <= 1e-3 and df <= 2e-2 and ds <= 5e-2
+
+
+def test_reloads_equal_fresh_contexts(pop):
+    """One context reloaded with batches of other sizes (more and fewer bins,
+    the one-launch and the multi-launch binning) gives bit for bit what a
+    fresh context gives: the binning leaves its counters cleared for the next
+    run."""
+    m, structs = pop
+    big = [ternary_structure(1100, 11.5, 71), ternary_structure(1500, 14.0, 72)]  # above the one-launch limit
+    batches = [big[:1], structs[:10], big, big[1:], structs[10:30], big[:1]]
+    ctx = gapx.Context(m)
+    for k, batch in enumerate(batches):
+        ctx.load(batch, 6.0)
+        for _ in range(2):
+            ctx.run(gapx.GRAD)
+        r = ctx.results()
+        f = evaluate(m, batch, gapx.GRAD)[0]
+        assert np.array_equal(r["energy"], f["energy"]) and np.array_equal(r["forces"], f["forces"]), k
+        assert np.array_equal(r["stress"], f["stress"]), k
+    ctx.close()
