@@ -460,8 +460,8 @@ static int initial_kcap(const Ctx &c) {
   return round_up(std::max(32, (int)(1.25 * kest) + 24), 8);
 }
 
-static int ctx_load(Ctx &c, int nstruct, const int *natoms, const int *species, const double *lat,
-                    const double *pos, double rcut, bool fortran) {
+static int ctx_load_batch(Ctx &c, int nstruct, const int *natoms, const int *species, const double *lat,
+                          const double *pos, double rcut, bool fortran) {
   if (nstruct <= 0 || !natoms || !species || !lat || !pos) return fail(GAPX_ERR_ARG, "empty batch");
   if (!(rcut > 0)) return fail(GAPX_ERR_ARG, "rcut must be positive");
   GAPX_HIP(hipSetDevice(c.device));
@@ -515,8 +515,24 @@ static int ctx_load(Ctx &c, int nstruct, const int *natoms, const int *species, 
   c.kcap = initial_kcap(c);
   c.tcap = initial_tcap(c);
   c.ran = false;
+  c.md_ready = false;
   c.invalidate_graph();
   return GAPX_OK;
+}
+
+// A load that fails part-way leaves the context empty (no structures), so no
+// later call can run kernels sized for the new batch on the old buffers.
+static int ctx_load(Ctx &c, int nstruct, const int *natoms, const int *species, const double *lat,
+                    const double *pos, double rcut, bool fortran) {
+  const int rc = ctx_load_batch(c, nstruct, natoms, species, lat, pos, rcut, fortran);
+  if (rc) {
+    c.N = 0;
+    c.nstruct = 0;
+    c.ran = false;
+    c.md_ready = false;
+    c.invalidate_graph();
+  }
+  return rc;
 }
 
 struct Bufs {
@@ -1787,10 +1803,17 @@ int gapx_ctx_update(gapx_ctx *c, const double *lat, const double *pos) {
   if (!c || !pos || c->N <= 0) return fail(GAPX_ERR_ARG, "gapx_ctx_update: nothing loaded");
   GAPX_HIP(hipSetDevice(c->device));
   if (lat) {
+    // a rejected lattice (singular) leaves the loaded cells as they were
+    const std::vector<double> lat_before = c->lat;
+    const std::vector<StructParams> sp_before = c->sp;
     std::memcpy(c->lat.data(), lat, 9 * sizeof(double) * c->nstruct);
     int nb_before = c->nbins;
     int rc = struct_params(*c);
-    if (rc) return rc;
+    if (rc) {
+      c->lat = lat_before;
+      c->sp = sp_before;
+      return rc;
+    }
     if (c->nbins != nb_before) c->invalidate_graph();
     GAPX_HIP(hipMemcpyAsync(c->ptr<StructParams>("sp"), c->sp.data(), sizeof(StructParams) * c->nstruct,
                             hipMemcpyHostToDevice, c->stream));

@@ -1,0 +1,144 @@
+"""Edge cases of the GPU path against the oracle (FP64 bar: 1e-9 eV/atom,
+1e-7 eV/A, 1e-6 GPa) and error paths that must leave the context usable.
+
+* An isolated atom (no neighbour within rcut), one atom in a cell smaller
+  than rcut (only its own periodic images as neighbours, the reference keeps
+  them: gap_calc.f90:97-101), a close pair inside the reference's
+  d < 0.5 A warning range (gap_calc.f90:102-105), and a ragged batch of 1-,
+  2- and 64-atom structures.
+* Empty input (no atoms, a structure with no atoms) is an argument error.
+* A load that fails part-way (an atomic number the model does not list)
+  leaves the context empty: a run then reports "no structures loaded", and
+  the next good load works as in a fresh context.
+* A rejected lattice update (singular cell) keeps the loaded cells.
+* More neighbours than the descriptor kernels' LDS layout holds is reported
+  as GAPX_ERR_NEIGHBORS, without the reference's 1000 cap as well.
+"""
+import numpy as np
+import pytest
+
+import gapx
+import oracle
+from helpers import fitted_small_model, small_model, ternary_structure
+
+pytestmark = pytest.mark.gpu
+
+TOL_E, TOL_F, TOL_S = 1e-9, 1e-7, 1e-6
+
+
+@pytest.fixture(scope="module")
+def model():
+    if gapx.device_count() == 0:
+        pytest.skip("no GPU visible")
+    rng = np.random.default_rng(17)
+    s = ternary_structure(48, 12.0, 171)
+    om = fitted_small_model(rng, s)
+    m = gapx.Model.from_model(om)
+    yield om, m
+    m.close()
+
+
+def check(om, m, species, lat, pos, rcut=6.0):
+    e, f, st, _ = m.calc(species, lat, pos, rcut)
+    r = oracle.gap_calc(om, species, lat, pos, rcut)
+    n = len(species)
+    assert abs(e - r["energy"]) / n <= TOL_E
+    assert np.abs(f - r["forces"]).max() <= TOL_F
+    assert np.abs(st - r["stress"]).max() <= TOL_S
+    return e, f
+
+
+def test_isolated_atom(model):
+    om, m = model
+    e, f = check(om, m, np.array([8], np.int32), np.eye(3) * 20.0, np.array([[3.0, 4.0, 5.0]]))
+    assert not np.any(f)
+
+
+def test_single_atom_in_cell_smaller_than_rcut(model):
+    om, m = model
+    lat = np.array([[2.6, 0.0, 0.0], [0.4, 2.8, 0.0], [0.3, -0.2, 3.1]])
+    check(om, m, np.array([22], np.int32), lat, np.array([[0.5, 0.7, 0.9]]))
+
+
+def test_close_pair(model):
+    om, m = model
+    pos = np.array([[5.0, 5.0, 5.0], [5.3, 5.0, 5.0], [6.4, 5.9, 5.2]])
+    check(om, m, np.array([1, 1, 8], np.int32), np.eye(3) * 10.0, pos)
+
+
+def test_ragged_batch_equals_single_structures(model):
+    om, m = model
+    structs = [dict(species=np.array([8], np.int32), lat=np.eye(3) * 9.0, pos=np.array([[1.0, 2.0, 3.0]])),
+               dict(species=np.array([1, 22], np.int32), lat=np.eye(3) * 7.5, pos=np.array([[0.0, 0.0, 0.0],
+                                                                                              [1.9, 0.3, 0.1]])),
+               ternary_structure(64, 12.0, 172)]
+    ene, forces, stress = m.calc_batch(structs, 6.0)
+    for k, s in enumerate(structs):
+        e, f = check(om, m, s["species"], s["lat"], s["pos"])
+        assert abs(ene[k] - e) / len(s["species"]) <= 1e-12
+        assert np.abs(forces[k] - f).max() <= 1e-12
+
+
+def test_empty_inputs_are_argument_errors(model):
+    om, m = model
+    with pytest.raises(gapx.GapxError) as e:
+        m.calc(np.zeros(0, np.int32), np.eye(3) * 5.0, np.zeros((0, 3)), 6.0)
+    assert e.value.code == 1
+    ctx = gapx.Context(m)
+    with pytest.raises(gapx.GapxError) as e:
+        ctx.load([dict(species=np.zeros(0, np.int32), lat=np.eye(3) * 5.0, pos=np.zeros((0, 3)))], 6.0)
+    assert e.value.code == 1
+    with pytest.raises(gapx.GapxError) as e:
+        ctx.run(gapx.GRAD)
+    assert e.value.code == 1
+    ctx.close()
+
+
+def test_failed_load_leaves_context_empty(model):
+    om, m = model
+    good = [ternary_structure(40, 12.0, 173)]
+    ctx = gapx.Context(m)
+    ctx.load(good, 6.0)
+    ctx.run(gapx.GRAD)
+    r0 = ctx.results()
+    bad = [ternary_structure(300, 12.0, 174)]
+    bad[0]["species"] = bad[0]["species"].copy()
+    bad[0]["species"][7] = 79  # not in the model
+    with pytest.raises(gapx.GapxError) as e:
+        ctx.load(bad, 6.0)
+    assert e.value.code == 4
+    for call in (lambda: ctx.run(gapx.GRAD), lambda: ctx.update(good[0]["pos"])):
+        with pytest.raises(gapx.GapxError) as e:
+            call()
+        assert e.value.code == 1
+    ctx.load(good, 6.0)
+    ctx.run(gapx.GRAD)
+    r1 = ctx.results()
+    assert np.array_equal(r0["forces"], r1["forces"]) and r0["energy"][0] == r1["energy"][0]
+    ctx.close()
+
+
+def test_rejected_lattice_update_keeps_cells(model):
+    om, m = model
+    s = ternary_structure(40, 12.0, 175)
+    ctx = gapx.Context(m)
+    ctx.load([s], 6.0)
+    ctx.run(gapx.GRAD)
+    r0 = ctx.results()
+    with pytest.raises(gapx.GapxError) as e:
+        ctx.update(s["pos"], lat=np.zeros((1, 3, 3)))
+    assert e.value.code == 1
+    ctx.run(gapx.GRAD)
+    r1 = ctx.results()
+    assert np.array_equal(r0["forces"], r1["forces"]) and np.array_equal(r0["stress"], r1["stress"])
+    ctx.close()
+
+
+def test_lds_neighbour_limit_is_reported(model):
+    om, m = model
+    lat = np.eye(3) * 4.0
+    pos = np.random.default_rng(2).random((40, 3)) * 4.0
+    for flags in (0, gapx.MIXED):
+        with pytest.raises(gapx.GapxError) as e:
+            m.calc(np.ones(40, np.int32), lat, pos, 9.0, flags=flags)
+        assert e.value.code == 5
