@@ -25,7 +25,7 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["api.hip", "neighbors.hip", "descriptors.hip", "gpr.hip", "gpr_mixed.hip", "bond.hip", "mlp.hip", "relax.hip",
-               "variance.hip", "md.hip"]
+               "variance.hip", "md.hip", "ceilings.hip"]
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wno-unused-result", "-Wno-unused-value",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

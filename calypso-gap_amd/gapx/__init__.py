@@ -47,7 +47,7 @@ EXPORTS = [
     "gapx_mlp_train", "gapx_ctx_export_labels", "gapx_ctx_run_surrogate", "gapx_ctx_relax",
     "gapx_ctx_positions", "gapx_model_set_qmm", "gapx_ctx_variance",
     "gapx_ctx_md", "gapx_ctx_momenta", "gapx_model_write", "gapx_model_save", "gapx_model_load",
-    "gapx_model_read_cached",
+    "gapx_model_read_cached", "gapx_probe_fma_rates",
 ]
 
 
@@ -108,6 +108,7 @@ def lib():
         L.gapx_ctx_positions.argtypes = [vp, vp]
         L.gapx_model_set_qmm.argtypes = [vp, vp, ci, ci]
         L.gapx_ctx_variance.argtypes = [vp, vp, vp]
+        L.gapx_probe_fma_rates.argtypes = [vp]
         L.gapx_model_write.argtypes = [vp, ctypes.c_char_p]
         L.gapx_model_save.argtypes = [vp, ctypes.c_char_p]
         L.gapx_model_load.argtypes = [ctypes.c_char_p, vp]
@@ -141,6 +142,13 @@ def device_count():
     n = ctypes.c_int(0)
     _check(lib().gapx_device_count(ctypes.byref(n)))
     return n.value
+
+
+def probe_fma_rates():
+    """Measured TFLOP/s of FP32 vector FMA (no packed instructions) and FP64 vector FMA."""
+    out = np.zeros(2)
+    _check(lib().gapx_probe_fma_rates(_p(out)))
+    return dict(fma_f32=float(out[0]), fma_f64=float(out[1]))
 
 
 def min_distance(lat, pos, rcut=6.0):

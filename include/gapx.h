@@ -50,6 +50,11 @@ typedef struct gapx_ctx gapx_ctx;
 typedef struct gapx_mlp gapx_mlp;
 
 int gapx_version(void);
+/* Measured vector-FMA throughput in TFLOP/s (no counterpart in the reference;
+ * a measurement aid for the benchmark): tflops[0] FP32 FMA without packed
+ * instructions (as the kernels are compiled), tflops[1] FP64 FMA.
+ * Register-only microkernels on the current device, about 20 ms in all. */
+int gapx_probe_fma_rates(double *tflops);
 const char *gapx_last_error(void);
 /* Number of visible GPUs (0 when no GPU or no driver). */
 int gapx_device_count(int *n);

@@ -147,3 +147,12 @@ def test_stage_timing():
     ctx.sync()
     st = ctx.stages()
     assert st["gpr_contraction"][1] == 3 and st["gpr_contraction"][0] > 0
+
+
+def test_fma_probe_is_plausible():
+    """The measurement aid returns finite rates below the published vector
+    peaks (FP32 157.3 TFLOP/s with packed instructions, FP64 78.6)."""
+    c = gapx.probe_fma_rates()
+    print("measured TFLOP/s:", c)
+    for k, peak in (("fma_f32", 157.3), ("fma_f64", 78.6)):
+        assert np.isfinite(c[k]) and 0.02 * peak < c[k] < 1.05 * peak, k
