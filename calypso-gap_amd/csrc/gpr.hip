@@ -27,13 +27,16 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int kPC = 32;  // sparse points per LDS chunk
+// sparse points per LDS chunk: 64 up to D = 128 (fewer barriers and chunk
+// loads per MFMA), 32 above (LDS)
+constexpr int pc_f64(int DP) { return DP <= 128 ? 64 : 32; }
 
 // FP64: v_mfma_f64_16x16x4_f64.  Operand layout per lane l (l16 = l % 16,
 // lq = l / 16): A[l16][lq], B[lq][l16]; result D[lq + 4 r][l16], r = 0..3
 // (rows strided by 4, unlike the FP16/BF16 forms; measured on gfx950).
 template <int DP>
 __global__ __launch_bounds__(256) void k_gpr_f64(GprArgs a) {
+  constexpr int kPC = pc_f64(DP);
   constexpr int SM = DP + 1;  // odd row stride (doubles): conflict-free column reads
   __shared__ double s_m[kPC * SM];
   __shared__ double s_mn[kPC], s_c[kPC];
@@ -66,36 +69,40 @@ __global__ __launch_bounds__(256) void k_gpr_f64(GprArgs a) {
       s_c[tid] = a.coeff[p0 + tid];
     }
     __syncthreads();
-    // S^T[p = lq + 4 r][row l16] for both 16-point blocks: four independent
-    // accumulator chains (even and odd feature steps) keep the MFMA pipe full
-    v4d Sa[kPC / 16], Sb[kPC / 16];
+    // the chunk in 32-point halves (registers as for one 32-point chunk)
+#pragma unroll 1
+    for (int q0 = 0; q0 < kPC; q0 += 32) {
+      // S^T[p = lq + 4 r][row l16] for both 16-point blocks: four independent
+      // accumulator chains (even and odd feature steps) keep the MFMA pipe full
+      v4d Sa[2], Sb[2];
 #pragma unroll
-    for (int h = 0; h < kPC / 16; ++h) Sa[h] = Sb[h] = v4d{0, 0, 0, 0};
+      for (int h = 0; h < 2; ++h) Sa[h] = Sb[h] = v4d{0, 0, 0, 0};
 #pragma unroll
-    for (int t = 0; t < DP / 4; t += 2)
+      for (int t = 0; t < DP / 4; t += 2)
 #pragma unroll
-      for (int h = 0; h < kPC / 16; ++h) {
-        Sa[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(s_m[(16 * h + l16) * SM + 4 * t + lq], xa[t], Sa[h], 0, 0, 0);
-        Sb[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(s_m[(16 * h + l16) * SM + 4 * t + 4 + lq], xa[t + 1], Sb[h], 0,
-                                                     0, 0);
-      }
+        for (int h = 0; h < 2; ++h) {
+          const double *mp = s_m + (q0 + 16 * h + l16) * SM + 4 * t + lq;
+          Sa[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(mp[0], xa[t], Sa[h], 0, 0, 0);
+          Sb[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(mp[4], xa[t + 1], Sb[h], 0, 0, 0);
+        }
 #pragma unroll
-    for (int h = 0; h < kPC / 16; ++h) {
-      double w[4];
+      for (int h = 0; h < 2; ++h) {
+        double w[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int p = 16 * h + lq + 4 * r;
-        w[r] = exp(-0.5 * (xnr + s_mn[p] - 2.0 * (Sa[h][r] + Sb[h][r]))) * s_c[p];
-        er += w[r];
-      }
-      if (a.grad) {
-        // K step r: points 16 h + 4 r + lq; W is lane (row, lq)'s w[r]
+        for (int r = 0; r < 4; ++r) {
+          const int p = q0 + 16 * h + lq + 4 * r;
+          w[r] = exp(-0.5 * (xnr + s_mn[p] - 2.0 * (Sa[h][r] + Sb[h][r]))) * s_c[p];
+          er += w[r];
+        }
+        if (a.grad) {
+          // K step r: points q0 + 16 h + 4 r + lq; W is lane (row, lq)'s w[r]
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+          for (int r = 0; r < 4; ++r)
 #pragma unroll
-          for (int b = 0; b < DP / 16; ++b)
-            G[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(s_m[(16 * h + 4 * r + lq) * SM + 16 * b + l16], w[r], G[b],
-                                                        0, 0, 0);
+            for (int b = 0; b < DP / 16; ++b)
+              G[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(s_m[(q0 + 16 * h + 4 * r + lq) * SM + 16 * b + l16], w[r],
+                                                          G[b], 0, 0, 0);
+        }
       }
     }
   }
@@ -140,7 +147,7 @@ __global__ __launch_bounds__(256) void k_gpr_finish(GprArgs a, int fp64) {
 }  // namespace
 
 hipError_t launch_gpr_fp64(hipStream_t s, const GprArgs &a) {
-  if (a.pts_per_split % kPC != 0 || a.Np % 64 != 0) return hipErrorInvalidValue;
+  if (a.pts_per_split % pc_f64(a.Dp) != 0 || a.Np % 64 != 0) return hipErrorInvalidValue;
   dim3 grid(a.Np / 64, a.splits);
   switch (a.Dp) {
     case 32: k_gpr_f64<32><<<grid, 256, 0, s>>>(a); break;

@@ -226,7 +226,7 @@ hipError_t launch_typed(hipStream_t s, const GprArgs &a) {
 
 }  // namespace
 
-int gpr_points_per_chunk(bool fp64, int Dp) { return fp64 ? 32 : (Dp <= 128 ? 64 : 32); }
+int gpr_points_per_chunk(bool fp64, int Dp) { return Dp <= 128 ? 64 : 32; }  // both precisions
 
 hipError_t launch_gpr_mixed(hipStream_t s, const GprArgs &a) {
   if (a.pts_per_split % gpr_points_per_chunk(false, a.Dp) != 0 || (a.rows != 64 && a.rows != 128) ||
