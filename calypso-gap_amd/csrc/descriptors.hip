@@ -526,14 +526,14 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
         for (int hh = 0; hh < NE; ++hh) {
           const int h = h0 + hh;
           if (h < sf.n_eta) {
-            const T E = Math<T>::ex(-eta_h[hh] * S2) * F3;
+            // plain and weighted sums as four FMAs per zeta (E w_ab formed once)
+            const T E = Math<T>::ex(-eta_h[hh] * S2) * F3, Ew = E * wab;
 #pragma unroll
             for (int k = 0; k < NZ; ++k) {
-              const T tp = E * Pp[k], tm = E * Pm[k];
-              acc[(hh * 2 + 0) * NZ + k] += tp;
-              acc[(hh * 2 + 1) * NZ + k] += tm;
-              acc[NV / 2 + (hh * 2 + 0) * NZ + k] += tp * wab;
-              acc[NV / 2 + (hh * 2 + 1) * NZ + k] += tm * wab;
+              acc[(hh * 2 + 0) * NZ + k] += E * Pp[k];
+              acc[(hh * 2 + 1) * NZ + k] += E * Pm[k];
+              acc[NV / 2 + (hh * 2 + 0) * NZ + k] += Ew * Pp[k];
+              acc[NV / 2 + (hh * 2 + 1) * NZ + k] += Ew * Pm[k];
             }
           }
         }
