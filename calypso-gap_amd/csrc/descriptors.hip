@@ -417,7 +417,7 @@ __device__ __forceinline__ void group_sum(T (&v)[NV], int wv, int lane, int tid,
 
 template <typename T, int NZ>
 struct GridShape {
-  static constexpr int CAP = sizeof(T) == 4 ? 32 : 16;             // (plain, weighted) pairs per pass
+  static constexpr int CAP = 32;  // (plain, weighted) pairs per pass
   static constexpr int NE = (CAP / (2 * NZ)) > 0 ? CAP / (2 * NZ) : 1;  // etas per pass
   static constexpr int NV = NE * 2 * NZ * 2;
 };
