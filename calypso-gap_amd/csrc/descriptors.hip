@@ -44,6 +44,8 @@ struct Math<double> {
   static __device__ __forceinline__ double ex(double x) { return exp(x); }
   static __device__ __forceinline__ double cs(double x) { return cos(x); }
   static __device__ __forceinline__ double sn(double x) { return sin(x); }
+  // sine and cosine of one argument with one argument reduction
+  static __device__ __forceinline__ void sc(double x, double &s, double &c) { sincos(x, &s, &c); }
   static __device__ __forceinline__ double sq(double x) { return sqrt(x); }
   static __device__ __forceinline__ double pw(double b, double e) { return pow(b, e); }
   static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
@@ -54,6 +56,10 @@ struct Math<float> {
   static __device__ __forceinline__ float ex(float x) { return __expf(x); }
   static __device__ __forceinline__ float cs(float x) { return __cosf(x); }
   static __device__ __forceinline__ float sn(float x) { return __sinf(x); }
+  static __device__ __forceinline__ void sc(float x, float &s, float &c) {
+    s = __sinf(x);
+    c = __cosf(x);
+  }
   static __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float pw(float b, float e) { return powf(b, e); }
   // hardware approximations (about 1 ulp) for the mixed path's inner loops
@@ -605,8 +611,10 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
   const int g = sf.g_ang;
   const T fa = s_fc[g * KP + a], dfa = s_dfc[g * KP + a], fb = s_fc[g * KP + b], dfb = s_dfc[g * KP + b];
   const bool in = e <= rc;
-  const T fe = in ? T(0.5) * (Math<T>::cs(pirc * e) + T(1)) : T(0);
-  const T dfe = in ? T(-0.5) * Math<T>::sn(pirc * e) * pirc : T(0);
+  T sine, cosine;
+  Math<T>::sc(pirc * e, sine, cosine);
+  const T fe = in ? T(0.5) * (cosine + T(1)) : T(0);
+  const T dfe = in ? T(-0.5) * sine * pirc : T(0);
   T Pp[NZ], Pm[NZ], dPp[NZ], dPm[NZ];
   grid_powers<T, NZ, true>(cth, sf, Pp, Pm, dPp, dPm);
   T SA = 0, SB = 0, SC = 0;
@@ -693,8 +701,10 @@ __device__ __forceinline__ void ang_bwd_term(uint32_t q, bool active, const SFDe
       fb = s_fc[g * KP + b];
       dfb = s_dfc[g * KP + b];
       const bool in = e <= rc;
-      fe = in ? T(0.5) * (Math<T>::cs((T)kPi * e / rc) + T(1)) : T(0);
-      dfe = in ? T(0.5) * (-Math<T>::sn((T)kPi * e / rc)) * ((T)kPi / rc) : T(0);
+      T sine, cosine;
+      Math<T>::sc((T)kPi * e / rc, sine, cosine);
+      fe = in ? T(0.5) * (cosine + T(1)) : T(0);
+      dfe = in ? T(0.5) * (-sine) * ((T)kPi / rc) : T(0);
     }
     const T eta = (T)sf.ang_eta[s];
     if (eta != cur_eta) {
