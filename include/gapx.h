@@ -148,7 +148,8 @@ int gapx_descriptors(gapx_model *m, int na, const int *species, const double *la
 int gapx_min_distance(int na, const double *lat, const double *pos, double rcut, int flags, double *dmin);
 
 /* ---- device-resident contexts ----------------------------------------------
- * A context owns a HIP stream and device buffers for a batch of structures.
+ * A context owns two HIP streams (its work stream, and a second one joined
+ * inside each run) and device buffers for a batch of structures.
  * Load once, then run repeatedly on the inputs already in HBM; results stay
  * on the device until fetched. */
 int gapx_ctx_create(gapx_model *m, gapx_ctx **out);
