@@ -623,10 +623,27 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
     const T E = Math<T>::ex(-eta * S2);
     const int s0 = h * 2 * NZ;
     T sa = 0, sb = 0;
+    T gA[2 * NZ], gW[2 * NZ];
+    if constexpr (sizeof(T) == 4 && NZ % 2 == 0) {
+      // 16-byte reads: s0 is a multiple of 4 and both blocks are aligned
+      const float4 *a4 = (const float4 *)(s_gA + s0), *w4 = (const float4 *)(s_gW + s0);
+#pragma unroll
+      for (int v = 0; v < NZ / 2; ++v) {
+        const float4 x = a4[v], y = w4[v];
+        gA[4 * v] = x.x, gA[4 * v + 1] = x.y, gA[4 * v + 2] = x.z, gA[4 * v + 3] = x.w;
+        gW[4 * v] = y.x, gW[4 * v + 1] = y.y, gW[4 * v + 2] = y.z, gW[4 * v + 3] = y.w;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 2 * NZ; ++j) {
+        gA[j] = s_gA[s0 + j];
+        gW[j] = s_gW[s0 + j];
+      }
+    }
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
-      const T gp = s_gA[s0 + k] + wab * s_gW[s0 + k];
-      const T gm = s_gA[s0 + NZ + k] + wab * s_gW[s0 + NZ + k];
+      const T gp = gA[k] + wab * gW[k];
+      const T gm = gA[NZ + k] + wab * gW[NZ + k];
       sa += gp * dPp[k] + gm * dPm[k];
       sb += gp * Pp[k] + gm * Pm[k];
     }
@@ -761,7 +778,10 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   T *s_dfc = s_fc + G * KP;
   T *s_g = s_dfc + G * KP;
   T *s_gA = s_g + Dp;  // per angular line: g of the plain and of the weighted feature
-  T *s_gW = s_gA + sf.n_ang;
+  // s_gW starts on a 4-value boundary: with 16-byte aligned blocks (every
+  // region above has a multiple of 4 values) the grid form reads four lines'
+  // g at once; n_ang <= Dp / 2, so both fit before s_eta
+  T *s_gW = s_gA + ((sf.n_ang + 3) & ~3);
   T *s_eta = s_g + 2 * Dp;                        // [Dp / 2] grid form: eta of each pass
 
   const int start = i * nb.kcap;
@@ -1036,6 +1056,8 @@ hipError_t launch_descriptors_bwd(hipStream_t s, bool fp64, int N, const int *sp
                                   const GradSrc &gs, double *slot_f, double *atom_vir, int waves, int KP) {
   Lds L{KP, sf.n_groups > 0 ? sf.n_groups : 1, Dp};
   if (N <= 0) return hipSuccess;
+  // the LDS layout's 16-byte reads of g need KP (slots per atom) a multiple of 4
+  if (KP % 4 != 0) return hipErrorInvalidValue;
   if (fp64) return bwd_fit<double>(waves, s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
   return bwd_fit<float>(waves, s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
 }
