@@ -66,6 +66,18 @@ struct Math<float> {
   static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 };
 
+// Two values of T read as one LDS access (the backward's per-neighbour pairs).
+template <typename T>
+struct Pair;
+template <>
+struct Pair<float> {
+  typedef float2 type;
+};
+template <>
+struct Pair<double> {
+  typedef double2 type;
+};
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
@@ -591,15 +603,16 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
 }
 
 template <typename T, int NZ>
-__device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDev &sf, const T *s_d, const T *s_ri,
-                                             const T *s_w, const T *s_fc, const T *s_dfc, const T *s_gA,
-                                             const T *s_gW, const T *s_eta, int KP, T rc, T pirc, double *s_T,
-                                             double *s_F, T (&vir)[6]) {
+__device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDev &sf, const T *s_d, const T *s_rw,
+                                             const T *s_fd, const T *s_gA, const T *s_gW, const T *s_eta, int KP,
+                                             T rc, T pirc, double *s_T, double *s_F, T (&vir)[6]) {
   if (!active) return;
   const int a = q & 0xffff, b = q >> 16;
   const T ax = s_d[4 * a], ay = s_d[4 * a + 1], az = s_d[4 * a + 2], ra = s_d[4 * a + 3];
   const T bx = s_d[4 * b], by = s_d[4 * b + 1], bz = s_d[4 * b + 2], rb = s_d[4 * b + 3];
-  const T ria = s_ri[a], rib = s_ri[b];
+  typedef typename Pair<T>::type P2;
+  const P2 rwa = ((const P2 *)s_rw)[a], rwb = ((const P2 *)s_rw)[b];  // (1/r, w)
+  const T ria = rwa.x, rib = rwb.x;
   const T ex = bx - ax, ey = by - ay, ez = bz - az;
   const T e2 = ex * ex + ey * ey + ez * ez;
   const T e = Math<T>::sq(e2);
@@ -607,9 +620,10 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
   const T cth = (ra * ra + rb * rb - e2) * inv2ab;
   const T S2 = ra * ra + rb * rb + e2;
   const T dca = rib - cth * ria, dcb = ria - cth * rib, dce = -e * (T(2) * inv2ab);
-  const T wab = s_w[a] * s_w[b];
+  const T wab = rwa.y * rwb.y;
   const int g = sf.g_ang;
-  const T fa = s_fc[g * KP + a], dfa = s_dfc[g * KP + a], fb = s_fc[g * KP + b], dfb = s_dfc[g * KP + b];
+  const P2 fda = ((const P2 *)s_fd)[g * KP + a], fdb = ((const P2 *)s_fd)[g * KP + b];  // (fc, dfc)
+  const T fa = fda.x, dfa = fda.y, fb = fdb.x, dfb = fdb.y;
   const bool in = e <= rc;
   T sine, cosine;
   Math<T>::sc(pirc * e, sine, cosine);
@@ -680,8 +694,8 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ void ang_bwd_term(uint32_t q, bool active, const SFDev &sf, const T *s_d,
-                                             const T *s_w, const T *s_fc, const T *s_dfc, const T *s_gA,
-                                             const T *s_gW, int KP, double *s_T, double *s_F, T (&vir)[6]) {
+                                             const T *s_rw, const T *s_fd, const T *s_gA, const T *s_gW, int KP,
+                                             double *s_T, double *s_F, T (&vir)[6]) {
   if (!active) return;
   const int a = q & 0xffff, b = q >> 16;
   const T ax = s_d[4 * a], ay = s_d[4 * a + 1], az = s_d[4 * a + 2], ra = s_d[4 * a + 3];
@@ -693,7 +707,7 @@ __device__ __forceinline__ void ang_bwd_term(uint32_t q, bool active, const SFDe
   const T cth = (ra * ra + rb * rb - e2) * inv2ab;
   const T S2 = ra * ra + rb * rb + e2;
   const T dca = T(1) / rb - cth / ra, dcb = T(1) / ra - cth / rb, dce = -e * (T(2) * inv2ab);
-  const T wab = s_w[a] * s_w[b];
+  const T wab = s_rw[2 * a + 1] * s_rw[2 * b + 1];
   T Ta = 0, Tb = 0, Te = 0;
   T SA = 0, SB = 0, SC = 0;
   T fa = 0, dfa = 0, fb = 0, dfb = 0, fe = 0, dfe = 0;
@@ -713,10 +727,10 @@ __device__ __forceinline__ void ang_bwd_term(uint32_t q, bool active, const SFDe
       cur_g = g;
       SA = SB = SC = T(0);
       const T rc = (T)sf.group_rc[g];
-      fa = s_fc[g * KP + a];
-      dfa = s_dfc[g * KP + a];
-      fb = s_fc[g * KP + b];
-      dfb = s_dfc[g * KP + b];
+      fa = s_fd[2 * (g * KP + a)];
+      dfa = s_fd[2 * (g * KP + a) + 1];
+      fb = s_fd[2 * (g * KP + b)];
+      dfb = s_fd[2 * (g * KP + b) + 1];
       const bool in = e <= rc;
       T sine, cosine;
       Math<T>::sc((T)kPi * e / rc, sine, cosine);
@@ -772,11 +786,9 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   double *s_T = s_vir + 6 * W;     // [W][KP]
   double *s_F = s_T + W * KP;      // [W][3 KP]
   T *s_d = (T *)(s_F + 3 * W * KP);
-  T *s_w = s_d + 4 * KP;
-  T *s_ri = s_w + KP;  // 1 / r
-  T *s_fc = s_ri + KP;
-  T *s_dfc = s_fc + G * KP;
-  T *s_g = s_dfc + G * KP;
+  T *s_rw = s_d + 4 * KP;    // [KP] pairs (1 / r, species weight)
+  T *s_fd = s_rw + 2 * KP;   // [G][KP] pairs (fc, dfc/dr) per cutoff group
+  T *s_g = s_fd + 2 * G * KP;
   T *s_gA = s_g + Dp;  // per angular line: g of the plain and of the weighted feature
   // s_gW starts on a 4-value boundary: with 16-byte aligned blocks (every
   // region above has a multiple of 4 values) the grid form reads four lines'
@@ -797,13 +809,13 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
     s_d[4 * a + 1] = (T)dd[1];
     s_d[4 * a + 2] = (T)dd[2];
     s_d[4 * a + 3] = (T)r;
-    s_ri[a] = (T)(1.0 / r);
-    s_w[a] = (T)wz[species[nb.nb_j[start + a]]];
+    s_rw[2 * a] = (T)(1.0 / r);
+    s_rw[2 * a + 1] = (T)wz[species[nb.nb_j[start + a]]];
     for (int g = 0; g < sf.n_groups; ++g) {
       const double rc = sf.group_rc[g];
       const bool in = r <= rc;
-      s_fc[g * KP + a] = in ? (T)(0.5 * (cos(kPi * r / rc) + 1.0)) : T(0);
-      s_dfc[g * KP + a] = in ? (T)(0.5 * (-sin(kPi * r / rc)) * (kPi / rc)) : T(0);
+      s_fd[2 * (g * KP + a)] = in ? (T)(0.5 * (cos(kPi * r / rc) + 1.0)) : T(0);
+      s_fd[2 * (g * KP + a) + 1] = in ? (T)(0.5 * (-sin(kPi * r / rc)) * (kPi / rc)) : T(0);
     }
   }
   for (int k = tid; k < 4 * W * KP; k += 64 * W) s_T[k] = 0.0;  // s_T and s_F
@@ -845,7 +857,7 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
 
   // radial: one neighbour per thread
   for (int a = tid; a < K; a += 64 * W) {
-    const T r = s_d[4 * a + 3], w = s_w[a];
+    const T r = s_d[4 * a + 3], w = s_rw[2 * a + 1];
     T acc = 0;
     for (int s = 0; s < sf.n_rad; ++s) {
       const int f = sf.rad_feat[s];
@@ -853,7 +865,7 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
       const T A = (T)sf.rad_a[s];
       const T dr = r - (T)sf.rad_rs[s];
       const T geff = s_g[f] + w * s_g[f + sf.nsf];
-      acc += geff * Math<T>::ex(-A * dr * dr) * (T(-2) * A * dr * s_fc[g * KP + a] + s_dfc[g * KP + a]);
+      acc += geff * Math<T>::ex(-A * dr * dr) * (T(-2) * A * dr * s_fd[2 * (g * KP + a)] + s_fd[2 * (g * KP + a) + 1]);
     }
     s_T[a] = (double)acc;  // wave 0's row
   }
@@ -874,10 +886,10 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
         const bool act = t0 + u * ST + lane < nt;
         const uint32_t q = pop_q4(qc);
         if constexpr (GNZ > 0)
-          ang_bwd_grid<T, GNZ>(q, act, sf, s_d, s_ri, s_w, s_fc, s_dfc, s_gA, s_gW, s_eta, KP, rcA, pirc,
-                               s_T + wv * KP, s_F + 3 * wv * KP, vir);
+          ang_bwd_grid<T, GNZ>(q, act, sf, s_d, s_rw, s_fd, s_gA, s_gW, s_eta, KP, rcA, pirc, s_T + wv * KP,
+                               s_F + 3 * wv * KP, vir);
         else
-          ang_bwd_term<T>(q, act, sf, s_d, s_w, s_fc, s_dfc, s_gA, s_gW, KP, s_T + wv * KP, s_F + 3 * wv * KP,
+          ang_bwd_term<T>(q, act, sf, s_d, s_rw, s_fd, s_gA, s_gW, KP, s_T + wv * KP, s_F + 3 * wv * KP,
                           vir);
       }
 #pragma unroll
