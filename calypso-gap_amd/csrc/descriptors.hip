@@ -452,11 +452,10 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
   const int i = blockIdx.x;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int KP = L.KP, Dp = L.Dp;
-  double *s_xn = (double *)smem;  // [W]
-  T *s_d = (T *)(s_xn + W);
-  T *s_w = s_d + 4 * KP;
-  T *s_ri = s_w + KP;           // 1 / r
-  T *s_fc = s_ri + KP;
+  double *s_xn = (double *)smem;  // [W], padded to 16 bytes
+  T *s_d = (T *)(s_xn + ((W + 1) & ~1));
+  T *s_rw = s_d + 4 * KP;       // [KP] pairs (1 / r, species weight)
+  T *s_fc = s_rw + 2 * KP;
   T *s_feat = s_fc + L.G * KP;
   T *s_part = s_feat + Dp;      // [W * NV]
   T *s_sum = s_part + W * NV;   // [NV]
@@ -474,8 +473,8 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
     s_d[4 * a + 1] = (T)dd[1];
     s_d[4 * a + 2] = (T)dd[2];
     s_d[4 * a + 3] = (T)r;
-    s_ri[a] = (T)(1.0 / r);
-    s_w[a] = (T)wz[species[nb.nb_j[start + a]]];
+    s_rw[2 * a] = (T)(1.0 / r);
+    s_rw[2 * a + 1] = (T)wz[species[nb.nb_j[start + a]]];
     for (int g = 0; g < sf.n_groups; ++g) {
       const double rc = sf.group_rc[g];
       s_fc[g * KP + a] = (r <= rc) ? (T)(0.5 * (cos(kPi * r / rc) + 1.0)) : T(0);
@@ -490,7 +489,7 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
 #pragma unroll
     for (int l = 0; l < RCH; ++l) acc[l] = accw[l] = T(0);
     for (int a = tid; a < K; a += 64 * W) {
-      const T r = s_d[4 * a + 3], w = s_w[a];
+      const T r = s_d[4 * a + 3], w = s_rw[2 * a + 1];
 #pragma unroll
       for (int l = 0; l < RCH; ++l) {
         const int s = c0 + l;
@@ -533,9 +532,11 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
         const T ex = bx - ax, ey = by - ay, ez = bz - az;
         const T e2 = ex * ex + ey * ey + ez * ez;
         const T e = Math<T>::sq(e2);
-        const T cth = (ra * ra + rb * rb - e2) * (T(0.5) * s_ri[a] * s_ri[b]);
+        typedef typename Pair<T>::type P2;
+        const P2 rwa = ((const P2 *)s_rw)[a], rwb = ((const P2 *)s_rw)[b];  // (1/r, w)
+        const T cth = (ra * ra + rb * rb - e2) * (T(0.5) * rwa.x * rwb.x);
         const T S2 = ra * ra + rb * rb + e2;
-        const T wab = s_w[a] * s_w[b];
+        const T wab = rwa.y * rwb.y;
         const T fe = (e <= rc) ? T(0.5) * (Math<T>::cs(pirc * e) + T(1)) : T(0);
         const T F3 = act ? s_fc[g * KP + a] * s_fc[g * KP + b] * fe : T(0);
         T Pp[NZ], Pm[NZ], dPp[NZ], dPm[NZ];
@@ -992,7 +993,7 @@ hipError_t bwd_grid(hipStream_t s, int N, const int *species, const double *wz, 
 template <typename T, int NZ, int W>
 size_t fwd_grid_lds(const Lds &L) {
   constexpr int NV = GridShape<T, NZ>::NV;
-  return W * sizeof(double) + (size_t)(6 * L.KP + L.G * L.KP + L.Dp + W * NV + NV) * sizeof(T);
+  return ((W + 1) & ~1) * sizeof(double) + (size_t)(6 * L.KP + L.G * L.KP + L.Dp + W * NV + NV) * sizeof(T);
 }
 
 template <typename T, int NZ, int W>
