@@ -138,8 +138,12 @@ __device__ __forceinline__ void line_power(const SFDev &sf, int s, T cth, PowSta
   }
 }
 
+// Launch geometry of the descriptor kernels: LDS layout (neighbour slots,
+// cutoff groups, padded features) and the first atom of the launch (one
+// workgroup per atom from atom0 on).
 struct Lds {
   int KP, G, Dp;
+  int atom0;
 };
 
 // Triplet indices of wave steps t, t + stride, .. (4 steps) for one lane; the
@@ -243,7 +247,7 @@ __global__ __launch_bounds__(256) void k_desc_fwd(int N, const int *__restrict__
                                                   SFDev sf, Lds L, const double *__restrict__ theta_inv,
                                                   const double *__restrict__ mu, int split_mode, DescOut out) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int i = blockIdx.x;
+  const int i = L.atom0 + blockIdx.x;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int KP = L.KP, Dp = L.Dp;
   double *s_xn = (double *)smem;  // [W]
@@ -449,7 +453,7 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
   constexpr int NE = GridShape<T, NZ>::NE, NV = GridShape<T, NZ>::NV;
   constexpr int RCH = 16;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int i = blockIdx.x;
+  const int i = L.atom0 + blockIdx.x;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int KP = L.KP, Dp = L.Dp;
   double *s_xn = (double *)smem;  // [W], padded to 16 bytes
@@ -778,7 +782,7 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
                                                   SFDev sf, Lds L, GradSrc gs, double *__restrict__ slot_f,
                                                   double *__restrict__ atom_vir) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int i = blockIdx.x;
+  const int i = L.atom0 + blockIdx.x;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int KP = L.KP, Dp = L.Dp, G = L.G;
   // per-wave accumulators in FP64 for both precisions (LDS FP64 atomics are
@@ -1037,8 +1041,8 @@ int max_neighbors_for_lds(bool fp64, int G, int Dp) {
 hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *species, const double *wz,
                                   const NeighborBufs &nb, const TripletBufs &tb, const SFDev &sf,
                                   const double *theta_inv, const double *mu, int split_mode, DescOut out, int chunk,
-                                  int waves, int KP) {
-  Lds L{KP, sf.n_groups > 0 ? sf.n_groups : 1, out.Dp};
+                                  int waves, int KP, int atom0) {
+  Lds L{KP, sf.n_groups > 0 ? sf.n_groups : 1, out.Dp, atom0};
   if (N <= 0) return hipSuccess;
   if (sf.grid) {
     if (fp64) return fwd_grid<double>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
@@ -1066,8 +1070,9 @@ hipError_t bwd_fit(int waves, hipStream_t s, int N, const int *species, const do
 
 hipError_t launch_descriptors_bwd(hipStream_t s, bool fp64, int N, const int *species, const double *wz,
                                   const NeighborBufs &nb, const TripletBufs &tb, const SFDev &sf, int Dp,
-                                  const GradSrc &gs, double *slot_f, double *atom_vir, int waves, int KP) {
-  Lds L{KP, sf.n_groups > 0 ? sf.n_groups : 1, Dp};
+                                  const GradSrc &gs, double *slot_f, double *atom_vir, int waves, int KP,
+                                  int atom0) {
+  Lds L{KP, sf.n_groups > 0 ? sf.n_groups : 1, Dp, atom0};
   if (N <= 0) return hipSuccess;
   // the LDS layout's 16-byte reads of g need KP (slots per atom) a multiple of 4
   if (KP % 4 != 0) return hipErrorInvalidValue;

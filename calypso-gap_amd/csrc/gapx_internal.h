@@ -166,10 +166,11 @@ struct TripletBufs {
 hipError_t launch_build_neighbors(hipStream_t s, int N, const int *atom_struct, const StructParams *sp,
                                   double rcut, NeighborBufs nb, double rcA, const TripletBufs *tb);
 
+// Descriptor forward for atoms [atom0, atom0 + N).
 hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *species,
                                   const double *wz, const NeighborBufs &nb, const TripletBufs &tb,
                                   const SFDev &sf, const double *theta_inv, const double *mu,
-                                  int split_mode, DescOut out, int chunk, int waves, int KP);
+                                  int split_mode, DescOut out, int chunk, int waves, int KP, int atom0 = 0);
 // dE/dx for the backward pass: either given (g64 or g32, splits = 0), or
 // finished from the contraction's split partials inside the backward kernel
 // (splits > 0): E_i = sum_s Epart[s][i] and g = -(xt E - sum_s G[s][i]) theta_inv,
@@ -187,10 +188,11 @@ struct GradSrc {
   double *g64_out = nullptr;
   float *g32_out = nullptr;
 };
+// Backward for atoms [atom0, atom0 + N).
 hipError_t launch_descriptors_bwd(hipStream_t s, bool fp64, int N, const int *species,
                                   const double *wz, const NeighborBufs &nb, const TripletBufs &tb,
                                   const SFDev &sf, int Dp, const GradSrc &gs,
-                                  double *slot_f, double *atom_vir, int waves, int KP);
+                                  double *slot_f, double *atom_vir, int waves, int KP, int atom0 = 0);
 // Largest per-atom neighbour count the descriptor kernels' LDS layout holds.
 int max_neighbors_for_lds(bool fp64, int G, int Dp);
 
@@ -209,6 +211,7 @@ hipError_t launch_forces_reduce(hipStream_t s, int N, int grad, const NeighborBu
 struct GprArgs {
   int N, Np, Dp, Mp, splits, pts_per_split;
   int rows;                       // rows per workgroup (64; the mixed kernel also 128)
+  int tile0 = 0, ntiles = 0;      // row tiles [tile0, tile0 + ntiles) of this launch (0: all)
   const double *xt64; const double *xn;
   const uint16_t *xt_hi, *xt_lo;
   const float *xt32;

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_gpr_f64(GprArgs a) {
   __shared__ double s_mn[kPC], s_c[kPC];
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int l16 = lane & 15, lq = lane >> 4;
-  const int row0 = blockIdx.x * 64 + wv * 16;
+  const int row0 = (a.tile0 + blockIdx.x) * 64 + wv * 16;
   const int split = blockIdx.y;
   const int pb = split * a.pts_per_split;
   const int pe = min(a.Mp, pb + a.pts_per_split);
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void k_gpr_finish(GprArgs a, int fp64) {
 
 hipError_t launch_gpr_fp64(hipStream_t s, const GprArgs &a) {
   if (a.pts_per_split % pc_f64(a.Dp) != 0 || a.Np % 64 != 0) return hipErrorInvalidValue;
-  dim3 grid(a.Np / 64, a.splits);
+  dim3 grid(a.ntiles > 0 ? a.ntiles : a.Np / 64, a.splits);
   switch (a.Dp) {
     case 32: k_gpr_f64<32><<<grid, 256, 0, s>>>(a); break;
     case 64: k_gpr_f64<64><<<grid, 256, 0, s>>>(a); break;

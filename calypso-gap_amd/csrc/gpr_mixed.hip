@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_gpr_mixed(GprArgs a) {
 
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int l16 = lane & 15, lq = lane >> 4;
-  const int row0 = blockIdx.x * (64 * R) + wv * (16 * R);
+  const int row0 = (a.tile0 + blockIdx.x) * (64 * R) + wv * (16 * R);
   const int split = blockIdx.y;
   const int pb = split * a.pts_per_split;
   const int pe = min(a.Mp, pb + a.pts_per_split);
@@ -202,9 +202,9 @@ __global__ __launch_bounds__(256) void k_gpr_mixed(GprArgs a) {
 template <bool BF, int DP, int PC>
 hipError_t launch_rows(hipStream_t s, const GprArgs &a) {
   if (a.rows == 128 && DP <= 128) {
-    k_gpr_mixed<BF, DP, PC, 2><<<dim3(a.Np / 128, a.splits), 256, 0, s>>>(a);
+    k_gpr_mixed<BF, DP, PC, 2><<<dim3(a.ntiles > 0 ? a.ntiles : a.Np / 128, a.splits), 256, 0, s>>>(a);
   } else {
-    k_gpr_mixed<BF, DP, PC, 1><<<dim3(a.Np / 64, a.splits), 256, 0, s>>>(a);
+    k_gpr_mixed<BF, DP, PC, 1><<<dim3(a.ntiles > 0 ? a.ntiles : a.Np / 64, a.splits), 256, 0, s>>>(a);
   }
   return hipGetLastError();
 }

@@ -158,3 +158,29 @@ def test_reloads_equal_fresh_contexts(pop):
         assert np.array_equal(r["energy"], f["energy"]) and np.array_equal(r["forces"], f["forces"]), k
         assert np.array_equal(r["stress"], f["stress"]), k
     ctx.close()
+
+
+@pytest.mark.parametrize("flags", [gapx.GRAD, gapx.GRAD | gapx.MIXED], ids=["fp64", "mixed"])
+def test_two_half_schedule_equals_one_piece(pop, flags):
+    """Batches of at least 16 384 atoms run as two halves on two streams; the
+    results equal the one-piece schedule (used while per-stage timing is on)
+    bit for bit, with graph replay as well."""
+    m, _ = pop
+    structs = synth.population(160, seed=3005)
+    assert sum(len(s["species"]) for s in structs) >= 16384
+    ctx = gapx.Context(m)
+    ctx.load(structs, 6.0)
+    ctx.timing(True)
+    ctx.run(flags)
+    one = ctx.results()
+    ctx.timing(False)
+    ctx.run(flags)
+    two = ctx.results()
+    ctx.use_graph(True)
+    for _ in range(2):
+        ctx.run(flags)
+    three = ctx.results()
+    for r in (two, three):
+        for k in ("energy", "forces", "stress", "eatom"):
+            assert np.array_equal(one[k], r[k]), k
+    ctx.close()
