@@ -579,7 +579,7 @@ static int neighbor_cap(Ctx &c, int flags) {
 // Neighbour list: bins, then one wavefront per atom writing up to kcap slots.
 // Nothing here waits for the device; counts above kcap are detected when the
 // results are fetched (the run is then repeated with a larger kcap).
-static int run_neighbors(Ctx &c, int flags, Bufs &b, bool grad) {
+static int run_neighbors(Ctx &c, int flags, Bufs &b) {
   NeighborBufs &nb = b.nb;
   int rc;
   int *stats;
@@ -659,7 +659,7 @@ static int gpr_layout(Ctx &c, GprArgs &g, bool fp64) {
   // Split the sparse points over grid.y until there are about gpr_blocks
   // workgroups (resident waves; more splits only add partial-sum traffic).
   const int tiles = c.Np / g.rows;
-  const int chunk = gpr_points_per_chunk(fp64, m.Dp);
+  const int chunk = gpr_points_per_chunk(m.Dp);
   int splits = std::max(1, (t.gpr_blocks + tiles - 1) / tiles);
   splits = std::min(splits, m.Mp / chunk);
   const int per = round_up((m.Mp + splits - 1) / splits, chunk);
@@ -720,10 +720,9 @@ static int enqueue_run(Ctx &c, int flags) {
   Bufs b;
   int rc;
   stage_begin(c, kNeighbors);
-  if ((rc = run_neighbors(c, flags, b, grad))) return rc;
+  if ((rc = run_neighbors(c, flags, b))) return rc;
   NeighborBufs &nb = b.nb;
   const int *d_species = c.ptr<int>("species");
-  const StructParams *d_sp = c.ptr<StructParams>("sp");
   stage_end(c, kNeighbors);
 
   // descriptors
@@ -869,7 +868,7 @@ static int enqueue_surrogate(Ctx &c, Mlp &mlp, int flags) {
   Bufs b;
   int rc;
   stage_begin(c, kNeighbors);
-  if ((rc = run_neighbors(c, flags, b, grad))) return rc;
+  if ((rc = run_neighbors(c, flags, b))) return rc;
   stage_end(c, kNeighbors);
   DescOut &o = b.out;
   o.N = c.N;

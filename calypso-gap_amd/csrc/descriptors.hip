@@ -247,6 +247,7 @@ __global__ __launch_bounds__(256) void k_desc_fwd(int N, const int *__restrict__
                                                   SFDev sf, Lds L, const double *__restrict__ theta_inv,
                                                   const double *__restrict__ mu, int split_mode, DescOut out) {
   extern __shared__ __align__(16) unsigned char smem[];
+  if ((int)blockIdx.x >= N) return;  // one workgroup per atom of [atom0, atom0 + N)
   const int i = L.atom0 + blockIdx.x;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int KP = L.KP, Dp = L.Dp;
@@ -453,6 +454,7 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
   constexpr int NE = GridShape<T, NZ>::NE, NV = GridShape<T, NZ>::NV;
   constexpr int RCH = 16;
   extern __shared__ __align__(16) unsigned char smem[];
+  if ((int)blockIdx.x >= N) return;  // one workgroup per atom of [atom0, atom0 + N)
   const int i = L.atom0 + blockIdx.x;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int KP = L.KP, Dp = L.Dp;
@@ -782,6 +784,7 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
                                                   SFDev sf, Lds L, GradSrc gs, double *__restrict__ slot_f,
                                                   double *__restrict__ atom_vir) {
   extern __shared__ __align__(16) unsigned char smem[];
+  if ((int)blockIdx.x >= N) return;  // one workgroup per atom of [atom0, atom0 + N)
   const int i = L.atom0 + blockIdx.x;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int KP = L.KP, Dp = L.Dp, G = L.G;

@@ -228,7 +228,7 @@ struct GprArgs {
 };
 hipError_t launch_gpr_fp64(hipStream_t s, const GprArgs &a);
 hipError_t launch_gpr_mixed(hipStream_t s, const GprArgs &a);
-int gpr_points_per_chunk(bool fp64, int Dp); // sparse points per LDS chunk
+int gpr_points_per_chunk(int Dp);  // sparse points per LDS chunk (both precisions)
 hipError_t launch_gpr_finish(hipStream_t s, const GprArgs &a, bool fp64);
 
 // MLP surrogate (mlp.hip), as used by the context.

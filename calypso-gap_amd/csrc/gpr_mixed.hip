@@ -226,10 +226,10 @@ hipError_t launch_typed(hipStream_t s, const GprArgs &a) {
 
 }  // namespace
 
-int gpr_points_per_chunk(bool fp64, int Dp) { return Dp <= 128 ? 64 : 32; }  // both precisions
+int gpr_points_per_chunk(int Dp) { return Dp <= 128 ? 64 : 32; }
 
 hipError_t launch_gpr_mixed(hipStream_t s, const GprArgs &a) {
-  if (a.pts_per_split % gpr_points_per_chunk(false, a.Dp) != 0 || (a.rows != 64 && a.rows != 128) ||
+  if (a.pts_per_split % gpr_points_per_chunk(a.Dp) != 0 || (a.rows != 64 && a.rows != 128) ||
       a.Np % a.rows != 0)
     return hipErrorInvalidValue;
   return a.bf16 ? launch_typed<true>(s, a) : launch_typed<false>(s, a);

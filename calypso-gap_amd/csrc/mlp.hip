@@ -322,6 +322,7 @@ __global__ void k_gather_rows(int rows, int n_valid, uint64_t seed, uint64_t ste
                               float *__restrict__ bg) {
   const int r = blockIdx.x;
   const int j = threadIdx.x;
+  if (r >= rows) return;
   // SplitMix64 of (seed, step, r): a uniform row of the data set
   uint64_t z = seed + 0x9E3779B97F4A7C15ull * (step * 1000003ull + (uint64_t)r + 1);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
