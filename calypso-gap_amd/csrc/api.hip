@@ -326,6 +326,7 @@ struct Ctx {
   int device = 0;
   int nstruct = 0, N = 0, Np = 0, nbins = 0;
   bool fused_bins = false;  // every structure small enough for k_bin_struct
+  int fused_atoms = 0, fused_bins_max = 0;  // its LDS sizes: largest structure and bin count
   double rcut = 6.0;
   std::vector<int> natoms, offsets;
   std::vector<double> lat;
@@ -437,11 +438,14 @@ static int struct_params(Ctx &c) {
   }
   c.nbins = bin_base;
   // one-launch binning when every structure fits its LDS layout
-  c.fused_bins = std::getenv("GAPX_NO_FUSED_BINS") == nullptr;
+  c.fused_atoms = c.fused_bins_max = 0;
   for (int s = 0; s < c.nstruct; ++s) {
     const StructParams &P = c.sp[s];
-    if (P.natoms > kFusedMaxAtoms || P.nb[0] * P.nb[1] * P.nb[2] > kFusedBins) c.fused_bins = false;
+    c.fused_atoms = std::max(c.fused_atoms, P.natoms);
+    c.fused_bins_max = std::max(c.fused_bins_max, P.nb[0] * P.nb[1] * P.nb[2]);
   }
+  c.fused_bins = std::getenv("GAPX_NO_FUSED_BINS") == nullptr && c.fused_atoms <= kFusedAtoms &&
+                 fused_bin_lds_bytes(c.fused_atoms, c.fused_bins_max) <= kFusedLdsBytes;
   return GAPX_OK;
 }
 
@@ -604,7 +608,8 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b) {
   const double *d_pos = c.ptr<double>("pos");
   const StructParams *d_sp = c.ptr<StructParams>("sp");
   if (c.fused_bins) {
-    GAPX_HIP(launch_bin_structures(c.stream, c.nstruct, d_pos, d_sp, nb, c.nbins, stats));
+    GAPX_HIP(launch_bin_structures(c.stream, c.nstruct, d_pos, d_sp, nb, c.nbins, c.fused_atoms, c.fused_bins_max,
+                                   stats));
   } else {
     // k_wrap_scan keeps bin_count and its ticket at zero between runs; clear
     // them (whole capacity) once after they are (re)allocated

@@ -401,6 +401,23 @@ __device__ __forceinline__ void grid_powers(T cth, const SFDev &sf, T (&Pp)[NZ],
 // (NV padded to 32 or 64; for 32 the two half-waves are folded first).  63
 // exchanges instead of NV full wave reductions.  The waves' totals are then
 // added in wave order (deterministic).
+// One butterfly level per instantiation, so every register index is a
+// compile-time constant (a loop over M left rolled by the compiler would
+// index the array at run time).
+template <typename T, int M, int NVP>
+__device__ __forceinline__ void butterfly(T (&x)[NVP], int lane) {
+  if constexpr (M >= 1) {
+    const bool upper = (lane & M) != 0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      const T send = upper ? x[j] : x[j + M];
+      const T keep = upper ? x[j + M] : x[j];
+      x[j] = keep + __shfl_xor(send, M, 64);
+    }
+    butterfly<T, M / 2, NVP>(x, lane);
+  }
+}
+
 template <typename T, int NV, int W>
 __device__ __forceinline__ void group_sum(T (&v)[NV], int wv, int lane, int tid, T *s_part, T *s_out) {
   constexpr int NVP = NV <= 32 ? 32 : 64;
@@ -412,16 +429,7 @@ __device__ __forceinline__ void group_sum(T (&v)[NV], int wv, int lane, int tid,
 #pragma unroll
     for (int j = 0; j < 32; ++j) x[j] += __shfl_xor(x[j], 32, 64);
   }
-#pragma unroll
-  for (int m = NVP / 2; m >= 1; m >>= 1) {
-    const bool upper = (lane & m) != 0;
-#pragma unroll
-    for (int j = 0; j < m; ++j) {
-      const T send = upper ? x[j] : x[j + m];
-      const T keep = upper ? x[j + m] : x[j];
-      x[j] = keep + __shfl_xor(send, m, 64);
-    }
-  }
+  butterfly<T, NVP / 2, NVP>(x, lane);
   if (W == 1) {
     if (lane < NV) s_out[lane] = x[0];
     __syncthreads();
@@ -489,11 +497,13 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
   for (int d = tid; d < Dp; d += 64 * W) s_feat[d] = T(0);
   __syncthreads();
 
-  // radial lines, RCH at a time
+  // radial lines, RCH at a time: plain sums in acc[0, RCH), weighted ones in
+  // acc[RCH, 2 RCH), reduced over the group with the angular pass's exchange
+  static_assert(2 * RCH <= NV, "radial sums reuse the angular reduction buffers");
   for (int c0 = 0; c0 < sf.n_rad; c0 += RCH) {
-    T acc[RCH], accw[RCH];
+    T acc[2 * RCH];
 #pragma unroll
-    for (int l = 0; l < RCH; ++l) acc[l] = accw[l] = T(0);
+    for (int l = 0; l < 2 * RCH; ++l) acc[l] = T(0);
     for (int a = tid; a < K; a += 64 * W) {
       const T r = s_d[4 * a + 3], w = s_rw[2 * a + 1];
 #pragma unroll
@@ -503,11 +513,16 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
           const T dr = r - (T)sf.rad_rs[s];
           const T t = Math<T>::ex(-(T)sf.rad_a[s] * dr * dr) * s_fc[sf.rad_group[s] * KP + a];
           acc[l] += t;
-          accw[l] += t * w;
+          acc[RCH + l] += t * w;
         }
       }
     }
-    chunk_reduce<T, RCH, W>(acc, accw, wv, lane, tid, c0, sf.n_rad, sf.rad_feat, sf.nsf, s_part, s_feat);
+    group_sum<T, 2 * RCH, W>(acc, wv, lane, tid, s_part, s_sum);
+    for (int j = tid; j < 2 * RCH; j += 64 * W) {
+      const int l = j % RCH;
+      if (c0 + l < sf.n_rad) s_feat[sf.rad_feat[c0 + l] + (j >= RCH ? sf.nsf : 0)] = s_sum[j];
+    }
+    __syncthreads();
   }
 
   // angular lines in grid form, NE etas per pass

@@ -130,13 +130,17 @@ struct NeighborBufs {
 hipError_t launch_wrap_scan(hipStream_t s, int N, int nbins, const double *pos, const int *atom_struct,
                             const StructParams *sp, NeighborBufs nb, int *ticket, int *stats);
 hipError_t launch_scatter_bins(hipStream_t s, int N, int nbins, NeighborBufs nb);
-// The whole binning of structures with at most kFusedAtoms atoms and
-// kFusedBins bins each, one workgroup per structure (also clears stats[0..7]).
-// The host uses it for batches of small structures only (kFusedMaxAtoms):
-// one large structure would leave all but one CU idle.
-constexpr int kFusedAtoms = 8192, kFusedBins = 2048, kFusedMaxAtoms = 1024;
+// The whole binning of structures with at most kFusedAtoms atoms each, one
+// workgroup per structure (also clears stats[0..7]); its LDS holds the
+// largest structure's bins and atoms (fused_bin_lds_bytes, at most
+// kFusedLdsBytes).  Measured on the 4096-atom workload, one workgroup takes
+// longer than the three launches (wrap + count + scan, scatter, rank), so
+// the one-launch path is for batches of structures up to kFusedAtoms atoms.
+constexpr int kFusedAtoms = 1024;
+constexpr size_t kFusedLdsBytes = 64 * 1024;
+size_t fused_bin_lds_bytes(int atoms_max, int bins_max);
 hipError_t launch_bin_structures(hipStream_t s, int nstruct, const double *pos, const StructParams *sp,
-                                 NeighborBufs nb, int nbins_total, int *stats);
+                                 NeighborBufs nb, int nbins_total, int atoms_max, int bins_max, int *stats);
 hipError_t launch_reverse_slots(hipStream_t s, int N, NeighborBufs nb);
 hipError_t launch_fill_int(hipStream_t s, int *p, int n, int v);
 hipError_t launch_zero_ints(hipStream_t s, int *a, int na, int *b, int nb, int *c, int nc);

@@ -152,43 +152,54 @@ __global__ __launch_bounds__(64) void k_rank_bins(int nbins, NeighborBufs nb) {
 // One workgroup per structure does the whole binning in LDS: wrap and bin
 // every atom, count per bin, exclusive scan, scatter, then order each bin's
 // atoms by index and write the bin-sorted records; workgroup 0 also clears
-// the run's statistics.  Same output as k_wrap_scan +
-// k_scatter + k_rank_bins, in one launch, for structures of at most
-// kFusedAtoms atoms and kFusedBins bins.
+// the run's statistics.  Same output as k_wrap_scan + k_scatter +
+// k_rank_bins, in one launch, for structures of at most kFusedAtoms atoms
+// (bins_max bins and atoms_max atoms size the LDS arrays, see
+// fused_bin_lds_bytes).  Thread t owns atoms t, t + 1024, .. from the wrap to
+// the sorted write and keeps their shifts and bins in registers (the wrapped
+// positions it wrote itself are read back once, all together, at the end).
 __global__ __launch_bounds__(1024) void k_bin_struct(const double *__restrict__ pos,
                                                      const StructParams *__restrict__ sp, NeighborBufs nb,
-                                                     int nbins_total, int *__restrict__ stats) {
-  __shared__ int s_cnt[kFusedBins + 1];  // counts, then local bin starts
-  __shared__ int s_cur[kFusedBins];      // scatter cursors
-  __shared__ int s_tmp[kFusedAtoms];     // local atom index per slot, unsorted inside each bin
-  __shared__ int s_part[1024];
+                                                     int nbins_total, int bins_max, int *__restrict__ stats) {
+  constexpr int PER = kFusedAtoms / 1024;
+  extern __shared__ int smem_i[];
+  // [bins_max + 1]: counts, then bin starts, then (after the scatter, whose
+  // cursors they are) bin ends
+  int *s_cnt = smem_i;
+  // [max(atoms_max, 1024)]: the scan's partial sums, then the local atom
+  // index of every slot, unsorted inside each bin
+  int *s_tmp = s_cnt + bins_max + 1;
+  int *s_part = s_tmp;
   const int s = blockIdx.x, t = threadIdx.x;
   const StructParams &P = sp[s];
   const int n = P.natoms, base = P.atom_base, nbs = P.nb[0] * P.nb[1] * P.nb[2];
   if (s == 0 && t < 8) stats[t] = 0;
-  for (int b = t; b < nbs; b += 1024) {
-    s_cnt[b] = 0;
-    s_cur[b] = 0;
-  }
+  for (int b = t; b < nbs; b += 1024) s_cnt[b] = 0;
   __syncthreads();
-  for (int a = t; a < n; a += 1024) {
-    const int i = base + a;
-    const double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-    int bk3[3], sh[3];
-    for (int k = 0; k < 3; ++k) {
-      const double f = r[0] * P.Linv[k] + r[1] * P.Linv[3 + k] + r[2] * P.Linv[6 + k];
-      const double fl = floor(f);
-      sh[k] = (int)fl;
-      const int bk = (int)floor((f - fl) * P.nb[k]);
-      bk3[k] = bk < 0 ? 0 : (bk >= P.nb[k] ? P.nb[k] - 1 : bk);
+  int sh[PER][3], lb[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int a = t + 1024 * u;
+    lb[u] = 0;
+    if (a < n) {
+      const int i = base + a;
+      const double r[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+      int bk3[3];
+      for (int k = 0; k < 3; ++k) {
+        const double f = r[0] * P.Linv[k] + r[1] * P.Linv[3 + k] + r[2] * P.Linv[6 + k];
+        const double fl = floor(f);
+        sh[u][k] = (int)fl;
+        const int bk = (int)floor((f - fl) * P.nb[k]);
+        bk3[k] = bk < 0 ? 0 : (bk >= P.nb[k] ? P.nb[k] - 1 : bk);
+      }
+      for (int c = 0; c < 3; ++c) {
+        nb.wpos[3 * i + c] = r[c] - sh[u][0] * P.L[c] - sh[u][1] * P.L[3 + c] - sh[u][2] * P.L[6 + c];
+        nb.shift[3 * i + c] = sh[u][c];
+      }
+      lb[u] = (bk3[0] * P.nb[1] + bk3[1]) * P.nb[2] + bk3[2];
+      nb.atom_bin[i] = P.bin_base + lb[u];
+      atomicAdd(&s_cnt[lb[u]], 1);
     }
-    for (int c = 0; c < 3; ++c) {
-      nb.wpos[3 * i + c] = r[c] - sh[0] * P.L[c] - sh[1] * P.L[3 + c] - sh[2] * P.L[6 + c];
-      nb.shift[3 * i + c] = sh[c];
-    }
-    const int lb = (bk3[0] * P.nb[1] + bk3[1]) * P.nb[2] + bk3[2];
-    nb.atom_bin[i] = P.bin_base + lb;
-    atomicAdd(&s_cnt[lb], 1);
   }
   __syncthreads();
   // exclusive scan of the counts
@@ -209,22 +220,31 @@ __global__ __launch_bounds__(1024) void k_bin_struct(const double *__restrict__ 
     s_cnt[b] = run;
     run += v;
   }
-  if (t == 0) s_cnt[nbs] = n;
   __syncthreads();
   for (int b = t; b < nbs; b += 1024) nb.bin_start[P.bin_base + b] = base + s_cnt[b];
   if (t == 0 && s == (int)gridDim.x - 1) nb.bin_start[nbins_total] = base + n;
-  for (int a = t; a < n; a += 1024) {
-    const int lb = nb.atom_bin[base + a] - P.bin_base;
-    s_tmp[s_cnt[lb] + atomicAdd(&s_cur[lb], 1)] = a;
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int a = t + 1024 * u;
+    if (a < n) s_tmp[atomicAdd(&s_cnt[lb[u]], 1)] = a;
   }
   __syncthreads();
-  for (int q = t; q < n; q += 1024) {
-    const int a = s_tmp[q];
-    const int lb = nb.atom_bin[base + a] - P.bin_base;
-    const int b0 = s_cnt[lb], b1 = s_cnt[lb + 1];
-    int rank = 0;
-    for (int k = b0; k < b1; ++k) rank += (s_tmp[k] < a) ? 1 : 0;
-    put_sorted(nb, base + b0 + rank, base + a);
+  // each atom's rank inside its bin = the number of smaller indices there
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int a = t + 1024 * u;
+    if (a < n) {
+      const int b0 = lb[u] > 0 ? s_cnt[lb[u] - 1] : 0, b1 = s_cnt[lb[u]];
+      int rank = 0;
+      for (int k = b0; k < b1; ++k) rank += (s_tmp[k] < a) ? 1 : 0;
+      const int q = base + b0 + rank, j = base + a;
+      nb.bin_atoms[q] = j;
+      double2 *p2 = (double2 *)(nb.spos + 4 * (size_t)q);
+      p2[0] = make_double2(nb.wpos[3 * j], nb.wpos[3 * j + 1]);
+      p2[1] = make_double2(nb.wpos[3 * j + 2], 0.0);
+      ((int4 *)nb.sinfo)[q] = make_int4(j, sh[u][0], sh[u][1], sh[u][2]);
+    }
   }
 }
 
@@ -420,17 +440,28 @@ __device__ void list_stats(int N, int kcap, const int *__restrict__ nb_count, co
   __shared__ long long red[NW][6];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   long long v[6] = {0, 0, 0, 0, 0, 0};
-  for (int a = t; a < N; a += NT) {
-    const int r = nb_raw[a];
-    v[0] += nb_count[a];
-    v[1] = max(v[1], (long long)r);
-    v[2] += (r > kcap) ? 1 : 0;
-    if (t_raw) {
-      const int tr = t_raw[a];
-      v[3] += t_count[a];
-      v[4] = max(v[4], (long long)tr);
-      v[5] += (tr > tcap) ? 1 : 0;
+  // four atoms per thread per step, their loads issued together
+  constexpr int U = 4;
+  for (int a0 = t; a0 < N; a0 += NT * U) {
+    int r[U], c[U], tr[U], tc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int a = min(a0 + NT * u, N - 1);
+      r[u] = nb_raw[a];
+      c[u] = nb_count[a];
+      tr[u] = t_raw ? t_raw[a] : 0;
+      tc[u] = t_raw ? t_count[a] : 0;
     }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (a0 + NT * u < N) {
+        v[0] += c[u];
+        v[1] = max(v[1], (long long)r[u]);
+        v[2] += (r[u] > kcap) ? 1 : 0;
+        v[3] += tc[u];
+        v[4] = max(v[4], (long long)tr[u]);
+        v[5] += (tr[u] > tcap) ? 1 : 0;
+      }
   }
   for (int o = 32; o > 0; o >>= 1) {
 #pragma unroll
@@ -458,12 +489,14 @@ __device__ __forceinline__ V wave_reduce_sum(V v) {
 }
 
 // Everything after the backward pass in one launch of 256-thread blocks:
-//   blocks [0, G)            forces, one wavefront per atom (four atoms per
-//                            block): F_m = sum of own slots - sum of reverse
-//                            slots in slot order (zeros without grad);
-//   blocks [G, G + nstruct)  energy and virial of one structure, stress in GPa
+//   blocks [0, nstruct)      energy and virial of one structure, stress in GPa
 //                            (gap_calc.f90:152-154, 189-203);
-//   block G + nstruct        list statistics (see launch_forces_reduce).
+//   block nstruct            list statistics (see launch_forces_reduce);
+//   blocks nstruct + 1 + g   forces, one wavefront per atom (four atoms per
+//                            block g < G): F_m = sum of own slots - sum of
+//                            reverse slots in slot order (zeros without grad).
+// The per-structure sums loop over their atoms, so they go first: they run
+// beside the gather blocks instead of after them.
 __global__ __launch_bounds__(256) void k_forces_reduce(int N, int G, int nstruct, NeighborBufs nb,
                                                        const double *__restrict__ slot_f, double *__restrict__ force,
                                                        const StructParams *__restrict__ sp,
@@ -472,8 +505,8 @@ __global__ __launch_bounds__(256) void k_forces_reduce(int N, int G, int nstruct
                                                        double *__restrict__ ene, double *__restrict__ stress6,
                                                        TripletBufs tb, int *stats_out) {
   const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  if (b < G) {
-    const int m = 4 * b + wv;
+  if (b > nstruct) {
+    const int m = 4 * (b - nstruct - 1) + wv;
     if (m >= N) return;
     double f0 = 0, f1 = 0, f2 = 0;
     if (grad) {
@@ -504,7 +537,7 @@ __global__ __launch_bounds__(256) void k_forces_reduce(int N, int G, int nstruct
     }
     return;
   }
-  const int s = b - G;
+  const int s = b;
   if (s == nstruct) {
     list_stats<256>(N, nb.kcap, nb.nb_count, nb.nb_raw, tb.tcap, tb.raw ? tb.count : nullptr, tb.raw, nb.total + 3,
                     stats_out);
@@ -513,11 +546,27 @@ __global__ __launch_bounds__(256) void k_forces_reduce(int N, int G, int nstruct
   const StructParams &P = sp[s];
   __shared__ double red[4][7];
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int a = t; a < P.natoms; a += 256) {
-    const int i = P.atom_base + a;
-    acc[0] += eatom[i];
-    if (grad)
-      for (int c = 0; c < 6; ++c) acc[1 + c] += vir[6 * i + c];
+  // four atoms per thread per step, their loads issued together (the sums
+  // keep the order a = t, t + 256, ..; more would cost the gather blocks of
+  // this launch occupancy)
+  constexpr int U = 4;
+  const int n = P.natoms;
+  for (int a0 = t; a0 < n; a0 += 256 * U) {
+    double e[U], v[U][6];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = P.atom_base + min(a0 + 256 * u, n - 1);
+      e[u] = eatom[i];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) v[u][c] = grad ? vir[6 * i + c] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (a0 + 256 * u < n) {
+        acc[0] += e[u];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[1 + c] += v[u][c];
+      }
   }
 #pragma unroll
   for (int c = 0; c < 7; ++c) {
@@ -570,9 +619,15 @@ hipError_t launch_scatter_bins(hipStream_t s, int N, int nbins, NeighborBufs nb)
   return hipGetLastError();
 }
 
+size_t fused_bin_lds_bytes(int atoms_max, int bins_max) {
+  return sizeof(int) * ((size_t)bins_max + 1 + (size_t)std::max(atoms_max, 1024));
+}
+
 hipError_t launch_bin_structures(hipStream_t s, int nstruct, const double *pos, const StructParams *sp,
-                                 NeighborBufs nb, int nbins_total, int *stats) {
-  k_bin_struct<<<nstruct, 1024, 0, s>>>(pos, sp, nb, nbins_total, stats);
+                                 NeighborBufs nb, int nbins_total, int atoms_max, int bins_max, int *stats) {
+  const size_t lds = fused_bin_lds_bytes(atoms_max, bins_max);
+  if (atoms_max > kFusedAtoms || lds > kFusedLdsBytes) return hipErrorInvalidValue;
+  k_bin_struct<<<nstruct, 1024, lds, s>>>(pos, sp, nb, nbins_total, bins_max, stats);
   return hipGetLastError();
 }
 
