@@ -847,15 +847,13 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   const size_t row = (size_t)i * Dp;
   if (gs.splits > 0) {
     // the contraction's finish for this atom (same order as k_gpr_finish)
-    double E = 0.0;
-    for (int sp = 0; sp < gs.splits; ++sp) E += gs.Epart[(size_t)sp * gs.Np + i];
+    const double E = split_sum(gs.splits, [&](int sp) { return gs.Epart[(size_t)sp * gs.Np + i]; });
     if (tid == 0) gs.eatom[i] = E;
     for (int d = tid; d < Dp; d += 64 * W) {
-      double G = 0.0;
-      if (gs.Gpart)
-        for (int sp = 0; sp < gs.splits; ++sp) G += gs.Gpart[((size_t)sp * gs.Np + i) * Dp + d];
-      else
-        for (int sp = 0; sp < gs.splits; ++sp) G += (double)gs.Gpart32[((size_t)sp * gs.Np + i) * Dp + d];
+      const double G =
+          gs.Gpart ? split_sum(gs.splits, [&](int sp) { return gs.Gpart[((size_t)sp * gs.Np + i) * Dp + d]; })
+                   : split_sum(gs.splits,
+                               [&](int sp) { return (double)gs.Gpart32[((size_t)sp * gs.Np + i) * Dp + d]; });
       const double xt = gs.xt64 ? gs.xt64[row + d] : (double)gs.xt32[row + d];
       const double g = -(xt * E - G) * gs.theta_inv[d];
       if (gs.g64_out) {

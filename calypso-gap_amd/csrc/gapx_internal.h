@@ -179,6 +179,23 @@ hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *sp
 // finished from the contraction's split partials inside the backward kernel
 // (splits > 0): E_i = sum_s Epart[s][i] and g = -(xt E - sum_s G[s][i]) theta_inv,
 // summed in split order; E_i goes to eatom and g to g64/g32 as well.
+// sum_{sp < n} load(sp) in ascending sp, with the loads issued eight at a
+// time (a plain loop's accumulation makes every load wait for the one
+// before it).  n >= 0; load is called with indices in [0, n) only.
+template <typename F>
+__device__ __forceinline__ double split_sum(int n, F load) {
+  double s = 0.0;
+  for (int s0 = 0; s0 < n; s0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = load(min(s0 + u, n - 1));
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (s0 + u < n) s += v[u];
+  }
+  return s;
+}
+
 struct GradSrc {
   const double *g64 = nullptr;
   const float *g32 = nullptr;

@@ -35,7 +35,7 @@ constexpr int pc_f64(int DP) { return DP <= 128 ? 64 : 32; }
 // lq = l / 16): A[l16][lq], B[lq][l16]; result D[lq + 4 r][l16], r = 0..3
 // (rows strided by 4, unlike the FP16/BF16 forms; measured on gfx950).
 template <int DP>
-__global__ __launch_bounds__(256) void k_gpr_f64(GprArgs a) {
+__global__ __launch_bounds__(256, DP <= 160 ? 2 : 1) void k_gpr_f64(GprArgs a) {
   constexpr int kPC = pc_f64(DP);
   constexpr int SM = DP + 1;  // odd row stride (doubles): conflict-free column reads
   __shared__ double s_m[kPC * SM];
@@ -58,11 +58,28 @@ __global__ __launch_bounds__(256) void k_gpr_f64(GprArgs a) {
   for (int b = 0; b < DP / 16; ++b) G[b] = v4d{0, 0, 0, 0};
   double er = 0.0;
 
+  // 16-byte pieces of the chunk per thread, loaded KB at a time with the
+  // loads of a batch in flight together (KB bounded by the register budget)
+  constexpr int NPT = kPC * DP / 512, KB = 4;
+  static_assert(kPC * DP % 512 == 0, "whole pieces per thread");
   for (int p0 = pb; p0 < pe; p0 += kPC) {
     __syncthreads();
-    for (int e = tid; e < kPC * DP; e += 256) {
-      const int pc = e / DP, d = e - pc * DP;
-      s_m[pc * SM + d] = a.mt[(size_t)(p0 + pc) * DP + d];
+#pragma unroll
+    for (int b0 = 0; b0 < NPT; b0 += KB) {
+      double2 v[KB];
+#pragma unroll
+      for (int u = 0; u < KB; ++u)
+        if (b0 + u < NPT) {
+          const int e = tid + 256 * (b0 + u), pc = e / (DP / 2), d2 = e - pc * (DP / 2);
+          v[u] = *(const double2 *)(a.mt + (size_t)(p0 + pc) * DP + 2 * d2);
+        }
+#pragma unroll
+      for (int u = 0; u < KB; ++u)
+        if (b0 + u < NPT) {
+          const int e = tid + 256 * (b0 + u), pc = e / (DP / 2), d2 = e - pc * (DP / 2);
+          s_m[pc * SM + 2 * d2] = v[u].x;
+          s_m[pc * SM + 2 * d2 + 1] = v[u].y;
+        }
     }
     if (tid < kPC) {
       s_mn[tid] = a.mn[p0 + tid];
@@ -124,16 +141,13 @@ __global__ __launch_bounds__(256) void k_gpr_f64(GprArgs a) {
 __global__ __launch_bounds__(256) void k_gpr_finish(GprArgs a, int fp64) {
   const int i = blockIdx.x;
   if (i >= a.N) return;
-  double E = 0.0;
-  for (int s = 0; s < a.splits; ++s) E += a.Epart[(size_t)s * a.Np + i];
+  const double E = split_sum(a.splits, [&](int s) { return a.Epart[(size_t)s * a.Np + i]; });
   if (threadIdx.x == 0) a.eatom[i] = E;
   if (!a.grad) return;
   for (int d = threadIdx.x; d < a.Dp; d += blockDim.x) {
-    double G = 0.0;
-    if (fp64)
-      for (int s = 0; s < a.splits; ++s) G += a.Gpart[((size_t)s * a.Np + i) * a.Dp + d];
-    else
-      for (int s = 0; s < a.splits; ++s) G += (double)a.Gpart32[((size_t)s * a.Np + i) * a.Dp + d];
+    const double G =
+        fp64 ? split_sum(a.splits, [&](int s) { return a.Gpart[((size_t)s * a.Np + i) * a.Dp + d]; })
+             : split_sum(a.splits, [&](int s) { return (double)a.Gpart32[((size_t)s * a.Np + i) * a.Dp + d]; });
     const size_t idx = (size_t)i * a.Dp + d;
     const double xt = fp64 ? a.xt64[idx] : (double)a.xt32[idx];
     const double g = -(xt * E - G) * a.theta_inv[d];

@@ -34,6 +34,7 @@ namespace {
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef __bf16 v8b __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));  // 16 bytes of halves
 
 template <bool BF>
 struct Half;
@@ -56,8 +57,57 @@ struct Half<true> {
   }
 };
 
+// One chunk of PC sparse points goes global -> registers -> LDS in two
+// batches, each with all its loads in flight together: the point-major tile
+// (PC rows of DP halves), then the feature-major tile (DP rows of PC halves,
+// the permuted K order), both split halves, 16 bytes per piece and NPT pieces
+// per thread (indices past the tile are clamped for the load and skipped for
+// the store).  Two batches rather than one keep the staging registers within
+// the budget of two waves per SIMD.
+template <int DP, int PC, int SM, int ST>
+__device__ __forceinline__ void load_chunk(const GprArgs &a, int p0, int tid, uint16_t *s_mh, uint16_t *s_ml,
+                                           uint16_t *s_th, uint16_t *s_tl) {
+  constexpr int NE = PC * DP / 8;
+  constexpr int NPT = (NE + 255) / 256;
+  v4u h[NPT], l[NPT];
+#pragma unroll
+  for (int u = 0; u < NPT; ++u) {
+    const int e = min(tid + 256 * u, NE - 1);
+    const int pc = e / (DP / 8), q = e - pc * (DP / 8);
+    const size_t g = (size_t)(p0 + pc) * DP + 8 * q;
+    h[u] = *(const v4u *)(a.mt_hi + g);
+    l[u] = *(const v4u *)(a.mt_lo + g);
+  }
+#pragma unroll
+  for (int u = 0; u < NPT; ++u) {
+    const int e = tid + 256 * u;
+    if (NE % 256 == 0 || e < NE) {
+      const int pc = e / (DP / 8), q = e - pc * (DP / 8);
+      *(v4u *)(s_mh + pc * SM + 8 * q) = h[u];
+      *(v4u *)(s_ml + pc * SM + 8 * q) = l[u];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NPT; ++u) {
+    const int e = min(tid + 256 * u, NE - 1);
+    const int d = e / (PC / 8), r = e - d * (PC / 8);
+    const size_t g = (size_t)d * a.Mp + p0 + 8 * r;
+    h[u] = *(const v4u *)(a.mtT_hi + g);
+    l[u] = *(const v4u *)(a.mtT_lo + g);
+  }
+#pragma unroll
+  for (int u = 0; u < NPT; ++u) {
+    const int e = tid + 256 * u;
+    if (NE % 256 == 0 || e < NE) {
+      const int d = e / (PC / 8), r = e - d * (PC / 8);
+      *(v4u *)(s_th + d * ST + 8 * r) = h[u];
+      *(v4u *)(s_tl + d * ST + 8 * r) = l[u];
+    }
+  }
+}
+
 template <bool BF, int DP, int PC, int R>
-__global__ __launch_bounds__(256) void k_gpr_mixed(GprArgs a) {
+__global__ __launch_bounds__(256, 2) void k_gpr_mixed(GprArgs a) {
   typedef typename Half<BF>::vec vec;
   typedef typename Half<BF>::scalar hs;
   constexpr int SM = DP + 8;  // row stride (halves) of the point-major tile
@@ -101,20 +151,7 @@ __global__ __launch_bounds__(256) void k_gpr_mixed(GprArgs a) {
 
   for (int p0 = pb; p0 < pe; p0 += PC) {
     __syncthreads();
-    // point-major tile: PC rows of DP halves (16-byte pieces)
-    for (int e = tid; e < PC * DP / 8; e += 256) {
-      const int pc = e / (DP / 8), q = e - pc * (DP / 8);
-      const size_t g = (size_t)(p0 + pc) * DP + 8 * q;
-      *(uint4 *)(s_mh + pc * SM + 8 * q) = *(const uint4 *)(a.mt_hi + g);
-      *(uint4 *)(s_ml + pc * SM + 8 * q) = *(const uint4 *)(a.mt_lo + g);
-    }
-    // feature-major tile (points in the permuted K order): DP rows of PC halves
-    for (int e = tid; e < DP * PC / 8; e += 256) {
-      const int d = e / (PC / 8), q = e - d * (PC / 8);
-      const size_t g = (size_t)d * a.Mp + p0 + 8 * q;
-      *(uint4 *)(s_th + d * ST + 8 * q) = *(const uint4 *)(a.mtT_hi + g);
-      *(uint4 *)(s_tl + d * ST + 8 * q) = *(const uint4 *)(a.mtT_lo + g);
-    }
+    load_chunk<DP, PC, SM, ST>(a, p0, tid, s_mh, s_ml, s_th, s_tl);
     if (tid < PC) {
       s_mn[tid] = a.mn[p0 + tid];
       s_c[tid] = (float)a.coeff[p0 + tid];
@@ -201,11 +238,13 @@ __global__ __launch_bounds__(256) void k_gpr_mixed(GprArgs a) {
 
 template <bool BF, int DP, int PC>
 hipError_t launch_rows(hipStream_t s, const GprArgs &a) {
-  if (a.rows == 128 && DP <= 128) {
-    k_gpr_mixed<BF, DP, PC, 2><<<dim3(a.ntiles > 0 ? a.ntiles : a.Np / 128, a.splits), 256, 0, s>>>(a);
-  } else {
-    k_gpr_mixed<BF, DP, PC, 1><<<dim3(a.ntiles > 0 ? a.ntiles : a.Np / 64, a.splits), 256, 0, s>>>(a);
+  if constexpr (DP <= 128) {
+    if (a.rows == 128) {
+      k_gpr_mixed<BF, DP, PC, 2><<<dim3(a.ntiles > 0 ? a.ntiles : a.Np / 128, a.splits), 256, 0, s>>>(a);
+      return hipGetLastError();
+    }
   }
+  k_gpr_mixed<BF, DP, PC, 1><<<dim3(a.ntiles > 0 ? a.ntiles : a.Np / 64, a.splits), 256, 0, s>>>(a);
   return hipGetLastError();
 }
 
