@@ -1047,11 +1047,15 @@ static int ctx_autotune(Ctx &c, int flags, bool force) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     rc = fail(GAPX_ERR_HIP, "autotune: could not create events");
-  // whole evaluations, back to back on the stream (as an application runs
-  // them): the mean of `reps` runs between two events, best of two trials
+  // whole evaluations, back to back on the stream, replayed from a graph
+  // captured for each candidate (so host launch costs, which do not depend
+  // on the launch shapes, stay out of the comparison): the mean of `reps`
+  // runs between two events, best of two trials
   const int reps = 10;
+  c.use_graph = true;
   auto measure = [&](double &out) -> int {
-    int r = ctx_run(c, flags);  // warm-up (buffers, caches)
+    c.invalidate_graph();
+    int r = ctx_run(c, flags);  // capture, warm-up (buffers, caches)
     if (r) return r;
     out = 1e300;
     for (int trial = 0; trial < 2; ++trial) {

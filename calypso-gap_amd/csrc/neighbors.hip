@@ -83,10 +83,21 @@ __global__ __launch_bounds__(256) void k_wrap_scan(int N, int nbins, const doubl
   __syncthreads();
   if (!s_last) return;
   __threadfence();
-  // exclusive scan of bin_count, 256 segments
+  // exclusive scan of bin_count, 256 segments; each thread reads its
+  // segment eight counts at a time, the loads of a batch in flight together
   const int per = (nbins + 255) / 256, lo = t * per, hi = min(nbins, lo + per);
+  auto count_at = [&](int k) {
+    return __hip_atomic_load(&nb.bin_count[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
   long long sum = 0;
-  for (int k = lo; k < hi; ++k) sum += __hip_atomic_load(&nb.bin_count[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int k0 = lo; k0 < hi; k0 += 8) {
+    int v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = count_at(min(k0 + u, hi - 1));
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k0 + u < hi) sum += v[u];
+  }
   part[t] = sum;
   __syncthreads();
   for (int off = 1; off < 256; off <<= 1) {
@@ -96,10 +107,18 @@ __global__ __launch_bounds__(256) void k_wrap_scan(int N, int nbins, const doubl
     __syncthreads();
   }
   long long run = part[t] - sum;
+  for (int k0 = lo; k0 < hi; k0 += 8) {
+    int v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = count_at(min(k0 + u, hi - 1));
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k0 + u < hi) {
+        nb.bin_start[k0 + u] = (int)run;
+        run += v[u];
+      }
+  }
   for (int k = lo; k < hi; ++k) {
-    const int c = __hip_atomic_load(&nb.bin_count[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    nb.bin_start[k] = (int)run;
-    run += c;
     nb.bin_count[k] = 0;
     nb.bin_cursor[k] = 0;
   }
@@ -260,34 +279,65 @@ __device__ __forceinline__ unsigned long long nb_key(int j, int n0, int n1, int 
 // form a run of K - d entries.  Runs d and K - d together hold K entries, so
 // one wave step takes both (full lanes for K <= 64).  Consecutive entries of
 // a run have distinct a and distinct b, so the backward pass's LDS atomics on
-// per-neighbour accumulators rarely collide within one instruction.  The
-// cutoff test is a squared comparison with a relative margin of 1e-6; a
-// triplet admitted by the margin contributes zero, since the descriptor
-// kernels apply each line's cutoff to every distance.
+// per-neighbour accumulators rarely collide within one instruction.  Entry c
+// of a step (c < K) always has neighbour c as one end (a = c in run d, b = c
+// in run K - d), so each lane keeps its own records in registers (K <= 256)
+// and reads only the partner's from LDS.  The cutoff test is a squared
+// comparison with a relative margin of 1e-6; a triplet admitted by the margin
+// contributes zero, since the descriptor kernels apply each line's cutoff to
+// every distance.
 __device__ __forceinline__ void triplets_from_lds(int i, int K, const float4 *s_p, double rcA, const TripletBufs &tb,
                                                   int lane) {
+  constexpr int kOwn = 4;  // register-held record chunks: K <= 64 kOwn
   const float rc = (float)rcA * (1.0f + 1e-6f);
   const float rc2 = rc * rc;
   const size_t out = (size_t)i * tb.tcap;
   int count = 0;
-  for (int d1 = 1; 2 * d1 <= K; ++d1) {
-    // run d1 (K - d1 pairs), then run d2 = K - d1 (d1 pairs) unless it is d1
-    const int d2 = K - d1;
-    const int n1 = K - d1, n = (d2 != d1) ? K : n1;
-    for (int c0 = 0; c0 < n; c0 += 64) {
-      const int c = c0 + lane;
-      const bool first = c < n1;
-      const int a = first ? c : c - n1, b = a + (first ? d1 : d2);
-      // both records read unconditionally (idle lanes read record 0) and
-      // the tests combined without short-circuits: one LDS round trip
-      const bool live = c < n;
-      const float4 pa = s_p[live ? a : 0], pb = s_p[live ? b : 0];
-      const float ex = pb.x - pa.x, ey = pb.y - pa.y, ez = pb.z - pa.z;
-      const bool ok = live & (pa.w <= rc) & (pb.w <= rc) & (ex * ex + ey * ey + ez * ez <= rc2);
-      const unsigned long long m = __ballot(ok);
-      const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
-      if (ok && pos < tb.tcap) tb.list[out + pos] = (uint32_t)a | ((uint32_t)b << 16);
-      count += __popcll(m);
+  auto emit = [&](bool ok, int a, int b) {
+    const unsigned long long m = __ballot(ok);
+    const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+    if (ok && pos < tb.tcap) tb.list[out + pos] = (uint32_t)a | ((uint32_t)b << 16);
+    count += __popcll(m);
+  };
+  if (K <= 64 * kOwn) {
+    float4 own[kOwn];
+    bool own_in[kOwn];
+#pragma unroll
+    for (int q = 0; q < kOwn; ++q) {
+      own[q] = s_p[min(64 * q + lane, max(K - 1, 0))];
+      own_in[q] = own[q].w <= rc;
+    }
+    for (int d1 = 1; 2 * d1 <= K; ++d1) {
+      const int d2 = K - d1;
+      const int n1 = K - d1, n = (d2 != d1) ? K : n1;
+#pragma unroll
+      for (int q = 0; q < kOwn; ++q) {
+        if (64 * q >= n) break;
+        const int c = 64 * q + lane;
+        const bool first = c < n1, live = c < n;
+        const int partner = first ? c + d1 : c - n1;  // b of run d1, or a of run d2
+        const float4 pp = s_p[live ? partner : 0];
+        const float ex = pp.x - own[q].x, ey = pp.y - own[q].y, ez = pp.z - own[q].z;
+        const bool ok = live & own_in[q] & (pp.w <= rc) & (ex * ex + ey * ey + ez * ez <= rc2);
+        emit(ok, first ? c : partner, first ? partner : c);
+      }
+    }
+  } else {
+    for (int d1 = 1; 2 * d1 <= K; ++d1) {
+      // run d1 (K - d1 pairs), then run d2 = K - d1 (d1 pairs) unless it is d1
+      const int d2 = K - d1;
+      const int n1 = K - d1, n = (d2 != d1) ? K : n1;
+      for (int c0 = 0; c0 < n; c0 += 64) {
+        const int c = c0 + lane;
+        const bool first = c < n1;
+        const int a = first ? c : c - n1, b = a + (first ? d1 : d2);
+        // both records read unconditionally (idle lanes read record 0) and
+        // the tests combined without short-circuits: one LDS round trip
+        const bool live = c < n;
+        const float4 pa = s_p[live ? a : 0], pb = s_p[live ? b : 0];
+        const float ex = pb.x - pa.x, ey = pb.y - pa.y, ez = pb.z - pa.z;
+        emit(live & (pa.w <= rc) & (pb.w <= rc) & (ex * ex + ey * ey + ez * ez <= rc2), a, b);
+      }
     }
   }
   if (lane == 0) {
