@@ -350,12 +350,23 @@ def probe(args):
     m = gapx.Model.from_model(model)
     ctx = gapx.Context(m)
     ctx.load([s], synth.WORKLOADS[args.workload]["rc"])
-    for flags in (gapx.GRAD, gapx.GRAD | gapx.MIXED):
-        # every launch shape the autotuner tries, then the evaluations at its choice
-        print("probe: tuning %s" % ctx.autotune(flags), flush=True)
-        for _ in range(args.probe):
-            ctx.run(flags)
-        ctx.results()
+    for flags, name in ((gapx.GRAD, "fp64"), (gapx.GRAD | gapx.MIXED, "mixed")):
+        # every launch shape the autotuner can choose (one knob at a time from
+        # the defaults, as it searches), `--probe` eager evaluations each; no
+        # graphs, whose replay the counter passes do not need
+        base = dict(waves_fwd=1, waves_bwd=1, gpr_rows=64, gpr_blocks=512)
+        knobs = [("waves_fwd", (1, 2, 4)), ("waves_bwd", (1, 2, 4)), ("gpr_blocks", (256, 512, 1024, 2048))]
+        if flags & gapx.MIXED:
+            knobs.insert(2, ("gpr_rows", (64, 128)))
+        for k, vals in knobs:
+            for v in vals:
+                t = dict(base)
+                t[k] = v
+                ctx.set_tuning(flags, **t)
+                for _ in range(args.probe):
+                    ctx.run(flags)
+                ctx.results()
+        print("probe: %s launch shapes done" % name, flush=True)
     ctx.close()
     m.close()
     print("probe: done", flush=True)
@@ -376,8 +387,7 @@ def main():
     ap.add_argument("--surrogate-steps", type=int, default=3000)
     ap.add_argument("--probe", type=int, default=0,
                     help="profiling probe: a 1 GiB device copy (calibrates HBM byte counters), then per precision "
-                         "the autotuner's candidate launch shapes and this many evaluations at its choice; "
-                         "no JSON line")
+                         "this many eager evaluations at every launch shape the autotuner can choose; no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this evaluator runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
