@@ -429,6 +429,8 @@ static int struct_params(Ctx &c) {
       nbk = std::max(1, std::min(nbk, 128));
       P.nb[k] = nbk;
       P.S[k] = (int)std::ceil(c.rcut * nbk / dk);
+      // the list kernel's FP32 index arithmetic wants fewer than 2^20 images
+      if (P.S[k] > 48) return fail(GAPX_ERR_ARG, "lattice plane spacing below rcut / 48: too many periodic images");
     }
     P.bin_base = bin_base;
     bin_base += P.nb[0] * P.nb[1] * P.nb[2];

@@ -25,7 +25,13 @@ namespace gapx {
 
 namespace {
 
-__device__ inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
+// floor(a / b) for b >= 1 and |a| < 2^22 through FP32, given invb = 1 / b:
+// (a + 1/2) / b is at least 1/(2b) from every integer, while the two
+// roundings (of invb and of the product) move it by at most |a + 1/2| 2^-23 / b,
+// which is smaller than that bound, so the floor is exact.
+__device__ __forceinline__ int floordiv_small(int a, float invb) {
+  return (int)floorf(((float)a + 0.5f) * invb);
+}
 
 __global__ void k_fill_int(int *p, int n, int v) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -369,6 +375,9 @@ __global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ ato
   const double xi = nb.wpos[3 * i], yi = nb.wpos[3 * i + 1], zi = nb.wpos[3 * i + 2];
   const int si0 = nb.shift[3 * i], si1 = nb.shift[3 * i + 1], si2 = nb.shift[3 * i + 2];
   const int m1 = 2 * P.S[1] + 1, m2 = 2 * P.S[2] + 1, nimg = (2 * P.S[0] + 1) * m1 * m2;
+  // image and bin indices stay below 2^20 (the host keeps S <= 48)
+  const float inv_m1 = 1.0f / m1, inv_m2 = 1.0f / m2;
+  const float inv_n0 = 1.0f / P.nb[0], inv_n1 = 1.0f / P.nb[1], inv_n2 = 1.0f / P.nb[2];
   const size_t base = (size_t)i * nb.kcap;
   const double2 *sp2 = (const double2 *)nb.spos;
   const int4 *sinfo = (const int4 *)nb.sinfo;
@@ -377,11 +386,12 @@ __global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ ato
     const int img = g0 + lane;
     int lo = 0, n = 0, t0 = 0, t1 = 0, t2 = 0;
     if (img < nimg) {
-      const int o2 = img % m2 - P.S[2], o1 = (img / m2) % m1 - P.S[1], o0 = img / (m1 * m2) - P.S[0];
+      const int q2 = floordiv_small(img, inv_m2), q1 = floordiv_small(q2, inv_m1);
+      const int o2 = img - q2 * m2 - P.S[2], o1 = q2 - q1 * m1 - P.S[1], o0 = q1 - P.S[0];
       const int bb0 = bi0 + o0, bb1 = bi1 + o1, bb2 = bi2 + o2;
-      t0 = floordiv(bb0, P.nb[0]);
-      t1 = floordiv(bb1, P.nb[1]);
-      t2 = floordiv(bb2, P.nb[2]);
+      t0 = floordiv_small(bb0, inv_n0);
+      t1 = floordiv_small(bb1, inv_n1);
+      t2 = floordiv_small(bb2, inv_n2);
       const int w0 = bb0 - t0 * P.nb[0], w1 = bb1 - t1 * P.nb[1], w2 = bb2 - t2 * P.nb[2];
       const int bin = P.bin_base + (w0 * P.nb[1] + w1) * P.nb[2] + w2;
       lo = nb.bin_start[bin];
@@ -560,14 +570,27 @@ __global__ __launch_bounds__(256) void k_forces_reduce(int N, int G, int nstruct
     if (m >= N) return;
     double f0 = 0, f1 = 0, f2 = 0;
     if (grad) {
+      // the atom's own slots and reverse indices are read over its whole
+      // capacity, in flight together with its count (slots at or beyond the
+      // count hold stale values and are not added), so the gather waits for
+      // memory twice: own slots, then the reverse slots they name
       const int K = nb.nb_count[m];
       const size_t base = (size_t)m * nb.kcap;
-      for (int a = lane; a < K; a += 64) {
+      for (int a = lane; a < nb.kcap; a += 64) {
         const size_t sl = base + a;
-        f0 += slot_f[3 * sl];
-        f1 += slot_f[3 * sl + 1];
-        f2 += slot_f[3 * sl + 2];
-        const int r = nb.nb_rev[sl];
+        const double o0 = slot_f[3 * sl], o1 = slot_f[3 * sl + 1], o2 = slot_f[3 * sl + 2];
+        const int rv = nb.nb_rev[sl];
+        // an empty asm that takes the loaded values: the compiler must issue
+        // the loads here, before the comparison with K, instead of moving
+        // them under it (no instruction is emitted)
+        asm volatile("" ::"v"(o0), "v"(o1), "v"(o2), "v"(rv));
+        const bool live = a < K;
+        const int r = live ? rv : -1;
+        if (live) {
+          f0 += o0;
+          f1 += o1;
+          f2 += o2;
+        }
         if (r >= 0) {
           f0 -= slot_f[3 * (size_t)r];
           f1 -= slot_f[3 * (size_t)r + 1];

@@ -142,3 +142,13 @@ def test_lds_neighbour_limit_is_reported(model):
         with pytest.raises(gapx.GapxError) as e:
             m.calc(np.ones(40, np.int32), lat, pos, 9.0, flags=flags)
         assert e.value.code == 5
+
+
+def test_too_many_periodic_images_is_an_argument_error(model):
+    """A lattice plane spacing below rcut / 48 would need more periodic
+    images than the list kernel indexes exactly; it is refused at load."""
+    om, m = model
+    with pytest.raises(gapx.GapxError) as e:
+        m.calc(np.array([8], np.int32), np.eye(3) * 0.1, np.array([[0.01, 0.02, 0.03]]), 6.0)
+    assert e.value.code == 1
+    check(om, m, np.array([8], np.int32), np.eye(3) * 2.0, np.array([[0.5, 0.7, 0.9]]), rcut=3.0)
