@@ -368,20 +368,36 @@ __global__ __launch_bounds__(256) void k_desc_fwd(int N, const int *__restrict__
 // b = 1 + lam cos, lam = +1 (p) and -1 (m).  zeta_k = 2^zlog[k] with zlog
 // ascending: one squaring chain, advanced by uniform trip counts (registers
 // only; an indexed table of powers would go to scratch memory).
-template <typename T, int NZ, bool DERIV>
+// Z16: the zeta list is {1, 2, 4, 16} (zlog {0, 1, 2, 4}, the usual grid),
+// known at compile time, so the chain is straight-line code with the same
+// multiplications in the same order.
+template <typename T, int NZ, bool DERIV, bool Z16>
 __device__ __forceinline__ void grid_powers(T cth, const SFDev &sf, T (&Pp)[NZ], T (&Pm)[NZ], T (&dPp)[NZ],
                                             T (&dPm)[NZ]) {
+  static_assert(!Z16 || NZ == 4, "the {1, 2, 4, 16} list has four zetas");
   T p = T(1) + cth, m = T(1) - cth;  // b^(2^t)
   T qp = T(1), qm = T(1);            // b^(2^t - 1)
   int t = 0;
 #pragma unroll
   for (int k = 0; k < NZ; ++k) {
-    const int j = sf.zlog[k];
-    for (; t < j; ++t) {
-      qp *= p;
-      qm *= m;
-      p *= p;
-      m *= m;
+    const int j = Z16 ? (k < 3 ? k : 4) : sf.zlog[k];
+    if constexpr (Z16) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)  // at most 4 squarings per zeta; t and j are constants here
+        if (t < j) {
+          qp *= p;
+          qm *= m;
+          p *= p;
+          m *= m;
+          ++t;
+        }
+    } else {
+      for (; t < j; ++t) {
+        qp *= p;
+        qm *= m;
+        p *= p;
+        m *= m;
+      }
     }
     const T pref = (T)sf.pref_k[k];
     Pp[k] = pref * p;
@@ -453,7 +469,7 @@ struct GridShape {
   static constexpr int NV = NE * 2 * NZ * 2;
 };
 
-template <typename T, int NZ, int W>
+template <typename T, int NZ, int W, bool Z16>
 __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restrict__ species,
                                                        const double *__restrict__ wz, NeighborBufs nb,
                                                        TripletBufs tb, SFDev sf, Lds L,
@@ -561,7 +577,7 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
         const T fe = (e <= rc) ? T(0.5) * (Math<T>::cs(pirc * e) + T(1)) : T(0);
         const T F3 = act ? s_fc[g * KP + a] * s_fc[g * KP + b] * fe : T(0);
         T Pp[NZ], Pm[NZ], dPp[NZ], dPm[NZ];
-        grid_powers<T, NZ, false>(cth, sf, Pp, Pm, dPp, dPm);
+        grid_powers<T, NZ, false, Z16>(cth, sf, Pp, Pm, dPp, dPm);
 #pragma unroll
         for (int hh = 0; hh < NE; ++hh) {
           const int h = h0 + hh;
@@ -624,7 +640,7 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
   }
 }
 
-template <typename T, int NZ>
+template <typename T, int NZ, bool Z16>
 __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDev &sf, const T *s_d, const T *s_rw,
                                              const T *s_fd, const T *s_gA, const T *s_gW, const T *s_eta, int KP,
                                              T rc, T pirc, double *s_T, double *s_F, T (&vir)[6]) {
@@ -652,7 +668,7 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
   const T fe = in ? T(0.5) * (cosine + T(1)) : T(0);
   const T dfe = in ? T(-0.5) * sine * pirc : T(0);
   T Pp[NZ], Pm[NZ], dPp[NZ], dPm[NZ];
-  grid_powers<T, NZ, true>(cth, sf, Pp, Pm, dPp, dPm);
+  grid_powers<T, NZ, true, Z16>(cth, sf, Pp, Pm, dPp, dPm);
   T SA = 0, SB = 0, SC = 0;
   for (int h = 0; h < sf.n_eta; ++h) {
     const T eta = s_eta[h];
@@ -793,7 +809,7 @@ __device__ __forceinline__ void ang_bwd_term(uint32_t q, bool active, const SFDe
   }
 }
 
-template <typename T, int W, int GNZ>
+template <typename T, int W, int GNZ, bool Z16>
 __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__ species,
                                                   const double *__restrict__ wz, NeighborBufs nb, TripletBufs tb,
                                                   SFDev sf, Lds L, GradSrc gs, double *__restrict__ slot_f,
@@ -907,7 +923,7 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
         const bool act = t0 + u * ST + lane < nt;
         const uint32_t q = pop_q4(qc);
         if constexpr (GNZ > 0)
-          ang_bwd_grid<T, GNZ>(q, act, sf, s_d, s_rw, s_fd, s_gA, s_gW, s_eta, KP, rcA, pirc, s_T + wv * KP,
+          ang_bwd_grid<T, GNZ, Z16>(q, act, sf, s_d, s_rw, s_fd, s_gA, s_gW, s_eta, KP, rcA, pirc, s_T + wv * KP,
                                s_F + 3 * wv * KP, vir);
         else
           ang_bwd_term<T>(q, act, sf, s_d, s_rw, s_fd, s_gA, s_gW, KP, s_T + wv * KP, s_F + 3 * wv * KP,
@@ -988,11 +1004,17 @@ hipError_t fwd_waves(int waves, hipStream_t s, int N, const int *species, const 
   return fwd_launch<T, NCH, 4>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
 }
 
-template <typename T, int W, int GNZ>
+// zeta list {1, 2, 4, 16}: the grid kernels' specialisation with a compile-time chain
+bool zeta_1_2_4_16(const SFDev &sf) {
+  return sf.nz == 4 && sf.zlog[0] == 0 && sf.zlog[1] == 1 && sf.zlog[2] == 2 && sf.zlog[3] == 4;
+}
+
+template <typename T, int W, int GNZ, bool Z16 = false>
 hipError_t bwd_launch(hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
                       const TripletBufs &tb, const SFDev &sf, Lds L, const GradSrc &gs, double *slot_f,
                       double *atom_vir) {
-  k_desc_bwd<T, W, GNZ><<<N, 64 * W, bwd_lds<T, W>(L), s>>>(N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+  k_desc_bwd<T, W, GNZ, Z16><<<N, 64 * W, bwd_lds<T, W>(L), s>>>(N, species, wz, nb, tb, sf, L, gs, slot_f,
+                                                                 atom_vir);
   return hipGetLastError();
 }
 
@@ -1005,7 +1027,9 @@ hipError_t bwd_grid(hipStream_t s, int N, const int *species, const double *wz, 
     case 1: return bwd_launch<T, W, 1>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
     case 2: return bwd_launch<T, W, 2>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
     case 3: return bwd_launch<T, W, 3>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
-    case 4: return bwd_launch<T, W, 4>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+    case 4:
+      if (zeta_1_2_4_16(sf)) return bwd_launch<T, W, 4, true>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+      return bwd_launch<T, W, 4>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
     default: return bwd_launch<T, W, 5>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
   }
 }
@@ -1016,12 +1040,12 @@ size_t fwd_grid_lds(const Lds &L) {
   return ((W + 1) & ~1) * sizeof(double) + (size_t)(6 * L.KP + L.G * L.KP + L.Dp + W * NV + NV) * sizeof(T);
 }
 
-template <typename T, int NZ, int W>
+template <typename T, int NZ, int W, bool Z16 = false>
 hipError_t fwd_grid_launch(hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
                            const TripletBufs &tb, const SFDev &sf, Lds L, const double *theta_inv, const double *mu,
                            int split_mode, DescOut out) {
-  k_desc_fwd_grid<T, NZ, W><<<N, 64 * W, fwd_grid_lds<T, NZ, W>(L), s>>>(N, species, wz, nb, tb, sf, L, theta_inv,
-                                                                         mu, split_mode, out);
+  k_desc_fwd_grid<T, NZ, W, Z16><<<N, 64 * W, fwd_grid_lds<T, NZ, W>(L), s>>>(N, species, wz, nb, tb, sf, L,
+                                                                              theta_inv, mu, split_mode, out);
   return hipGetLastError();
 }
 
@@ -1029,15 +1053,18 @@ template <typename T>
 hipError_t fwd_grid(int waves, hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
                     const TripletBufs &tb, const SFDev &sf, Lds L, const double *theta_inv, const double *mu,
                     int split_mode, DescOut out) {
-#define GAPX_FWD_GRID(NZ)                                                                                     \
-  return (waves >= 2) ? fwd_grid_launch<T, NZ, 2>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out) \
-                      : fwd_grid_launch<T, NZ, 1>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out)
+#define GAPX_FWD_GRID(NZ, Z)                                                                                    \
+  return (waves >= 2)                                                                                           \
+             ? fwd_grid_launch<T, NZ, 2, Z>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out) \
+             : fwd_grid_launch<T, NZ, 1, Z>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out)
   switch (sf.nz) {
-    case 1: GAPX_FWD_GRID(1);
-    case 2: GAPX_FWD_GRID(2);
-    case 3: GAPX_FWD_GRID(3);
-    case 4: GAPX_FWD_GRID(4);
-    default: GAPX_FWD_GRID(5);
+    case 1: GAPX_FWD_GRID(1, false);
+    case 2: GAPX_FWD_GRID(2, false);
+    case 3: GAPX_FWD_GRID(3, false);
+    case 4:
+      if (zeta_1_2_4_16(sf)) GAPX_FWD_GRID(4, true);
+      GAPX_FWD_GRID(4, false);
+    default: GAPX_FWD_GRID(5, false);
   }
 #undef GAPX_FWD_GRID
 }
