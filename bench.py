@@ -58,14 +58,23 @@ BYTES_NEIGHBOR = 4 + 8 + 32 + 4                # per pair: index, key, (dx,dy,dz
 SINGLE_KERNEL_STAGES = ("descriptors_fwd", "gpr_contraction", "gpr_finish", "descriptors_bwd")
 
 
-def stage_kernel(stage, mixed, tune):
+def zeta_1_2_4_16(header):
+    """True when the angular lines' zeta list is {1, 2, 4, 16}: the grid
+    descriptor kernels then run their specialisation for that list."""
+    ang = np.isin(header["sf_type"], (2, 4))
+    return sorted(set(np.round(header["sf_zeta"][ang]).astype(int).tolist())) == [1, 2, 4, 16]
+
+
+def stage_kernel(stage, mixed, tune, z16):
     """The one kernel of a single-kernel stage under the given launch shapes,
-    as rocprofv3 names it (grid-form tables with 4 zetas, D = 128)."""
+    as rocprofv3 names it (grid-form tables with 4 zetas, D = 128; z16: the
+    zeta list is {1, 2, 4, 16})."""
     t = "float" if mixed else "double"
+    z = "true" if z16 else "false"
     if stage == "descriptors_fwd":
-        return "k_desc_fwd_grid<%s, 4, %d>" % (t, tune["waves_fwd"])
+        return "k_desc_fwd_grid<%s, 4, %d, %s>" % (t, min(tune["waves_fwd"], 2), z)
     if stage == "descriptors_bwd":
-        return "k_desc_bwd<%s, %d, 4>" % (t, tune["waves_bwd"])
+        return "k_desc_bwd<%s, %d, 4, %s>" % (t, tune["waves_bwd"], z)
     if stage == "gpr_contraction":
         return "k_gpr_mixed<false, 128, 64, %d>" % (tune["gpr_rows"] // 64) if mixed else "k_gpr_f64<128>"
     return "k_gpr_finish"
@@ -438,7 +447,7 @@ def main():
     rates = stage_rates(st, N, D, M, pairs, triplets, m, mixed=True)
     # the dominant kernel: the longest of the stages that are one kernel each
     dom = max(SINGLE_KERNEL_STAGES, key=lambda k: st[k])
-    kname = stage_kernel(dom, True, tuning["mixed"])
+    kname = stage_kernel(dom, True, tuning["mixed"], zeta_1_2_4_16(m.header()))
     roof = {kk: rates[dom][kk] for kk in ("bound", "achieved", "peak", "unit", "frac")}
     roof["traffic"] = measured_traffic(args.workload, kname)
     roof["kernel"] = kname
