@@ -566,8 +566,8 @@ __global__ __launch_bounds__(256) void k_forces_reduce(int N, int G, int nstruct
                                                        TripletBufs tb, int *stats_out) {
   const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
   if (b > nstruct) {
-    const int m = 4 * (b - nstruct - 1) + wv;
-    if (m >= N) return;
+    const int g = b - nstruct - 1, m = 4 * g + wv;
+    if (g >= G || m >= N) return;
     double f0 = 0, f1 = 0, f2 = 0;
     if (grad) {
       // the atom's own slots and reverse indices are read over its whole
