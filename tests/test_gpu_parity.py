@@ -156,3 +156,22 @@ def test_fma_probe_is_plausible():
     print("measured TFLOP/s:", c)
     for k, peak in (("fma_f32", 157.3), ("fma_f64", 78.6)):
         assert np.isfinite(c[k]) and 0.02 * peak < c[k] < 1.05 * peak, k
+
+
+@pytest.mark.parametrize("zeta", [(1, 2, 4, 8), (1, 2, 4, 16), (1, 4, 16), (2, 8), (1, 2, 4, 8, 16)],
+                         ids=["1-2-4-8", "1-2-4-16", "1-4-16", "2-8", "1-2-4-8-16"])
+def test_grid_zeta_lists_match_oracle(zeta):
+    """The grid-form descriptor kernels for several zeta lists: the {1, 2, 4, 16}
+    list runs a compile-time power chain, the others the run-time chain; both
+    match the oracle in FP64, and the mixed path stays within its bar."""
+    rng = np.random.default_rng(41)
+    s = ternary_structure(60, 11.0, 411)
+    om = fitted_small_model(rng, s, zeta=zeta)
+    m = gapx.Model.from_model(om)
+    res = m.calc(s["species"], s["lat"], s["pos"], 6.0)
+    ref = oracle.gap_calc(om, s["species"], s["lat"], s["pos"], 6.0)
+    check(res, ref, len(s["species"]))
+    mix = m.calc(s["species"], s["lat"], s["pos"], 6.0, mixed=True)
+    assert abs(mix[0] - res[0]) / len(s["species"]) <= 1e-3
+    assert np.abs(mix[1] - res[1]).max() <= 2e-2
+    m.close()
