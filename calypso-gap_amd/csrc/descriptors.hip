@@ -643,7 +643,7 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
 template <typename T, int NZ, bool Z16>
 __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDev &sf, const T *s_d, const T *s_rw,
                                              const T *s_fd, const T *s_gA, const T *s_gW, const T *s_eta, int KP,
-                                             T rc, T pirc, double *s_T, double *s_F, T (&vir)[6]) {
+                                             T rc, T pirc, double *s_T, double *s_F) {
   if (!active) return;
   const int a = q & 0xffff, b = q >> 16;
   const T ax = s_d[4 * a], ay = s_d[4 * a + 1], az = s_d[4 * a + 2], ra = s_d[4 * a + 3];
@@ -718,12 +718,6 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
     atomicAdd(&s_F[3 * a], (double)-ux);
     atomicAdd(&s_F[3 * a + 1], (double)-uy);
     atomicAdd(&s_F[3 * a + 2], (double)-uz);
-    vir[0] += ux * ex;
-    vir[1] += uy * ey;
-    vir[2] += uz * ez;
-    vir[3] += ux * ey;
-    vir[4] += uy * ez;
-    vir[5] += ux * ez;
   }
 }
 
@@ -733,7 +727,7 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
 template <typename T>
 __device__ __forceinline__ void ang_bwd_term(uint32_t q, bool active, const SFDev &sf, const T *s_d,
                                              const T *s_rw, const T *s_fd, const T *s_gA, const T *s_gW, int KP,
-                                             double *s_T, double *s_F, T (&vir)[6]) {
+                                             double *s_T, double *s_F) {
   if (!active) return;
   const int a = q & 0xffff, b = q >> 16;
   const T ax = s_d[4 * a], ay = s_d[4 * a + 1], az = s_d[4 * a + 2], ra = s_d[4 * a + 3];
@@ -800,12 +794,6 @@ __device__ __forceinline__ void ang_bwd_term(uint32_t q, bool active, const SFDe
     atomicAdd(&s_F[3 * a], (double)-ux);
     atomicAdd(&s_F[3 * a + 1], (double)-uy);
     atomicAdd(&s_F[3 * a + 2], (double)-uz);
-    vir[0] += ux * ex;
-    vir[1] += uy * ey;
-    vir[2] += uz * ez;
-    vir[3] += ux * ey;
-    vir[4] += uy * ez;
-    vir[5] += ux * ez;
   }
 }
 
@@ -908,7 +896,7 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   }
   __syncthreads();
 
-  T vir[6] = {0, 0, 0, 0, 0, 0};
+
   if (sf.n_ang > 0) {
     const uint32_t *tl = tb.list + (size_t)i * tb.tcap;
     const int nt = tb.count[i];
@@ -924,10 +912,9 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
         const uint32_t q = pop_q4(qc);
         if constexpr (GNZ > 0)
           ang_bwd_grid<T, GNZ, Z16>(q, act, sf, s_d, s_rw, s_fd, s_gA, s_gW, s_eta, KP, rcA, pirc, s_T + wv * KP,
-                               s_F + 3 * wv * KP, vir);
+                                    s_F + 3 * wv * KP);
         else
-          ang_bwd_term<T>(q, act, sf, s_d, s_rw, s_fd, s_gA, s_gW, KP, s_T + wv * KP, s_F + 3 * wv * KP,
-                          vir);
+          ang_bwd_term<T>(q, act, sf, s_d, s_rw, s_fd, s_gA, s_gW, KP, s_T + wv * KP, s_F + 3 * wv * KP);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) qc[u] = qn[u];
@@ -935,7 +922,11 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   }
   __syncthreads();
 
-  // per-slot force contributions dE_i/dR(slot atom) and the radial virial
+  // per-slot force contributions dE_i/dR(slot atom) and the virial: the
+  // radial part T d d^T / r plus F_slot d^T for the r_jk forces (each triplet's
+  // u (r_k - r_j)^T is u r_k^T - u r_j^T, so summing F_slot d^T over the slots
+  // gives the triplet sum with one product per slot instead of per triplet)
+  double vir[6] = {0, 0, 0, 0, 0, 0};
   for (int a = tid; a < K; a += 64 * W) {
     const T r = s_d[4 * a + 3];
     double Ts = 0.0, Fx = 0.0, Fy = 0.0, Fz = 0.0;
@@ -952,16 +943,16 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
     o[0] = sc * dx + Fx;
     o[1] = sc * dy + Fy;
     o[2] = sc * dz + Fz;
-    vir[0] += (T)(sc * dx * dx);
-    vir[1] += (T)(sc * dy * dy);
-    vir[2] += (T)(sc * dz * dz);
-    vir[3] += (T)(sc * dx * dy);
-    vir[4] += (T)(sc * dy * dz);
-    vir[5] += (T)(sc * dx * dz);
+    vir[0] += sc * dx * dx + Fx * dx;
+    vir[1] += sc * dy * dy + Fy * dy;
+    vir[2] += sc * dz * dz + Fz * dz;
+    vir[3] += sc * dx * dy + Fx * dy;
+    vir[4] += sc * dy * dz + Fy * dz;
+    vir[5] += sc * dx * dz + Fx * dz;
   }
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
-    const double v = wave_sum((double)vir[c]);
+    const double v = wave_sum(vir[c]);
     if (lane == 0) s_vir[6 * wv + c] = v;
   }
   __syncthreads();
