@@ -13,6 +13,8 @@
 * A rejected lattice update (singular cell) keeps the loaded cells.
 * More neighbours than the descriptor kernels' LDS layout holds is reported
   as GAPX_ERR_NEIGHBORS, without the reference's 1000 cap as well.
+* A cell too small for rcut / 48 is refused; lists above 256 slots (the
+  triplet builder's other path) match the oracle.
 """
 import numpy as np
 import pytest
@@ -152,3 +154,25 @@ def test_too_many_periodic_images_is_an_argument_error(model):
         m.calc(np.array([8], np.int32), np.eye(3) * 0.1, np.array([[0.01, 0.02, 0.03]]), 6.0)
     assert e.value.code == 1
     check(om, m, np.array([8], np.int32), np.eye(3) * 2.0, np.array([[0.5, 0.7, 0.9]]), rcut=3.0)
+
+
+def test_more_than_256_neighbours_match_oracle(model):
+    """Lists longer than 256 slots take the triplet builder's path that reads
+    both records of a pair from LDS (shorter lists keep each lane's own records
+    in registers); FP64 against the oracle, mixed within its bar."""
+    om, m = model
+    s = ternary_structure(40, 12.0, 176)
+    e, f, st, _ = m.calc(s["species"], s["lat"], s["pos"], 10.0)
+    r = oracle.gap_calc(om, s["species"], s["lat"], s["pos"], 10.0)
+    n = len(s["species"])
+    assert abs(e - r["energy"]) / n <= TOL_E
+    assert np.abs(f - r["forces"]).max() <= TOL_F
+    assert np.abs(st - r["stress"]).max() <= TOL_S
+    ctx = gapx.Context(m)
+    ctx.load([s], 10.0)
+    ctx.run(gapx.GRAD)
+    pairs, _ = ctx.counts()
+    ctx.close()
+    assert pairs / n > 256
+    mix = m.calc(s["species"], s["lat"], s["pos"], 10.0, mixed=True)
+    assert abs(mix[0] - e) / n <= 1e-3 and np.abs(mix[1] - f).max() <= 2e-2
