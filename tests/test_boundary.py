@@ -27,7 +27,8 @@ def in_model_dir(tmp_path, monkeypatch):
 def test_f2py_signatures_match_the_reference():
     import libgap.libgap as L
     doc = L.__doc__
-    # gappy's generated signatures: same names, same order
+    # the signatures f2py generates from gappy/libgap/*.f90 (setup.py:36-45):
+    # same names, same order
     assert "ene,force,stress,variance = fgap_calc(species,lat,pos,theta,mm,qmm,coeff,rcut,lgrad" in doc
     assert "nsparsex,des_len,theta,mm,invcmm,coeff = fgap_read()" in doc
     assert "min_bond = fget_bond(lat,elements,pos,rcut" in doc
