@@ -102,7 +102,7 @@ def build_fortran():
 
 
 def build_oracle():
-    """The checker: C oracle always; the reference Fortran when it is mounted."""
+    """The oracle: its C restatement always; the reference Fortran when it is mounted."""
     run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     if os.path.isdir("/root/reference/gappy/libgap"):
         run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"])

@@ -1,7 +1,7 @@
 /*
  * gap_oracle.c -- CPU FP64 restatement of libgap's energy/force/stress path.
  *
- * TEST INFRASTRUCTURE ONLY: this file is the checker for the HIP evaluator.
+ * TEST INFRASTRUCTURE ONLY: this file is the oracle the HIP evaluator is tested against.
  * Only the tests, the smoke check and the benchmark's CPU-baseline leg may
  * load it; the product path never does.
  *

@@ -1,8 +1,8 @@
 /*
  * gap_oracle.h -- CPU FP64 restatement of libgap's energy/force/stress path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
- * evaluator: only the tests, the smoke check and the benchmark's CPU-baseline
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle the HIP evaluator is
+ * tested against: only the tests, the smoke check and the benchmark's CPU-baseline
  * leg may load it.  The product path never calls it.
  *
  * Layout conventions (C, row-major):

@@ -1,6 +1,6 @@
 """Python access to the CPU FP64 oracle (test infrastructure only).
 
-This module is the checker for the HIP evaluator.  Only the tests, the smoke
+This module is the oracle the HIP evaluator is tested against.  Only the tests, the smoke
 check and the benchmark's CPU-baseline leg import it; the product path never
 does.  It wraps ``oracle/liboracle.so`` (the C restatement of libgap's
 energy/force/stress path, gap_oracle.c) and, where it has been built,

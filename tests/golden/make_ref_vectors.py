@@ -89,7 +89,7 @@ def _desc(header, species, lat, pos, rc):
     return oracle.descriptors(m, species, lat, pos, rc)
 
 
-def config1_model():
+def c64_single_species_model():
     w = synth.WORKLOADS["c64_zeta1"]
     d = synth.workload_model("c64_zeta1", _desc)
     d["sf_zeta"] = np.ones(len(d["sf_type"]))
@@ -111,7 +111,7 @@ def main():
     models = {
         "example": oracle.parse_model(os.path.join(HERE, "c64_diamond", "gap_parameters")),
         "ternary": ternary_model(rng),
-        "config1": config1_model(),
+        "c64_single_species": c64_single_species_model(),
     }
     cases = []
     sp, lat, pos = read_xyz_b8c56(REF_XYZ)
@@ -130,7 +130,7 @@ def main():
     s = synth.structure(8, 8 * 14.0, (1, 8, 22), np.random.default_rng(15))
     cases.append(("ternary", "t8_small_cell", s["species"], s["lat"], s["pos"], 6.0, True))
     s = synth.workload_structure("c64_zeta1")
-    cases.append(("config1", "config1_c64", s["species"], s["lat"], s["pos"], 6.0, True))
+    cases.append(("c64_single_species", "c64_single_species", s["species"], s["lat"], s["pos"], 6.0, True))
 
     out = {}
     for name, m in models.items():

@@ -45,8 +45,8 @@ def test_mixed_vs_fp64_ternary4k(bf16):
 def test_mixed_reference_vectors(ref_vectors, case):
     """The reference's own vectors through the mixed path.
 
-    The models fitted for this build (ternary, config1) have standardised
-    descriptors and meet the 1 meV/atom, 20 meV/A bar.  The shipped example
+    The fitted workload models (the 4096-atom ternary and the 64-atom single
+    species) have standardised descriptors and meet the 1 meV/atom, 20 meV/A bar.  The shipped example
     model does not standardise (theta spans 1.6 to 8.4e6, |xt|^2 reaches ~1e5
     while the kernel-carrying distances are O(1)), so the FP32-accumulated
     expanded distance loses about three digits there; its stated bar is
