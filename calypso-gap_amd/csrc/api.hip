@@ -344,6 +344,8 @@ struct Ctx {
   void *blas = nullptr;  // rocBLAS handle on the stream (variance only, made on first use)
   bool md_ready = false;  // md_p holds the momenta of the last MD run
   unsigned long long bins_clean = ~0ULL;  // allocation generations of bin_count/bin_ticket last cleared
+  int red_parts = 0;                      // reduction parts over all structures (see ReduceBufs)
+  unsigned long long red_clean = ~0ULL;   // allocation generations of red_ticket/stats_acc last cleared
   // timing
   bool timing = false;
   hipEvent_t ev[2 * kNumStages] = {};
@@ -520,6 +522,14 @@ static int ctx_load_batch(Ctx &c, int nstruct, const int *natoms, const int *spe
   GAPX_HIP(hipMemcpyAsync(d_astruct, astruct.data(), sizeof(int) * c.N, hipMemcpyHostToDevice, c.stream));
   GAPX_HIP(hipMemcpyAsync(d_pos, hpos.data(), sizeof(double) * 3 * c.N, hipMemcpyHostToDevice, c.stream));
   GAPX_HIP(hipMemcpyAsync(d_sp, c.sp.data(), sizeof(StructParams) * nstruct, hipMemcpyHostToDevice, c.stream));
+  // the reduction's structure parts (kReduceAtoms atoms each, consecutive per structure)
+  std::vector<int2> part_map;
+  for (int s = 0; s < nstruct; ++s)
+    for (int p = 0; p * kReduceAtoms < std::max(1, c.natoms[s]); ++p) part_map.push_back(make_int2(s, p));
+  int2 *d_map;
+  if ((rc = c.get("red_map", part_map.size(), &d_map))) return rc;
+  GAPX_HIP(hipMemcpyAsync(d_map, part_map.data(), sizeof(int2) * part_map.size(), hipMemcpyHostToDevice, c.stream));
+  c.red_parts = (int)part_map.size();
   GAPX_HIP(hipStreamSynchronize(c.stream));
   c.kcap = initial_kcap(c);
   c.tcap = initial_tcap(c);
@@ -709,10 +719,31 @@ static int enqueue_forces(Ctx &c, Bufs &b, bool fp64, bool grad, const double *e
   // forces, energies, stress, and the list statistics straight into pinned
   // host memory (read by ctx_settle), in one launch; the force gather needs
   // the reverse-slot map from the second stream
+  ReduceBufs rb;
+  double *part_sum;
+  int *ticket, *stats_acc;
+  if ((rc = c.get("red_sum", 8 * (size_t)c.red_parts, &part_sum)) || (rc = c.get("red_ticket", c.nstruct, &ticket)) ||
+      (rc = c.get("stats_acc", 8, &stats_acc)))
+    return rc;
+  {
+    // the tickets and accumulators stay zero between runs; clear them once
+    // after they are (re)allocated
+    const unsigned long long g = ((unsigned long long)c.mem["red_ticket"].gen << 32) | c.mem["stats_acc"].gen;
+    if (g != c.red_clean) {
+      GAPX_HIP(launch_zero_ints(c.stream, ticket, (int)(c.mem["red_ticket"].cap / sizeof(int)), stats_acc, 8,
+                                nullptr, 0));
+      c.red_clean = g;
+    }
+  }
+  rb.part_map = c.ptr<int2>("red_map");
+  rb.nparts = c.red_parts;
+  rb.part_sum = part_sum;
+  rb.ticket = ticket;
+  rb.stats_acc = stats_acc;
   if (grad) GAPX_HIP(hipStreamWaitEvent(c.stream, c.join_ev, 0));
   stage_begin(c, kForces);
   GAPX_HIP(launch_forces_reduce(c.stream, c.N, grad ? 1 : 0, nb, b.slot_f, b.force, c.nstruct, d_sp, eatom,
-                                b.atom_vir, b.ene, b.stress, b.tb, c.host_stats_dev));
+                                b.atom_vir, b.ene, b.stress, b.tb, rb, c.host_stats_dev));
   stage_end(c, kForces);
   return GAPX_OK;
 }

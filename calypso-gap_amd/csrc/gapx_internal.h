@@ -217,15 +217,32 @@ hipError_t launch_descriptors_bwd(hipStream_t s, bool fp64, int N, const int *sp
 // Largest per-atom neighbour count the descriptor kernels' LDS layout holds.
 int max_neighbors_for_lds(bool fp64, int G, int Dp);
 
+// Work buffers of the reduction after the backward pass.  Per-structure sums
+// run in parts of at most kReduceAtoms atoms, one workgroup each; the last
+// part of a structure to finish (a per-structure ticket) adds the parts'
+// partial sums in part order, so the result does not depend on timing.  The
+// list statistics are folded by ceil(N / kReduceAtoms) workgroups with integer
+// atomics (exact, so also independent of order); the last of them writes the
+// output and clears the accumulators.  ticket and stats_acc are zero between
+// runs (cleared once when allocated, then by the kernel itself).
+constexpr int kReduceAtoms = 1024;
+struct ReduceBufs {
+  const int2 *part_map;  // [nparts] (structure, part index); a structure's parts are consecutive
+  int nparts;
+  double *part_sum;      // [nparts][8]
+  int *ticket;           // [nstruct]
+  int *stats_acc;        // [8], [7] the statistics ticket
+};
+
 // After the backward pass, one launch: per-atom forces (zeros without grad),
-// per-structure energy and stress, and (one more workgroup) the list
-// statistics: out[0..7] = kept pairs, largest raw pair count, atoms above
-// kcap, descriptor guard (nb.total[3]), kept triplets, largest raw triplet
-// count, atoms above tcap, 0.  stats_out may be pinned host memory.
+// per-structure energy and stress, and the list statistics: out[0..7] = kept
+// pairs, largest raw pair count, atoms above kcap, descriptor guard
+// (nb.total[3]), kept triplets, largest raw triplet count, atoms above tcap, 0.
+// stats_out may be pinned host memory.
 hipError_t launch_forces_reduce(hipStream_t s, int N, int grad, const NeighborBufs &nb, const double *slot_f,
                                 double *force, int nstruct, const StructParams *sp, const double *eatom,
                                 const double *atom_vir, double *ene, double *stress6, const TripletBufs &tb,
-                                int *stats_out);
+                                const ReduceBufs &rb, int *stats_out);
 
 // Contraction: E_i = sum_k exp(-|xt_i - mt_k|^2 / 2) c_k and
 // g_i = -(xt_i E_i - sum_k W_ik mt_k) * theta_inv.

@@ -490,169 +490,191 @@ __global__ __launch_bounds__(64) void k_reverse(int N, NeighborBufs nb) {
   }
 }
 
-// Fold the per-atom counts into the run's statistics (one workgroup):
-// per-thread values, wavefront shuffles, then the 16 wavefront results.
-template <int NT>
-__device__ void list_stats(int N, int kcap, const int *__restrict__ nb_count, const int *__restrict__ nb_raw,
-                           int tcap, const int *__restrict__ t_count, const int *__restrict__ t_raw,
-                           const int *__restrict__ guard, int *__restrict__ out) {
-  constexpr int NW = NT / 64;
-  __shared__ long long red[NW][6];
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  long long v[6] = {0, 0, 0, 0, 0, 0};
-  // four atoms per thread per step, their loads issued together
-  constexpr int U = 4;
-  for (int a0 = t; a0 < N; a0 += NT * U) {
-    int r[U], c[U], tr[U], tc[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int a = min(a0 + NT * u, N - 1);
-      r[u] = nb_raw[a];
-      c[u] = nb_count[a];
-      tr[u] = t_raw ? t_raw[a] : 0;
-      tc[u] = t_raw ? t_count[a] : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (a0 + NT * u < N) {
-        v[0] += c[u];
-        v[1] = max(v[1], (long long)r[u]);
-        v[2] += (r[u] > kcap) ? 1 : 0;
-        v[3] += tc[u];
-        v[4] = max(v[4], (long long)tr[u]);
-        v[5] += (tr[u] > tcap) ? 1 : 0;
-      }
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      const long long w = __shfl_xor(v[k], o, 64);
-      v[k] = (k == 1 || k == 4) ? max(v[k], w) : v[k] + w;
-    }
-  }
-  if (lane == 0)
-    for (int k = 0; k < 6; ++k) red[wv][k] = v[k];
-  __syncthreads();
-  if (t < 6) {
-    long long a = 0;
-    for (int w = 0; w < NW; ++w) a = (t == 1 || t == 4) ? max(a, red[w][t]) : a + red[w][t];
-    out[t < 3 ? t : t + 1] = (int)a;
-  }
-  if (t == 6) out[3] = guard[0];
-  if (t == 7) out[7] = 0;
-}
-
 template <typename V>
 __device__ __forceinline__ V wave_reduce_sum(V v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
 
-// Everything after the backward pass in one launch of 256-thread blocks:
-//   blocks [0, nstruct)      energy and virial of one structure, stress in GPa
-//                            (gap_calc.f90:152-154, 189-203);
-//   block nstruct            list statistics (see launch_forces_reduce);
-//   blocks nstruct + 1 + g   forces, one wavefront per atom (four atoms per
-//                            block g < G): F_m = sum of own slots - sum of
-//                            reverse slots in slot order (zeros without grad).
-// The per-structure sums loop over their atoms, so they go first: they run
-// beside the gather blocks instead of after them.
+// Everything after the backward pass in one launch of 256-thread blocks
+// (P = rb.nparts structure parts, T = ceil(N / kReduceAtoms) statistics parts):
+//   blocks [0, P)          energy and virial of one part of a structure (at
+//                          most kReduceAtoms atoms, four per thread, their
+//                          loads in flight together); a structure in one part
+//                          writes its energy and stress in GPa directly
+//                          (gap_calc.f90:152-154, 189-203), else the last part
+//                          to finish adds the parts' sums in part order;
+//   blocks [P, P + T)      the list statistics of kReduceAtoms atoms each,
+//                          folded with integer atomics; the last block writes
+//                          them out and clears the accumulators;
+//   blocks P + T + g       forces, one wavefront per atom (four atoms per
+//                          block g < G): F_m = sum of own slots - sum of
+//                          reverse slots in slot order (zeros without grad).
+// The sums go first so that they run beside the gather blocks.
 __global__ __launch_bounds__(256) void k_forces_reduce(int N, int G, int nstruct, NeighborBufs nb,
                                                        const double *__restrict__ slot_f, double *__restrict__ force,
                                                        const StructParams *__restrict__ sp,
                                                        const double *__restrict__ eatom,
                                                        const double *__restrict__ vir, int grad,
                                                        double *__restrict__ ene, double *__restrict__ stress6,
-                                                       TripletBufs tb, int *stats_out) {
+                                                       TripletBufs tb, ReduceBufs rb, int *stats_out) {
   const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  if (b > nstruct) {
-    const int g = b - nstruct - 1, m = 4 * g + wv;
-    if (g >= G || m >= N) return;
-    double f0 = 0, f1 = 0, f2 = 0;
-    if (grad) {
-      // the atom's own slots and reverse indices are read over its whole
-      // capacity, in flight together with its count (slots at or beyond the
-      // count hold stale values and are not added), so the gather waits for
-      // memory twice: own slots, then the reverse slots they name
-      const int K = nb.nb_count[m];
-      const size_t base = (size_t)m * nb.kcap;
-      for (int a = lane; a < nb.kcap; a += 64) {
-        const size_t sl = base + a;
-        const double o0 = slot_f[3 * sl], o1 = slot_f[3 * sl + 1], o2 = slot_f[3 * sl + 2];
-        const int rv = nb.nb_rev[sl];
-        // an empty asm that takes the loaded values: the compiler must issue
-        // the loads here, before the comparison with K, instead of moving
-        // them under it (no instruction is emitted)
-        asm volatile("" ::"v"(o0), "v"(o1), "v"(o2), "v"(rv));
-        const bool live = a < K;
-        const int r = live ? rv : -1;
-        if (live) {
-          f0 += o0;
-          f1 += o1;
-          f2 += o2;
-        }
-        if (r >= 0) {
-          f0 -= slot_f[3 * (size_t)r];
-          f1 -= slot_f[3 * (size_t)r + 1];
-          f2 -= slot_f[3 * (size_t)r + 2];
-        }
-      }
-      for (int o = 32; o > 0; o >>= 1) {
-        f0 += __shfl_xor(f0, o, 64);
-        f1 += __shfl_xor(f1, o, 64);
-        f2 += __shfl_xor(f2, o, 64);
-      }
-    }
-    if (lane == 0) {
-      force[3 * m] = f0;
-      force[3 * m + 1] = f1;
-      force[3 * m + 2] = f2;
-    }
-    return;
-  }
-  const int s = b;
-  if (s == nstruct) {
-    list_stats<256>(N, nb.kcap, nb.nb_count, nb.nb_raw, tb.tcap, tb.raw ? tb.count : nullptr, tb.raw, nb.total + 3,
-                    stats_out);
-    return;
-  }
-  const StructParams &P = sp[s];
-  __shared__ double red[4][7];
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  // four atoms per thread per step, their loads issued together (the sums
-  // keep the order a = t, t + 256, ..; more would cost the gather blocks of
-  // this launch occupancy)
-  constexpr int U = 4;
-  const int n = P.natoms;
-  for (int a0 = t; a0 < n; a0 += 256 * U) {
+  constexpr int U = kReduceAtoms / 256;
+  const int T = (N + kReduceAtoms - 1) / kReduceAtoms;
+  __shared__ int s_last;
+  if (b < rb.nparts) {
+    const int2 pm = rb.part_map[b];
+    const int s = pm.x, p = pm.y;
+    if (s >= nstruct) return;
+    const StructParams &P = sp[s];
+    const int parts = (P.natoms + kReduceAtoms - 1) / kReduceAtoms;
+    const int lo = p * kReduceAtoms, hi = min(P.natoms, lo + kReduceAtoms);
+    __shared__ double red[4][7];
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     double e[U], v[U][6];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int i = P.atom_base + min(a0 + 256 * u, n - 1);
+      const int i = P.atom_base + min(lo + t + 256 * u, hi - 1);
       e[u] = eatom[i];
 #pragma unroll
       for (int c = 0; c < 6; ++c) v[u][c] = grad ? vir[6 * i + c] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (a0 + 256 * u < n) {
+      if (lo + t + 256 * u < hi) {
         acc[0] += e[u];
 #pragma unroll
         for (int c = 0; c < 6; ++c) acc[1 + c] += v[u][c];
       }
-  }
 #pragma unroll
-  for (int c = 0; c < 7; ++c) {
-    const double v = wave_reduce_sum(acc[c]);
-    if (lane == 0) red[wv][c] = v;
+    for (int c = 0; c < 7; ++c) {
+      const double w = wave_reduce_sum(acc[c]);
+      if (lane == 0) red[wv][c] = w;
+    }
+    __syncthreads();
+    double tot = 0.0;
+    if (t < 7) tot = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+    if (parts > 1) {
+      if (t < 7) rb.part_sum[8 * (size_t)b + t] = tot;
+      __threadfence();
+      __syncthreads();
+      if (t == 0) s_last = (atomicAdd(&rb.ticket[s], 1) == parts - 1);
+      __syncthreads();
+      if (!s_last) return;
+      __threadfence();
+      if (t < 7) {
+        const size_t first = (size_t)(b - p);  // the structure's first part block
+        tot = split_sum(parts, [&](int q) {
+          return __hip_atomic_load(&rb.part_sum[8 * (first + q) + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        });
+      }
+      if (t == 0) rb.ticket[s] = 0;
+    }
+    if (t < 7) {
+      if (t == 0)
+        ene[s] = tot;
+      else
+        stress6[6 * s + t - 1] = grad ? tot * (-(1.0 / kGPa) / P.vol) : 0.0;
+    }
+    return;
   }
-  __syncthreads();
-  if (t < 7) {
-    const double v = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
-    if (t == 0)
-      ene[s] = v;
-    else
-      stress6[6 * s + t - 1] = grad ? v * (-(1.0 / kGPa) / P.vol) : 0.0;
+  if (b < rb.nparts + T) {
+    const int lo = (b - rb.nparts) * kReduceAtoms;
+    __shared__ long long red[4][6];
+    long long v[6] = {0, 0, 0, 0, 0, 0};
+    int r[U], c[U], tr[U], tc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int a = min(lo + t + 256 * u, N - 1);
+      r[u] = nb.nb_raw[a];
+      c[u] = nb.nb_count[a];
+      tr[u] = tb.raw ? tb.raw[a] : 0;
+      tc[u] = tb.raw ? tb.count[a] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (lo + t + 256 * u < N) {
+        v[0] += c[u];
+        v[1] = max(v[1], (long long)r[u]);
+        v[2] += (r[u] > nb.kcap) ? 1 : 0;
+        v[3] += tc[u];
+        v[4] = max(v[4], (long long)tr[u]);
+        v[5] += (tr[u] > tb.tcap) ? 1 : 0;
+      }
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const long long w = __shfl_xor(v[k], o, 64);
+        v[k] = (k == 1 || k == 4) ? max(v[k], w) : v[k] + w;
+      }
+    }
+    if (lane == 0)
+      for (int k = 0; k < 6; ++k) red[wv][k] = v[k];
+    __syncthreads();
+    if (t < 6) {
+      long long a = 0;
+      for (int w = 0; w < 4; ++w) a = (t == 1 || t == 4) ? max(a, red[w][t]) : a + red[w][t];
+      int *dst = &rb.stats_acc[t < 3 ? t : t + 1];  // the output layout, [3] and [7] left out
+      if (t == 1 || t == 4)
+        atomicMax(dst, (int)a);
+      else
+        atomicAdd(dst, (int)a);
+    }
+    __threadfence();
+    __syncthreads();
+    if (t == 0) s_last = (atomicAdd(&rb.stats_acc[7], 1) == T - 1);
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    int val = 0;
+    if (t < 8 && t != 3 && t != 7) val = __hip_atomic_load(&rb.stats_acc[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t == 3) val = nb.total[3];
+    if (t < 8) stats_out[t] = val;
+    __syncthreads();
+    if (t < 8) rb.stats_acc[t] = 0;
+    return;
+  }
+  const int g = b - rb.nparts - T, m = 4 * g + wv;
+  if (g >= G || m >= N) return;
+  double f0 = 0, f1 = 0, f2 = 0;
+  if (grad) {
+    // the atom's own slots and reverse indices are read over its whole
+    // capacity, in flight together with its count (slots at or beyond the
+    // count hold stale values and are not added), so the gather waits for
+    // memory twice: own slots, then the reverse slots they name
+    const int K = nb.nb_count[m];
+    const size_t base = (size_t)m * nb.kcap;
+    for (int a = lane; a < nb.kcap; a += 64) {
+      const size_t sl = base + a;
+      const double o0 = slot_f[3 * sl], o1 = slot_f[3 * sl + 1], o2 = slot_f[3 * sl + 2];
+      const int rv = nb.nb_rev[sl];
+      // an empty asm that takes the loaded values: the compiler must issue
+      // the loads here, before the comparison with K, instead of moving
+      // them under it (no instruction is emitted)
+      asm volatile("" ::"v"(o0), "v"(o1), "v"(o2), "v"(rv));
+      const bool live = a < K;
+      const int r = live ? rv : -1;
+      if (live) {
+        f0 += o0;
+        f1 += o1;
+        f2 += o2;
+      }
+      if (r >= 0) {
+        f0 -= slot_f[3 * (size_t)r];
+        f1 -= slot_f[3 * (size_t)r + 1];
+        f2 -= slot_f[3 * (size_t)r + 2];
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      f0 += __shfl_xor(f0, o, 64);
+      f1 += __shfl_xor(f1, o, 64);
+      f2 += __shfl_xor(f2, o, 64);
+    }
+  }
+  if (lane == 0) {
+    force[3 * m] = f0;
+    force[3 * m + 1] = f1;
+    force[3 * m + 2] = f2;
   }
 }
 
@@ -721,10 +743,11 @@ hipError_t launch_reverse_slots(hipStream_t s, int N, NeighborBufs nb) {
 hipError_t launch_forces_reduce(hipStream_t s, int N, int grad, const NeighborBufs &nb, const double *slot_f,
                                 double *force, int nstruct, const StructParams *sp, const double *eatom,
                                 const double *atom_vir, double *ene, double *stress6, const TripletBufs &tb,
-                                int *stats_out) {
-  const int G = (N + 3) / 4;
-  k_forces_reduce<<<G + nstruct + 1, 256, 0, s>>>(N, G, nstruct, nb, slot_f, force, sp, eatom, atom_vir, grad, ene,
-                                                  stress6, tb, stats_out);
+                                const ReduceBufs &rb, int *stats_out) {
+  if (N <= 0 || nstruct <= 0 || rb.nparts < nstruct) return hipErrorInvalidValue;
+  const int G = (N + 3) / 4, T = (N + kReduceAtoms - 1) / kReduceAtoms;
+  k_forces_reduce<<<rb.nparts + T + G, 256, 0, s>>>(N, G, nstruct, nb, slot_f, force, sp, eatom, atom_vir, grad, ene,
+                                                    stress6, tb, rb, stats_out);
   return hipGetLastError();
 }
 
