@@ -91,3 +91,32 @@ def test_mixed_batch_matches_single():
         # single structure, so FP32 partial sums round differently
         assert abs(e - ene[k]) < 1e-6 * max(1, abs(e))
         np.testing.assert_allclose(forces[k], f, atol=1e-5)
+
+
+def test_row_tiles_agree():
+    """The 64- and 128-row tiles of the mixed contraction (1 and 2 16-row
+    blocks per wavefront) do the same per-row arithmetic in the same order;
+    with the sparse points split the same way (8 splits of the 4096-atom
+    workload) their results agree to rounding."""
+    from gapx import synth
+    om = workload_model("ternary4k")
+    s = synth.workload_structure("ternary4k")
+    m = gapx.Model.from_model(om)
+    flags = gapx.GRAD | gapx.MIXED
+    out = {}
+    for rows in (64, 128):
+        ctx = gapx.Context(m)
+        ctx.load([s], 6.0)
+        ctx.set_tuning(flags, waves_fwd=1, waves_bwd=2, gpr_rows=rows, gpr_blocks=8 * 4096 // rows)
+        assert ctx.tuning(flags)["gpr_rows"] == rows
+        ctx.use_graph(False)
+        ctx.run(flags)
+        out[rows] = ctx.results()
+        ctx.close()
+    m.close()
+    for rows in (128,):
+        print("rows %d vs 64: dE %.3e  dF %.3e" % (rows, abs(out[rows]["energy"][0] - out[64]["energy"][0]),
+                                                 float(np.abs(out[rows]["forces"] - out[64]["forces"]).max())))
+        assert abs(out[rows]["energy"][0] - out[64]["energy"][0]) <= 1e-9 * max(1.0, abs(out[64]["energy"][0]))
+        np.testing.assert_allclose(out[rows]["forces"], out[64]["forces"], rtol=0, atol=1e-9)
+        np.testing.assert_allclose(out[rows]["stress"], out[64]["stress"], rtol=0, atol=1e-9)
