@@ -327,7 +327,6 @@ struct Ctx {
   int nstruct = 0, N = 0, Np = 0, nbins = 0;
   bool fused_bins = false;  // every structure small enough for k_bin_struct
   int fused_atoms = 0, fused_bins_max = 0;  // its LDS sizes: largest structure and bin count
-  bool ballot_bins = false;  // otherwise: every structure small enough for k_fill_bins
   double rcut = 6.0;
   std::vector<int> natoms, offsets;
   std::vector<double> lat;
@@ -451,7 +450,6 @@ static int struct_params(Ctx &c) {
   }
   c.fused_bins = std::getenv("GAPX_NO_FUSED_BINS") == nullptr && c.fused_atoms <= kFusedAtoms &&
                  fused_bin_lds_bytes(c.fused_atoms, c.fused_bins_max) <= kFusedLdsBytes;
-  c.ballot_bins = std::getenv("GAPX_SCATTER_BINS") == nullptr && c.fused_atoms <= kBallotAtoms;
   return GAPX_OK;
 }
 
@@ -636,7 +634,7 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b) {
       c.bins_clean = g;
     }
     GAPX_HIP(launch_wrap_scan(c.stream, c.N, c.nbins, d_pos, d_astruct, d_sp, nb, ticket, stats));
-    GAPX_HIP(launch_scatter_bins(c.stream, c.N, c.nbins, nb, c.nstruct, d_sp, c.ballot_bins));
+    GAPX_HIP(launch_scatter_bins(c.stream, c.N, c.nbins, nb));
   }
   // triplet lists for the angular lines (forward and backward share them),
   // built by the same wavefronts as the neighbour lists

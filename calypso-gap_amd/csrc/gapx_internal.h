@@ -129,13 +129,7 @@ struct NeighborBufs {
 // entry and leaves them zero (with bin_cursor[0, nbins) and stats[0..7]).
 hipError_t launch_wrap_scan(hipStream_t s, int N, int nbins, const double *pos, const int *atom_struct,
                             const StructParams *sp, NeighborBufs nb, int *ticket, int *stats);
-// Place the atoms in their bins ordered by index and write the bin-sorted
-// records: with ballot, one wavefront per bin reading its structure's atoms
-// in order (structures of at most kBallotAtoms atoms); else an atomic scatter
-// and a per-bin rank sort.
-constexpr int kBallotAtoms = 16384;
-hipError_t launch_scatter_bins(hipStream_t s, int N, int nbins, NeighborBufs nb, int nstruct, const StructParams *sp,
-                               bool ballot);
+hipError_t launch_scatter_bins(hipStream_t s, int N, int nbins, NeighborBufs nb);
 // The whole binning of structures with at most kFusedAtoms atoms each, one
 // workgroup per structure (also clears stats[0..7]); its LDS holds the
 // largest structure's bins and atoms (fused_bin_lds_bytes, at most

@@ -174,62 +174,6 @@ __global__ __launch_bounds__(64) void k_rank_bins(int nbins, NeighborBufs nb) {
   }
 }
 
-// Fill every bin straight from its structure's atoms in index order: one
-// wavefront per bin finds its structure (binary search over the structures'
-// bin offsets) and reads the structure's atom bins 4096 at a time (sixteen
-// 16-byte loads in flight per lane, lane l holding atoms 4 l .. 4 l + 3 of
-// each 256); a lane's matches are placed after those of the lanes below it
-// (exclusive prefix of the per-lane counts 0..4 from three ballots, one per
-// bit), so each bin comes out ordered by index without atomics or a sort.
-// The loop stops once the bin is full.  Entries read outside the structure
-// belong to other structures' bins and never match.  Same output as
-// k_scatter + k_rank_bins; each wave may read its whole structure, so it
-// serves structures of at most kBallotAtoms atoms.
-__global__ __launch_bounds__(64) void k_fill_bins(int N, int nbins, int nstruct, const StructParams *__restrict__ sp,
-                                                  NeighborBufs nb) {
-  const int b = blockIdx.x, lane = threadIdx.x;
-  if (b >= nbins) return;
-  int lo = 0, hi = nstruct - 1;  // the last structure whose bins start at or before b
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (sp[mid].bin_base <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  const int a_hi = sp[lo].atom_base + sp[lo].natoms;
-  const int qend = nb.bin_start[b + 1];
-  int q = nb.bin_start[b];
-  constexpr int U = 16;
-  const unsigned long long below = (1ull << lane) - 1ull;
-  for (int a0 = sp[lo].atom_base & ~3; a0 < a_hi && q < qend; a0 += 256 * U) {
-    int4 v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int a = a0 + 256 * u + 4 * lane;  // a multiple of 4: 16-byte aligned
-      if (a + 3 < N) {
-        v[u] = *(const int4 *)(nb.atom_bin + a);
-      } else {
-        v[u].x = a < N ? nb.atom_bin[a] : -1;
-        v[u].y = a + 1 < N ? nb.atom_bin[a + 1] : -1;
-        v[u].z = a + 2 < N ? nb.atom_bin[a + 2] : -1;
-        v[u].w = -1;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const bool m0 = v[u].x == b, m1 = v[u].y == b, m2 = v[u].z == b, m3 = v[u].w == b;
-      const int cnt = (int)m0 + (int)m1 + (int)m2 + (int)m3;
-      const unsigned long long c0 = __ballot(cnt & 1), c1 = __ballot(cnt & 2), c2 = __ballot(cnt & 4);
-      int p = q + __popcll(c0 & below) + 2 * __popcll(c1 & below) + 4 * __popcll(c2 & below);
-      const int a = a0 + 256 * u + 4 * lane;
-      if (m0) put_sorted(nb, p++, a);
-      if (m1) put_sorted(nb, p++, a + 1);
-      if (m2) put_sorted(nb, p++, a + 2);
-      if (m3) put_sorted(nb, p, a + 3);
-      q += __popcll(c0) + 2 * __popcll(c1) + 4 * __popcll(c2);
-    }
-  }
-}
-
 // One workgroup per structure does the whole binning in LDS: wrap and bin
 // every atom, count per bin, exclusive scan, scatter, then order each bin's
 // atoms by index and write the bin-sorted records; workgroup 0 also clears
@@ -762,12 +706,7 @@ hipError_t launch_wrap_scan(hipStream_t s, int N, int nbins, const double *pos, 
   return hipGetLastError();
 }
 
-hipError_t launch_scatter_bins(hipStream_t s, int N, int nbins, NeighborBufs nb, int nstruct, const StructParams *sp,
-                               bool ballot) {
-  if (ballot) {
-    k_fill_bins<<<nbins, 64, 0, s>>>(N, nbins, nstruct, sp, nb);
-    return hipGetLastError();
-  }
+hipError_t launch_scatter_bins(hipStream_t s, int N, int nbins, NeighborBufs nb) {
   k_scatter<<<(N + 255) / 256, 256, 0, s>>>(N, nb);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

@@ -188,26 +188,23 @@ def test_two_half_schedule_equals_one_piece(pop, flags):
 
 def test_one_launch_binning_equals_three_launches(pop, monkeypatch):
     """Batches of structures of up to 1024 atoms are binned by one workgroup
-    per structure in one launch; the two-launch path (wrap + count + scan,
-    then one wavefront per bin filling it in index order) and the three-launch
-    path (wrap + count + scan, atomic scatter, rank sort) give the same
-    neighbour order, hence bitwise-equal results."""
+    per structure in one launch; the three-launch path (wrap + count + scan,
+    scatter, rank) gives the same neighbour order, hence bitwise-equal
+    results."""
     m, _ = pop
     structs = [ternary_structure(1024, 1.0 / 0.07, 3101), ternary_structure(700, 1.0 / 0.07, 3102)]
     structs += synth.population(6, seed=3103)
     out = []
-    for env in ((), ("GAPX_NO_FUSED_BINS",), ("GAPX_NO_FUSED_BINS", "GAPX_SCATTER_BINS")):
-        for k in env:
-            monkeypatch.setenv(k, "1")
+    for off in (False, True):
+        if off:
+            monkeypatch.setenv("GAPX_NO_FUSED_BINS", "1")
         ctx = gapx.Context(m)
         ctx.load(structs, 6.0)
         ctx.run(gapx.GRAD)
         out.append(ctx.results())
         pairs, _ = ctx.counts()
         ctx.close()
-        for k in env:
-            monkeypatch.delenv(k)
+    monkeypatch.delenv("GAPX_NO_FUSED_BINS")
     assert pairs > 30 * sum(len(s["species"]) for s in structs)
-    for r in out[1:]:
-        for k in ("energy", "forces", "stress", "eatom"):
-            assert np.array_equal(out[0][k], r[k]), k
+    for k in ("energy", "forces", "stress", "eatom"):
+        assert np.array_equal(out[0][k], out[1][k]), k
