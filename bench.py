@@ -69,10 +69,12 @@ def zeta_1_2_4_16(header):
     return sorted(set(np.round(header["sf_zeta"][ang]).astype(int).tolist())) == [1, 2, 4, 16]
 
 
-def stage_kernel(stage, mixed, tune, z16):
+def stage_kernel(stage, mixed, tune, z16, des_len):
     """The one kernel of a single-kernel stage under the given launch shapes,
-    as rocprofv3 names it (grid-form tables with 4 zetas, D = 128; z16: the
-    zeta list is {1, 2, 4, 16})."""
+    as rocprofv3 names it (grid-form tables with 4 zetas; z16: the zeta list
+    is {1, 2, 4, 16}).  The contraction is instantiated for the descriptor
+    length padded to 32, with 64-point sparse chunks up to 128 features and
+    32-point chunks above, and 128-row tiles only up to 128 features."""
     t = "float" if mixed else "double"
     z = "true" if z16 else "false"
     if stage == "descriptors_fwd":
@@ -80,7 +82,11 @@ def stage_kernel(stage, mixed, tune, z16):
     if stage == "descriptors_bwd":
         return "k_desc_bwd<%s, %d, 4, %s>" % (t, tune["waves_bwd"], z)
     if stage == "gpr_contraction":
-        return "k_gpr_mixed<false, 128, 64, %d>" % (tune["gpr_rows"] // 64) if mixed else "k_gpr_f64<128>"
+        dp = (des_len + 31) // 32 * 32
+        if not mixed:
+            return "k_gpr_f64<%d>" % dp
+        r = tune["gpr_rows"] // 64 if dp <= 128 else 1
+        return "k_gpr_mixed<false, %d, %d, %d>" % (dp, 64 if dp <= 128 else 32, r)
     return "k_gpr_finish"
 
 
@@ -451,7 +457,7 @@ def main():
     rates = stage_rates(st, N, D, M, pairs, triplets, m, mixed=True)
     # the dominant kernel: the longest of the stages that are one kernel each
     dom = max(SINGLE_KERNEL_STAGES, key=lambda k: st[k])
-    kname = stage_kernel(dom, True, tuning["mixed"], zeta_1_2_4_16(m.header()))
+    kname = stage_kernel(dom, True, tuning["mixed"], zeta_1_2_4_16(m.header()), D)
     roof = {kk: rates[dom][kk] for kk in ("bound", "achieved", "peak", "unit", "frac")}
     roof["traffic"] = measured_traffic(args.workload, kname)
     roof["kernel"] = kname
