@@ -202,6 +202,34 @@ static int upload_model(Model &m) {
   int e2 = (cmax > 0) ? (int)std::floor(std::log2(16384.0 / cmax)) : 10;
   e2 = std::max(-24, std::min(e2, 10));
   m.w_scale = std::ldexp(1.0f, e2);
+  // Whether the mixed contraction can hold the energy: E_i = sum_k W_ik is
+  // computed from FP32 kernel values (relative error about 2^-21 each), so
+  // its error grows with A_i = sum_k |W_ik|, not with |E_i|.  A_i is probed
+  // on the model's own sparse points (the descriptor distribution the model
+  // was fitted on), up to 64 of them, in FP64 with direct differences.  A
+  // model whose terms cancel (A * 2^-21 above 1e-4 eV, a tenth of the
+  // 1 meV/atom bar) or whose scaled points leave the FP16 range runs its
+  // contraction in FP64 on the mixed path; the descriptors stay FP32 (their
+  // rounding moves E by about 1e-6 eV/atom even on such models).
+  double mt_abs = 0.0;
+  for (size_t q = 0; q < mt.size(); ++q) mt_abs = std::max(mt_abs, std::fabs(mt[q]));
+  double amax = 0.0;
+  const int nprobe = std::min(M, 64);
+  for (int p = 0; p < nprobe; ++p) {
+    const int j = (nprobe > 1) ? (int)((long long)p * (M - 1) / (nprobe - 1)) : 0;
+    double a = 0.0;
+    for (int k = 0; k < M; ++k) {
+      double d2 = 0.0;
+      for (int d = 0; d < D; ++d) {
+        const double v = mt[(size_t)j * Dp + d] - mt[(size_t)k * Dp + d];
+        d2 += v * v;
+      }
+      a += std::fabs(m.coeff[k]) * std::exp(-0.5 * d2);
+    }
+    amax = std::max(amax, a);
+  }
+  m.cancel_probe = amax;
+  m.mixed_f64 = (amax * std::ldexp(1.0, -21) > 1e-4) || !(mt_abs <= 16384.0);
   std::vector<uint16_t> hi((size_t)Mp * Dp), lo((size_t)Mp * Dp), bhi((size_t)Mp * Dp), blo((size_t)Mp * Dp);
   std::vector<uint16_t> thi((size_t)Mp * Dp), tlo((size_t)Mp * Dp), tbhi((size_t)Mp * Dp), tblo((size_t)Mp * Dp);
   // Feature-major copies: within every group of 32 points, point p sits at
@@ -357,6 +385,15 @@ struct Ctx {
   hipGraphExec_t gexec = nullptr;
   int graph_flags = -1;
   int graph_kcap = -1, graph_tcap = -1, graph_N = -1;
+  // launch parameters of the binning that the graph fixes at capture: the
+  // bin count (grid of the rank kernel), which binning kernel runs and the
+  // one-launch kernel's LDS sizes (largest structure and bin count)
+  int graph_nbins = -1, graph_fused = -1, graph_fused_atoms = -1, graph_fused_bins_max = -1;
+  bool graph_matches(int flags) const {
+    return gexec && graph_flags == flags && graph_kcap == kcap && graph_tcap == tcap && graph_N == N &&
+           graph_nbins == nbins && graph_fused == (fused_bins ? 1 : 0) && graph_fused_atoms == fused_atoms &&
+           graph_fused_bins_max == fused_bins_max;
+  }
 
   template <typename T>
   int get(const char *name, size_t count, T **out) {
@@ -665,14 +702,16 @@ static int fork_reverse(Ctx &c, Bufs &b) {
   return GAPX_OK;
 }
 
-static int gpr_layout(Ctx &c, GprArgs &g, bool fp64) {
+// f64_kernel: the contraction runs k_gpr_f64 (64-row tiles); tune: the launch
+// shapes of the run's precision (0 FP64, 1 mixed).
+static int gpr_layout(Ctx &c, GprArgs &g, bool f64_kernel, int tune) {
   const Model &m = *c.m;
-  const Tuning &t = c.tune[fp64 ? 0 : 1];
+  const Tuning &t = c.tune[tune];
   g.N = c.N;
   g.Np = c.Np;
   g.Dp = m.Dp;
   g.Mp = m.Mp;
-  g.rows = (!fp64 && m.Dp <= 128 && t.gpr_rows == 128) ? 128 : 64;
+  g.rows = (!f64_kernel && m.Dp <= 128 && t.gpr_rows == 128) ? 128 : 64;
   // Split the sparse points over grid.y until there are about gpr_blocks
   // workgroups (resident waves; more splits only add partial-sum traffic).
   const int tiles = c.Np / g.rows;
@@ -752,7 +791,8 @@ static int enqueue_forces(Ctx &c, Bufs &b, bool fp64, bool grad, const double *e
 // device, so the sequence can be captured into a graph.
 static int enqueue_run(Ctx &c, int flags) {
   Model &m = *c.m;
-  const bool fp64 = !(flags & GAPX_MIXED);
+  const bool fp64 = !(flags & GAPX_MIXED);  // descriptor precision
+  const bool c64 = fp64 || m.mixed_f64;      // contraction precision
   const bool grad = (flags & GAPX_GRAD) != 0;
   const int Dp = m.Dp;
   Bufs b;
@@ -772,26 +812,30 @@ static int enqueue_run(Ctx &c, int flags) {
   if (fp64) {
     if ((rc = c.get("x64", ND, &o.x64)) || (rc = c.get("xt64", ND, &o.xt64))) return rc;
   } else {
-    if ((rc = c.get("x32", ND, &o.x32)) || (rc = c.get("xt32", ND, &o.xt32)) || (rc = c.get("xt_hi", ND, &o.xt_hi)) ||
-        (rc = c.get("xt_lo", ND, &o.xt_lo)))
+    if ((rc = c.get("x32", ND, &o.x32)) || (rc = c.get("xt32", ND, &o.xt32))) return rc;
+    // the contraction's operands: FP64 xt (from the FP32 features) or the 16-bit split
+    if (c64) {
+      if ((rc = c.get("xt64", ND, &o.xt64))) return rc;
+    } else if ((rc = c.get("xt_hi", ND, &o.xt_hi)) || (rc = c.get("xt_lo", ND, &o.xt_lo))) {
       return rc;
+    }
   }
   // padding rows [N, Np) of the contraction operands stay zero
   if (c.Np > c.N) {
-    if (fp64) GAPX_HIP(launch_fill_double(c.stream, o.xt64 + (size_t)c.N * Dp, (size_t)(c.Np - c.N) * Dp, 0.0));
+    if (c64) GAPX_HIP(launch_fill_double(c.stream, o.xt64 + (size_t)c.N * Dp, (size_t)(c.Np - c.N) * Dp, 0.0));
     GAPX_HIP(launch_fill_double(c.stream, o.xn + c.N, c.Np - c.N, 0.0));
-    if (!fp64) {
+    if (!c64) {
       GAPX_HIP(launch_fill_int(c.stream, (int *)(o.xt_hi + (size_t)c.N * Dp), (c.Np - c.N) * Dp / 2, 0));
       GAPX_HIP(launch_fill_int(c.stream, (int *)(o.xt_lo + (size_t)c.N * Dp), (c.Np - c.N) * Dp / 2, 0));
     }
   }
-  const int split_mode = fp64 ? 0 : ((flags & GAPX_BF16) ? 2 : 1);
+  const int split_mode = c64 ? 0 : ((flags & GAPX_BF16) ? 2 : 1);
   const int chunk = c.chunk > 0 ? c.chunk : 16;
   const Tuning &tu = c.tune[fp64 ? 0 : 1];
 
   // contraction arguments
   GprArgs &g = b.g;
-  gpr_layout(c, g, fp64);
+  gpr_layout(c, g, c64, fp64 ? 0 : 1);
   g.xt64 = o.xt64;
   g.xn = o.xn;
   g.xt_hi = o.xt_hi;
@@ -811,7 +855,7 @@ static int enqueue_run(Ctx &c, int flags) {
   g.wscale = m.w_scale;
   if ((rc = c.get("Epart", (size_t)g.splits * c.Np, &g.Epart)) || (rc = c.get("eatom", c.Np, &g.eatom))) return rc;
   if (grad) {
-    if (fp64) {
+    if (c64) {
       if ((rc = c.get("Gpart", (size_t)g.splits * ND, &g.Gpart))) return rc;
     } else {
       if ((rc = c.get("Gpart32", (size_t)g.splits * ND, &g.Gpart32))) return rc;
@@ -824,12 +868,13 @@ static int enqueue_run(Ctx &c, int flags) {
   }
   GradSrc gs;
   if (grad) {
-    // the finish (sum of split partials, dE/dx) runs inside the backward kernel
+    // the finish (sum of split partials, dE/dx) runs inside the backward
+    // kernel, with the xt the contraction used (FP64 whenever it ran in FP64)
     gs.splits = g.splits;
     gs.Np = g.Np;
     gs.Epart = g.Epart;
-    gs.Gpart = fp64 ? g.Gpart : nullptr;
-    gs.Gpart32 = fp64 ? nullptr : g.Gpart32;
+    gs.Gpart = c64 ? g.Gpart : nullptr;
+    gs.Gpart32 = c64 ? nullptr : g.Gpart32;
     gs.xt64 = o.xt64;
     gs.xt32 = o.xt32;
     gs.theta_inv = m.d_theta_inv;
@@ -838,7 +883,7 @@ static int enqueue_run(Ctx &c, int flags) {
     gs.g32_out = g.g32;
   }
   auto contraction = [&](hipStream_t st, const GprArgs &ga) -> int {
-    if (fp64)
+    if (c64)
       GAPX_HIP(launch_gpr_fp64(st, ga));
     else
       GAPX_HIP(launch_gpr_mixed(st, ga));
@@ -888,7 +933,7 @@ static int enqueue_run(Ctx &c, int flags) {
   stage_end(c, kContraction);
   if (!grad) {
     stage_begin(c, kFinish);
-    GAPX_HIP(launch_gpr_finish(c.stream, g, fp64));
+    GAPX_HIP(launch_gpr_finish(c.stream, g, c64));
     stage_end(c, kFinish);
   }
   return enqueue_forces(c, b, fp64, grad, g.eatom, gs, tu.waves_bwd);
@@ -963,9 +1008,7 @@ static int ctx_run(Ctx &c, int flags, Mlp *mlp = nullptr) {
   if (c.mlp) {  // surrogate runs are eager
     if ((rc = enqueue_surrogate(c, *c.mlp, flags))) return rc;
   } else if (c.use_graph && !c.timing) {
-    const bool valid = c.gexec && c.graph_flags == flags && c.graph_kcap == c.kcap && c.graph_tcap == c.tcap &&
-                       c.graph_N == c.N;
-    if (!valid) {
+    if (!c.graph_matches(flags)) {
       c.invalidate_graph();
       if ((rc = enqueue_run(c, flags))) return rc;
       hipGraph_t graph = nullptr;
@@ -987,6 +1030,10 @@ static int ctx_run(Ctx &c, int flags, Mlp *mlp = nullptr) {
       c.graph_kcap = c.kcap;
       c.graph_tcap = c.tcap;
       c.graph_N = c.N;
+      c.graph_nbins = c.nbins;
+      c.graph_fused = c.fused_bins ? 1 : 0;
+      c.graph_fused_atoms = c.fused_atoms;
+      c.graph_fused_bins_max = c.fused_bins_max;
     }
     GAPX_HIP(hipGraphLaunch(c.gexec, c.stream));
   } else {
@@ -1733,6 +1780,13 @@ int gapx_model_header(const gapx_model *m, int *z, double *w, int *sf_type, doub
   return GAPX_OK;
 }
 
+int gapx_model_precision(const gapx_model *m, double *cancellation, int *mixed_contraction_fp64) {
+  if (!m) return fail(GAPX_ERR_ARG, "null model");
+  if (cancellation) *cancellation = m->cancel_probe;
+  if (mixed_contraction_fp64) *mixed_contraction_fp64 = m->mixed_f64 ? 1 : 0;
+  return GAPX_OK;
+}
+
 int gapx_model_write(const gapx_model *m, const char *path) {
   if (!m || !path) return fail(GAPX_ERR_ARG, "gapx_model_write: null argument");
   return write_model_text(*m, path);
@@ -1909,14 +1963,15 @@ int gapx_ctx_update(gapx_ctx *c, const double *lat, const double *pos) {
     const std::vector<double> lat_before = c->lat;
     const std::vector<StructParams> sp_before = c->sp;
     std::memcpy(c->lat.data(), lat, 9 * sizeof(double) * c->nstruct);
-    int nb_before = c->nbins;
     int rc = struct_params(*c);
     if (rc) {
       c->lat = lat_before;
       c->sp = sp_before;
+      struct_params(*c);  // restores nbins and the binning choice of the loaded cells
       return rc;
     }
-    if (c->nbins != nb_before) c->invalidate_graph();
+    // a captured graph whose binning launch parameters no longer match is
+    // replaced at the next run (Ctx::graph_matches)
     GAPX_HIP(hipMemcpyAsync(c->ptr<StructParams>("sp"), c->sp.data(), sizeof(StructParams) * c->nstruct,
                             hipMemcpyHostToDevice, c->stream));
   }

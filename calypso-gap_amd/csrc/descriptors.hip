@@ -138,6 +138,14 @@ __device__ __forceinline__ void line_power(const SFDev &sf, int s, T cth, PowSta
   }
 }
 
+// A feature's FP16 contraction operand: xt clamped to [-2^15, 2^15], so the
+// hi part is finite and the lo part exact.  The host keeps every scaled
+// sparse-point component within 2^14 on this path, so a clamped component
+// still differs from each of them by more than 2^14: the kernel value is
+// exp(-d2 / 2) with d2 > 2^28 whether the norm of the clamped or of the true
+// row enters d2, i.e. 0 in FP32 and in FP64 alike.
+__device__ __forceinline__ double fp16_operand(double xt) { return fmin(fmax(xt, -32768.0), 32768.0); }
+
 // Launch geometry of the descriptor kernels: LDS layout (neighbour slots,
 // cutoff groups, padded features) and the first atom of the launch (one
 // workgroup per atom from atom0 on).
@@ -335,9 +343,10 @@ __global__ __launch_bounds__(256) void k_desc_fwd(int N, const int *__restrict__
     xn += xt * xt;
     if (out.xt64) out.xt64[row + d] = xt;
     if (out.xt32) out.xt32[row + d] = (float)xt;
-    if (split_mode == 1) {  // FP16 hi + lo
-      const _Float16 hi = (_Float16)xt;
-      const _Float16 lo = (_Float16)(xt - (double)hi);
+    if (split_mode == 1) {  // FP16 hi + lo of xt clamped to the FP16 range (see fp16_operand)
+      const double xc = fp16_operand(xt);
+      const _Float16 hi = (_Float16)xc;
+      const _Float16 lo = (_Float16)(xc - (double)hi);
       out.xt_hi[row + d] = __builtin_bit_cast(uint16_t, hi);
       out.xt_lo[row + d] = __builtin_bit_cast(uint16_t, lo);
     } else if (split_mode == 2) {  // BF16 hi + lo
@@ -618,8 +627,9 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
     if (out.xt64) out.xt64[row + d] = xt;
     if (out.xt32) out.xt32[row + d] = (float)xt;
     if (split_mode == 1) {
-      const _Float16 hi = (_Float16)xt;
-      const _Float16 lo = (_Float16)(xt - (double)hi);
+      const double xc = fp16_operand(xt);
+      const _Float16 hi = (_Float16)xc;
+      const _Float16 lo = (_Float16)(xc - (double)hi);
       out.xt_hi[row + d] = __builtin_bit_cast(uint16_t, hi);
       out.xt_lo[row + d] = __builtin_bit_cast(uint16_t, lo);
     } else if (split_mode == 2) {

@@ -77,6 +77,8 @@ struct Model {
   uint16_t *d_mtT_bhi = nullptr, *d_mtT_blo = nullptr; // BF16 split, transposed
   double *d_qmm = nullptr;               // [Mp][Mp] variance matrix Q, zero padded (own allocation)
   float w_scale = 1024.f;                // power of 2 applied to W before its FP16 split
+  double cancel_probe = 0.0;             // largest sum_k |W_jk| over the probed sparse points j
+  bool mixed_f64 = false;                // mixed runs use the FP64 contraction (see upload_model)
   size_t dev_bytes = 0;
   struct Ctx *cached_ctx = nullptr;      // workspace for gapx_calc
 };

@@ -161,9 +161,9 @@ contains
       status = 2
       return
     end if
+    call free_header()
     read(u, *, iostat=ios) hdr_nspecies
     if (ios /= 0 .or. hdr_nspecies <= 0) goto 90
-    if (allocated(hdr_z)) deallocate(hdr_z, hdr_w, hdr_type, hdr_alpha, hdr_cut, hdr_zeta)
     allocate(hdr_z(hdr_nspecies), hdr_w(hdr_nspecies))
     do k = 1, hdr_nspecies
       read(u, *, iostat=ios) hdr_z(k), hdr_w(k)
@@ -193,6 +193,22 @@ contains
     return
 90  close(u)
     status = 2
+    ! a header read that failed part-way leaves no header and no stamps, so
+    ! the next call reads the file again from the start
+    call free_header()
+    stamp_size = -1
+    stamp_mtime = -1
+  end subroutine
+
+  ! Every header array that is allocated, each under its own check (a failed
+  ! read can leave some allocated and some not).
+  subroutine free_header()
+    if (allocated(hdr_z)) deallocate(hdr_z)
+    if (allocated(hdr_w)) deallocate(hdr_w)
+    if (allocated(hdr_type)) deallocate(hdr_type)
+    if (allocated(hdr_alpha)) deallocate(hdr_alpha)
+    if (allocated(hdr_cut)) deallocate(hdr_cut)
+    if (allocated(hdr_zeta)) deallocate(hdr_zeta)
   end subroutine
 
   ! Device model for (header, theta, MM, coeff); rebuilt when the header file

@@ -47,7 +47,7 @@ EXPORTS = [
     "gapx_mlp_train", "gapx_ctx_export_labels", "gapx_ctx_run_surrogate", "gapx_ctx_relax",
     "gapx_ctx_positions", "gapx_model_set_qmm", "gapx_ctx_variance",
     "gapx_ctx_md", "gapx_ctx_momenta", "gapx_model_write", "gapx_model_save", "gapx_model_load",
-    "gapx_model_read_cached", "gapx_probe_fma_rates",
+    "gapx_model_read_cached", "gapx_probe_fma_rates", "gapx_model_precision",
 ]
 
 
@@ -74,6 +74,7 @@ def lib():
         L.gapx_model_arrays.argtypes = [vp, vp, vp, ci, ci, vp]
         L.gapx_model_header.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.gapx_model_free.argtypes = [vp]
+        L.gapx_model_precision.argtypes = [vp, vp, vp]
         L.gapx_model_free.restype = None
         L.gapx_file_stamp.argtypes = [ctypes.c_char_p, vp, vp]
         L.gapx_calc.argtypes = [vp, ci, vp, vp, vp, cd, ci, vp, vp, vp, vp]
@@ -264,6 +265,13 @@ class Model:
                                      float(rcut), fl, _p(ene), _p(force), _p(stress)))
         offs = np.concatenate([[0], np.cumsum(natoms)])
         return ene, [force[offs[k]:offs[k + 1]] for k in range(len(structures))], stress
+
+    def precision(self):
+        """(cancellation probe A, whether GAPX_MIXED runs contract in FP64)."""
+        a = ctypes.c_double()
+        f = ctypes.c_int()
+        _check(lib().gapx_model_precision(self._h, ctypes.byref(a), ctypes.byref(f)))
+        return dict(cancellation=a.value, mixed_contraction_fp64=bool(f.value))
 
     def set_qmm(self, qmm):
         """Q of the predictive variance (nsparse x nsparse), or None for Q = 0."""

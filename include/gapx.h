@@ -90,6 +90,16 @@ int gapx_model_header(const gapx_model *m, int *z, double *w, int *sf_type, doub
 
 void gapx_model_free(gapx_model *m);
 
+/* How the mixed path treats this model (no reference counterpart; the
+ * reference computes in FP64 only).  The energy E_i = sum_k W_ik is a sum of
+ * terms W_ik = K_ik c_k that can cancel: its FP32 error grows with
+ * A_i = sum_k |W_ik|.  cancellation = the largest A_j over up to 64 of the
+ * model's own sparse points j (FP64, at load); mixed_contraction_fp64 = 1
+ * when A * 2^-21 exceeds 1e-4 eV or a scaled sparse-point component exceeds
+ * 2^14, and then GAPX_MIXED runs keep FP32 descriptors but contract in FP64.
+ * Either pointer may be NULL. */
+int gapx_model_precision(const gapx_model *m, double *cancellation, int *mixed_contraction_fp64);
+
 /* Model I/O beyond FGAP_READ (gap_calc.f90:303-364 reads the text once per
  * call and keeps no cache).
  * gapx_model_write: the model as a gap_parameters text file in the layout

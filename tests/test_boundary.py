@@ -99,3 +99,37 @@ def test_ase_adapter_imports_against_ase_interface():
         assert mod.GAP.implemented_properties == ["energy", "forces", "stress"]
     finally:
         sys.path.remove(os.path.join(ROOT, "tests", "stub_ase"))
+
+
+@pytest.mark.parametrize("bad", ["species", "nsf", "sf_line"])
+def test_failed_header_read_is_reported_again_not_aborted(tmp_path, bad):
+    """A gap_parameters header that fails part-way (a bad species line, SF
+    count or SF line) is a status error on every call, not a Fortran abort on
+    the second call: the reader frees what it allocated and forgets the
+    file's stamp.  After the file is fixed, the next read goes past the header
+    (here to the model upload, which needs a GPU: status 6, or 0 on one)."""
+    lines = open(MODEL).read().splitlines()
+    ns = int(lines[0].split()[0])
+    k = {"species": 1, "nsf": 1 + ns, "sf_line": 2 + ns}[bad]
+    broken = list(lines)
+    broken[k] = "x y z"
+    (tmp_path / "gap_parameters").write_text("\n".join(broken) + "\n")
+    code = """
+import numpy as np, os, time
+import libgap.libgap as L
+m = np.zeros((2, 66), order="F"); th = np.ones(66); c = np.zeros(2)
+lat = np.eye(3) * 8.0; pos = np.zeros((1, 3), order="F"); sp = np.array([6], np.int32)
+out = [L.fgap_calc_status(sp, lat, pos, th, m, c, 6.0, True, 0)[-1] for _ in range(3)]
+os.replace("gap_parameters.good", "gap_parameters")
+t = time.time() + 2.0
+os.utime("gap_parameters", (t, t))
+out.append(L.fgap_calc_status(sp, lat, pos, th, m, c, 6.0, True, 0)[-1])
+print("statuses", out)
+"""
+    shutil.copy(MODEL, tmp_path / "gap_parameters.good")
+    r = _run(code, tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith("statuses")][0]
+    st = eval(line.split(" ", 1)[1])
+    assert st[:3] == [3, 3, 3]
+    assert st[3] in (0, 6, 7)

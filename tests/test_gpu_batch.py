@@ -208,3 +208,38 @@ def test_one_launch_binning_equals_three_launches(pop, monkeypatch):
     assert pairs > 30 * sum(len(s["species"]) for s in structs)
     for k in ("energy", "forces", "stress", "eatom"):
         assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_lattice_update_with_same_bin_total_recaptures(pop):
+    """A captured graph fixes the binning's launch parameters (which kernel,
+    its LDS sizes, the bin count).  Two structures whose bin grids change from
+    4x4x2 + 2x4x5 (32 + 40 bins) to 4x4x4 + 2x2x2 (64 + 8) keep the total at
+    72 while the largest structure's bin count rises from 40 to 64; with graph
+    replay on, a lattice update must not replay the old launch.  The updated
+    context equals an eager run loaded with the new cells bit for bit."""
+    m, _ = pop
+    a2 = ternary_structure(40, 1728.0 / 40, 3201)   # 12 x 12 x 12 cell after the update
+    b2 = ternary_structure(20, 216.0 / 20, 3202)    # 6 x 6 x 6
+    a2["lat"], b2["lat"] = np.eye(3) * 12.0, np.eye(3) * 6.0
+    a2["pos"] = synth.random_positions(a2["lat"], 40, np.random.default_rng(3203))
+    b2["pos"] = synth.random_positions(b2["lat"], 20, np.random.default_rng(3204))
+    lat1 = [np.diag([12.0, 12.0, 6.0]), np.diag([6.0, 12.0, 15.0])]   # 4x4x2 and 2x4x5 bins
+    first = []
+    for s, l1 in zip((a2, b2), lat1):
+        frac = s["pos"] @ np.linalg.inv(s["lat"])
+        first.append(dict(species=s["species"], lat=l1, pos=frac @ l1))
+    flags = gapx.GRAD
+    ctx = gapx.Context(m)
+    ctx.load(first, 6.0)
+    ctx.use_graph(True)
+    for _ in range(2):
+        ctx.run(flags)   # eager run, capture, replay
+    ctx.results()
+    ctx.update(np.concatenate([a2["pos"], b2["pos"]]), lat=np.stack([a2["lat"], b2["lat"]]))
+    for _ in range(2):
+        ctx.run(flags)
+    rg = ctx.results()
+    ctx.close()
+    re = evaluate(m, [a2, b2], flags)[0]
+    for k in ("energy", "forces", "stress"):
+        assert np.array_equal(rg[k], re[k]), k
