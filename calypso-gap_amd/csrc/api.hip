@@ -625,7 +625,7 @@ static void collect_timing(Ctx &c) {
 
 // Largest neighbour count the run accepts (LDS layout, or the reference's cap).
 static int neighbor_cap(Ctx &c, int flags) {
-  const int lds = std::min(1024, max_neighbors_for_lds(!(flags & GAPX_MIXED), std::max(1, c.m->sf.n_groups), c.m->Dp));
+  const int lds = max_neighbors_for_lds(!(flags & GAPX_MIXED), c.m->sfd, c.m->Dp);
   return (flags & GAPX_REF_NEIGHBOR_CAP) ? std::min(lds, kRefNeighborCap) : lds;
 }
 
@@ -653,6 +653,15 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b) {
   nb.total = stats;
   nb.overflow = stats + 2;
   nb.kcap = c.kcap;
+  // cutoff tables in global memory for lists too long for the descriptor
+  // kernels' LDS layouts
+  nb.fc_scratch = nullptr;
+  const size_t tbytes = desc_table_scratch_bytes(!(flags & GAPX_MIXED), c.m->sfd, c.m->Dp, c.kcap, c.N);
+  if (tbytes) {
+    unsigned char *t;
+    if ((rc = c.get("fc_scratch", tbytes, &t))) return rc;
+    nb.fc_scratch = t;
+  }
   const int *d_astruct = c.ptr<int>("astruct");
   const double *d_pos = c.ptr<double>("pos");
   const StructParams *d_sp = c.ptr<StructParams>("sp");
@@ -1050,10 +1059,13 @@ static int ctx_run(Ctx &c, int flags, Mlp *mlp = nullptr) {
 
 // Wait for the last run and check its neighbour counts: above kcap the run
 // is repeated with room for the largest count; above the cap it is an error.
+// At most two reruns are needed: the raw neighbour counts do not depend on
+// kcap, but the triplet counts of a run whose lists were cut at kcap do, so
+// the rerun with the full lists can still overflow tcap once.
 static int ctx_settle(Ctx &c) {
   GAPX_HIP(hipStreamSynchronize(c.stream));
   collect_timing(c);
-  for (int attempt = 0; attempt < 2; ++attempt) {
+  for (int attempt = 0; attempt < 4; ++attempt) {
     const int *h = c.host_stats;
     const int cap = neighbor_cap(c, c.last_flags);
     if (h[1] > cap) {

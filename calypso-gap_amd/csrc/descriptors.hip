@@ -30,6 +30,11 @@
 // after each chunk of lines.  Lines are sorted by (cutoff group, eta, lambda,
 // zeta), so exponentials, cutoff products and the powers (1 + lambda cos)^zeta
 // are updated incrementally along a chunk (zeta = 2, 4, 16 by squaring).
+#include <atomic>
+#include <mutex>
+#include <set>
+#include <utility>
+
 #include "gapx_internal.h"
 
 namespace gapx {
@@ -249,7 +254,10 @@ __device__ __forceinline__ void chunk_reduce(T (&acc)[NCH], T (&accw)[NCH], int 
   __syncthreads();
 }
 
-template <typename T, int NCH, int W>
+// FG: the per-group cutoff tables live in the global buffer nb.fc_scratch
+// (row of atom i) instead of LDS, for neighbour lists too long for the LDS
+// layout (launched with one wavefront per atom).
+template <typename T, int NCH, int W, bool FG>
 __global__ __launch_bounds__(256) void k_desc_fwd(int N, const int *__restrict__ species,
                                                   const double *__restrict__ wz, NeighborBufs nb, TripletBufs tb,
                                                   SFDev sf, Lds L, const double *__restrict__ theta_inv,
@@ -262,8 +270,8 @@ __global__ __launch_bounds__(256) void k_desc_fwd(int N, const int *__restrict__
   double *s_xn = (double *)smem;  // [W]
   T *s_d = (T *)(s_xn + W);
   T *s_w = s_d + 4 * KP;
-  T *s_fc = s_w + KP;
-  T *s_feat = s_fc + L.G * KP;
+  T *s_fc = FG ? (T *)nb.fc_scratch + (size_t)i * L.G * KP : s_w + KP;
+  T *s_feat = FG ? s_w + KP : s_fc + L.G * KP;
   T *s_part = s_feat + Dp;  // [W * 2 * NCH]
 
   const int start = i * nb.kcap;
@@ -478,7 +486,7 @@ struct GridShape {
   static constexpr int NV = NE * 2 * NZ * 2;
 };
 
-template <typename T, int NZ, int W, bool Z16>
+template <typename T, int NZ, int W, bool Z16, bool FG>
 __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restrict__ species,
                                                        const double *__restrict__ wz, NeighborBufs nb,
                                                        TripletBufs tb, SFDev sf, Lds L,
@@ -494,8 +502,8 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
   double *s_xn = (double *)smem;  // [W], padded to 16 bytes
   T *s_d = (T *)(s_xn + ((W + 1) & ~1));
   T *s_rw = s_d + 4 * KP;       // [KP] pairs (1 / r, species weight)
-  T *s_fc = s_rw + 2 * KP;
-  T *s_feat = s_fc + L.G * KP;
+  T *s_fc = FG ? (T *)nb.fc_scratch + (size_t)i * L.G * KP : s_rw + 2 * KP;
+  T *s_feat = FG ? s_rw + 2 * KP : s_fc + L.G * KP;
   T *s_part = s_feat + Dp;      // [W * NV]
   T *s_sum = s_part + W * NV;   // [NV]
 
@@ -807,7 +815,7 @@ __device__ __forceinline__ void ang_bwd_term(uint32_t q, bool active, const SFDe
   }
 }
 
-template <typename T, int W, int GNZ, bool Z16>
+template <typename T, int W, int GNZ, bool Z16, bool FG>
 __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__ species,
                                                   const double *__restrict__ wz, NeighborBufs nb, TripletBufs tb,
                                                   SFDev sf, Lds L, GradSrc gs, double *__restrict__ slot_f,
@@ -824,8 +832,10 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   double *s_F = s_T + W * KP;      // [W][3 KP]
   T *s_d = (T *)(s_F + 3 * W * KP);
   T *s_rw = s_d + 4 * KP;    // [KP] pairs (1 / r, species weight)
-  T *s_fd = s_rw + 2 * KP;   // [G][KP] pairs (fc, dfc/dr) per cutoff group
-  T *s_g = s_fd + 2 * G * KP;
+  // [G][KP] pairs (fc, dfc/dr) per cutoff group, in LDS or (FG) in the
+  // atom's row of the global buffer
+  T *s_fd = FG ? (T *)nb.fc_scratch + (size_t)i * 2 * G * KP : s_rw + 2 * KP;
+  T *s_g = FG ? s_rw + 2 * KP : s_fd + 2 * G * KP;
   T *s_gA = s_g + Dp;  // per angular line: g of the plain and of the weighted feature
   // s_gW starts on a 4-value boundary: with 16-byte aligned blocks (every
   // region above has a multiple of 4 values) the grid form reads four lines'
@@ -974,25 +984,92 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   }
 }
 
-template <typename T, int NCH, int W>
+// LDS bytes of each kernel's layout; FG: the cutoff tables in global memory.
+template <typename T, int NCH, int W, bool FG>
 size_t fwd_lds(const Lds &L) {
-  return W * sizeof(double) + (size_t)(4 * L.KP + L.KP + L.G * L.KP + L.Dp + W * 2 * NCH) * sizeof(T);
+  return W * sizeof(double) + (size_t)(4 * L.KP + L.KP + (FG ? 0 : L.G * L.KP) + L.Dp + W * 2 * NCH) * sizeof(T);
 }
 
-template <typename T, int W>
+template <typename T, int W, bool FG>
 size_t bwd_lds(const Lds &L) {
   return (6 * W + 4 * W * (size_t)L.KP) * sizeof(double) +
-         (size_t)(6 * L.KP + 2 * L.G * L.KP + 2 * L.Dp + L.Dp / 2) * sizeof(T);
+         (size_t)(6 * L.KP + (FG ? 0 : 2 * L.G * L.KP) + 2 * L.Dp + L.Dp / 2) * sizeof(T);
 }
 
+template <typename T, int NZ, int W, bool FG>
+size_t fwd_grid_lds(const Lds &L) {
+  constexpr int NV = GridShape<T, NZ>::NV;
+  return ((W + 1) & ~1) * sizeof(double) +
+         (size_t)(6 * L.KP + (FG ? 0 : L.G * L.KP) + L.Dp + W * NV + NV) * sizeof(T);
+}
+
+// Layouts with several wavefronts per atom stay within 64 KB, so that several
+// workgroups share a CU.  A one-wave layout may take the device's whole
+// per-workgroup LDS (160 KB on gfx950) for long neighbour lists; past that the
+// cutoff tables move to global memory (FG).
 constexpr size_t kLdsBudget = 64 * 1024;
 
-template <typename T, int NCH, int W>
+// Per-workgroup LDS limit of the current device, queried once per device.
+size_t lds_limit() {
+  static std::atomic<long long> cache[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return kLdsBudget;
+  long long v = cache[dev].load(std::memory_order_relaxed);
+  if (v == 0) {
+    int b = 0;
+    if (hipDeviceGetAttribute(&b, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        b < (int)kLdsBudget)
+      b = (int)kLdsBudget;
+    v = b;
+    cache[dev].store(v, std::memory_order_relaxed);
+  }
+  return (size_t)v;
+}
+
+// A kernel launched with more than 64 KB of dynamic LDS is first allowed the
+// device's limit (once per kernel and device, outside any stream capture:
+// the eager run that precedes a capture makes the same launches).
+hipError_t allow_lds(const void *kernel, size_t bytes) {
+  if (bytes <= kLdsBudget) return hipSuccess;
+  static std::mutex mu;
+  static std::set<std::pair<int, const void *>> done;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(mu);
+  if (done.count({dev, kernel})) return hipSuccess;
+  e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit());
+  if (e == hipSuccess) done.insert({dev, kernel});
+  return e;
+}
+
+// Launch shape of a descriptor kernel: the most wavefronts per atom (up to
+// `waves`, at most `wmax`) whose layout fits 64 KB, else one wavefront within
+// the device's limit, else one wavefront with the cutoff tables in global
+// memory.  bytes(W, fg) is the layout's LDS size.  False if nothing fits
+// (the host caps the list length so that the last form always does).
+template <typename F>
+bool pick_shape(int waves, int wmax, F bytes, int &W, bool &fg) {
+  fg = false;
+  for (int w = 4; w >= 2; w /= 2)
+    if (w <= wmax && waves >= w && bytes(w, false) <= kLdsBudget) {
+      W = w;
+      return true;
+    }
+  W = 1;
+  if (bytes(1, false) <= lds_limit()) return true;
+  fg = true;
+  return bytes(1, true) <= lds_limit();
+}
+
+template <typename T, int NCH, int W, bool FG>
 hipError_t fwd_launch(hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
                       const TripletBufs &tb, const SFDev &sf, Lds L, const double *theta_inv, const double *mu,
                       int split_mode, DescOut out) {
-  k_desc_fwd<T, NCH, W><<<N, 64 * W, fwd_lds<T, NCH, W>(L), s>>>(N, species, wz, nb, tb, sf, L, theta_inv, mu,
-                                                                  split_mode, out);
+  const size_t lds = fwd_lds<T, NCH, W, FG>(L);
+  const hipError_t e = allow_lds((const void *)k_desc_fwd<T, NCH, W, FG>, lds);
+  if (e != hipSuccess) return e;
+  k_desc_fwd<T, NCH, W, FG><<<N, 64 * W, lds, s>>>(N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
   return hipGetLastError();
 }
 
@@ -1000,9 +1077,18 @@ template <typename T, int NCH>
 hipError_t fwd_waves(int waves, hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
                      const TripletBufs &tb, const SFDev &sf, Lds L, const double *theta_inv, const double *mu,
                      int split_mode, DescOut out) {
-  if (waves <= 1) return fwd_launch<T, NCH, 1>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
-  if (waves == 2) return fwd_launch<T, NCH, 2>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
-  return fwd_launch<T, NCH, 4>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+  int W;
+  bool fg;
+  auto bytes = [&](int w, bool g) {
+    if (w == 4) return fwd_lds<T, NCH, 4, false>(L);
+    if (w == 2) return fwd_lds<T, NCH, 2, false>(L);
+    return g ? fwd_lds<T, NCH, 1, true>(L) : fwd_lds<T, NCH, 1, false>(L);
+  };
+  if (!pick_shape(waves, 4, bytes, W, fg) || (fg && !nb.fc_scratch)) return hipErrorInvalidValue;
+  if (fg) return fwd_launch<T, NCH, 1, true>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+  if (W == 1) return fwd_launch<T, NCH, 1, false>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+  if (W == 2) return fwd_launch<T, NCH, 2, false>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+  return fwd_launch<T, NCH, 4, false>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
 }
 
 // zeta list {1, 2, 4, 16}: the grid kernels' specialisation with a compile-time chain
@@ -1010,76 +1096,136 @@ bool zeta_1_2_4_16(const SFDev &sf) {
   return sf.nz == 4 && sf.zlog[0] == 0 && sf.zlog[1] == 1 && sf.zlog[2] == 2 && sf.zlog[3] == 4;
 }
 
-template <typename T, int W, int GNZ, bool Z16 = false>
+template <typename T, int W, int GNZ, bool Z16, bool FG>
 hipError_t bwd_launch(hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
                       const TripletBufs &tb, const SFDev &sf, Lds L, const GradSrc &gs, double *slot_f,
                       double *atom_vir) {
-  k_desc_bwd<T, W, GNZ, Z16><<<N, 64 * W, bwd_lds<T, W>(L), s>>>(N, species, wz, nb, tb, sf, L, gs, slot_f,
-                                                                 atom_vir);
+  const size_t lds = bwd_lds<T, W, FG>(L);
+  const hipError_t e = allow_lds((const void *)k_desc_bwd<T, W, GNZ, Z16, FG>, lds);
+  if (e != hipSuccess) return e;
+  k_desc_bwd<T, W, GNZ, Z16, FG><<<N, 64 * W, lds, s>>>(N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
   return hipGetLastError();
 }
 
-template <typename T, int W>
+template <typename T, int W, bool FG>
 hipError_t bwd_grid(hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
                     const TripletBufs &tb, const SFDev &sf, Lds L, const GradSrc &gs, double *slot_f,
                     double *atom_vir) {
-  if (!sf.grid) return bwd_launch<T, W, 0>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+  if (!sf.grid) return bwd_launch<T, W, 0, false, FG>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
   switch (sf.nz) {
-    case 1: return bwd_launch<T, W, 1>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
-    case 2: return bwd_launch<T, W, 2>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
-    case 3: return bwd_launch<T, W, 3>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+    case 1: return bwd_launch<T, W, 1, false, FG>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+    case 2: return bwd_launch<T, W, 2, false, FG>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+    case 3: return bwd_launch<T, W, 3, false, FG>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
     case 4:
-      if (zeta_1_2_4_16(sf)) return bwd_launch<T, W, 4, true>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
-      return bwd_launch<T, W, 4>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
-    default: return bwd_launch<T, W, 5>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+      if (zeta_1_2_4_16(sf))
+        return bwd_launch<T, W, 4, true, FG>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+      return bwd_launch<T, W, 4, false, FG>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+    default: return bwd_launch<T, W, 5, false, FG>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
   }
 }
 
-template <typename T, int NZ, int W>
-size_t fwd_grid_lds(const Lds &L) {
-  constexpr int NV = GridShape<T, NZ>::NV;
-  return ((W + 1) & ~1) * sizeof(double) + (size_t)(6 * L.KP + L.G * L.KP + L.Dp + W * NV + NV) * sizeof(T);
-}
-
-template <typename T, int NZ, int W, bool Z16 = false>
+template <typename T, int NZ, int W, bool Z16, bool FG>
 hipError_t fwd_grid_launch(hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
                            const TripletBufs &tb, const SFDev &sf, Lds L, const double *theta_inv, const double *mu,
                            int split_mode, DescOut out) {
-  k_desc_fwd_grid<T, NZ, W, Z16><<<N, 64 * W, fwd_grid_lds<T, NZ, W>(L), s>>>(N, species, wz, nb, tb, sf, L,
-                                                                              theta_inv, mu, split_mode, out);
+  const size_t lds = fwd_grid_lds<T, NZ, W, FG>(L);
+  const hipError_t e = allow_lds((const void *)k_desc_fwd_grid<T, NZ, W, Z16, FG>, lds);
+  if (e != hipSuccess) return e;
+  k_desc_fwd_grid<T, NZ, W, Z16, FG><<<N, 64 * W, lds, s>>>(N, species, wz, nb, tb, sf, L, theta_inv, mu,
+                                                            split_mode, out);
   return hipGetLastError();
+}
+
+template <typename T, int NZ, bool Z>
+hipError_t fwd_grid_nz(int waves, hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
+                       const TripletBufs &tb, const SFDev &sf, Lds L, const double *theta_inv, const double *mu,
+                       int split_mode, DescOut out) {
+  int W;
+  bool fg;
+  auto bytes = [&](int w, bool g) {
+    if (w == 2) return fwd_grid_lds<T, NZ, 2, false>(L);
+    return g ? fwd_grid_lds<T, NZ, 1, true>(L) : fwd_grid_lds<T, NZ, 1, false>(L);
+  };
+  if (!pick_shape(waves, 2, bytes, W, fg) || (fg && !nb.fc_scratch)) return hipErrorInvalidValue;
+  if (fg) return fwd_grid_launch<T, NZ, 1, Z, true>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+  if (W == 2)
+    return fwd_grid_launch<T, NZ, 2, Z, false>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+  return fwd_grid_launch<T, NZ, 1, Z, false>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
 }
 
 template <typename T>
 hipError_t fwd_grid(int waves, hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
                     const TripletBufs &tb, const SFDev &sf, Lds L, const double *theta_inv, const double *mu,
                     int split_mode, DescOut out) {
-#define GAPX_FWD_GRID(NZ, Z)                                                                                    \
-  return (waves >= 2)                                                                                           \
-             ? fwd_grid_launch<T, NZ, 2, Z>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out) \
-             : fwd_grid_launch<T, NZ, 1, Z>(s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out)
   switch (sf.nz) {
-    case 1: GAPX_FWD_GRID(1, false);
-    case 2: GAPX_FWD_GRID(2, false);
-    case 3: GAPX_FWD_GRID(3, false);
+    case 1: return fwd_grid_nz<T, 1, false>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+    case 2: return fwd_grid_nz<T, 2, false>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+    case 3: return fwd_grid_nz<T, 3, false>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
     case 4:
-      if (zeta_1_2_4_16(sf)) GAPX_FWD_GRID(4, true);
-      GAPX_FWD_GRID(4, false);
-    default: GAPX_FWD_GRID(5, false);
+      if (zeta_1_2_4_16(sf))
+        return fwd_grid_nz<T, 4, true>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+      return fwd_grid_nz<T, 4, false>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
+    default: return fwd_grid_nz<T, 5, false>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
   }
-#undef GAPX_FWD_GRID
+}
+
+// Whether some descriptor kernel of this table needs its cutoff tables in
+// global memory at KP slots (its one-wave LDS layout exceeds the device's limit).
+template <typename T>
+bool needs_global_tables(const SFDev &sf, const Lds &L) {
+  const size_t lim = lds_limit();
+  if (bwd_lds<T, 1, false>(L) > lim) return true;
+  if (!sf.grid) return fwd_lds<T, 16, 1, false>(L) > lim;
+  return fwd_grid_lds<T, 5, 1, false>(L) > lim;  // the widest grid layout
+}
+
+// Whether every descriptor kernel of this table has a layout for KP slots.
+template <typename T>
+bool fits(const SFDev &sf, const Lds &L) {
+  const size_t lim = lds_limit();
+  if (bwd_lds<T, 1, true>(L) > lim) return false;
+  if (!sf.grid) return fwd_lds<T, 16, 1, true>(L) <= lim;
+  return fwd_grid_lds<T, 5, 1, true>(L) <= lim;
+}
+
+template <typename T>
+hipError_t bwd_fit(int waves, hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
+                   const TripletBufs &tb, const SFDev &sf, Lds L, const GradSrc &gs, double *slot_f,
+                   double *atom_vir) {
+  int W;
+  bool fg;
+  auto bytes = [&](int w, bool g) {
+    if (w == 4) return bwd_lds<T, 4, false>(L);
+    if (w == 2) return bwd_lds<T, 2, false>(L);
+    return g ? bwd_lds<T, 1, true>(L) : bwd_lds<T, 1, false>(L);
+  };
+  if (!pick_shape(waves, 4, bytes, W, fg) || (fg && !nb.fc_scratch)) return hipErrorInvalidValue;
+  if (fg) return bwd_grid<T, 1, true>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+  if (W == 4) return bwd_grid<T, 4, false>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+  if (W == 2) return bwd_grid<T, 2, false>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
+  return bwd_grid<T, 1, false>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
 }
 
 }  // namespace
 
-// Largest neighbour count the LDS layout allows (checked by the host).
-int max_neighbors_for_lds(bool fp64, int G, int Dp) {
-  const size_t budget = kLdsBudget - 64;
-  const size_t bytes = fp64 ? 8 : 4;
-  // the backward with one wave is the largest single-wave layout: (6 + 2G) KP + 2.5 Dp
-  // values of the compute type and 4 KP FP64 accumulators
-  long kp = (long)(budget - (2 * Dp + Dp / 2) * bytes) / (long)((6 + 2 * G) * bytes + 4 * sizeof(double));
-  return (int)(kp / 8 * 8);
+// Largest per-atom neighbour count the descriptor kernels take (every
+// kernel of the table then has a layout within the device's LDS limit, with
+// the cutoff tables in global memory where needed), capped at kMaxNeighbors.
+int max_neighbors_for_lds(bool fp64, const SFDev &sf, int Dp) {
+  const int G = sf.n_groups > 0 ? sf.n_groups : 1;
+  int kp = kMaxNeighbors;
+  for (; kp > 8; kp -= 8) {
+    const Lds L{kp, G, Dp, 0};
+    if (fp64 ? fits<double>(sf, L) : fits<float>(sf, L)) break;
+  }
+  return kp;
+}
+
+size_t desc_table_scratch_bytes(bool fp64, const SFDev &sf, int Dp, int KP, int N) {
+  const int G = sf.n_groups > 0 ? sf.n_groups : 1;
+  const Lds L{KP, G, Dp, 0};
+  if (!(fp64 ? needs_global_tables<double>(sf, L) : needs_global_tables<float>(sf, L))) return 0;
+  return (size_t)N * 2 * G * KP * (fp64 ? sizeof(double) : sizeof(float));
 }
 
 hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *species, const double *wz,
@@ -1098,18 +1244,6 @@ hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *sp
   }
   if (chunk <= 8) return fwd_waves<float, 8>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
   return fwd_waves<float, 16>(waves, s, N, species, wz, nb, tb, sf, L, theta_inv, mu, split_mode, out);
-}
-
-// The most waves per atom (up to `waves`) whose layout fits the LDS budget.
-template <typename T>
-hipError_t bwd_fit(int waves, hipStream_t s, int N, const int *species, const double *wz, const NeighborBufs &nb,
-                   const TripletBufs &tb, const SFDev &sf, Lds L, const GradSrc &gs, double *slot_f,
-                   double *atom_vir) {
-  if (waves >= 4 && bwd_lds<T, 4>(L) <= kLdsBudget)
-    return bwd_grid<T, 4>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
-  if (waves >= 2 && bwd_lds<T, 2>(L) <= kLdsBudget)
-    return bwd_grid<T, 2>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
-  return bwd_grid<T, 1>(s, N, species, wz, nb, tb, sf, L, gs, slot_f, atom_vir);
 }
 
 hipError_t launch_descriptors_bwd(hipStream_t s, bool fp64, int N, const int *species, const double *wz,

@@ -124,6 +124,10 @@ struct NeighborBufs {
   int *sinfo;           // [N*4] in bin order: atom index, its three cell shifts
   int *overflow;        // = total + 2
   int *total;           // [8] see launch_forces_reduce; [3] is the descriptor kernels' guard
+  // per-group cutoff tables of the descriptor kernels, [N][2 G kcap] values of
+  // the run's precision, for lists too long for the kernels' LDS layouts
+  // (desc_table_scratch_bytes); null otherwise
+  void *fc_scratch;
 };
 
 // Wrap positions into their cells, bin and count the atoms, and scan the
@@ -216,8 +220,14 @@ hipError_t launch_descriptors_bwd(hipStream_t s, bool fp64, int N, const int *sp
                                   const double *wz, const NeighborBufs &nb, const TripletBufs &tb,
                                   const SFDev &sf, int Dp, const GradSrc &gs,
                                   double *slot_f, double *atom_vir, int waves, int KP, int atom0 = 0);
-// Largest per-atom neighbour count the descriptor kernels' LDS layout holds.
-int max_neighbors_for_lds(bool fp64, int G, int Dp);
+// Largest per-atom neighbour count the descriptor kernels take: within the
+// device's per-workgroup LDS (160 KB on gfx950) with the cutoff tables moved
+// to global memory where needed, and at most kMaxNeighbors.
+constexpr int kMaxNeighbors = 4096;
+int max_neighbors_for_lds(bool fp64, const SFDev &sf, int Dp);
+// Bytes of NeighborBufs::fc_scratch the descriptor kernels need for N atoms
+// with KP slots each (0: their LDS layouts hold the cutoff tables).
+size_t desc_table_scratch_bytes(bool fp64, const SFDev &sf, int Dp, int KP, int N);
 
 // Work buffers of the reduction after the backward pass.  Per-structure sums
 // run in parts of at most kReduceAtoms atoms, one workgroup each; the last

@@ -51,4 +51,4 @@ def golden_cases(ref):
 
 
 GOLDEN_CASE_NAMES = ["b8c56", "bc24_triclinic", "bc6_small_cell", "bc6_small_cell_nograd", "t16", "t40",
-                     "t40_nograd", "t8_small_cell", "c64_single_species"]
+                     "t40_nograd", "t8_small_cell", "c64_single_species", "bc8_dense_920", "t8_dense_920"]

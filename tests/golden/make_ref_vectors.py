@@ -11,8 +11,9 @@ the reference is mounted.
 Cases cover what the shipped ase.traj does not: negative and multi-species
 weights (w_j w_k in angular terms), triclinic cells, cells smaller than the
 cutoff (more than one image per axis, self-images), lgrad = .false., SF
-type 3, per-line cutoffs below Rcut, and the 64-atom / D = 96 / M = 500
-single-species workload itself.
+type 3, per-line cutoffs below Rcut, the 64-atom / D = 96 / M = 500
+single-species workload itself, and two dense cells with about 920 neighbours
+per atom (the reference stops above 1000).
 
 FGET_BOND (get_bond.f90:4-117) is run on the same structures at two cutoffs.
 
@@ -97,6 +98,21 @@ def c64_single_species_model():
     return as_model(d)
 
 
+def dense_model(base, species_pool, seed):
+    """A base model's header (species weights and SF table) with theta, MM and
+    coeff fitted on the descriptors of dense cells (1 A^3 per atom, about 920
+    neighbours within 6 A), so that such cells have O(1) kernel values."""
+    header = dict(z=base.z, w=base.w, sf=(base.sf_type, base.sf_alpha, base.sf_cut,
+                                           np.ones(base.nsf) if base.sf_zeta is None else base.sf_zeta))
+    xs = []
+    for p in range(4):
+        s = synth.structure(8, 8 * 1.0, species_pool, np.random.default_rng(seed + p), cell="cubic", min_dist=0.9)
+        xs.append(_desc(header, s["species"], s["lat"], s["pos"], 6.0))
+    d = synth.make_model(base.z, base.w, header["sf"], np.concatenate(xs), 24, 0.1, np.random.default_rng(seed))
+    d["theta"] = d["theta"] * 2.0
+    return as_model(d)
+
+
 def run_ref(model, species, lat, pos, rcut, lgrad):
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "gap_parameters")
@@ -113,6 +129,8 @@ def main():
         "ternary": ternary_model(rng),
         "c64_single_species": c64_single_species_model(),
     }
+    models["example_dense"] = dense_model(models["example"], (5, 6), 900)
+    models["ternary_dense"] = dense_model(models["ternary"], (1, 8, 22), 910)
     cases = []
     sp, lat, pos = read_xyz_b8c56(REF_XYZ)
     cases.append(("example", "b8c56", sp, lat, pos, 6.0, True))
@@ -131,6 +149,12 @@ def main():
     cases.append(("ternary", "t8_small_cell", s["species"], s["lat"], s["pos"], 6.0, True))
     s = synth.workload_structure("c64_zeta1")
     cases.append(("c64_single_species", "c64_single_species", s["species"], s["lat"], s["pos"], 6.0, True))
+    # about 920 neighbours per atom within 6 A (8 atoms in a 2 A cube): near
+    # the reference's limit of 1000 (gap_calc.f90:68, 107-111)
+    s = synth.structure(8, 8 * 1.0, (5, 6), np.random.default_rng(16), cell="cubic", min_dist=0.9)
+    cases.append(("example_dense", "bc8_dense_920", s["species"], s["lat"], s["pos"], 6.0, True))
+    s = synth.structure(8, 8 * 1.0, (1, 8, 22), np.random.default_rng(17), cell="cubic", min_dist=0.9)
+    cases.append(("ternary_dense", "t8_dense_920", s["species"], s["lat"], s["pos"], 6.0, True))
 
     out = {}
     for name, m in models.items():

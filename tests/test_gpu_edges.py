@@ -11,8 +11,11 @@
   leaves the context empty: a run then reports "no structures loaded", and
   the next good load works as in a fresh context.
 * A rejected lattice update (singular cell) keeps the loaded cells.
-* More neighbours than the descriptor kernels' LDS layout holds is reported
-  as GAPX_ERR_NEIGHBORS, without the reference's 1000 cap as well.
+* Lists above the reference's 1000 (gap_calc.f90:107-111) stop FGAP_CALC's
+  flags like the reference and are evaluated whole without them; lists
+  longer than the descriptor kernels take (about 2000) are reported as
+  GAPX_ERR_NEIGHBORS.  Lists of about 920 (under the reference's limit) are
+  pinned by the reference's own vectors (tests/test_gpu_parity.py).
 * A cell too small for rcut / 48 is refused; lists above 256 slots (the
   triplet builder's other path) match the oracle.
 """
@@ -136,14 +139,57 @@ def test_rejected_lattice_update_keeps_cells(model):
     ctx.close()
 
 
-def test_lds_neighbour_limit_is_reported(model):
+def test_neighbour_limit_is_reported(model):
+    """About 4500 neighbours per atom (40 atoms in a 4 A cube, rcut 12): above
+    the list length the descriptor kernels take (about 2000 in FP64 for this
+    table, kept within the device's LDS with the cutoff tables in global
+    memory), so GAPX_ERR_NEIGHBORS in both precisions."""
     om, m = model
     lat = np.eye(3) * 4.0
     pos = np.random.default_rng(2).random((40, 3)) * 4.0
     for flags in (0, gapx.MIXED):
         with pytest.raises(gapx.GapxError) as e:
-            m.calc(np.ones(40, np.int32), lat, pos, 9.0, flags=flags)
+            m.calc(np.ones(40, np.int32), lat, pos, 12.0, flags=flags)
         assert e.value.code == 5
+
+
+def dense_cell(vpa, seed):
+    from gapx import synth
+    return synth.structure(8, 8 * vpa, (1, 8, 22), np.random.default_rng(seed), cell="cubic", min_dist=0.9)
+
+
+@pytest.mark.parametrize("mixed", [False, True], ids=["fp64", "mixed"])
+def test_above_the_reference_cap_without_it(mixed):
+    """About 1090 neighbours per atom (8 atoms at 0.83 A^3 each) with a model
+    fitted on the cell's own descriptors: FGAP_CALC's flags
+    (GAPX_REF_NEIGHBOR_CAP) stop above 1000 like the reference
+    (gap_calc.f90:107-111); without the flag the lists are taken whole and
+    match the oracle's sparse mode (FP64 bar, the stress to 1e-9 of its
+    4e4 GPa; mixed within 1 meV/atom and 20 meV/A of FP64)."""
+    s = dense_cell(0.83, 41)
+    om = fitted_small_model(np.random.default_rng(42), s)
+    m = gapx.Model.from_model(om)
+    ctx = gapx.Context(m)
+    ctx.load([s], 6.0)
+    ctx.run(gapx.GRAD | (gapx.MIXED if mixed else 0))
+    ctx.results()
+    pairs, _ = ctx.counts()
+    ctx.close()
+    assert pairs / 8 > 1000
+    with pytest.raises(gapx.GapxError) as e:
+        m.calc(s["species"], s["lat"], s["pos"], 6.0, flags=gapx.REF_NEIGHBOR_CAP | (gapx.MIXED if mixed else 0))
+    assert e.value.code == 5
+    e64, f64, s64, _ = m.calc(s["species"], s["lat"], s["pos"], 6.0)
+    if mixed:
+        r = m.calc(s["species"], s["lat"], s["pos"], 6.0, mixed=True)
+        assert abs(r[0] - e64) / 8 <= 1e-3 and np.abs(r[1] - f64).max() <= 2e-2
+    else:
+        ref = oracle.gap_calc(om, s["species"], s["lat"], s["pos"], 6.0, mode="sparse", nthreads=8)
+        print("dense 1090: E %.6f |F| %.4f S %.1f" % (e64, np.abs(f64).max(), s64[0]))
+        assert abs(e64 - ref["energy"]) / 8 <= TOL_E
+        assert np.abs(f64 - ref["forces"]).max() <= TOL_F
+        assert np.abs(s64 - ref["stress"]).max() <= max(TOL_S, 1e-9 * np.abs(ref["stress"]).max())
+    m.close()
 
 
 def test_too_many_periodic_images_is_an_argument_error(model):
