@@ -1,28 +1,34 @@
 """Benchmark: atom energy+force+stress evaluations per second on one MI355X.
 
-Workload (default): 4096-atom ternary structure (Z 1/8/22, weights = Z),
-triclinic cell at 0.07 atoms/A^3, rcut 6 A, 32 radial + 32 angular wACSF
-lines (D = 128), M = 4000 sparse points, synthetic seeded data and a
-synthetic model built from GPU descriptors of seeded pool structures.
+Headline workload (default, ``--workload population``): the batched form, 512
+uneven periodic structures of 8-256 atoms (10-20 A^3/atom, triclinic cells,
+1-4 species from {1, 6, 8, 22}, weights = Z), 32 radial + 32 angular wACSF
+lines (zeta in {1, 2, 4, 16}, D = 128), M = 4000 sparse points, rcut 6 A; a
+synthetic model built from GPU descriptors of a separately seeded population.
+All structures are one device-resident batch, autotuned and replayed from a
+captured HIP graph.  A step is one full E+F+S evaluation of every structure
+from positions already in HBM: neighbour lists, descriptors, contraction,
+backward pass, forces and stress.  ``value`` is the mixed-precision rate of
+the whole population; the FP64 rate, their ratio and the largest
+per-structure mixed errors against FP64 are in the same line.
 
-A step is one full E+F+S evaluation of the structure from positions already
-in HBM: neighbour list, descriptors, contraction, backward pass, forces and
-stress.  Prints one JSON line (see the fields below).
+Extra records in the same line: ``ternary32k`` (the 32768-atom cell, rcut
+6.5 A, D = 256, M = 8000) and ``ternary4k`` (the 4096-atom cell, D = 128,
+M = 4000), both precisions each; ``relax`` and ``md`` (batched FIRE
+relaxation and NVT MD of the population); ``surrogate`` (the MLP pre-screen,
+trained on GPU GAP labels); ``variance`` (predictive variance of the
+4096-atom cell).  Skip them with --no-large, --no-4k, --no-dynamics,
+--no-surrogate.  ``cpu_baseline`` is the reference FGAP_CALC compiled from its
+own sources (oracle/_ref) on the first structures of the population, one
+core, about --cpu-seconds of work (--no-cpu skips it).
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload ternary4k]
-
-The JSON line also carries "population": the batched form (512 uneven
-structures, 8-256 atoms, D = 128, M = 4000), autotuned and replayed from a
-HIP graph, in mixed precision and in FP64, with the per-structure mixed
-error against FP64 (skip with --no-population), and "variance": the
-predictive variance of the workload structure after an FP64 evaluation
-(rocBLAS FP64 GEMMs S = xt mt^T and U = K Q plus the row kernels), timed
-apart from the headline step.
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload population]
 """
 import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -58,7 +64,7 @@ BYTES_NEIGHBOR = 4 + 8 + 32 + 4                # per pair: index, key, (dx,dy,dz
 # the headline metric this record reports (value: the mixed-precision rate;
 # the FP64 rate, their ratio, the mixed errors and the roofline fraction
 # are in the same record)
-METRIC = "atom E+F+stress evals/sec, mixed vs FP64 GPU at \u22641 meV/atom; roofline frac"
+METRIC = "atom E+F+stress evals/sec, mixed vs FP64 GPU at ≤1 meV/atom; roofline frac"
 SINGLE_KERNEL_STAGES = ("descriptors_fwd", "gpr_contraction", "gpr_finish", "descriptors_bwd")
 
 
@@ -69,25 +75,44 @@ def zeta_1_2_4_16(header):
     return sorted(set(np.round(header["sf_zeta"][ang]).astype(int).tolist())) == [1, 2, 4, 16]
 
 
-def stage_kernel(stage, mixed, tune, z16, des_len):
+def contraction_grid(n_atoms, des_len, nsparse, mixed, tune):
+    """Threads of one contraction launch over n_atoms rows (the host's split
+    of the sparse points over grid.y for the tuned workgroup target)."""
+    dp = (des_len + 31) // 32 * 32
+    mp = (nsparse + 63) // 64 * 64
+    rows = tune["gpr_rows"] if (mixed and dp <= 128) else 64
+    tiles = (n_atoms + 127) // 128 * 128 // rows
+    chunk = 64 if dp <= 128 else 32
+    splits = min(max(1, -(-tune["gpr_blocks"] // tiles)), mp // chunk)
+    per = -(-(-(-mp // splits)) // chunk) * chunk
+    splits = -(-mp // per)
+    return tiles * splits * 256
+
+
+def stage_kernel(stage, mixed, tune, z16, des_len, n_atoms, nsparse, c64=False):
     """The one kernel of a single-kernel stage under the given launch shapes,
-    as rocprofv3 names it (grid-form tables with 4 zetas; z16: the zeta list
-    is {1, 2, 4, 16}).  The contraction is instantiated for the descriptor
-    length padded to 32, with 64-point sparse chunks up to 128 features and
-    32-point chunks above, and 128-row tiles only up to 128 features."""
+    as rocprofv3 names it, and its grid in threads for a one-piece launch over
+    n_atoms (grid-form tables with 4 zetas; z16: the zeta list is {1, 2, 4,
+    16}; c64: the mixed run contracts in FP64).  The contraction is
+    instantiated for the descriptor length padded to 32, with 64-point sparse
+    chunks up to 128 features and 32-point chunks above, and 128-row tiles
+    only up to 128 features."""
     t = "float" if mixed else "double"
     z = "true" if z16 else "false"
     if stage == "descriptors_fwd":
-        return "k_desc_fwd_grid<%s, 4, %d, %s>" % (t, min(tune["waves_fwd"], 2), z)
+        w = min(tune["waves_fwd"], 2)
+        return "k_desc_fwd_grid<%s, 4, %d, %s, false>" % (t, w, z), n_atoms * 64 * w
     if stage == "descriptors_bwd":
-        return "k_desc_bwd<%s, %d, 4, %s>" % (t, tune["waves_bwd"], z)
+        w = tune["waves_bwd"]
+        return "k_desc_bwd<%s, %d, 4, %s, false>" % (t, w, z), n_atoms * 64 * w
     if stage == "gpr_contraction":
         dp = (des_len + 31) // 32 * 32
-        if not mixed:
-            return "k_gpr_f64<%d>" % dp
+        grid = contraction_grid(n_atoms, des_len, nsparse, mixed and not c64, tune)
+        if not mixed or c64:
+            return "k_gpr_f64<%d>" % dp, grid
         r = tune["gpr_rows"] // 64 if dp <= 128 else 1
-        return "k_gpr_mixed<false, %d, %d, %d>" % (dp, 64 if dp <= 128 else 32, r)
-    return "k_gpr_finish"
+        return "k_gpr_mixed<false, %d, %d, %d>" % (dp, 64 if dp <= 128 else 32, r), grid
+    return "k_gpr_finish", n_atoms * 128
 
 
 def stage_rates(st, N, D, M, pairs, triplets, m, mixed):
@@ -95,7 +120,7 @@ def stage_rates(st, N, D, M, pairs, triplets, m, mixed):
     n_ang = int(np.sum((hdr["sf_type"] == 2) | (hdr["sf_type"] == 4)))
     n_rad = int(np.sum((hdr["sf_type"] == 1) | (hdr["sf_type"] == 3)))
     vpeak = PEAK["fp32"] if mixed else PEAK["fp64"]
-    mpeak = PEAK["fp16"] if mixed else PEAK["fp64"]
+    mpeak = PEAK["fp16"] if (mixed and not m.precision()["mixed_contraction_fp64"]) else PEAK["fp64"]
     work = {
         "neighbor_list": ("hbm", pairs * BYTES_NEIGHBOR * 2 + N * 64, 8000.0, "GB/s", 1e9),
         "descriptors_fwd": ("valu", pairs * n_rad * FLOPS_RAD_FWD + triplets * n_ang * FLOPS_ANG_FWD, vpeak,
@@ -115,22 +140,18 @@ def stage_rates(st, N, D, M, pairs, triplets, m, mixed):
     return out
 
 
-def measured_traffic(workload, kernel):
-    """HBM bytes per launch of a kernel from the committed counter summary
-    (profiles/traffic.json, two rocprofv3 passes scaled by a copy of known
-    size), or None when that summary does not cover this workload/kernel."""
+def measured_traffic(workload, kernel, grid):
+    """HBM bytes per launch of a kernel at a grid (threads) from the committed
+    counter summary (profiles/traffic.json: two rocprofv3 passes per workload,
+    scaled by a copy of known size), or None when it does not cover them."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("workload") != workload:
-        return None
-    for v in t["kernels"].values():
-        if v["kernel"] == kernel:
-            return v["traffic_bytes"]
-    return None
+    v = t.get("workloads", {}).get(workload, {}).get("kernels", {}).get("%s grid %d" % (kernel, grid))
+    return None if v is None else v["traffic_bytes"]
 
 
 def time_runs(ctx, flags, steps, warmup):
@@ -145,6 +166,8 @@ def time_runs(ctx, flags, steps, warmup):
 
 
 def stage_times(ctx, flags, steps):
+    """Per-stage milliseconds from HIP events on the context's stream (eager
+    one-piece launches: each single-kernel stage is one launch over all atoms)."""
     ctx.timing(True)
     for _ in range(steps):
         ctx.run(flags)
@@ -154,30 +177,66 @@ def stage_times(ctx, flags, steps):
     return {k: (v[0] / max(v[1], 1)) for k, v in st.items()}
 
 
-def population_bench(steps, warmup):
-    """Batched population in both precisions: autotuned, graph replay."""
-    t0 = time.perf_counter()
-    structs = synth.population(synth.POPULATION["n_struct"])
-    model = synth.population_model(gpu_descriptor_fn)
-    print("bench: population ready in %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
-    m = gapx.Model.from_model(model)
+def measure(m, structs, rc, steps, warmup, label):
+    """Both precisions on one loaded batch: autotuned launch shapes, graph
+    replay for the rate, eager per-stage events for the stage times.  Returns
+    the context (still loaded) and the record."""
     ctx = gapx.Context(m)
-    ctx.load(structs, synth.POPULATION["rc"])
+    ctx.load(structs, rc)
     natoms = np.array([len(s["species"]) for s in structs])
-    out, res = {}, {}
+    N = int(natoms.sum())
+    runs = {}
     for name, flags in (("fp64", gapx.GRAD), ("mixed", gapx.GRAD | gapx.MIXED)):
-        tune = ctx.autotune(flags)
-        ctx.use_graph(True)
+        tune = ctx.autotune(flags)  # launch shapes for this batch (cached per shape class)
+        ctx.use_graph(True)  # replay the captured launch sequence
         sec = time_runs(ctx, flags, steps, warmup)
-        ctx.use_graph(False)
+        ctx.use_graph(False)  # per-stage events need eager launches
         st = stage_times(ctx, flags, max(3, steps // 4))
         ctx.use_graph(True)
         ctx.run(flags)
-        res[name] = ctx.results()
-        out[name] = dict(value=round(float(natoms.sum()) / sec, 1), ms_per_step=round(sec * 1e3, 4), tuning=tune,
-                         stages_ms={k: round(v, 4) for k, v in st.items()})
-        print("bench: population %s %.3f ms/step tuning %s" % (name, sec * 1e3, tune), file=sys.stderr, flush=True)
-    # batched relaxation of the whole population on the device (mixed, graph replay)
+        runs[name] = dict(sec=sec, stages=st, res=ctx.results(), tune=tune)
+        print("bench: %s %s %.3f ms/step tuning %s stages %s" % (label, name, sec * 1e3, tune,
+                                                                 {k: round(v, 4) for k, v in st.items()}),
+              file=sys.stderr, flush=True)
+    pairs, triplets = ctx.counts()
+    r64, rmx = runs["fp64"]["res"], runs["mixed"]["res"]
+    offs = np.concatenate([[0], np.cumsum(natoms)])
+    dF = [float(np.abs(rmx["forces"][offs[k]:offs[k + 1]] - r64["forces"][offs[k]:offs[k + 1]]).max())
+          for k in range(len(structs))]
+    err = dict(energy_meV_per_atom=float(np.abs((rmx["energy"] - r64["energy"]) / natoms).max()) * 1e3,
+               force_meV_per_A=max(dF) * 1e3,
+               stress_GPa=float(np.abs(rmx["stress"] - r64["stress"]).max()))
+    D, M = m.des_len, m.nsparse
+    c64 = m.precision()["mixed_contraction_fp64"]
+    st = runs["mixed"]["stages"]
+    rates = stage_rates(st, N, D, M, pairs, triplets, m, mixed=True)
+    # the dominant kernel: the longest of the stages that are one kernel each
+    dom = max(SINGLE_KERNEL_STAGES, key=lambda k: st[k])
+    kname, grid = stage_kernel(dom, True, runs["mixed"]["tune"], zeta_1_2_4_16(m.header()), D, N, M, c64)
+    roof = {kk: rates[dom][kk] for kk in ("bound", "achieved", "peak", "unit", "frac")}
+    roof["traffic"] = measured_traffic(label, kname, grid)
+    roof.update(kernel=kname, grid=grid, stage=dom, ms_per_launch=round(st[dom], 4), work_per_launch=rates[dom]["work"])
+    roof["stages"] = {k: dict(ms=round(st[k], 4), **{kk: vv for kk, vv in rates[k].items()
+                                                     if kk in ("achieved", "frac", "bound", "unit")}) for k in st}
+    sec = runs["mixed"]["sec"]
+    rec = dict(
+        value=round(N / sec, 1), ms_per_step=round(sec * 1e3, 4),
+        config=dict(workload=label, structures=len(structs), atoms=N, des_len=D, nsparse=M, rcut=rc,
+                    pairs=pairs, triplets=triplets, mixed_contraction_fp64=bool(c64),
+                    tuning={k: runs[k]["tune"] for k in runs}),
+        fp64=dict(value=round(N / runs["fp64"]["sec"], 1), ms_per_step=round(runs["fp64"]["sec"] * 1e3, 4),
+                  stages_ms={k: round(v, 4) for k, v in runs["fp64"]["stages"].items()}),
+        mixed_stages_ms={k: round(v, 4) for k, v in st.items()},
+        mixed_over_fp64=round(runs["fp64"]["sec"] / sec, 3),
+        mixed_error_vs_fp64_max={k: float("%.4g" % v) for k, v in err.items()},
+        roofline=roof)
+    return ctx, rec
+
+
+def dynamics_bench(ctx, structs, rc):
+    """Batched FIRE relaxation and NVT MD of the loaded population (mixed
+    precision, graph replay)."""
+    natoms = np.array([len(s["species"]) for s in structs])
     ctx.use_graph(True)
     t0 = time.perf_counter()
     rel = ctx.relax(gapx.GRAD | gapx.MIXED, max_steps=200, fmax=0.05, maxstep=0.2)
@@ -185,10 +244,9 @@ def population_bench(steps, warmup):
     relax = dict(steps=rel["steps"], converged=rel["converged"], seconds=round(t_rel, 3),
                  structure_steps_per_s=round(len(structs) * rel["steps"] / t_rel, 1), fmax=0.05, max_steps=200)
     print("bench: population relax %s" % relax, file=sys.stderr, flush=True)
-    # batched NVT MD of the whole population (NVTBerendsen as the reference's
-    # MD driver configures it: 1 fs, 300 K, taut 500 fs), from the starting
-    # positions with Maxwell-Boltzmann momenta; mixed precision, graph replay
-    ctx.load(structs, synth.POPULATION["rc"])
+    # NVTBerendsen as the reference's MD driver configures it (1 fs, 300 K,
+    # taut 500 fs), from the starting positions with Maxwell-Boltzmann momenta
+    ctx.load(structs, rc)
     mass_of = {1: 1.008, 6: 12.011, 8: 15.999, 22: 47.867}
     masses = np.concatenate([[mass_of[int(z)] for z in s["species"]] for s in structs])
     p0 = np.random.default_rng(3).normal(size=(len(masses), 3)) * np.sqrt(masses * 300.0 * gapx.KB)[:, None]
@@ -197,19 +255,10 @@ def population_bench(steps, warmup):
     log = ctx.md(masses, md_steps, dt_fs=1.0, temperature=300.0, taut_fs=500.0, momenta=p0,
                  flags=gapx.GRAD | gapx.MIXED)
     t_md = time.perf_counter() - t0
-    md = dict(steps=md_steps, seconds=round(t_md, 3), atom_steps_per_s=round(natoms.sum() * md_steps / t_md, 1),
-              ms_per_step=round(t_md / md_steps * 1e3, 4), mean_T_end=round(float(log[-1, :, 2].mean()), 1),
-              note="includes the initial force evaluation and one device sync per step (list capacity check)")
+    md = dict(steps=md_steps, seconds=round(t_md, 3), atom_steps_per_s=round(float(natoms.sum()) * md_steps / t_md, 1),
+              ms_per_step=round(t_md / md_steps * 1e3, 4), mean_T_end=round(float(log[-1, :, 2].mean()), 1))
     print("bench: population md %s" % md, file=sys.stderr, flush=True)
-    ctx.close()
-    m.close()
-    r64, rmx = res["fp64"], res["mixed"]
-    err = dict(energy_meV_per_atom=float(np.abs((rmx["energy"] - r64["energy"]) / natoms).max()) * 1e3,
-               force_meV_per_A=float(np.abs(rmx["forces"] - r64["forces"]).max()) * 1e3,
-               stress_GPa=float(np.abs(rmx["stress"] - r64["stress"]).max()))
-    return dict(structures=len(structs), atoms=int(natoms.sum()), unit="atom E+F+S evals/s", **out, relax=relax, md=md,
-                mixed_over_fp64=round(out["mixed"]["value"] / out["fp64"]["value"], 3),
-                mixed_error_vs_fp64_max={k: float("%.4g" % v) for k, v in err.items()})
+    return relax, md
 
 
 def surrogate_bench(m, n_train, n_cand, steps, batch=4096):
@@ -297,7 +346,7 @@ def variance_bench(m, ctx, N, reps=10):
     m.set_qmm(0.5 * (A + A.T))
     ctx.run(gapx.GRAD)
     ctx.results()
-    ctx.variance()  # rocBLAS handle and workspace
+    ctx.variance()  # workspace
     t = time.perf_counter()
     for _ in range(reps):
         ctx.variance()
@@ -307,38 +356,70 @@ def variance_bench(m, ctx, N, reps=10):
     flops = 2.0 * N * Mp * Dp + 2.0 * N * Mp * Mp
     return dict(ms=round(sec * 1e3, 4), atoms_per_s=round(N / sec, 1), gemm_tflops=round(flops / sec / 1e12, 2),
                 frac_fp64_peak=round(flops / sec / 1e12 / PEAK["fp64"], 3),
-                note="wall clock per gapx_ctx_variance call (enqueue, GEMMs and row kernels, sync, copy-back); "
+                note="wall clock per gapx_ctx_variance call (enqueue, kernels, sync, copy-back); "
                      "flops = 2 N Mp Dp + 2 N Mp^2")
 
 
-def cpu_baseline(model, s, rc, seconds):
-    """The CPU oracle (sparse FP64 path, OpenMP over centre atoms) on a
-    bounded sample: the first n centre atoms of the same structure."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    om = oracle.Model(z=model["z"], w=model["w"], sf_type=model["sf_type"], sf_alpha=model["sf_alpha"],
-                      sf_cut=model["sf_cut"], theta=model["theta"], mm=model["mm"], coeff=model["coeff"],
-                      sf_zeta=model["sf_zeta"])
+    return oracle
+
+
+def _oracle_model(oracle, model):
+    return oracle.Model(z=model["z"], w=model["w"], sf_type=model["sf_type"], sf_alpha=model["sf_alpha"],
+                        sf_cut=model["sf_cut"], theta=model["theta"], mm=model["mm"], coeff=model["coeff"],
+                        sf_zeta=model["sf_zeta"])
+
+
+def cpu_baseline(model, structs, rc, seconds):
+    """The reference FGAP_CALC compiled from its own sources (oracle/_ref;
+    one core: its OpenMP directives are commented out) on the first
+    structures of the same population until about `seconds` of work, the
+    model written as ./gap_parameters in a temporary folder (the reference
+    reads its header there); beside it the oracle's sparse FP64 path on the
+    same sample with all host cores (up to 16).  Without the compiled
+    reference the oracle's rate is the baseline (kind "port")."""
+    oracle = _oracle()
+    om = _oracle_model(oracle, model)
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     cores = max(1, min(cores, 16))
-    n = 64
-    t = 0.0
-    while True:
+    ref = None
+    k, n_ref, t_ref = 0, 0, 0.0
+    if oracle.ref_available():
+        with tempfile.TemporaryDirectory() as tmp:
+            path = os.path.join(tmp, "gap_parameters")
+            oracle.write_model(path, om)
+            while k < len(structs) and t_ref < seconds:
+                s = structs[k]
+                t0 = time.perf_counter()
+                oracle.ref_gap_calc(path, s["species"], s["lat"], s["pos"], om.theta, om.mm, om.coeff, rc, True)
+                t_ref += time.perf_counter() - t0
+                n_ref += len(s["species"])
+                k += 1
+        ref = dict(value=n_ref / t_ref, structures=k, atoms=n_ref, seconds=round(t_ref, 2))
+    # the oracle (sparse mode, OpenMP over centre atoms) on the same sample,
+    # or on the first structures until about `seconds` without the reference
+    kk, t_port, n_port = 0, 0.0, 0
+    while kk < len(structs) and (kk < k if ref else t_port < seconds):
+        s = structs[kk]
         t0 = time.perf_counter()
-        oracle.gap_calc_range(om, s["species"], s["lat"], s["pos"], rc, 0, n, nthreads=cores)
-        t = time.perf_counter() - t0
-        if t > seconds / 4 or n >= len(s["species"]):
-            break
-        n = min(len(s["species"]), int(n * max(2.0, seconds / 4 / max(t, 1e-3))))
-    # the same oracle on one thread, over the first 256 centre atoms
-    n1 = min(256, len(s["species"]))
-    t0 = time.perf_counter()
-    oracle.gap_calc_range(om, s["species"], s["lat"], s["pos"], rc, 0, n1, nthreads=1)
-    t1 = time.perf_counter() - t0
-    return dict(value=n / t, unit="atom E+F+S evals/s", cores=cores, kind="port",
-                sample="first %d of %d centre atoms of the same structure, %.1f s" % (n, len(s["species"]), t),
-                value_1_thread=round(n1 / t1, 1),
-                sample_1_thread="first %d centre atoms on one thread, %.2f s" % (n1, t1))
+        oracle.gap_calc(om, s["species"], s["lat"], s["pos"], rc, mode="sparse", nthreads=cores)
+        t_port += time.perf_counter() - t0
+        n_port += len(s["species"])
+        kk += 1
+    port = dict(value=n_port / t_port, cores=cores, structures=kk, atoms=n_port, seconds=round(t_port, 2))
+    if ref:
+        return dict(value=round(ref["value"], 1), unit="atom E+F+S evals/s", cores=1, kind="reference",
+                    sample="FGAP_CALC (gappy/libgap/gap_calc.f90, wacsf.f90; amdflang -O3) on the first %d of %d "
+                           "population structures (%d atoms), %.1f s; it reads zeta = 1 for every angular line "
+                           "(the zeta column is ignored by its list-directed read), the same number of lines and "
+                           "so the same cost" % (k, len(structs), n_ref, t_ref),
+                    port=dict(value=round(port["value"], 1), cores=cores, kind="port",
+                              sample="the oracle's sparse FP64 path on the same structures, %.2f s" % t_port))
+    return dict(value=round(port["value"], 1), unit="atom E+F+S evals/s", cores=cores, kind="port",
+                sample="the oracle's sparse FP64 path on the first %d population structures (%d atoms), %.1f s"
+                       % (kk, n_port, t_port))
 
 
 def device_copy(nbytes):
@@ -360,15 +441,23 @@ def device_copy(nbytes):
         hip.hipFree(b)
 
 
+def load_workload(name):
+    """(model dict, structures, rcut) of a workload; its model is built from
+    GPU descriptors of separately seeded structures."""
+    if name == "population":
+        return synth.population_model(gpu_descriptor_fn), synth.population(synth.POPULATION["n_struct"]), \
+            synth.POPULATION["rc"]
+    return synth.workload_model(name, gpu_descriptor_fn), [synth.workload_structure(name)], synth.WORKLOADS[name]["rc"]
+
+
 def probe(args):
     """Copy of known size, then evaluations (for rocprofv3 counter passes)."""
     device_copy(1 << 30)
     print("probe: copied %d bytes" % (1 << 30), flush=True)
-    model = synth.workload_model(args.workload, gpu_descriptor_fn)
-    s = synth.workload_structure(args.workload)
+    model, structs, rc = load_workload(args.workload)
     m = gapx.Model.from_model(model)
     ctx = gapx.Context(m)
-    ctx.load([s], synth.WORKLOADS[args.workload]["rc"])
+    ctx.load(structs, rc)
     for flags, name in ((gapx.GRAD, "fp64"), (gapx.GRAD | gapx.MIXED, "mixed")):
         # every launch shape the autotuner can choose (one knob at a time from
         # the defaults, as it searches), `--probe` eager evaluations each; no
@@ -396,10 +485,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ternary4k")
+    ap.add_argument("--workload", default="population", choices=["population", "ternary4k", "ternary32k", "c64"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-population", action="store_true")
+    ap.add_argument("--no-large", action="store_true", help="skip the 32768-atom record")
+    ap.add_argument("--no-4k", action="store_true", help="skip the 4096-atom record (and variance, surrogate)")
+    ap.add_argument("--no-dynamics", action="store_true", help="skip the population's relaxation and MD")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--surrogate-train", type=int, default=20000,
                     help="training structures (the surrogate form names 200000)")
@@ -414,105 +505,73 @@ def main():
     if gapx.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
-
-    w = synth.WORKLOADS[args.workload]
-    rc = w["rc"]
     if args.probe:
         return probe(args)
     t0 = time.perf_counter()
     # measured vector-FMA throughput (register-only microkernels)
     ceil = {k: round(v, 1) for k, v in gapx.probe_fma_rates().items()}
-    model = synth.workload_model(args.workload, gpu_descriptor_fn)
-    s = synth.workload_structure(args.workload)
+    model, structs, rc = load_workload(args.workload)
     print("bench: workload %s ready in %.1f s; measured FMA rates %s TFLOP/s" % (
         args.workload, time.perf_counter() - t0, ceil), file=sys.stderr, flush=True)
     m = gapx.Model.from_model(model)
-    ctx = gapx.Context(m)
-    ctx.load([s], rc)
-    N = len(s["species"])
-    D, M = m.des_len, m.nsparse
-
-    runs = {}
-    tuning = {}
-    for name, flags in (("fp64", gapx.GRAD), ("mixed", gapx.GRAD | gapx.MIXED)):
-        tuning[name] = ctx.autotune(flags)  # launch shapes for this workload (cached per shape)
-        ctx.use_graph(True)  # replay the captured launch sequence
-        sec = time_runs(ctx, flags, args.steps, args.warmup)
-        ctx.use_graph(False)  # per-stage events need eager launches
-        st = stage_times(ctx, flags, max(3, args.steps // 4))
-        ctx.run(flags)
-        res = ctx.results()
-        runs[name] = dict(sec=sec, stages=st, res=res)
-        print("bench: %s %.3f ms/step; stages %s" % (name, sec * 1e3, {k: round(v, 4) for k, v in st.items()}),
-              file=sys.stderr, flush=True)
-    pairs, triplets = ctx.counts()
-    r64, rmx = runs["fp64"]["res"], runs["mixed"]["res"]
-    err = dict(energy_meV_per_atom=abs(rmx["energy"][0] - r64["energy"][0]) / N * 1e3,
-               force_meV_per_A=float(np.abs(rmx["forces"] - r64["forces"]).max()) * 1e3,
-               stress_GPa=float(np.abs(rmx["stress"] - r64["stress"]).max()))
-
-    sec = runs["mixed"]["sec"]
-    value = N / sec
-    st = runs["mixed"]["stages"]
-    rates = stage_rates(st, N, D, M, pairs, triplets, m, mixed=True)
-    # the dominant kernel: the longest of the stages that are one kernel each
-    dom = max(SINGLE_KERNEL_STAGES, key=lambda k: st[k])
-    kname = stage_kernel(dom, True, tuning["mixed"], zeta_1_2_4_16(m.header()), D)
-    roof = {kk: rates[dom][kk] for kk in ("bound", "achieved", "peak", "unit", "frac")}
-    roof["traffic"] = measured_traffic(args.workload, kname)
-    roof["kernel"] = kname
-    roof["stage"] = dom
-    roof["ms_per_launch"] = round(st[dom], 4)
-    roof["work_per_launch"] = rates[dom]["work"]
-    roof["stages"] = {k: dict(ms=round(st[k], 4), **{kk: vv for kk, vv in rates[k].items()
-                                                     if kk in ("achieved", "frac", "bound", "unit")}) for k in st}
-    var = variance_bench(m, ctx, N)
-    print("bench: variance %s" % var, file=sys.stderr, flush=True)
-    # the FP64 contraction beside what a plain FP64 library GEMM sustains on
-    # the same part in the same run (the variance's K Q, also MFMA FP64)
-    roof["fp64_contraction"] = dict(
-        tflops=round(4.0 * D * M * N / (runs["fp64"]["stages"]["gpr_contraction"] * 1e-3) / 1e12, 1),
-        rocblas_dgemm_tflops=var["gemm_tflops"])
+    ctx, head = measure(m, structs, rc, args.steps, args.warmup, args.workload)
+    relax = md = None
+    if args.workload == "population" and not args.no_dynamics:
+        relax, md = dynamics_bench(ctx, structs, rc)
     ctx.close()
-    surr = None
-    if not args.no_surrogate and args.workload == "ternary4k":
-        surr = surrogate_bench(m, args.surrogate_train, synth.SURROGATE["candidates"], args.surrogate_steps)
-        print("bench: surrogate %s" % surr, file=sys.stderr, flush=True)
     m.close()
-    popr = None
-    if not args.no_population:
-        popr = population_bench(args.steps, args.warmup)
+
+    extra = {}
+    for name, skip in (("ternary32k", args.no_large), ("ternary4k", args.no_4k)):
+        if skip or name == args.workload:
+            continue
+        t0 = time.perf_counter()
+        emodel, estructs, erc = load_workload(name)
+        print("bench: workload %s ready in %.1f s" % (name, time.perf_counter() - t0), file=sys.stderr, flush=True)
+        em = gapx.Model.from_model(emodel)
+        ectx, rec = measure(em, estructs, erc, args.steps, args.warmup, name)
+        if name == "ternary4k":
+            rec["variance"] = variance_bench(em, ectx, rec["config"]["atoms"])
+            print("bench: variance %s" % rec["variance"], file=sys.stderr, flush=True)
+        ectx.close()
+        if name == "ternary4k" and not args.no_surrogate:
+            extra["surrogate"] = surrogate_bench(em, args.surrogate_train, synth.SURROGATE["candidates"],
+                                                 args.surrogate_steps)
+            print("bench: surrogate %s" % extra["surrogate"], file=sys.stderr, flush=True)
+        em.close()
+        extra[name] = rec
+
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(model, s, rc, args.cpu_seconds)
+        cpu = cpu_baseline(model, structs, rc, args.cpu_seconds)
+        print("bench: cpu baseline %s" % cpu, file=sys.stderr, flush=True)
     out = {
         "metric": METRIC,
-        "value": round(value, 1),
+        "value": head["value"],
         "unit": "atom E+F+S evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(sec * 1e3, 4),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 descriptors + f16 hi/lo MFMA (f32 accumulate) + f64 norms/reductions",
         "data": "synthetic (seeded structures and model)",
-        "config": {"workload": args.workload, "atoms": N, "des_len": D, "nsparse": M, "rcut": rc,
-                   "precision": "mixed", "pairs": pairs, "triplets": triplets, "tuning": tuning},
-        "fp64": {"value": round(N / runs["fp64"]["sec"], 1), "ms_per_step": round(runs["fp64"]["sec"] * 1e3, 4),
-                 "stages_ms": {k: round(v, 4) for k, v in runs["fp64"]["stages"].items()}},
-        "mixed_over_fp64": round(runs["fp64"]["sec"] / sec, 3),
-        "mixed_error_vs_fp64": {k: float("%.4g" % v) for k, v in err.items()},
-        "roofline": roof,
+        "config": dict(head["config"], precision="mixed"),
+        "fp64": head["fp64"],
+        "mixed_over_fp64": head["mixed_over_fp64"],
+        "mixed_error_vs_fp64": head["mixed_error_vs_fp64_max"],
+        "mixed_stages_ms": head["mixed_stages_ms"],
+        "roofline": head["roofline"],
         "cpu_baseline": cpu,
-        "population": popr,
-        "surrogate": surr,
-        "variance": var,
+        "relax": relax,
+        "md": md,
         "fma_rates_tflops": dict(ceil, note="measured by gapx_probe_fma_rates on this box: register-only FMA "
                                             "microkernels, FP32 without packed instructions as the kernels "
                                             "are compiled; the roofline peaks stay the published ones"),
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

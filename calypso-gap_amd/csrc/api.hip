@@ -1352,6 +1352,147 @@ static int ctx_variance(Ctx &c, double *var, double *sigma2) {
   return GAPX_OK;
 }
 
+// CAR2ACSF from a caller's neighbour table (wacsf.f90:2-796): the table goes
+// into the neighbour slots (image offset from the centre, the table's
+// distance, weight and index), the triplet lists are built from the slots,
+// the FP64 forward gives xx and, with lgrad, one backward pass per feature
+// column f with dE/dx = the unit vector of column f gives the per-slot
+// gradients d xx(f, i) / d pos(j) and the per-atom virials, from which dxdy
+// and strs are assembled on the host in slot order.  The context holds the
+// table afterwards, not a loaded batch.
+static int ctx_car2acsf(Ctx &c, int na, int max_nb, int nf, const double *pos, const double *neighbor,
+                        const int *count, double *xx, double *dxdy, double *strs, bool lgrad) {
+  Model &m = *c.m;
+  if (na <= 0 || max_nb < 0 || !pos || (!neighbor && max_nb > 0) || !count || !xx)
+    return fail(GAPX_ERR_ARG, "gapx_car2acsf: invalid argument");
+  if (nf != m.D) return fail(GAPX_ERR_ARG, "gapx_car2acsf: nf must be des_len = 2 nsf of the model header");
+  int kmax = 0;
+  for (int i = 0; i < na; ++i) {
+    if (count[i] < 0 || count[i] > max_nb)
+      return fail(GAPX_ERR_ARG, "gapx_car2acsf: neighbour count outside [0, max_neighbor]");
+    kmax = std::max(kmax, count[i]);
+  }
+  const int cap = max_neighbors_for_lds(true, m.sfd, m.Dp);
+  if (kmax > cap) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "gapx_car2acsf: an atom has %d neighbours, above the limit of %d", kmax, cap);
+    return fail(GAPX_ERR_NEIGHBORS, buf);
+  }
+  GAPX_HIP(hipSetDevice(c.device));
+  c.N = 0;
+  c.nstruct = 0;
+  c.ran = false;
+  c.md_ready = false;
+  c.invalidate_graph();
+  const int N = na, Dp = m.Dp, kcap = round_up(std::max(8, kmax), 8);
+  const size_t slots = (size_t)N * kcap;
+  std::vector<int> hj(slots, 0);
+  std::vector<double> hd(4 * slots, 0.0), hw(slots, 0.0);
+  // NEIGHBOR(i, a, col), Fortran order; columns: image x y z, distance,
+  // weight, neighbour index as a real (gap_calc.f90:113-116)
+  auto tab = [&](int i, int a, int col) { return neighbor[i + (size_t)na * (a + (size_t)max_nb * col)]; };
+  for (int i = 0; i < N; ++i)
+    for (int a = 0; a < count[i]; ++a) {
+      const size_t sl = (size_t)i * kcap + a;
+      for (int x = 0; x < 3; ++x) hd[4 * sl + x] = tab(i, a, x) - pos[i + (size_t)na * x];
+      hd[4 * sl + 3] = tab(i, a, 3);
+      hw[sl] = tab(i, a, 4);
+      const double jd = tab(i, a, 5);
+      if (!(jd >= 1.0 && jd < na + 1.0)) return fail(GAPX_ERR_ARG, "gapx_car2acsf: neighbour index outside [1, NA]");
+      if (!(hd[4 * sl + 3] > 0.0)) return fail(GAPX_ERR_ARG, "gapx_car2acsf: neighbour distance must be positive");
+      hj[sl] = (int)jd - 1;
+    }
+  NeighborBufs nb{};
+  TripletBufs tb{};
+  DescOut o{};
+  int *stats;
+  double *nbw;
+  int rc;
+  if ((rc = c.get("nb_count", N, &nb.nb_count)) || (rc = c.get("nb_j", slots, &nb.nb_j)) ||
+      (rc = c.get("nb_d", 4 * slots, &nb.nb_d)) || (rc = c.get("nb_w", slots, &nbw)) ||
+      (rc = c.get("stats", 8, &stats)) || (rc = c.get("x64", (size_t)N * Dp, &o.x64)) ||
+      (rc = c.get("xt64", (size_t)N * Dp, &o.xt64)) || (rc = c.get("xn", N, &o.xn)))
+    return rc;
+  nb.nb_w = nbw;
+  nb.kcap = kcap;
+  nb.total = stats;
+  nb.overflow = stats + 2;
+  const size_t tbytes = desc_table_scratch_bytes(true, m.sfd, Dp, kcap, N);
+  if (tbytes) {
+    unsigned char *t;
+    if ((rc = c.get("fc_scratch", tbytes, &t))) return rc;
+    nb.fc_scratch = t;
+  }
+  GAPX_HIP(hipMemcpyAsync(nb.nb_count, count, sizeof(int) * N, hipMemcpyHostToDevice, c.stream));
+  GAPX_HIP(hipMemcpyAsync(nb.nb_j, hj.data(), sizeof(int) * slots, hipMemcpyHostToDevice, c.stream));
+  GAPX_HIP(hipMemcpyAsync(nb.nb_d, hd.data(), sizeof(double) * 4 * slots, hipMemcpyHostToDevice, c.stream));
+  GAPX_HIP(hipMemcpyAsync(nbw, hw.data(), sizeof(double) * slots, hipMemcpyHostToDevice, c.stream));
+  GAPX_HIP(launch_fill_int(c.stream, stats, 8, 0));
+  // triplet lists: at most K (K - 1) / 2 pairs per atom, so no capacity retry
+  tb.tcap = round_up(std::max(64, kmax * (kmax - 1) / 2), 64);
+  if ((rc = c.get("trip_list", (size_t)N * tb.tcap, &tb.list)) || (rc = c.get("trip_count", N, &tb.count)) ||
+      (rc = c.get("trip_raw", N, &tb.raw)))
+    return rc;
+  if (m.sf.n_ang > 0)
+    GAPX_HIP(launch_triplets_from_slots(c.stream, N, nb, m.sf.rc_max_ang, tb));
+  else
+    GAPX_HIP(launch_fill_int(c.stream, tb.count, N, 0));
+  o.N = N;
+  o.Dp = Dp;
+  GAPX_HIP(launch_descriptors_fwd(c.stream, true, N, nullptr, m.d_wz, nb, tb, m.sfd, m.d_theta_inv, m.d_mu, 0, o,
+                                  c.chunk > 0 ? c.chunk : 16, c.tune[0].waves_fwd, kcap));
+  std::vector<double> hx((size_t)N * Dp);
+  GAPX_HIP(hipMemcpyAsync(hx.data(), o.x64, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, c.stream));
+  GAPX_HIP(hipStreamSynchronize(c.stream));
+  for (int i = 0; i < N; ++i)
+    for (int f = 0; f < nf; ++f) xx[f + (size_t)nf * i] = hx[(size_t)i * Dp + f];
+  if (dxdy) std::fill(dxdy, dxdy + (size_t)nf * na * na * 3, 0.0);
+  if (strs) std::fill(strs, strs + (size_t)9 * nf * na, 0.0);
+  if (lgrad && (dxdy || strs)) {
+    double *g64, *slot_f, *vir;
+    if ((rc = c.get("g64", (size_t)N * Dp, &g64)) || (rc = c.get("slot_f", 3 * slots, &slot_f)) ||
+        (rc = c.get("atom_vir", 6 * (size_t)N, &vir)))
+      return rc;
+    GradSrc gs;
+    gs.g64 = g64;
+    std::vector<double> hg((size_t)N * Dp, 0.0), hs(3 * slots), hv(6 * (size_t)N);
+    for (int f = 0; f < nf; ++f) {
+      for (int i = 0; i < N; ++i) {
+        if (f > 0) hg[(size_t)i * Dp + f - 1] = 0.0;
+        hg[(size_t)i * Dp + f] = 1.0;
+      }
+      GAPX_HIP(hipMemcpyAsync(g64, hg.data(), sizeof(double) * hg.size(), hipMemcpyHostToDevice, c.stream));
+      GAPX_HIP(launch_descriptors_bwd(c.stream, true, N, nullptr, m.d_wz, nb, tb, m.sfd, Dp, gs, slot_f, vir,
+                                      c.tune[0].waves_bwd, kcap));
+      GAPX_HIP(hipMemcpyAsync(hs.data(), slot_f, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, c.stream));
+      GAPX_HIP(hipMemcpyAsync(hv.data(), vir, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, c.stream));
+      GAPX_HIP(hipStreamSynchronize(c.stream));
+      if (dxdy)  // d xx(f, i) / d pos(n, x): + the slot's gradient for its atom n, - it for the centre i
+        for (int i = 0; i < N; ++i)
+          for (int a = 0; a < count[i]; ++a) {
+            const size_t sl = (size_t)i * kcap + a;
+            const size_t n = (size_t)hj[sl];
+            for (int x = 0; x < 3; ++x) {
+              const double v = hs[3 * sl + x];
+              dxdy[f + (size_t)nf * (i + (size_t)na * (n + (size_t)na * x))] += v;
+              dxdy[f + (size_t)nf * (i + (size_t)na * (i + (size_t)na * x))] -= v;
+            }
+          }
+      if (strs)  // strs(a, c, f, i) = sum over slots of delta_a d xx / d pos(j, c); symmetric
+        for (int i = 0; i < N; ++i) {
+          const double *v = &hv[6 * (size_t)i];
+          const double S[3][3] = {{v[0], v[3], v[5]}, {v[3], v[1], v[4]}, {v[5], v[4], v[2]}};
+          for (int a = 0; a < 3; ++a)
+            for (int cc = 0; cc < 3; ++cc) strs[a + 3 * (cc + 3 * (f + (size_t)nf * i))] = S[a][cc];
+        }
+    }
+  }
+  int guard = 0;
+  GAPX_HIP(hipMemcpy(&guard, stats + 3, sizeof(int), hipMemcpyDeviceToHost));
+  if (guard) return fail(GAPX_ERR_NEIGHBORS, "descriptor kernels saw more neighbours than their LDS layout holds");
+  return GAPX_OK;
+}
+
 static void ctx_free(Ctx *c);
 
 static int ctx_create(Model *m, Ctx **out) {
@@ -1950,6 +2091,14 @@ int gapx_descriptors(gapx_model *m, int na, const int *species, const double *la
   for (int i = 0; i < na; ++i)
     for (int d = 0; d < m->D; ++d) x[(size_t)i * m->D + d] = tmp[(size_t)i * m->Dp + d];
   return GAPX_OK;
+}
+
+int gapx_car2acsf(gapx_model *m, int na, int max_neighbor, int nf, const double *pos, const double *neighbor,
+                  const int *neighbor_count, double *xx, double *dxdy, double *strs, int lgrad) {
+  Ctx *c;
+  int rc = cached(m, &c);
+  if (rc) return rc;
+  return ctx_car2acsf(*c, na, max_neighbor, nf, pos, neighbor, neighbor_count, xx, dxdy, strs, lgrad != 0);
 }
 
 int gapx_ctx_create(gapx_model *m, gapx_ctx **out) {

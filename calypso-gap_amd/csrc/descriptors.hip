@@ -151,6 +151,13 @@ __device__ __forceinline__ void line_power(const SFDev &sf, int s, T cth, PowSta
 // row enters d2, i.e. 0 in FP32 and in FP64 alike.
 __device__ __forceinline__ double fp16_operand(double xt) { return fmin(fmax(xt, -32768.0), 32768.0); }
 
+// Weight of a neighbour slot: the caller's (CAR2ACSF's neighbour table), or
+// that of the neighbour's species.
+__device__ __forceinline__ double slot_weight(const NeighborBufs &nb, const double *wz, const int *species,
+                                              size_t slot) {
+  return nb.nb_w ? nb.nb_w[slot] : wz[species[nb.nb_j[slot]]];
+}
+
 // Launch geometry of the descriptor kernels: LDS layout (neighbour slots,
 // cutoff groups, padded features) and the first atom of the launch (one
 // workgroup per atom from atom0 on).
@@ -287,7 +294,7 @@ __global__ __launch_bounds__(256) void k_desc_fwd(int N, const int *__restrict__
     s_d[4 * a + 1] = (T)dd[1];
     s_d[4 * a + 2] = (T)dd[2];
     s_d[4 * a + 3] = (T)r;
-    s_w[a] = (T)wz[species[nb.nb_j[start + a]]];
+    s_w[a] = (T)slot_weight(nb, wz, species, start + a);
     for (int g = 0; g < sf.n_groups; ++g) {
       const double rc = sf.group_rc[g];
       s_fc[g * KP + a] = (r <= rc) ? (T)(0.5 * (cos(kPi * r / rc) + 1.0)) : T(0);
@@ -521,7 +528,7 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
     s_d[4 * a + 2] = (T)dd[2];
     s_d[4 * a + 3] = (T)r;
     s_rw[2 * a] = (T)(1.0 / r);
-    s_rw[2 * a + 1] = (T)wz[species[nb.nb_j[start + a]]];
+    s_rw[2 * a + 1] = (T)slot_weight(nb, wz, species, start + a);
     for (int g = 0; g < sf.n_groups; ++g) {
       const double rc = sf.group_rc[g];
       s_fc[g * KP + a] = (r <= rc) ? (T)(0.5 * (cos(kPi * r / rc) + 1.0)) : T(0);
@@ -857,7 +864,7 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
     s_d[4 * a + 2] = (T)dd[2];
     s_d[4 * a + 3] = (T)r;
     s_rw[2 * a] = (T)(1.0 / r);
-    s_rw[2 * a + 1] = (T)wz[species[nb.nb_j[start + a]]];
+    s_rw[2 * a + 1] = (T)slot_weight(nb, wz, species, start + a);
     for (int g = 0; g < sf.n_groups; ++g) {
       const double rc = sf.group_rc[g];
       const bool in = r <= rc;

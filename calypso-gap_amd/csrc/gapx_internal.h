@@ -128,6 +128,9 @@ struct NeighborBufs {
   // the run's precision, for lists too long for the kernels' LDS layouts
   // (desc_table_scratch_bytes); null otherwise
   void *fc_scratch;
+  // per-slot neighbour weights given by the caller (CAR2ACSF's neighbour
+  // table); null: the weight of the neighbour's species from the model
+  const double *nb_w;
 };
 
 // Wrap positions into their cells, bin and count the atoms, and scan the
@@ -175,6 +178,9 @@ struct TripletBufs {
 // for angular cutoff rcA, from the same wave.
 hipError_t launch_build_neighbors(hipStream_t s, int N, const int *atom_struct, const StructParams *sp,
                                   double rcut, NeighborBufs nb, double rcA, const TripletBufs *tb);
+// Triplet lists (as launch_build_neighbors makes them) of neighbour slots
+// that are already filled: nb_count, nb_d (and kcap) are read.
+hipError_t launch_triplets_from_slots(hipStream_t s, int N, NeighborBufs nb, double rcA, TripletBufs tb);
 
 // Descriptor forward for atoms [atom0, atom0 + N).
 hipError_t launch_descriptors_fwd(hipStream_t s, bool fp64, int N, const int *species,

@@ -454,6 +454,24 @@ __global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ ato
   }
 }
 
+// Triplet lists of slots filled by the caller (CAR2ACSF's neighbour table):
+// one wavefront per atom stages the slots' (dx, dy, dz, r) in LDS as FP32 and
+// builds the list as k_build<true> does.
+__global__ __launch_bounds__(64) void k_triplets_from_slots(int N, NeighborBufs nb, double rcA, TripletBufs tb) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float4 *s_p = (float4 *)smem;  // [kcap]
+  const int i = blockIdx.x, lane = threadIdx.x;
+  if (i >= N) return;
+  const int K = min(nb.nb_count[i], nb.kcap);
+  const size_t base = (size_t)i * nb.kcap;
+  for (int a = lane; a < K; a += 64) {
+    const double *d = nb.nb_d + 4 * (base + a);
+    s_p[a] = make_float4((float)d[0], (float)d[1], (float)d[2], (float)d[3]);
+  }
+  __syncthreads();
+  triplets_from_lds(i, K, s_p, rcA, tb, lane);
+}
+
 // For slot (i -> j, n) find the slot (j -> i, -n); -1 if absent (a pair within
 // one rounding step of rcut can be seen from one side only).  One wavefront
 // per centre atom, one slot per lane; j's packed keys are read 16 at a time
@@ -732,6 +750,13 @@ hipError_t launch_build_neighbors(hipStream_t s, int N, const int *atom_struct, 
     k_build<true><<<N, 64, 4 * sizeof(float) * (size_t)nb.kcap, s>>>(N, atom_struct, sp, rcut, nb, rcA, *tb);
   else
     k_build<false><<<N, 64, 0, s>>>(N, atom_struct, sp, rcut, nb, 0.0, TripletBufs{});
+  return hipGetLastError();
+}
+
+hipError_t launch_triplets_from_slots(hipStream_t s, int N, NeighborBufs nb, double rcA, TripletBufs tb) {
+  if (N <= 0) return hipSuccess;
+  if (nb.kcap < 1 || nb.kcap > kMaxNeighbors) return hipErrorInvalidValue;
+  k_triplets_from_slots<<<N, 64, 4 * sizeof(float) * (size_t)nb.kcap, s>>>(N, nb, rcA, tb);
   return hipGetLastError();
 }
 

@@ -5,6 +5,8 @@
 !   FGAP_CALC   gappy/libgap/gap_calc.f90:1-46    energy, forces, stress
 !   FGAP_READ   gappy/libgap/gap_calc.f90:303-314 model reader (fixed caps)
 !   FGET_BOND   gappy/libgap/get_bond.f90:4-12    minimum interatomic distance
+!   CAR2ACSF    gappy/libgap/wacsf.f90:1-12       descriptors and their derivatives
+!   WRITE_ARRAY_2DIM  gappy/libgap/wacsf.f90:798-810  a matrix as text
 ! and, with no reference counterpart:
 !   FGAP_CALC_STATUS   FGAP_CALC returning a status instead of stopping
 !   FGAP_CALC_BATCH    many structures in one call
@@ -274,6 +276,58 @@ subroutine fget_bond(na, lat, elements, pos, rcut, min_bond)
     stop
   end if
 end subroutine fget_bond
+
+! CAR2ACSF (wacsf.f90:1-12): descriptors xx(nf, na) and, with lgrad, their
+! dense derivatives dxdy(nf, na, na, 3) and strs(3, 3, nf, na) from the
+! caller's neighbour table, with the SF table of ./gap_parameters; stops like
+! the reference when the file is missing (wacsf.f90:41-44) and on any other
+! error with the message of the failed call.
+subroutine car2acsf(na, max_neighbor, nf, pos, neighbor, neighbor_count, xx, dxdy, strs, lgrad)
+  use gapx_iface
+  implicit none
+  integer, intent(in) :: na, max_neighbor, nf
+  double precision, intent(in), dimension(na, 3) :: pos
+  double precision, intent(in), dimension(na, max_neighbor, 6) :: neighbor
+  integer, intent(in), dimension(na) :: neighbor_count
+  double precision, intent(out), dimension(nf, na) :: xx
+  double precision, intent(out), dimension(nf, na, na, 3) :: dxdy
+  double precision, intent(out), dimension(3, 3, nf, na) :: strs
+  logical, intent(in) :: lgrad
+  integer :: status
+  integer(c_int) :: lg
+  call ensure_header_model(status)
+  if (status == 2) then
+    print *, "gap_parameters file does not exist!"
+    stop
+  end if
+  lg = 0
+  if (lgrad) lg = 1
+  if (status == GAPX_OK) status = gapx_car2acsf(hmodel, int(na, c_int), int(max_neighbor, c_int), int(nf, c_int), &
+                                                pos, neighbor, neighbor_count, xx, dxdy, strs, lg)
+  if (status /= GAPX_OK) then
+    print *, "gapx: ", last_error()
+    stop
+  end if
+end subroutine car2acsf
+
+! write_array_2dim (wacsf.f90:798-810): a(n, m) as text, one row per line,
+! each value in F20.10, to the file named by name (leading and trailing
+! blanks removed).
+subroutine write_array_2dim(n, m, a, name)
+  implicit none
+  integer, intent(in) :: n, m
+  double precision, intent(in), dimension(n, m) :: a
+  character(len=*), intent(in) :: name
+  integer :: u, r, c
+  open(newunit=u, file=trim(adjustl(name)), status="replace", action="write")
+  do r = 1, n
+    do c = 1, m
+      write(u, "(F20.10)", advance="no") a(r, c)
+    end do
+    write(u, *)
+  end do
+  close(u)
+end subroutine write_array_2dim
 
 ! Message of the last failed gapx call (for callers of the *_status entries).
 subroutine fgap_last_error(msg)

@@ -69,6 +69,16 @@ module gapx_iface
       real(c_double), intent(out) :: dmin
     end function
 
+    integer(c_int) function gapx_car2acsf(m, na, max_neighbor, nf, pos, neighbor, neighbor_count, xx, dxdy, &
+                                          strs, lgrad) bind(C, name="gapx_car2acsf")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: na, max_neighbor, nf, lgrad
+      real(c_double), intent(in) :: pos(*), neighbor(*)
+      integer(c_int), intent(in) :: neighbor_count(*)
+      real(c_double), intent(out) :: xx(*), dxdy(*), strs(*)
+    end function
+
     integer(c_int) function gapx_file_stamp(path, fsize, mtime) bind(C, name="gapx_file_stamp")
       import :: c_int, c_char, c_long_long
       character(kind=c_char), intent(in) :: path(*)
@@ -92,6 +102,13 @@ module gapx_iface
   integer(c_int64_t), save :: model_hash = 0
   integer, save :: model_m = -1, model_d = -1
   integer, save :: hdr_nspecies = 0, hdr_nsf = 0
+  ! bumped on every successful header read; each device model records the
+  ! generation it was built from
+  integer, save :: hdr_gen = 0, model_gen = -1
+  ! header-only model for CAR2ACSF (species table and SF lines, one zero
+  ! sparse point), rebuilt when the header changes
+  type(c_ptr), save :: hmodel = c_null_ptr
+  integer, save :: hmodel_gen = -1
   integer(c_int), allocatable, save :: hdr_z(:), hdr_type(:)
   real(c_double), allocatable, save :: hdr_w(:), hdr_alpha(:), hdr_cut(:), hdr_zeta(:)
 
@@ -190,6 +207,7 @@ contains
     stamp_size = fsize
     stamp_mtime = mtime
     changed = .true.
+    hdr_gen = hdr_gen + 1
     return
 90  close(u)
     status = 2
@@ -232,7 +250,7 @@ contains
     call hash_doubles(h, theta, des_len)
     call hash_doubles(h, mm, nsparse * des_len)
     call hash_doubles(h, coeff, nsparse)
-    if (.not. changed .and. c_associated(model) .and. h == model_hash .and. nsparse == model_m &
+    if (model_gen == hdr_gen .and. c_associated(model) .and. h == model_hash .and. nsparse == model_m &
         .and. des_len == model_d) then
       status = GAPX_OK
       return
@@ -246,7 +264,40 @@ contains
       model_hash = h
       model_m = nsparse
       model_d = des_len
+      model_gen = hdr_gen
     end if
+  end subroutine
+
+  ! Header-only device model of ./gap_parameters for CAR2ACSF, which reads
+  ! only the species table and the SF lines (wacsf.f90:40-56).  status: gapx
+  ! status code (2: file missing, 3: unreadable header).
+  subroutine ensure_header_model(status)
+    integer, intent(out) :: status
+    integer :: hstat
+    logical :: changed
+    real(c_double), allocatable :: theta(:), mm(:), coeff(:)
+    call load_header(hstat, changed)
+    if (hstat == 1) then
+      status = 2
+      return
+    else if (hstat /= 0) then
+      status = 3
+      return
+    end if
+    if (hmodel_gen == hdr_gen .and. c_associated(hmodel)) then
+      status = GAPX_OK
+      return
+    end if
+    if (c_associated(hmodel)) call gapx_model_free(hmodel)
+    hmodel = c_null_ptr
+    allocate(theta(2 * hdr_nsf), mm(2 * hdr_nsf), coeff(1))
+    theta = 1.0_c_double
+    mm = 0.0_c_double
+    coeff = 0.0_c_double
+    status = gapx_model_create(int(hdr_nspecies, c_int), hdr_z, hdr_w, int(hdr_nsf, c_int), hdr_type, hdr_alpha, &
+                               hdr_cut, hdr_zeta, int(2 * hdr_nsf, c_int), 1_c_int, theta, mm, 0_c_int, 0_c_int, &
+                               coeff, hmodel)
+    if (status == GAPX_OK) hmodel_gen = hdr_gen
   end subroutine
 
 end module gapx_iface

@@ -47,7 +47,7 @@ EXPORTS = [
     "gapx_mlp_train", "gapx_ctx_export_labels", "gapx_ctx_run_surrogate", "gapx_ctx_relax",
     "gapx_ctx_positions", "gapx_model_set_qmm", "gapx_ctx_variance",
     "gapx_ctx_md", "gapx_ctx_momenta", "gapx_model_write", "gapx_model_save", "gapx_model_load",
-    "gapx_model_read_cached", "gapx_probe_fma_rates", "gapx_model_precision",
+    "gapx_model_read_cached", "gapx_probe_fma_rates", "gapx_model_precision", "gapx_car2acsf",
 ]
 
 
@@ -124,6 +124,7 @@ def lib():
         L.gapx_ctx_device_bytes.argtypes = [vp, vp]
         L.gapx_ctx_counts.argtypes = [vp, vp, vp]
         L.gapx_min_distance.argtypes = [ci, vp, vp, cd, ci, vp]
+        L.gapx_car2acsf.argtypes = [vp, ci, ci, ci, vp, vp, vp, vp, vp, vp, ci]
         L.gapx_ctx_free.argtypes = [vp]
         L.gapx_ctx_free.restype = None
         _lib = L
@@ -289,6 +290,22 @@ class Model:
         _check(lib().gapx_descriptors(self._h, len(species), _p(species), _p(np.ascontiguousarray(lat, np.float64)),
                                       _p(np.ascontiguousarray(pos, np.float64)), float(rcut), 0, _p(x)))
         return x
+
+    def car2acsf(self, pos, neighbor, neighbor_count, lgrad=True):
+        """CAR2ACSF: (xx(nf, na), dxdy(nf, na, na, 3), strs(3, 3, nf, na)),
+        Fortran-ordered, from a neighbour table NEIGHBOR(na, max_neighbor, 6)
+        (image x y z, distance, weight, 1-based index) and its counts."""
+        pos = np.asfortranarray(pos, np.float64)
+        nb = np.asfortranarray(neighbor, np.float64)
+        cnt = np.ascontiguousarray(neighbor_count, np.int32)
+        na, max_nb = nb.shape[0], nb.shape[1]
+        D = self.des_len
+        xx = np.zeros((D, na), order="F")
+        dxdy = np.zeros((D, na, na, 3), order="F")
+        strs = np.zeros((3, 3, D, na), order="F")
+        _check(lib().gapx_car2acsf(self._h, na, max_nb, D, _p(pos), _p(nb), _p(cnt), _p(xx), _p(dxdy), _p(strs),
+                                   int(bool(lgrad))))
+        return xx, dxdy, strs
 
     def close(self):
         if self._h:

@@ -151,6 +151,21 @@ int gapx_calc_batch(gapx_model *m, int nstruct, const int *natoms, const int *sp
 int gapx_descriptors(gapx_model *m, int na, const int *species, const double *lat,
                      const double *pos, double rcut, int flags, double *x);
 
+/* CAR2ACSF (gappy/libgap/wacsf.f90:2-12): wACSF descriptors and their dense
+ * derivatives from a neighbour table the caller built, in the reference's
+ * Fortran layouts.  pos(NA, 3); neighbor(NA, max_neighbor, 6) with, per slot,
+ * the image position x y z, the distance, the weight and the 1-based index of
+ * the neighbour atom as a real (the table FGAP_CALC builds,
+ * gap_calc.f90:89-120); neighbor_count(NA).  Outputs (FP64): xx(nf, NA);
+ * dxdy(nf, NA, NA, 3) = d xx(f, i) / d pos(n, c); strs(3, 3, nf, NA) =
+ * sum over the neighbour terms of delta_a d xx(f, i) / d pos(j, c).  The SF
+ * table is the model's (only its header is used); nf must be its des_len.
+ * With lgrad = 0, or for a NULL pointer, dxdy and strs are not computed (a
+ * non-NULL one is zeroed, as the reference zeroes both).  A distance of 0 or
+ * an index outside [1, NA] is GAPX_ERR_ARG. */
+int gapx_car2acsf(gapx_model *m, int na, int max_neighbor, int nf, const double *pos, const double *neighbor,
+                  const int *neighbor_count, double *xx, double *dxdy, double *strs, int lgrad);
+
 /* FGET_BOND (gappy/libgap/get_bond.f90:4-117): min(10, shortest distance
  * within rcut between any atom and any periodic image of any atom, the
  * atom's own zero image excluded), over the reference's image range.

@@ -175,6 +175,10 @@ def lib():
         _lib.orc_descriptors.argtypes = [vp, ci, vp, vp, vp, cd, ci, vp]
         _lib.orc_neighbor_set.argtypes = [ci, vp, vp, cd, ctypes.c_long, vp, vp, vp]
         _lib.orc_neighbor_set.restype = ctypes.c_long
+        _lib.orc_species_weights.argtypes = [vp, ci, vp, vp]
+        _lib.orc_neighbors_ref.argtypes = [ci, vp, vp, vp, cd, ci, vp, vp]
+        _lib.orc_car2acsf_dense.argtypes = [vp, ci, ci, vp, vp, vp, vp, vp, vp, ci]
+        _lib.orc_car2acsf_dense.restype = None
     return _lib
 
 
@@ -270,6 +274,53 @@ def descriptors(model, species, lat, pos, rcut, nthreads=0):
     return xx
 
 
+def neighbor_table(model, species, lat, pos, rcut, max_nb=1000):
+    """FGAP_CALC's neighbour table (gap_calc.f90:89-120) in the layout CAR2ACSF
+    takes (wacsf.f90:5-6): Fortran NEIGHBOR(NA, max_nb, 6) = image x y z,
+    distance, species weight, 1-based atom index; and the counts (int32)."""
+    pm = _Pinned(model)
+    species = np.ascontiguousarray(species, dtype=np.int32)
+    lat = np.ascontiguousarray(lat, dtype=np.float64)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    na = len(species)
+    w = np.zeros(na)
+    L = lib()
+    if L.orc_species_weights(ctypes.byref(pm.cm), na, _ptr(species), _ptr(w)):
+        raise RuntimeError("oracle: species not listed in the model")
+    nb = np.zeros((na, max_nb, 6))
+    cnt = np.zeros(na, dtype=np.int32)
+    if L.orc_neighbors_ref(na, _ptr(lat), _ptr(pos), _ptr(w), rcut, max_nb, _ptr(nb), _ptr(cnt)):
+        raise RuntimeError("oracle: more than %d neighbours" % max_nb)
+    nb[:, :, 5] += 1.0  # 1-based, as FGAP_CALC stores real(j) (gap_calc.f90:115)
+    for i in range(na):
+        nb[i, cnt[i]:, :] = 0.0
+    return np.asfortranarray(nb), cnt
+
+
+def car2acsf(model, pos, neighbor, count, lgrad=True):
+    """CAR2ACSF restated (wacsf.f90:2-796) on a neighbour table in the
+    reference's layout (neighbor_table): xx(nf, na), dxdy(nf, na, na, 3),
+    strs(3, 3, nf, na), Fortran-ordered arrays."""
+    pm = _Pinned(model)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    na, max_nb = neighbor.shape[0], neighbor.shape[1]
+    nb = np.ascontiguousarray(neighbor, dtype=np.float64).copy()
+    nb[:, :, 5] -= 1.0  # the restatement indexes atoms from 0
+    cnt = np.ascontiguousarray(count, dtype=np.int32)
+    D = model.des_len
+    xx = np.zeros(na * D)
+    dxdy = np.zeros(na * na * 3 * D) if lgrad else None
+    strs = np.zeros(na * 9 * D) if lgrad else None
+    lib().orc_car2acsf_dense(ctypes.byref(pm.cm), na, max_nb, _ptr(pos), _ptr(nb), _ptr(cnt), _ptr(xx),
+                             _ptr(dxdy), _ptr(strs), int(lgrad))
+    out_x = np.asfortranarray(xx.reshape(na, D).T)
+    if not lgrad:
+        return out_x, np.zeros((D, na, na, 3), order="F"), np.zeros((3, 3, D, na), order="F")
+    out_d = np.asfortranarray(dxdy.reshape(na, na, 3, D).transpose(3, 0, 1, 2))
+    out_s = np.asfortranarray(strs.reshape(na, 3, 3, D).transpose(1, 2, 3, 0))
+    return out_x, out_d, out_s
+
+
 def neighbor_set(lat, pos, rcut):
     """Sorted (i, j, n1, n2, n3) rows and distances of every pair within rcut."""
     lat = np.ascontiguousarray(lat, dtype=np.float64)
@@ -360,6 +411,41 @@ def ref_gap_calc(model_path, species, lat, pos, theta, mm, coeff, rcut, lgrad=Tr
         os.chdir(cwd)
     return dict(energy=args["ENE"].value, forces=np.array(args["FORCE"]), stress=args["STRESS"].copy(),
                 variance=args["VAR"].value)
+
+
+def ref_car2acsf(model_path, pos, neighbor, count, nf, lgrad=True):
+    """The reference CAR2ACSF (compiled Fortran, wacsf.f90:2-12), run in the
+    model file's directory (it reads the SF table from ./gap_parameters)."""
+    ref = ctypes.CDLL(REF_PATH)
+    na = len(count)
+    max_nb = neighbor.shape[1]
+    P = np.asfortranarray(pos, dtype=np.float64)
+    NB = np.asfortranarray(neighbor, dtype=np.float64)
+    C = np.asfortranarray(count, dtype=np.int32)
+    xx = np.zeros((nf, na), order="F")
+    dxdy = np.zeros((nf, na, na, 3), order="F")
+    strs = np.zeros((3, 3, nf, na), order="F")
+    args = [ctypes.c_int(na), ctypes.c_int(max_nb), ctypes.c_int(nf)]
+    lg = ctypes.c_int(1 if lgrad else 0)
+    cwd = os.getcwd()
+    os.chdir(os.path.dirname(os.path.abspath(model_path)))
+    try:
+        ref.car2acsf_(*(ctypes.byref(a) for a in args), P.ctypes.data_as(ctypes.c_void_p),
+                      NB.ctypes.data_as(ctypes.c_void_p), C.ctypes.data_as(ctypes.c_void_p),
+                      xx.ctypes.data_as(ctypes.c_void_p), dxdy.ctypes.data_as(ctypes.c_void_p),
+                      strs.ctypes.data_as(ctypes.c_void_p), ctypes.byref(lg))
+    finally:
+        os.chdir(cwd)
+    return xx, dxdy, strs
+
+
+def ref_write_array_2dim(a, path):
+    """The reference write_array_2dim (compiled Fortran, wacsf.f90:798-810)."""
+    ref = ctypes.CDLL(REF_PATH)
+    A = np.asfortranarray(a, dtype=np.float64)
+    name = os.fsencode(path)
+    ref.write_array_2dim_(ctypes.byref(ctypes.c_int(A.shape[0])), ctypes.byref(ctypes.c_int(A.shape[1])),
+                          A.ctypes.data_as(ctypes.c_void_p), ctypes.c_char_p(name), ctypes.c_size_t(len(name)))
 
 
 def ref_get_bond(lat, species, pos, rcut):

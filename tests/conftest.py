@@ -52,3 +52,14 @@ def golden_cases(ref):
 
 GOLDEN_CASE_NAMES = ["b8c56", "bc24_triclinic", "bc6_small_cell", "bc6_small_cell_nograd", "t16", "t40",
                      "t40_nograd", "t8_small_cell", "c64_single_species", "bc8_dense_920", "t8_dense_920"]
+
+
+CAR_CASE_NAMES = ["bc6_small_cell", "t8_small_cell", "bc8_dense_920"]
+
+
+def car_case(ref, name):
+    """A CAR2ACSF golden case: FGAP_CALC's neighbour table of a structure and
+    the reference CAR2ACSF's xx, dxdy, strs on it (Fortran layouts)."""
+    p = "car_%s_" % name
+    return dict(name=name, model=str(ref[p + "model"]), pos=ref[p + "pos"], neighbor=ref[p + "neighbor"],
+                count=ref[p + "count"], xx=ref[p + "xx"], dxdy=ref[p + "dxdy"], strs=ref[p + "strs"])

@@ -17,7 +17,11 @@ per atom (the reference stops above 1000).
 
 FGET_BOND (get_bond.f90:4-117) is run on the same structures at two cutoffs.
 
-Output: tests/golden/ref_vectors.npz (inputs, model arrays, reference outputs).
+CAR2ACSF (wacsf.f90:2-12) is run on FGAP_CALC's neighbour tables of three of
+the cases, and write_array_2dim (wacsf.f90:798-810) on a small matrix.
+
+Output: tests/golden/ref_vectors.npz (inputs, model arrays, reference outputs)
+and tests/golden/write_array_2dim.txt (the reference's text).
 Usage: python tests/golden/make_ref_vectors.py
 """
 import os
@@ -178,6 +182,40 @@ def main():
                                           oracle.ref_get_bond(lat, sp, pos, 0.9)])
         print("%-24s N=%3d E=%.10f |F|max=%.6f S0=%.6f" % (cname, len(sp), res["energy"],
                                                           np.abs(res["forces"]).max(), res["stress"][0]))
+    # CAR2ACSF (wacsf.f90:2-12) on FGAP_CALC's own neighbour tables of three
+    # of the cases: xx, dense dxdy and strs as the reference returns them
+    car = ["bc6_small_cell", "t8_small_cell", "bc8_dense_920"]
+    out["car_names"] = np.array(car)
+    for mname, cname, sp, lat, pos, rcut, lgrad in cases:
+        if cname not in car:
+            continue
+        m = models[mname]
+        nb, cnt = oracle.neighbor_table(m, sp, lat, pos, rcut)
+        nb = np.asfortranarray(nb[:, :max(int(cnt.max()), 1), :])  # max_neighbor = the largest count
+        with tempfile.TemporaryDirectory() as tmp:
+            path = os.path.join(tmp, "gap_parameters")
+            oracle.write_model(path, m)
+            xx, dxdy, strs = oracle.ref_car2acsf(path, pos, nb, cnt, m.des_len, True)
+        pre = "car_%s_" % cname
+        out[pre + "model"] = np.array(mname)
+        out[pre + "pos"] = np.asarray(pos)
+        out[pre + "neighbor"] = nb
+        out[pre + "count"] = cnt
+        out[pre + "xx"] = xx
+        out[pre + "dxdy"] = dxdy
+        out[pre + "strs"] = strs
+        print("car2acsf %-16s N=%d max_nb=%d |xx|max=%.4g |dxdy|max=%.4g" % (cname, len(sp), nb.shape[1],
+                                                                           np.abs(xx).max(), np.abs(dxdy).max()))
+    # write_array_2dim (wacsf.f90:798-810): the text the reference writes for a
+    # small matrix with mixed signs and magnitudes
+    wa = np.array([[1.5, -2.25, 1e-11, 123456.789012345], [-0.0, 3.0e-4, -9876.54321, 2.0 / 3.0],
+                   [np.pi, -np.e, 1e5, -1e-10]])
+    out["write_array_input"] = wa
+    with tempfile.TemporaryDirectory() as tmp:
+        oracle.ref_write_array_2dim(wa, os.path.join(tmp, "wa.txt"))
+        text = open(os.path.join(tmp, "wa.txt"), "rb").read()
+    with open(os.path.join(HERE, "write_array_2dim.txt"), "wb") as f:
+        f.write(text)
     path = os.path.join(HERE, "ref_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

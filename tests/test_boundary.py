@@ -32,6 +32,8 @@ def test_f2py_signatures_match_the_reference():
     assert "ene,force,stress,variance = fgap_calc(species,lat,pos,theta,mm,qmm,coeff,rcut,lgrad" in doc
     assert "nsparsex,des_len,theta,mm,invcmm,coeff = fgap_read()" in doc
     assert "min_bond = fget_bond(lat,elements,pos,rcut" in doc
+    assert "xx,dxdy,strs = car2acsf(nf,pos,neighbor,neighbor_count,lgrad" in doc
+    assert "write_array_2dim(a,name" in doc
 
 
 def test_fortran_reader_matches_oracle_parser(in_model_dir):
@@ -133,3 +135,15 @@ print("statuses", out)
     st = eval(line.split(" ", 1)[1])
     assert st[:3] == [3, 3, 3]
     assert st[3] in (0, 6, 7)
+
+
+def test_write_array_2dim_writes_the_reference_text(tmp_path):
+    """write_array_2dim (wacsf.f90:798-810) writes, byte for byte, the text the
+    compiled reference wrote for the same matrix (tests/golden)."""
+    import libgap.libgap as L
+    ref = np.load(os.path.join(GOLDEN, "ref_vectors.npz"))
+    a = np.asfortranarray(ref["write_array_input"])
+    L.write_array_2dim(a, "  " + str(tmp_path / "out.txt") + "  ")  # blanks around the name are dropped
+    got = open(tmp_path / "out.txt", "rb").read()
+    want = open(os.path.join(GOLDEN, "write_array_2dim.txt"), "rb").read()
+    assert got == want

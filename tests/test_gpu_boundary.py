@@ -1,6 +1,6 @@
 """The drop-in boundary on the GPU: libgap.libgap.fgap_calc (reference
-signature), libgap.GAP.Calculator, the ASE adapter and FGET_BOND against the
-reference's own outputs; the batched and status entry points."""
+signature), libgap.GAP.Calculator, the ASE adapter, FGET_BOND and CAR2ACSF
+against the reference's own outputs; the batched and status entry points."""
 import os
 import shutil
 import sys
@@ -10,7 +10,7 @@ import pytest
 
 import gapx
 import oracle
-from conftest import GOLDEN, GOLDEN_CASE_NAMES, ROOT, golden_cases, golden_model
+from conftest import CAR_CASE_NAMES, GOLDEN, GOLDEN_CASE_NAMES, ROOT, car_case, golden_cases, golden_model
 
 pytestmark = pytest.mark.gpu
 PKG = os.path.join(ROOT, "calypso-gap_amd")
@@ -118,3 +118,36 @@ def test_fgap_descriptors_match_oracle(in_model_dir, frames):
     om = oracle.parse_model("gap_parameters")
     xo = oracle.descriptors(om, frames["numbers"], frames["cells"][2], frames["positions"][2], 6.0)
     np.testing.assert_allclose(xx.T, xo, rtol=1e-11, atol=1e-9)
+
+
+def _close(got, ref, rel):
+    assert got.shape == ref.shape
+    np.testing.assert_allclose(got, ref, rtol=0, atol=rel * max(1.0, float(np.abs(ref).max())))
+
+
+@pytest.mark.parametrize("case", CAR_CASE_NAMES)
+def test_car2acsf_matches_reference(tmp_path, monkeypatch, ref_vectors, case):
+    """CAR2ACSF through the f2py module with the reference's signature (the SF
+    table read from ./gap_parameters) and through the C ABI, against the
+    compiled reference's xx, dense dxdy and strs on FGAP_CALC's own neighbour
+    tables (6 and 8 atoms; 8 atoms with 920 neighbours each under a 7-cutoff
+    table, whose FP64 backward keeps its cutoff tables in global memory).
+    FP64 bar: 1e-11 of each array's largest entry (the GPU sums in another
+    order); lgrad = .false. gives xx and zeros."""
+    c = car_case(ref_vectors, case)
+    om = golden_model(ref_vectors, c["model"])
+    oracle.write_model(str(tmp_path / "gap_parameters"), om)
+    monkeypatch.chdir(tmp_path)
+    import libgap.libgap as L
+    nf = om.des_len
+    xx, dxdy, strs = L.car2acsf(nf, np.asfortranarray(c["pos"]), c["neighbor"], c["count"], True)
+    for got, ref in ((xx, c["xx"]), (dxdy, c["dxdy"]), (strs, c["strs"])):
+        _close(got, ref, 1e-11)
+    xx0, d0, s0 = L.car2acsf(nf, np.asfortranarray(c["pos"]), c["neighbor"], c["count"], False)
+    _close(xx0, c["xx"], 1e-11)
+    assert not np.any(d0) and not np.any(s0)
+    m = gapx.Model.from_model(om)
+    xx2, dxdy2, strs2 = m.car2acsf(c["pos"], c["neighbor"], c["count"])
+    for got, ref in ((xx2, c["xx"]), (dxdy2, c["dxdy"]), (strs2, c["strs"])):
+        _close(got, ref, 1e-11)
+    m.close()

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import GOLDEN_CASE_NAMES, golden_cases, golden_model
+from conftest import CAR_CASE_NAMES, GOLDEN_CASE_NAMES, car_case, golden_cases, golden_model
 from helpers import fitted_small_model, ternary_structure
 
 # parity bar of the FP64 build: 1e-9 eV/atom, 1e-7 eV/A, 1e-6 GPa; for the
@@ -259,3 +259,21 @@ def test_cell_list_equals_brute_force(n, vpa, seed):
     pos = s["pos"] + rng.normal(0, 3.0, size=s["pos"].shape)  # unwrapped positions
     rows, _ = oracle.neighbor_set(s["lat"], pos, 6.0)
     assert [tuple(int(v) for v in r) for r in rows] == brute_force_pairs(s["lat"], pos, 6.0)
+
+
+@pytest.mark.parametrize("case", CAR_CASE_NAMES)
+def test_car2acsf_restatement_matches_reference(ref_vectors, case):
+    """CAR2ACSF restated (dense dxdy/strs) against the compiled reference's
+    outputs on FGAP_CALC's own neighbour tables (tests/golden), and the
+    neighbour-table builder against the stored table."""
+    c = car_case(ref_vectors, case)
+    m = golden_model(ref_vectors, c["model"])
+    xx, dxdy, strs = oracle.car2acsf(m, c["pos"], c["neighbor"], c["count"])
+    for got, ref in ((xx, c["xx"]), (dxdy, c["dxdy"]), (strs, c["strs"])):
+        assert got.shape == ref.shape
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * max(1.0, np.abs(ref).max()))
+    g = [x for x in golden_cases(ref_vectors) if x["name"] == case][0]
+    nb, cnt = oracle.neighbor_table(m, g["species"], g["lat"], g["pos"], g["rcut"])
+    assert np.array_equal(cnt, c["count"])
+    k = c["neighbor"].shape[1]
+    np.testing.assert_array_equal(nb[:, :k, :], c["neighbor"])
