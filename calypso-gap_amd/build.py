@@ -58,8 +58,7 @@ def build_hip(jobs=8):
     if failed:
         raise SystemExit("HIP compilation failed")
     out = os.path.join(LIB, "libgapx.so")
-    run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", out] + objs +
-        ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
+    run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", out] + objs)
     return out
 
 

@@ -369,7 +369,6 @@ struct Ctx {
   int chunk = 0;  // lines per forward pass of the generic kernels (0: 16)
   Tuning tune[2];  // launch shapes for FP64 [0] and mixed [1] runs
   Mlp *mlp = nullptr;  // set while the last run used the surrogate
-  void *blas = nullptr;  // rocBLAS handle on the stream (variance only, made on first use)
   bool md_ready = false;  // md_p holds the momenta of the last MD run
   unsigned long long bins_clean = ~0ULL;  // allocation generations of bin_count/bin_ticket last cleared
   int red_parts = 0;                      // reduction parts over all structures (see ReduceBufs)
@@ -445,32 +444,47 @@ static void inv3(const double *A, double *B) {
   B[8] = (A[0] * A[4] - A[1] * A[3]) / det;
 }
 
+}  // namespace gapx
+
+// Cell-list parameters of one structure (declared in gapx_internal.h).
+int gapx::cell_params(const double *L9, double rcut, int nb_max, StructParams &P) {
+  std::memcpy(P.L, L9, 9 * sizeof(double));
+  const double *L = P.L;
+  // Det() of gap_calc.f90:290-299
+  const double det = L[0] * (L[4] * L[8] - L[5] * L[7]) - L[1] * (L[3] * L[8] - L[5] * L[6]) +
+                     L[2] * (L[3] * L[7] - L[6] * L[4]);
+  if (!(std::fabs(det) > 1e-12)) return fail(GAPX_ERR_ARG, "singular lattice");
+  P.vol = std::fabs(det);
+  inv3(P.L, P.Linv);
+  for (int k = 0; k < 3; ++k) {
+    double g = std::sqrt(P.Linv[k] * P.Linv[k] + P.Linv[3 + k] * P.Linv[3 + k] + P.Linv[6 + k] * P.Linv[6 + k]);
+    double dk = 1.0 / g;
+    // bins at least rcut/2 wide (search half-width S = 2 where the cell
+    // allows): on the 4096-atom cell a centre tests ~300 candidates instead
+    // of ~510 with rcut-wide bins, and the neighbour stage takes 6 us less
+    int nbk = (int)std::floor(2.0 * dk / rcut);
+    nbk = std::max(1, std::min(nbk, nb_max));
+    P.nb[k] = nbk;
+    P.S[k] = (int)std::ceil(rcut * nbk / dk);
+    // the list kernel's FP32 index arithmetic wants fewer than 2^20 images
+    if (P.S[k] > 48) return fail(GAPX_ERR_ARG, "lattice plane spacing below rcut / 48: too many periodic images");
+  }
+  P.bin_base = 0;
+  P.atom_base = 0;
+  P.natoms = 0;
+  P.pad = 0;
+  return GAPX_OK;
+}
+
+namespace gapx {
+
 static int struct_params(Ctx &c) {
   c.sp.resize(c.nstruct);
   int bin_base = 0;
   for (int s = 0; s < c.nstruct; ++s) {
     StructParams &P = c.sp[s];
-    std::memcpy(P.L, &c.lat[9 * s], 9 * sizeof(double));
-    const double *L = P.L;
-    // Det() of gap_calc.f90:290-299
-    const double det = L[0] * (L[4] * L[8] - L[5] * L[7]) - L[1] * (L[3] * L[8] - L[5] * L[6]) +
-                       L[2] * (L[3] * L[7] - L[6] * L[4]);
-    if (!(std::fabs(det) > 1e-12)) return fail(GAPX_ERR_ARG, "singular lattice");
-    P.vol = std::fabs(det);
-    inv3(P.L, P.Linv);
-    for (int k = 0; k < 3; ++k) {
-      double g = std::sqrt(P.Linv[k] * P.Linv[k] + P.Linv[3 + k] * P.Linv[3 + k] + P.Linv[6 + k] * P.Linv[6 + k]);
-      double dk = 1.0 / g;
-      // bins at least rcut/2 wide (search half-width S = 2 where the cell
-      // allows): on the 4096-atom cell a centre tests ~300 candidates instead
-      // of ~510 with rcut-wide bins, and the neighbour stage takes 6 us less
-      int nbk = (int)std::floor(2.0 * dk / c.rcut);
-      nbk = std::max(1, std::min(nbk, 128));
-      P.nb[k] = nbk;
-      P.S[k] = (int)std::ceil(c.rcut * nbk / dk);
-      // the list kernel's FP32 index arithmetic wants fewer than 2^20 images
-      if (P.S[k] > 48) return fail(GAPX_ERR_ARG, "lattice plane spacing below rcut / 48: too many periodic images");
-    }
+    const int rc = cell_params(&c.lat[9 * s], c.rcut, 128, P);
+    if (rc) return rc;
     P.bin_base = bin_base;
     bin_base += P.nb[0] * P.nb[1] * P.nb[2];
     P.atom_base = c.offsets[s];
@@ -1243,6 +1257,59 @@ static int ctx_relax(Ctx &c, int flags, int max_steps, double fmax, double maxst
   return GAPX_OK;
 }
 
+// Relax every structure of the batch at fixed cell with LBFGS as ASE runs it
+// (relax.hip): each step is one E+F evaluation (graph replay when enabled)
+// and one launch that checks convergence, updates the history and moves the
+// structures that have not converged; the step waits for the device once
+// (list capacities and convergence).  memory is capped at max_steps (the
+// history never holds more entries than steps taken).
+static int ctx_relax_lbfgs(Ctx &c, int flags, int max_steps, double fmax, double maxstep, int memory, double alpha,
+                           int *steps_done, int *n_converged) {
+  if (c.N <= 0) return fail(GAPX_ERR_ARG, "no structures loaded");
+  if (!(fmax > 0) || !(maxstep > 0) || max_steps < 0 || memory < 1 || memory > kLbfgsMaxMemory || !(alpha > 0))
+    return fail(GAPX_ERR_ARG, "gapx_ctx_relax_lbfgs: invalid argument");
+  GAPX_HIP(hipSetDevice(c.device));
+  flags = (flags | GAPX_GRAD) & ~GAPX_LAYOUT_F;
+  const int mem = std::max(1, std::min(memory, max_steps));
+  const size_t n3 = 3 * (size_t)c.N;
+  double *r0, *f0, *S, *Y, *rho, *q;
+  LbfgsState *st;
+  int rc;
+  if ((rc = c.get("lbfgs_r0", n3, &r0)) || (rc = c.get("lbfgs_f0", n3, &f0)) ||
+      (rc = c.get("lbfgs_S", (size_t)mem * n3, &S)) || (rc = c.get("lbfgs_Y", (size_t)mem * n3, &Y)) ||
+      (rc = c.get("lbfgs_rho", (size_t)mem * c.nstruct, &rho)) || (rc = c.get("lbfgs_q", n3, &q)) ||
+      (rc = c.get("lbfgs_state", c.nstruct, &st)))
+    return rc;
+  std::vector<LbfgsState> hs(c.nstruct, LbfgsState{0, 0, 0, 0, 0.0});
+  GAPX_HIP(hipMemcpyAsync(st, hs.data(), sizeof(LbfgsState) * c.nstruct, hipMemcpyHostToDevice, c.stream));
+  const LbfgsParams lp{maxstep, 1.0 / alpha, fmax, mem, 0};
+  int step = 0, conv = 0;
+  for (; step < max_steps; ++step) {
+    if ((rc = ctx_run(c, flags)) || (rc = ctx_settle(c))) return rc;
+    GAPX_HIP(launch_lbfgs_step(c.stream, c.nstruct, c.N, c.ptr<StructParams>("sp"), c.ptr<double>("force"),
+                               c.ptr<double>("pos"), r0, f0, S, Y, rho, q, st, lp));
+    GAPX_HIP(hipMemcpyAsync(hs.data(), st, sizeof(LbfgsState) * c.nstruct, hipMemcpyDeviceToHost, c.stream));
+    GAPX_HIP(hipStreamSynchronize(c.stream));
+    conv = 0;
+    for (const LbfgsState &L : hs) conv += L.converged;
+    if (conv == c.nstruct) break;
+  }
+  // results at the final positions; after max_steps steps, ASE's last
+  // convergence test on them (the kernel moves nothing in check_only mode)
+  if ((rc = ctx_run(c, flags)) || (rc = ctx_settle(c))) return rc;
+  if (step == max_steps) {
+    GAPX_HIP(launch_lbfgs_step(c.stream, c.nstruct, c.N, c.ptr<StructParams>("sp"), c.ptr<double>("force"),
+                               c.ptr<double>("pos"), r0, f0, S, Y, rho, q, st, LbfgsParams{maxstep, 0.0, fmax, mem, 1}));
+    GAPX_HIP(hipMemcpyAsync(hs.data(), st, sizeof(LbfgsState) * c.nstruct, hipMemcpyDeviceToHost, c.stream));
+    GAPX_HIP(hipStreamSynchronize(c.stream));
+    conv = 0;
+    for (const LbfgsState &L : hs) conv += L.converged;
+  }
+  if (steps_done) *steps_done = step;
+  if (n_converged) *n_converged = conv;
+  return GAPX_OK;
+}
+
 // Batched NVT MD: nsteps of ASE's NVTBerendsen over the loaded batch (md.hip
 // has the step).  The forces of the starting positions come from one
 // evaluation; each step is the velocity update and drift (two kernels), an
@@ -1316,7 +1383,6 @@ static int ctx_results(Ctx &c, double *ene, double *force, double *stress6, doub
 }
 
 // Predictive variance over the last run's features (after it settled).
-// Eager: the GEMMs are library calls, kept out of captured graphs.
 static int ctx_variance(Ctx &c, double *var, double *sigma2) {
   if (!c.ran) return fail(GAPX_ERR_ARG, "no evaluation has run on this context");
   int rc = ctx_settle(c);
@@ -1328,8 +1394,8 @@ static int ctx_variance(Ctx &c, double *var, double *sigma2) {
   v.Dp = m.Dp;
   v.Mp = m.Mp;
   v.nstruct = c.nstruct;
-  // K and U blocks of at most 512 MiB each
-  v.rows = (int)std::max<size_t>(64, std::min<size_t>(c.N, ((size_t)1 << 26) / m.Mp));
+  // K in blocks of at most 512 MiB, a multiple of 64 rows
+  v.rows = (int)std::max<size_t>(64, std::min<size_t>(round_up(c.N, 64), ((size_t)1 << 26) / m.Mp) / 64 * 64);
   v.xt64 = fp64 ? c.ptr<double>("xt64") : nullptr;
   v.xt32 = fp64 ? nullptr : c.ptr<float>("xt32");
   if (!v.xt64 && !v.xt32) return fail(GAPX_ERR_ARG, "the last run kept no features");
@@ -1340,12 +1406,11 @@ static int ctx_variance(Ctx &c, double *var, double *sigma2) {
   if ((rc = c.get("var_sigma2", c.N, &v.sigma2)) || (rc = c.get("var_struct", c.nstruct, &v.var))) return rc;
   if (v.qmm) {
     if ((rc = c.get("var_xn", c.N, &v.xn)) || (rc = c.get("var_K", (size_t)v.rows * m.Mp, &v.K)) ||
-        (rc = c.get("var_U", (size_t)v.rows * m.Mp, &v.U)))
+        (rc = c.get("var_partial", (size_t)(m.Mp / 64) * c.N, &v.partial)))
       return rc;
     if (!fp64 && (rc = c.get("var_xw", (size_t)c.N * m.Dp, &v.xw))) return rc;
-    if (!c.blas && (rc = variance_create_handle(&c.blas, c.stream))) return rc;
   }
-  if ((rc = enqueue_variance(c.blas, c.stream, v))) return rc;
+  if ((rc = enqueue_variance(c.stream, v))) return rc;
   if (var) GAPX_HIP(hipMemcpyAsync(var, v.var, sizeof(double) * c.nstruct, hipMemcpyDeviceToHost, c.stream));
   if (sigma2) GAPX_HIP(hipMemcpyAsync(sigma2, v.sigma2, sizeof(double) * c.N, hipMemcpyDeviceToHost, c.stream));
   GAPX_HIP(hipStreamSynchronize(c.stream));
@@ -1542,7 +1607,6 @@ static void ctx_free(Ctx *c) {
   if (c->aux) hipStreamSynchronize(c->aux);
   if (c->aux2) hipStreamSynchronize(c->aux2);
   c->invalidate_graph();
-  variance_destroy_handle(c->blas);
   for (auto &kv : c->mem)
     if (kv.second.p) hipFree(kv.second.p);
   for (auto &e : c->ev)
@@ -2268,6 +2332,13 @@ int gapx_ctx_relax(gapx_ctx *c, int flags, int max_steps, double fmax, double ma
   if (!c) return fail(GAPX_ERR_ARG, "null context");
   collect_timing(*c);
   return ctx_relax(*c, flags, max_steps, fmax, maxstep, steps_done, n_converged);
+}
+
+int gapx_ctx_relax_lbfgs(gapx_ctx *c, int flags, int max_steps, double fmax, double maxstep, int memory,
+                         double alpha, int *steps_done, int *n_converged) {
+  if (!c) return fail(GAPX_ERR_ARG, "null context");
+  collect_timing(*c);
+  return ctx_relax_lbfgs(*c, flags, max_steps, fmax, maxstep, memory, alpha, steps_done, n_converged);
 }
 
 int gapx_ctx_positions(gapx_ctx *c, double *pos) {

@@ -96,6 +96,12 @@ struct StructParams {
   int pad;
 };
 
+// Cell-list parameters of one structure from its lattice rows L9: volume,
+// inverse, bins per axis (at least rcut / 2 wide, at most nb_max) and search
+// half-widths; GAPX_ERR_ARG for a singular lattice or a plane spacing below
+// rcut / 48.  bin_base, atom_base and natoms are left 0.
+int cell_params(const double *L9, double rcut, int nb_max, StructParams &P);
+
 // Thread-local error message.
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
@@ -178,6 +184,13 @@ struct TripletBufs {
 // for angular cutoff rcA, from the same wave.
 hipError_t launch_build_neighbors(hipStream_t s, int N, const int *atom_struct, const StructParams *sp,
                                   double rcut, NeighborBufs nb, double rcA, const TripletBufs *tb);
+// FGET_BOND over the binned structure 0 of sp: *out = the bit pattern of
+// min(init, every |pos_i - (pos_j + n.L)| <= rcut with |n_k| <= nabc_k and
+// (j, n) != (i, 0)), distances in the reference's order of operations;
+// candidates are taken within rsel (>= rcut) of the wrapped positions.
+hipError_t launch_min_bond(hipStream_t s, int N, const StructParams *sp, const double *pos, NeighborBufs nb,
+                           double rsel, double rcut, const int nabc[3], unsigned long long *out,
+                           unsigned long long init);
 // Triplet lists (as launch_build_neighbors makes them) of neighbour slots
 // that are already filled: nb_count, nb_d (and kcap) are read.
 hipError_t launch_triplets_from_slots(hipStream_t s, int N, NeighborBufs nb, double rcA, TripletBufs tb);
@@ -315,6 +328,25 @@ struct FireParams {
 hipError_t launch_fire_step(hipStream_t s, int nstruct, int N, const StructParams *sp, const int *atom_struct,
                             const double *force, double *vel, double *pos, FireState *st, const FireParams &fp);
 
+// Batched LBFGS (relax.hip): per-structure state and parameters.
+constexpr int kLbfgsMaxMemory = 1024;
+struct LbfgsState {
+  int iter;       // steps taken
+  int count;      // history entries held, min(iter - 1, memory) once started
+  int head;       // ring slot of the next entry
+  int converged;
+  double fmax;    // largest |F_i| at the last evaluation
+};
+struct LbfgsParams {
+  double maxstep, h0, fmax;
+  int memory;
+  int check_only;  // only the convergence test (after the last step)
+};
+// S, Y: [memory][3 N]; rho: [nstruct][memory]; r0, f0, q: [3 N].
+hipError_t launch_lbfgs_step(hipStream_t s, int nstruct, int N, const StructParams *sp, const double *force,
+                             double *pos, double *r0, double *f0, double *S, double *Y, double *rho, double *q,
+                             LbfgsState *st, const LbfgsParams &lp);
+
 // Batched NVT MD (md.hip): ASE NVTBerendsen steps in ASE units.
 struct MdParams {
   double dt, temperature, taut, kB;
@@ -333,7 +365,7 @@ hipError_t launch_md_log(hipStream_t s, int nstruct, const StructParams *sp, con
 // last run's standardised features, in blocks of `rows` atoms.
 struct VarianceArgs {
   int N = 0, Dp = 0, Mp = 0, nstruct = 0;
-  int rows = 0;                   // atoms per block of K (K and U hold rows x Mp)
+  int rows = 0;                   // atoms per block of K (a multiple of 64; K holds rows x Mp)
   const double *xt64 = nullptr;   // [N][Dp] the run's features: FP64 run ...
   const float *xt32 = nullptr;    // ... or mixed run
   const double *mt = nullptr, *mn = nullptr;
@@ -341,12 +373,11 @@ struct VarianceArgs {
   const StructParams *sp = nullptr;
   double *xw = nullptr;           // [N][Dp] FP64 copy of xt32
   double *xn = nullptr;           // [N]
-  double *K = nullptr, *U = nullptr;
+  double *K = nullptr;            // [rows][Mp]
+  double *partial = nullptr;      // [Mp / 64][N] per-column-tile sums of the quadratic form
   double *sigma2 = nullptr;       // [N]
   double *var = nullptr;          // [nstruct]
 };
-int variance_create_handle(void **handle, hipStream_t s);
-void variance_destroy_handle(void *handle);
-int enqueue_variance(void *handle, hipStream_t s, const VarianceArgs &v);
+int enqueue_variance(hipStream_t s, const VarianceArgs &v);
 
 }  // namespace gapx

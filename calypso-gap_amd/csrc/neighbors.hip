@@ -472,6 +472,113 @@ __global__ __launch_bounds__(64) void k_triplets_from_slots(int N, NeighborBufs 
   triplets_from_lds(i, K, s_p, rcA, tb, lane);
 }
 
+// |pos_i - (pos_j + n1 L1 + n2 L2 + n3 L3)| in the order of operations of
+// FGET_BOND (gappy/libgap/get_bond.f90:58-60): each product rounded, the
+// sums left to right, no fused multiply-adds.
+__device__ __forceinline__ double bond_distance(const double *pi, const double *pj, int n0, int n1, int n2,
+                                                const double *L) {
+#pragma clang fp contract(off)
+  double dr[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double xyz = ((pj[c] + (double)n0 * L[c]) + (double)n1 * L[3 + c]) + (double)n2 * L[6 + c];
+    dr[c] = pi[c] - xyz;
+  }
+  return sqrt((dr[0] * dr[0] + dr[1] * dr[1]) + dr[2] * dr[2]);
+}
+
+// FGET_BOND over a cell list (structure 0 of sp): one wavefront per atom
+// walks the bin images as k_build does; a candidate within rsel of the
+// wrapped positions gets its image index n relative to the positions as given
+// and, when |n_k| <= nabc_k (the reference's image range), the reference's
+// distance, which enters the minimum when it is at most rcut.  The minimum
+// is taken over the wave (shuffles), over the workgroup's four waves (LDS),
+// and across workgroups by one atomicMin on the bit pattern (positive
+// doubles order like their bits), so the result does not depend on order.
+__global__ __launch_bounds__(256) void k_min_bond(int N, const StructParams *__restrict__ sp,
+                                                  const double *__restrict__ pos, NeighborBufs nb, double rsel,
+                                                  double rcut, int na0, int na1, int na2,
+                                                  unsigned long long *__restrict__ out) {
+  __shared__ double s_best[4];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + wv;
+  double best = 1e300;
+  if (i < N) {
+    const StructParams &P = sp[0];
+    const int b = nb.atom_bin[i] - P.bin_base;
+    const int bi2 = b % P.nb[2], bi1 = (b / P.nb[2]) % P.nb[1], bi0 = b / (P.nb[1] * P.nb[2]);
+    const double xi = nb.wpos[3 * i], yi = nb.wpos[3 * i + 1], zi = nb.wpos[3 * i + 2];
+    const int si0 = nb.shift[3 * i], si1 = nb.shift[3 * i + 1], si2 = nb.shift[3 * i + 2];
+    const double pi[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    const int m1 = 2 * P.S[1] + 1, m2 = 2 * P.S[2] + 1, nimg = (2 * P.S[0] + 1) * m1 * m2;
+    const float inv_m1 = 1.0f / m1, inv_m2 = 1.0f / m2;
+    const float inv_n0 = 1.0f / P.nb[0], inv_n1 = 1.0f / P.nb[1], inv_n2 = 1.0f / P.nb[2];
+    const double2 *sp2 = (const double2 *)nb.spos;
+    const int4 *sinfo = (const int4 *)nb.sinfo;
+    for (int g0 = 0; g0 < nimg; g0 += 64) {
+      const int img = g0 + lane;
+      int lo = 0, n = 0, t0 = 0, t1 = 0, t2 = 0;
+      if (img < nimg) {
+        const int q2 = floordiv_small(img, inv_m2), q1 = floordiv_small(q2, inv_m1);
+        const int o2 = img - q2 * m2 - P.S[2], o1 = q2 - q1 * m1 - P.S[1], o0 = q1 - P.S[0];
+        const int bb0 = bi0 + o0, bb1 = bi1 + o1, bb2 = bi2 + o2;
+        t0 = floordiv_small(bb0, inv_n0);
+        t1 = floordiv_small(bb1, inv_n1);
+        t2 = floordiv_small(bb2, inv_n2);
+        const int w0 = bb0 - t0 * P.nb[0], w1 = bb1 - t1 * P.nb[1], w2 = bb2 - t2 * P.nb[2];
+        const int bin = P.bin_base + (w0 * P.nb[1] + w1) * P.nb[2] + w2;
+        lo = nb.bin_start[bin];
+        n = nb.bin_start[bin + 1] - lo;
+      }
+      int incl = n;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+      }
+      const int total = __shfl(incl, 63, 64);
+      const int first = lo - (incl - n);
+      for (int c0 = 0; c0 < total; c0 += 64) {
+        const int c = c0 + lane;
+        int src = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+          if (__shfl(incl, src + step - 1, 64) <= c) src += step;
+        const int q = __shfl(first, src, 64) + c;
+        const int T0 = __shfl(t0, src, 64), T1 = __shfl(t1, src, 64), T2 = __shfl(t2, src, 64);
+        if (c < total) {
+          const int4 inf = sinfo[q];
+          if (!(T0 == 0 && T1 == 0 && T2 == 0 && inf.x == i)) {
+            const double2 pxy = sp2[2 * (size_t)q], pz = sp2[2 * (size_t)q + 1];
+            const double sx = T0 * P.L[0] + T1 * P.L[3] + T2 * P.L[6];
+            const double sy = T0 * P.L[1] + T1 * P.L[4] + T2 * P.L[7];
+            const double sz = T0 * P.L[2] + T1 * P.L[5] + T2 * P.L[8];
+            const double dx = pxy.x + sx - xi, dy = pxy.y + sy - yi, dz = pz.x + sz - zi;
+            const int n0 = T0 - inf.y + si0, n1 = T1 - inf.z + si1, n2 = T2 - inf.w + si2;
+            if (dx * dx + dy * dy + dz * dz <= rsel * rsel && abs(n0) <= na0 && abs(n1) <= na1 && abs(n2) <= na2) {
+              const double pj[3] = {pos[3 * inf.x], pos[3 * inf.x + 1], pos[3 * inf.x + 2]};
+              const double dis = bond_distance(pi, pj, n0, n1, n2, P.L);
+              if (!(dis > rcut)) best = fmin(best, dis);
+            }
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) best = fmin(best, __shfl_xor(best, o, 64));
+  if (lane == 0) s_best[wv] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double m = fmin(fmin(s_best[0], s_best[1]), fmin(s_best[2], s_best[3]));
+    if (m < 1e300) atomicMin(out, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
 // For slot (i -> j, n) find the slot (j -> i, -n); -1 if absent (a pair within
 // one rounding step of rcut can be seen from one side only).  One wavefront
 // per centre atom, one slot per lane; j's packed keys are read 16 at a time
@@ -757,6 +864,17 @@ hipError_t launch_triplets_from_slots(hipStream_t s, int N, NeighborBufs nb, dou
   if (N <= 0) return hipSuccess;
   if (nb.kcap < 1 || nb.kcap > kMaxNeighbors) return hipErrorInvalidValue;
   k_triplets_from_slots<<<N, 64, 4 * sizeof(float) * (size_t)nb.kcap, s>>>(N, nb, rcA, tb);
+  return hipGetLastError();
+}
+
+hipError_t launch_min_bond(hipStream_t s, int N, const StructParams *sp, const double *pos, NeighborBufs nb,
+                           double rsel, double rcut, const int nabc[3], unsigned long long *out,
+                           unsigned long long init) {
+  if (N <= 0) return hipErrorInvalidValue;
+  k_set_u64<<<1, 64, 0, s>>>(out, init);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_min_bond<<<(N + 3) / 4, 256, 0, s>>>(N, sp, pos, nb, rsel, rcut, nabc[0], nabc[1], nabc[2], out);
   return hipGetLastError();
 }
 

@@ -48,6 +48,7 @@ EXPORTS = [
     "gapx_ctx_positions", "gapx_model_set_qmm", "gapx_ctx_variance",
     "gapx_ctx_md", "gapx_ctx_momenta", "gapx_model_write", "gapx_model_save", "gapx_model_load",
     "gapx_model_read_cached", "gapx_probe_fma_rates", "gapx_model_precision", "gapx_car2acsf",
+    "gapx_ctx_relax_lbfgs",
 ]
 
 
@@ -106,6 +107,7 @@ def lib():
         L.gapx_ctx_export_labels.argtypes = [vp, vp]
         L.gapx_ctx_run_surrogate.argtypes = [vp, vp, ci]
         L.gapx_ctx_relax.argtypes = [vp, ci, ci, cd, cd, vp, vp]
+        L.gapx_ctx_relax_lbfgs.argtypes = [vp, ci, ci, cd, cd, ci, cd, vp, vp]
         L.gapx_ctx_positions.argtypes = [vp, vp]
         L.gapx_model_set_qmm.argtypes = [vp, vp, ci, ci]
         L.gapx_ctx_variance.argtypes = [vp, vp, vp]
@@ -421,12 +423,20 @@ class Context:
         _check(lib().gapx_ctx_get_tuning(self._h, int(flags), *[ctypes.byref(x) for x in v]))
         return dict(zip(("waves_fwd", "waves_bwd", "gpr_rows", "gpr_blocks"), (x.value for x in v)))
 
-    def relax(self, flags=GRAD, max_steps=500, fmax=0.05, maxstep=0.2):
-        """FIRE relaxation of every loaded structure at fixed cell, on the device."""
+    def relax(self, flags=GRAD, max_steps=500, fmax=0.05, maxstep=0.2, optimizer="fire", memory=100, alpha=70.0):
+        """Relaxation of every loaded structure at fixed cell, on the device:
+        optimizer "fire" (ASE's FIRE parameters) or "lbfgs" (ASE's LBFGS,
+        the reference's optimiser; memory and alpha as there)."""
         steps = ctypes.c_int()
         conv = ctypes.c_int()
-        _check(lib().gapx_ctx_relax(self._h, int(flags), int(max_steps), float(fmax), float(maxstep),
-                                    ctypes.byref(steps), ctypes.byref(conv)))
+        if optimizer == "lbfgs":
+            _check(lib().gapx_ctx_relax_lbfgs(self._h, int(flags), int(max_steps), float(fmax), float(maxstep),
+                                              int(memory), float(alpha), ctypes.byref(steps), ctypes.byref(conv)))
+        elif optimizer == "fire":
+            _check(lib().gapx_ctx_relax(self._h, int(flags), int(max_steps), float(fmax), float(maxstep),
+                                        ctypes.byref(steps), ctypes.byref(conv)))
+        else:
+            raise ValueError("optimizer must be 'fire' or 'lbfgs'")
         return dict(steps=steps.value, converged=conv.value)
 
     def positions(self):

@@ -221,6 +221,17 @@ int gapx_ctx_use_graph(gapx_ctx *c, int on);
 int gapx_ctx_relax(gapx_ctx *c, int flags, int max_steps, double fmax, double maxstep, int *steps_done,
                    int *n_converged);
 int gapx_ctx_positions(gapx_ctx *c, double *pos);
+/* Relax every loaded structure at fixed cell with LBFGS as ASE 3.18.1 runs
+ * it, the reference's optimiser (gappy/example/ASE-GAPPY/run_opt.py:108-109:
+ * LBFGS(atoms) at its defaults, run(fmax = -EDIFFG, steps = NSW)): history
+ * of `memory` steps (ASE: 100), initial inverse Hessian 1 / alpha (ASE: 70),
+ * no line search, each step scaled so that no atom moves more than maxstep
+ * (ASE: 0.2 A); a structure stops when max_i |F_i|^2 < fmax^2.  At most
+ * max_steps steps per structure; *steps_done = steps of the slowest
+ * structure, *n_converged = structures converged at their final positions.
+ * gapx_ctx_results / gapx_ctx_positions then give the final state. */
+int gapx_ctx_relax_lbfgs(gapx_ctx *c, int flags, int max_steps, double fmax, double maxstep, int memory,
+                         double alpha, int *steps_done, int *n_converged);
 
 /* Batched NVT molecular dynamics on the device: nsteps of ASE's NVTBerendsen
  * step (ASE 3.18.1), which the reference's MD driver runs for one structure

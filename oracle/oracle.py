@@ -14,6 +14,7 @@ tokens across records until k values are read and then discards the rest of
 the last record; a READ with no items skips one record.
 """
 import ctypes
+import math
 import os
 import subprocess
 from dataclasses import dataclass, field
@@ -319,6 +320,40 @@ def car2acsf(model, pos, neighbor, count, lgrad=True):
     out_d = np.asfortranarray(dxdy.reshape(na, na, 3, D).transpose(3, 0, 1, 2))
     out_s = np.asfortranarray(strs.reshape(na, 3, 3, D).transpose(1, 2, 3, 0))
     return out_x, out_d, out_s
+
+
+def min_bond(lat, pos, rcut, chunk=256):
+    """FGET_BOND restated (gappy/libgap/get_bond.f90:31-78): min(10, the
+    shortest |pos_i - (pos_j + n1 L1 + n2 L2 + n3 L3)| <= rcut over every
+    pair and every image n in [-n_k, n_k], (i, i, 0) skipped), with
+    n_k = ceil(rcut |b_k| / 2 pi) from the reciprocal vectors of the COLUMNS
+    of LAT (get_bond.f90:40-43, 89-98) and the distance in the reference's
+    order of operations (each product rounded, sums left to right)."""
+    lat = np.asarray(lat, dtype=np.float64)
+    pos = np.asarray(pos, dtype=np.float64)
+    col = lat.T  # col[c] = LAT(:, c + 1)
+    rc = np.stack([np.cross(col[1], col[2]), np.cross(col[2], col[0]), np.cross(col[0], col[1])])
+    vol = abs(np.sum(col[0] * rc[0]))
+    nabc = []
+    for r in range(3):
+        v = np.array([rc[0][r], rc[1][r], rc[2][r]]) / vol * 3.141592654 * 2.0
+        nabc.append(int(math.ceil(rcut * math.sqrt(v[0] ** 2 + v[1] ** 2 + v[2] ** 2) / 3.141592654 / 2.0)))
+    best = 10.0
+    na = len(pos)
+    for n1 in range(-nabc[0], nabc[0] + 1):
+        for n2 in range(-nabc[1], nabc[1] + 1):
+            for n3 in range(-nabc[2], nabc[2] + 1):
+                img = ((pos + float(n1) * lat[0]) + float(n2) * lat[1]) + float(n3) * lat[2]  # xyz of every j
+                for i0 in range(0, na, chunk):
+                    dr = pos[i0:i0 + chunk, None, :] - img[None, :, :]
+                    dis = np.sqrt((dr[..., 0] * dr[..., 0] + dr[..., 1] * dr[..., 1]) + dr[..., 2] * dr[..., 2])
+                    if n1 == 0 and n2 == 0 and n3 == 0:
+                        ii = np.arange(i0, min(i0 + chunk, na))
+                        dis[ii - i0, ii] = np.inf
+                    dis = dis[dis <= rcut]
+                    if dis.size:
+                        best = min(best, float(dis.min()))
+    return best
 
 
 def neighbor_set(lat, pos, rcut):

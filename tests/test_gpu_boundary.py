@@ -83,7 +83,38 @@ def test_fget_bond_matches_reference(ref_vectors, case):
     c = [x for x in golden_cases(ref_vectors) if x["name"] == case][0]
     want = ref_vectors["case_%s_min_bond" % case]
     got = [L.fget_bond(c["lat"], c["species"], c["pos"], c["rcut"]), L.fget_bond(c["lat"], c["species"], c["pos"], 0.9)]
-    np.testing.assert_allclose(got, want, rtol=1e-14, atol=0)
+    assert got == list(want)  # bit for bit
+
+
+def bond_cases():
+    from gapx import synth
+    out = []
+    s = synth.structure(1500, 1500 * 11.0, (1, 8, 22), np.random.default_rng(601))
+    out.append(("1500 atoms, three-launch binning", s["lat"], s["pos"], 6.0))
+    shift = np.random.default_rng(602).integers(-2, 3, size=(1500, 3)) @ s["lat"]
+    out.append(("1500 atoms unwrapped by up to two cells", s["lat"], s["pos"] + shift, 6.0))
+    s = synth.structure(3, 3 * 5.0, (1, 8), np.random.default_rng(603), min_dist=1.0)
+    out.append(("3 atoms, cell far smaller than rcut", s["lat"], s["pos"], 6.0))
+    lat = np.array([[9.0, 0.0, 0.0], [7.5, 4.0, 0.0], [-3.0, 2.5, 6.0]])  # strongly sheared
+    pos = np.random.default_rng(604).random((40, 3)) @ lat
+    out.append(("40 atoms, sheared cell", lat, pos, 6.0))
+    out.append(("40 atoms, rcut below every distance", lat, pos, 0.05))
+    return out
+
+
+@pytest.mark.parametrize("k", range(5))
+def test_min_distance_cell_list_equals_restatement(k):
+    """FGET_BOND on the cell list against the restated reference (oracle,
+    itself equal to the compiled reference on the golden cases), bit for bit:
+    many atoms (the three-launch binning), positions outside the cell (the
+    reference's image range counts from the positions as given), a cell far
+    below rcut and a strongly sheared one (the image counts from the columns
+    of LAT, as the reference takes them)."""
+    name, lat, pos, rc = bond_cases()[k]
+    got = gapx.min_distance(lat, pos, rc)
+    want = oracle.min_bond(lat, pos, rc)
+    print(name, got, want)
+    assert got == want
 
 
 def test_fgap_calc_batch_and_status(in_model_dir, ref_vectors):

@@ -100,3 +100,109 @@ def test_relaxation_lowers_energy_and_converges(setup, mixed):
     assert out["converged"] >= len(structs) // 2
     assert np.sum(fm <= 0.05) >= out["converged"]
     ctx.close()
+
+
+def lbfgs_host(m, structs, flags, steps, fmax, maxstep=0.2, memory=100, alpha=70.0):
+    """ASE 3.18.1's LBFGS (ase/optimize/lbfgs.py: LBFGS.step, update,
+    determine_step; Optimizer.converged), restated in numpy for every
+    structure, each step driven by the GPU forces at the host's positions."""
+    ctx = gapx.Context(m)
+    ctx.load(structs, 6.0)
+    pos = np.concatenate([s["pos"] for s in structs]).copy()
+    offs = np.concatenate([[0], np.cumsum([len(s["species"]) for s in structs])])
+    st = [dict(it=0, s=[], y=[], rho=[], r0=None, f0=None, conv=False) for _ in structs]
+    for _ in range(steps):
+        ctx.update(pos)
+        ctx.run(flags)
+        F = ctx.results()["forces"]
+        for k, S in enumerate(st):
+            sl = slice(offs[k], offs[k + 1])
+            f, r = F[sl], pos[sl].copy()
+            if S["conv"] or (f ** 2).sum(axis=1).max() < fmax ** 2:
+                S["conv"] = True
+                continue
+            if S["it"] > 0:
+                s0 = (r - S["r0"]).reshape(-1)
+                y0 = (S["f0"] - f).reshape(-1)
+                S["s"].append(s0)
+                S["y"].append(y0)
+                S["rho"].append(1.0 / np.dot(y0, s0))
+            if S["it"] > memory:
+                S["s"].pop(0)
+                S["y"].pop(0)
+                S["rho"].pop(0)
+            L = min(memory, S["it"])
+            a = np.empty(L)
+            q = -f.reshape(-1)
+            for i in range(L - 1, -1, -1):
+                a[i] = S["rho"][i] * np.dot(S["s"][i], q)
+                q -= a[i] * S["y"][i]
+            z = q / alpha
+            for i in range(L):
+                b = S["rho"][i] * np.dot(S["y"][i], z)
+                z += S["s"][i] * (a[i] - b)
+            dr = -z.reshape(-1, 3)
+            longest = np.sqrt((dr ** 2).sum(1)).max()
+            if longest >= maxstep:
+                dr *= maxstep / longest
+            pos[sl] = r + dr
+            S["it"] += 1
+            S["r0"], S["f0"] = r, f.copy()
+    ctx.close()
+    return pos
+
+
+def test_device_lbfgs_equals_host_lbfgs(setup):
+    """The device LBFGS moves every structure as ASE's LBFGS does (the
+    reference's optimiser) when both are driven by the same forces: positions
+    after 25 steps within 1e-9 A."""
+    m, structs = setup
+    ref = lbfgs_host(m, structs, gapx.GRAD, 25, 1e-6)
+    ctx = gapx.Context(m)
+    ctx.load(structs, 6.0)
+    out = ctx.relax(gapx.GRAD, max_steps=25, fmax=1e-6, optimizer="lbfgs")
+    assert out["steps"] == 25
+    d = np.abs(ctx.positions() - ref).max()
+    print("device vs host LBFGS after 25 steps: max |dx| %.2e A" % d)
+    assert d <= 1e-9
+    ctx.close()
+
+
+def test_lbfgs_history_beyond_its_memory(setup):
+    """With a history of 4 steps the ring buffer wraps several times in 25
+    steps; the device still follows ASE's LBFGS with memory = 4."""
+    m, structs = setup
+    ref = lbfgs_host(m, structs[:4], gapx.GRAD, 25, 1e-6, memory=4)
+    ctx = gapx.Context(m)
+    ctx.load(structs[:4], 6.0)
+    ctx.relax(gapx.GRAD, max_steps=25, fmax=1e-6, optimizer="lbfgs", memory=4)
+    d = np.abs(ctx.positions() - ref).max()
+    print("memory 4: max |dx| %.2e A" % d)
+    assert d <= 1e-9
+    ctx.close()
+
+
+def test_lbfgs_and_fire_reach_low_forces_and_energies(setup):
+    """Both device optimisers relax the same batch to max |F_i| < 0.05 eV/A
+    (ASE's test); LBFGS, the reference's choice, converges in no more steps
+    than FIRE on most structures, and the two relaxed energies are reported
+    per atom."""
+    m, structs = setup
+    res = {}
+    for opt in ("fire", "lbfgs"):
+        ctx = gapx.Context(m)
+        ctx.load(structs, 6.0)
+        ctx.run(gapx.GRAD)
+        e0 = ctx.results()["energy"].copy()
+        ctx.use_graph(True)
+        out = ctx.relax(gapx.GRAD, max_steps=300, fmax=0.05, optimizer=opt)
+        r = ctx.results()
+        res[opt] = dict(out=out, e=r["energy"].copy(), e0=e0)
+        ctx.close()
+        assert np.all(r["energy"] < e0)
+    n = np.array([len(s["species"]) for s in structs])
+    de = (res["lbfgs"]["e"] - res["fire"]["e"]) / n
+    print("fire %s, lbfgs %s; E_lbfgs - E_fire per atom: median %.2e, min %.2e, max %.2e eV"
+          % (res["fire"]["out"], res["lbfgs"]["out"], np.median(de), de.min(), de.max()))
+    assert res["lbfgs"]["out"]["converged"] >= len(structs) // 2
+    assert res["lbfgs"]["out"]["steps"] <= 300
