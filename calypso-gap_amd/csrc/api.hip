@@ -209,27 +209,46 @@ static int upload_model(Model &m) {
   // was fitted on), up to 64 of them, in FP64 with direct differences.  A
   // model whose terms cancel (A * 2^-21 above 1e-4 eV, a tenth of the
   // 1 meV/atom bar) or whose scaled points leave the FP16 range runs its
-  // contraction in FP64 on the mixed path; the descriptors stay FP32 (their
-  // rounding moves E by about 1e-6 eV/atom even on such models).
+  // contraction in FP64 on the mixed path.
+  // The FP32 descriptors themselves: a relative error u in the raw feature
+  // x_d moves xt_d by u |x_d| / theta_d, and E_j to first order by
+  // B_j u with B_j = sum_d |x_jd / theta_d| |dE_j / dxt_d| and
+  // dE_j / dxt_d = sum_k c_k K_jk (mt_kd - mt_jd).  Features that are large
+  // against their length scale theta (sums over about 10^3 neighbours in a
+  // dense cell, with theta fitted on a narrow pool) make B large; at
+  // B * 2^-24 above 1e-4 eV (measured: the FP32 sums err by up to 4 times
+  // that figure, 1.35 meV/atom on a model with B * 2^-24 = 3.4e-4) mixed runs
+  // of the model are FP64 runs.
   double mt_abs = 0.0;
   for (size_t q = 0; q < mt.size(); ++q) mt_abs = std::max(mt_abs, std::fabs(mt[q]));
-  double amax = 0.0;
+  double amax = 0.0, bmax = 0.0;
   const int nprobe = std::min(M, 64);
+  std::vector<double> gj(D);
   for (int p = 0; p < nprobe; ++p) {
     const int j = (nprobe > 1) ? (int)((long long)p * (M - 1) / (nprobe - 1)) : 0;
     double a = 0.0;
+    std::fill(gj.begin(), gj.end(), 0.0);
     for (int k = 0; k < M; ++k) {
       double d2 = 0.0;
       for (int d = 0; d < D; ++d) {
         const double v = mt[(size_t)j * Dp + d] - mt[(size_t)k * Dp + d];
         d2 += v * v;
       }
-      a += std::fabs(m.coeff[k]) * std::exp(-0.5 * d2);
+      const double kv = std::exp(-0.5 * d2);
+      a += std::fabs(m.coeff[k]) * kv;
+      const double w = m.coeff[k] * kv;
+      if (w != 0.0)
+        for (int d = 0; d < D; ++d) gj[d] += w * (mt[(size_t)k * Dp + d] - mt[(size_t)j * Dp + d]);
     }
+    double b = 0.0;
+    for (int d = 0; d < D; ++d) b += std::fabs(mt[(size_t)j * Dp + d] + mu[d]) * std::fabs(gj[d]);
     amax = std::max(amax, a);
+    bmax = std::max(bmax, b);
   }
   m.cancel_probe = amax;
   m.mixed_f64 = (amax * std::ldexp(1.0, -21) > 1e-4) || !(mt_abs <= 16384.0);
+  m.desc_probe = bmax;
+  m.mixed_desc_f64 = !(bmax * std::ldexp(1.0, -24) <= 1e-4);
   std::vector<uint16_t> hi((size_t)Mp * Dp), lo((size_t)Mp * Dp), bhi((size_t)Mp * Dp), blo((size_t)Mp * Dp);
   std::vector<uint16_t> thi((size_t)Mp * Dp), tlo((size_t)Mp * Dp), tbhi((size_t)Mp * Dp), tblo((size_t)Mp * Dp);
   // Feature-major copies: within every group of 32 points, point p sits at
@@ -1023,11 +1042,18 @@ static int enqueue_surrogate(Ctx &c, Mlp &mlp, int flags) {
 // is captured once per (flags, atom count, list capacities) and replayed: an
 // eager run first brings every workspace buffer to its final size, so the
 // capture itself allocates nothing.
+// The flags a run of this model executes with: a model whose FP32 features
+// cannot hold the mixed path's bar (upload_model) runs GAPX_MIXED as FP64.
+static int run_flags(const Model &m, int flags) {
+  return m.mixed_desc_f64 ? (flags & ~(GAPX_MIXED | GAPX_BF16)) : flags;
+}
+
 static int ctx_run(Ctx &c, int flags, Mlp *mlp = nullptr) {
   if (c.N <= 0) return fail(GAPX_ERR_ARG, "no structures loaded");
   GAPX_HIP(hipSetDevice(c.device));
   int rc;
   c.mlp = mlp;
+  if (!mlp) flags = run_flags(*c.m, flags);
   if (c.mlp) {  // surrogate runs are eager
     if ((rc = enqueue_surrogate(c, *c.mlp, flags))) return rc;
   } else if (c.use_graph && !c.timing) {
@@ -1133,6 +1159,7 @@ static void reset_timing(Ctx &c) {
 
 static int ctx_autotune(Ctx &c, int flags, bool force) {
   if (c.N <= 0) return fail(GAPX_ERR_ARG, "no structures loaded");
+  flags = run_flags(*c.m, flags);
   const int p = (flags & GAPX_MIXED) ? 1 : 0;
   const std::string key = tune_key(c, flags);
   if (!force) {
@@ -1997,10 +2024,13 @@ int gapx_model_header(const gapx_model *m, int *z, double *w, int *sf_type, doub
   return GAPX_OK;
 }
 
-int gapx_model_precision(const gapx_model *m, double *cancellation, int *mixed_contraction_fp64) {
+int gapx_model_precision(const gapx_model *m, double *cancellation, int *mixed_contraction_fp64,
+                         double *feature_sensitivity, int *mixed_runs_fp64) {
   if (!m) return fail(GAPX_ERR_ARG, "null model");
   if (cancellation) *cancellation = m->cancel_probe;
-  if (mixed_contraction_fp64) *mixed_contraction_fp64 = m->mixed_f64 ? 1 : 0;
+  if (mixed_contraction_fp64) *mixed_contraction_fp64 = (m->mixed_f64 || m->mixed_desc_f64) ? 1 : 0;
+  if (feature_sensitivity) *feature_sensitivity = m->desc_probe;
+  if (mixed_runs_fp64) *mixed_runs_fp64 = m->mixed_desc_f64 ? 1 : 0;
   return GAPX_OK;
 }
 
@@ -2357,7 +2387,7 @@ int gapx_ctx_autotune(gapx_ctx *c, int flags, int force) {
 
 int gapx_ctx_get_tuning(gapx_ctx *c, int flags, int *waves_fwd, int *waves_bwd, int *gpr_rows, int *gpr_blocks) {
   if (!c) return fail(GAPX_ERR_ARG, "null context");
-  const Tuning &t = c->tune[(flags & GAPX_MIXED) ? 1 : 0];
+  const Tuning &t = c->tune[(run_flags(*c->m, flags) & GAPX_MIXED) ? 1 : 0];
   if (waves_fwd) *waves_fwd = t.waves_fwd;
   if (waves_bwd) *waves_bwd = t.waves_bwd;
   if (gpr_rows) *gpr_rows = t.gpr_rows;
@@ -2371,7 +2401,7 @@ int gapx_ctx_set_tuning(gapx_ctx *c, int flags, int waves_fwd, int waves_bwd, in
   if (!ok_w(waves_fwd) || !ok_w(waves_bwd) || (gpr_rows != 64 && gpr_rows != 128) || gpr_blocks < 1 ||
       gpr_blocks > (1 << 20))
     return fail(GAPX_ERR_ARG, "gapx_ctx_set_tuning: waves must be 1, 2 or 4, rows 64 or 128, blocks in [1, 2^20]");
-  Tuning &t = c->tune[(flags & GAPX_MIXED) ? 1 : 0];
+  Tuning &t = c->tune[(run_flags(*c->m, flags) & GAPX_MIXED) ? 1 : 0];
   t.waves_fwd = waves_fwd;
   t.waves_bwd = waves_bwd;
   t.gpr_rows = gpr_rows;

@@ -79,6 +79,8 @@ struct Model {
   float w_scale = 1024.f;                // power of 2 applied to W before its FP16 split
   double cancel_probe = 0.0;             // largest sum_k |W_jk| over the probed sparse points j
   bool mixed_f64 = false;                // mixed runs use the FP64 contraction (see upload_model)
+  double desc_probe = 0.0;               // largest first-order dE_j per unit relative feature error
+  bool mixed_desc_f64 = false;           // mixed runs are FP64 runs (FP32 features too coarse)
   size_t dev_bytes = 0;
   struct Ctx *cached_ctx = nullptr;      // workspace for gapx_calc
 };

@@ -220,6 +220,10 @@ __global__ __launch_bounds__(256) void k_lbfgs_step(int nstruct, const StructPar
     const double c = s_a[k] - bk;
     for (int e = t; e < n3; e += 256) q[b0 + e] += c * Sk[e];
   }
+  // Until here every thread read and wrote only its own elements of q; the
+  // per-atom lengths below read the three components of an atom, which
+  // other threads wrote.
+  __syncthreads();
   // step p = -z, scaled to maxstep by its longest per-atom displacement
   double lm = 0.0;
   for (int a = t; a < P.natoms; a += 256) {

@@ -75,7 +75,7 @@ def lib():
         L.gapx_model_arrays.argtypes = [vp, vp, vp, ci, ci, vp]
         L.gapx_model_header.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.gapx_model_free.argtypes = [vp]
-        L.gapx_model_precision.argtypes = [vp, vp, vp]
+        L.gapx_model_precision.argtypes = [vp, vp, vp, vp, vp]
         L.gapx_model_free.restype = None
         L.gapx_file_stamp.argtypes = [ctypes.c_char_p, vp, vp]
         L.gapx_calc.argtypes = [vp, ci, vp, vp, vp, cd, ci, vp, vp, vp, vp]
@@ -270,11 +270,15 @@ class Model:
         return ene, [force[offs[k]:offs[k + 1]] for k in range(len(structures))], stress
 
     def precision(self):
-        """(cancellation probe A, whether GAPX_MIXED runs contract in FP64)."""
-        a = ctypes.c_double()
-        f = ctypes.c_int()
-        _check(lib().gapx_model_precision(self._h, ctypes.byref(a), ctypes.byref(f)))
-        return dict(cancellation=a.value, mixed_contraction_fp64=bool(f.value))
+        """How GAPX_MIXED runs treat the model: the cancellation probe A and
+        whether they contract in FP64; the feature sensitivity B and whether
+        they run wholly in FP64."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        f, g = ctypes.c_int(), ctypes.c_int()
+        _check(lib().gapx_model_precision(self._h, ctypes.byref(a), ctypes.byref(f), ctypes.byref(b),
+                                          ctypes.byref(g)))
+        return dict(cancellation=a.value, mixed_contraction_fp64=bool(f.value), feature_sensitivity=b.value,
+                    mixed_runs_fp64=bool(g.value))
 
     def set_qmm(self, qmm):
         """Q of the predictive variance (nsparse x nsparse), or None for Q = 0."""

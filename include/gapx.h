@@ -97,8 +97,14 @@ void gapx_model_free(gapx_model *m);
  * model's own sparse points j (FP64, at load); mixed_contraction_fp64 = 1
  * when A * 2^-21 exceeds 1e-4 eV or a scaled sparse-point component exceeds
  * 2^14, and then GAPX_MIXED runs keep FP32 descriptors but contract in FP64.
- * Either pointer may be NULL. */
-int gapx_model_precision(const gapx_model *m, double *cancellation, int *mixed_contraction_fp64);
+ * The FP32 features: a relative error u of feature d moves E_j by about
+ * u |x_jd / theta_d| |dE_j/dxt_d|; feature_sensitivity = the largest sum of
+ * these over d at the same probed points (u = 1), and mixed_runs_fp64 = 1
+ * when feature_sensitivity * 2^-24 exceeds 1e-4 eV: GAPX_MIXED runs of the
+ * model are then FP64 runs (features, contraction and backward; results
+ * equal the FP64 path's).  Any pointer may be NULL. */
+int gapx_model_precision(const gapx_model *m, double *cancellation, int *mixed_contraction_fp64,
+                         double *feature_sensitivity, int *mixed_runs_fp64);
 
 /* Model I/O beyond FGAP_READ (gap_calc.f90:303-364 reads the text once per
  * call and keeps no cache).

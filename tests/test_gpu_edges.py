@@ -158,6 +158,18 @@ def dense_cell(vpa, seed):
     return synth.structure(8, 8 * vpa, (1, 8, 22), np.random.default_rng(seed), cell="cubic", min_dist=0.9)
 
 
+def dense_model(s, mixed):
+    """A model fitted on the dense cell's own descriptors (kernel values
+    O(1)).  theta = 1.5 std over the cell's 8 atoms makes some features 4e4
+    times their length scale; with mixed=True theta also carries 2^-10 of the
+    feature's mean, which keeps the FP32 features within the mixed bar."""
+    om = fitted_small_model(np.random.default_rng(42), s)
+    if mixed:
+        x = oracle.descriptors(om, s["species"], s["lat"], s["pos"], 6.0)
+        om.theta = x.std(axis=0) * 1.5 + 2.0 ** -10 * np.abs(x.mean(axis=0)) + 1e-3
+    return om
+
+
 @pytest.mark.parametrize("mixed", [False, True], ids=["fp64", "mixed"])
 def test_above_the_reference_cap_without_it(mixed):
     """About 1090 neighbours per atom (8 atoms at 0.83 A^3 each) with a model
@@ -165,11 +177,13 @@ def test_above_the_reference_cap_without_it(mixed):
     (GAPX_REF_NEIGHBOR_CAP) stop above 1000 like the reference
     (gap_calc.f90:107-111); without the flag the lists are taken whole and
     match the oracle's sparse mode (FP64 bar, the stress to 1e-8 of its
-    5.6e5 GPa: a virial of some 10^4 terms, measured 2.5e-9; mixed within
-    1 meV/atom and 20 meV/A of FP64)."""
+    5.6e5 GPa: a virial of some 10^4 terms, measured 2.5e-9; mixed, on the
+    16-bit split contraction with FP32 features, within 1 meV/atom and
+    20 meV/A of FP64)."""
     s = dense_cell(0.83, 41)
-    om = fitted_small_model(np.random.default_rng(42), s)
+    om = dense_model(s, mixed)
     m = gapx.Model.from_model(om)
+    assert not m.precision()["mixed_runs_fp64"]
     ctx = gapx.Context(m)
     ctx.load([s], 6.0)
     ctx.run(gapx.GRAD | (gapx.MIXED if mixed else 0))
@@ -183,6 +197,7 @@ def test_above_the_reference_cap_without_it(mixed):
     e64, f64, s64, _ = m.calc(s["species"], s["lat"], s["pos"], 6.0)
     if mixed:
         r = m.calc(s["species"], s["lat"], s["pos"], 6.0, mixed=True)
+        print("dense 1090 mixed vs fp64: dE/atom %.3e dF %.3e" % (abs(r[0] - e64) / 8, np.abs(r[1] - f64).max()))
         assert abs(r[0] - e64) / 8 <= 1e-3 and np.abs(r[1] - f64).max() <= 2e-2
     else:
         ref = oracle.gap_calc(om, s["species"], s["lat"], s["pos"], 6.0, mode="sparse", nthreads=8)
@@ -190,6 +205,22 @@ def test_above_the_reference_cap_without_it(mixed):
         assert abs(e64 - ref["energy"]) / 8 <= TOL_E
         assert np.abs(f64 - ref["forces"]).max() <= TOL_F
         assert np.abs(s64 - ref["stress"]).max() <= max(TOL_S, 1e-8 * np.abs(ref["stress"]).max())
+    m.close()
+
+
+def test_mixed_runs_of_a_badly_scaled_model_are_fp64_runs():
+    """The same cell under the model whose features reach 4e4 times theta:
+    its FP32 features would move E by about 1.4 meV/atom (measured before the
+    probe routed it), so the model is recognised at load (B 2^-24 = 3.4e-4
+    eV) and GAPX_MIXED runs equal FP64 runs bit for bit."""
+    s = dense_cell(0.83, 41)
+    m = gapx.Model.from_model(dense_model(s, False))
+    prec = m.precision()
+    assert prec["mixed_runs_fp64"] and prec["feature_sensitivity"] * 2.0 ** -24 > 1e-4, prec
+    e64, f64, s64, _ = m.calc(s["species"], s["lat"], s["pos"], 6.0)
+    for flags in (gapx.MIXED, gapx.MIXED | gapx.BF16):
+        r = m.calc(s["species"], s["lat"], s["pos"], 6.0, flags=flags)
+        assert r[0] == e64 and np.array_equal(r[1], f64) and np.array_equal(r[2], s64)
     m.close()
 
 

@@ -34,6 +34,26 @@ def errors(a, b, n):
     return (abs(a[0] - b[0]) / n, float(np.abs(a[1] - b[1]).max()), float(np.abs(a[2] - b[2]).max()))
 
 
+def probes(om):
+    """The model-load probes, restated in numpy: A = max_j sum_k |c_k| K_jk
+    and B = max_j sum_d |mm_jd / theta_d| |sum_k c_k K_jk (mt_kd - mt_jd)| over
+    the same (up to 64 evenly spaced) sparse points j, with mt = mm / theta
+    centred by its column mean."""
+    th, mm, c = np.asarray(om.theta, float), np.asarray(om.mm, float), np.asarray(om.coeff, float)
+    m1 = mm / th
+    mt = m1 - m1.mean(axis=0)
+    M = len(mm)
+    npb = min(M, 64)
+    a = b = 0.0
+    for p in range(npb):
+        j = p * (M - 1) // (npb - 1) if npb > 1 else 0
+        K = np.exp(-0.5 * ((mt[j] - mt) ** 2).sum(axis=1))
+        a = max(a, float((np.abs(c) * K).sum()))
+        g = ((c * K)[:, None] * (mt - mt[j])).sum(axis=0)
+        b = max(b, float((np.abs(m1[j]) * np.abs(g)).sum()))
+    return a, b
+
+
 @pytest.mark.parametrize("bf16", [False, True], ids=["fp16x3", "bf16x3"])
 def test_mixed_vs_fp64_ternary4k(bf16):
     from gapx import synth
@@ -58,13 +78,20 @@ def test_mixed_reference_vectors(ref_vectors, case, bf16):
     to about 3e5, so FP32 kernel values (relative error about 2^-21) would
     leave 0.1-1 meV/atom.  The model is recognised at load (its cancellation
     probe is above 210) and its mixed runs contract in FP64 with FP32
-    descriptors; the fitted workload models (probe about 1) use the 16-bit
-    split contraction.
+    descriptors.  The two models fitted on cells at 1 A^3 per atom have
+    features 4e4-1.6e5 times their length scales theta (sums over about 900
+    neighbours), so their FP32 features alone can move E by about
+    B 2^-24 = 1.1e-4 and 3.2e-4 eV: their mixed runs are FP64 runs.  The
+    fitted ternary model (probes about 1) uses the 16-bit split contraction.
     """
     c = [x for x in golden_cases(ref_vectors) if x["name"] == case][0]
-    m = gapx.Model.from_model(golden_model(ref_vectors, c["model"]))
+    om = golden_model(ref_vectors, c["model"])
+    m = gapx.Model.from_model(om)
     prec = m.precision()
-    assert prec["mixed_contraction_fp64"] == (c["model"] == "example"), prec
+    a, b = probes(om)
+    assert abs(prec["cancellation"] - a) <= 1e-9 * a and abs(prec["feature_sensitivity"] - b) <= 1e-9 * b, (prec, a, b)
+    assert prec["mixed_runs_fp64"] == (c["model"] in ("example_dense", "ternary_dense")), prec
+    assert prec["mixed_contraction_fp64"] == (c["model"] != "ternary"), prec
     r = m.calc(c["species"], c["lat"], c["pos"], c["rcut"], grad=c["lgrad"], mixed=True,
                flags=gapx.BF16 if bf16 else 0)
     de, df, ds = errors(r, (c["energy"], c["forces"], c["stress"]), len(c["species"]))
