@@ -40,7 +40,7 @@ EXPORTS = [
     "gapx_calc", "gapx_calc_batch", "gapx_descriptors", "gapx_ctx_create", "gapx_ctx_load",
     "gapx_ctx_update", "gapx_ctx_run", "gapx_ctx_sync", "gapx_ctx_results", "gapx_ctx_features",
     "gapx_ctx_timing", "gapx_ctx_stage_count", "gapx_ctx_stage_info", "gapx_ctx_use_graph",
-    "gapx_ctx_device_bytes", "gapx_ctx_free", "gapx_ctx_counts", "gapx_min_distance",
+    "gapx_ctx_device_bytes", "gapx_ctx_free", "gapx_ctx_counts", "gapx_min_distance", "gapx_min_distance_noalloc",
     "gapx_ctx_autotune", "gapx_ctx_get_tuning", "gapx_ctx_set_tuning",
     "gapx_mlp_param_count", "gapx_mlp_create", "gapx_mlp_free", "gapx_mlp_get_params", "gapx_mlp_set_params",
     "gapx_mlp_forward", "gapx_mlp_loss_grad", "gapx_mlp_data_add", "gapx_mlp_data_clear", "gapx_mlp_data_count",
@@ -126,6 +126,7 @@ def lib():
         L.gapx_ctx_device_bytes.argtypes = [vp, vp]
         L.gapx_ctx_counts.argtypes = [vp, vp, vp]
         L.gapx_min_distance.argtypes = [ci, vp, vp, cd, ci, vp]
+        L.gapx_min_distance_noalloc.argtypes = [ci, vp, vp, cd, ci, vp]
         L.gapx_car2acsf.argtypes = [vp, ci, ci, ci, vp, vp, vp, vp, vp, vp, ci]
         L.gapx_ctx_free.argtypes = [vp]
         L.gapx_ctx_free.restype = None

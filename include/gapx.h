@@ -178,6 +178,19 @@ int gapx_car2acsf(gapx_model *m, int na, int max_neighbor, int nf, const double 
  * lat/pos in the layout given by flags (GAPX_LAYOUT_F or C). */
 int gapx_min_distance(int na, const double *lat, const double *pos, double rcut, int flags, double *dmin);
 
+/* gapx_min_distance without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight screening loops, harnesses that forbid hipMalloc).  Same
+ * arguments and result as gapx_min_distance (bit for bit); the workspace is
+ * the library's fixed 16 MiB device array, one per GPU, kept until the
+ * process ends.  Inputs whose workspace (staged positions, bins, binned
+ * copies: 148 bytes per atom plus 8 to 12 per bin) exceeds it are refused
+ * with GAPX_ERR_ARG before any GPU work, roughly above 110 000 atoms.  Every call
+ * on one GPU works in that GPU's single array: calls that overlap on one GPU
+ * (from several host threads) race; serialise them. */
+int gapx_min_distance_noalloc(int na, const double *lat, const double *pos, double rcut, int flags,
+                              double *dmin);
+
 /* ---- device-resident contexts ----------------------------------------------
  * A context owns two HIP streams (its work stream, and a second one joined
  * inside each run) and device buffers for a batch of structures.
