@@ -804,14 +804,14 @@ static int enqueue_forces(Ctx &c, Bufs &b, bool fp64, bool grad, const double *e
   double *part_sum;
   int *ticket, *stats_acc;
   if ((rc = c.get("red_sum", 8 * (size_t)c.red_parts, &part_sum)) || (rc = c.get("red_ticket", c.nstruct, &ticket)) ||
-      (rc = c.get("stats_acc", 8, &stats_acc)))
+      (rc = c.get("stats_acc", 16, &stats_acc)))
     return rc;
   {
     // the tickets and accumulators stay zero between runs; clear them once
     // after they are (re)allocated
     const unsigned long long g = ((unsigned long long)c.mem["red_ticket"].gen << 32) | c.mem["stats_acc"].gen;
     if (g != c.red_clean) {
-      GAPX_HIP(launch_zero_ints(c.stream, ticket, (int)(c.mem["red_ticket"].cap / sizeof(int)), stats_acc, 8,
+      GAPX_HIP(launch_zero_ints(c.stream, ticket, (int)(c.mem["red_ticket"].cap / sizeof(int)), stats_acc, 16,
                                 nullptr, 0));
       c.red_clean = g;
     }
@@ -821,6 +821,7 @@ static int enqueue_forces(Ctx &c, Bufs &b, bool fp64, bool grad, const double *e
   rb.part_sum = part_sum;
   rb.ticket = ticket;
   rb.stats_acc = stats_acc;
+  rb.sticky = stats_acc + 8;
   if (grad) GAPX_HIP(hipStreamWaitEvent(c.stream, c.join_ev, 0));
   stage_begin(c, kForces);
   GAPX_HIP(launch_forces_reduce(c.stream, c.N, grad ? 1 : 0, nb, b.slot_f, b.force, c.nstruct, d_sp, eatom,
@@ -1102,24 +1103,35 @@ static int ctx_run(Ctx &c, int flags, Mlp *mlp = nullptr) {
 // At most two reruns are needed: the raw neighbour counts do not depend on
 // kcap, but the triplet counts of a run whose lists were cut at kcap do, so
 // the rerun with the full lists can still overflow tcap once.
+// Verdict on list statistics h (the layout of launch_forces_reduce): GAPX_OK
+// with *grow false when every list fitted; GAPX_OK with *grow true after
+// raising kcap and/or tcap to the largest counts plus the given headroom
+// (the graph is invalidated); an error when a count is above the cap.
+static int capacity_verdict(Ctx &c, const int *h, int kroom, int troom, bool *grow) {
+  *grow = false;
+  const int cap = neighbor_cap(c, c.last_flags);
+  if (h[1] > cap) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "an atom has %d neighbours within rcut, above the limit of %d", h[1], cap);
+    return fail(GAPX_ERR_NEIGHBORS, buf);
+  }
+  if (h[3]) return fail(GAPX_ERR_NEIGHBORS, "descriptor kernels saw more neighbours than their LDS layout holds");
+  if (h[2] == 0 && h[6] == 0) return GAPX_OK;
+  if (h[2]) c.kcap = round_up(std::min(h[1] + kroom, std::max(cap, h[1] + 8)), 8);
+  if (h[6]) c.tcap = round_up(h[5] + troom, 64);
+  c.invalidate_graph();
+  *grow = true;
+  return GAPX_OK;
+}
+
 static int ctx_settle(Ctx &c) {
   GAPX_HIP(hipStreamSynchronize(c.stream));
   collect_timing(c);
   for (int attempt = 0; attempt < 4; ++attempt) {
-    const int *h = c.host_stats;
-    const int cap = neighbor_cap(c, c.last_flags);
-    if (h[1] > cap) {
-      char buf[160];
-      std::snprintf(buf, sizeof buf, "an atom has %d neighbours within rcut, above the limit of %d", h[1], cap);
-      return fail(GAPX_ERR_NEIGHBORS, buf);
-    }
-    if (h[3]) return fail(GAPX_ERR_NEIGHBORS, "descriptor kernels saw more neighbours than their LDS layout holds");
-    if (h[2] == 0 && h[6] == 0) return GAPX_OK;
-    if (h[2]) c.kcap = round_up(h[1] + 8, 8);
-    if (h[6]) c.tcap = round_up(h[5] + 64, 64);
-    c.invalidate_graph();
-    int rc = ctx_run(c, c.last_flags, c.mlp);
-    if (rc) return rc;
+    bool grow;
+    int rc = capacity_verdict(c, c.host_stats, 8, 64, &grow);
+    if (rc || !grow) return rc;
+    if ((rc = ctx_run(c, c.last_flags, c.mlp))) return rc;
     GAPX_HIP(hipStreamSynchronize(c.stream));
     collect_timing(c);
   }
@@ -1342,7 +1354,17 @@ static int ctx_relax_lbfgs(Ctx &c, int flags, int max_steps, double fmax, double
 // evaluation; each step is the velocity update and drift (two kernels), an
 // E+F evaluation (graph replay when enabled), and the second half-kick.
 // Optional log rows: [nsteps + 1][nstruct][E_pot, E_kin, T].
+//
+// The host does not wait for every step's list statistics.  Steps run in
+// blocks of kMdCheckEvery; every evaluation folds its statistics into a
+// running maximum on the device (ReduceBufs::sticky), which the host reads
+// once per block.  Capacities carry headroom (an eighth of the largest
+// count) so that moving atoms rarely outgrow them; when a block's lists did
+// overflow, the positions, momenta, forces and energies saved at the block's
+// start are restored and the block is run again with the larger capacities,
+// so the trajectory is the one every-step checking would give.
 static constexpr double kBoltzmannEv = 8.6173303e-5;  // eV/K, CODATA 2014 (ASE 3.18.1 units)
+static constexpr int kMdCheckEvery = 16;
 
 static int ctx_md(Ctx &c, int flags, int nsteps, double dt, double temperature, double taut, int fixcm,
                   const double *masses, const double *momenta, double *log) {
@@ -1353,33 +1375,79 @@ static int ctx_md(Ctx &c, int flags, int nsteps, double dt, double temperature, 
     if (!(masses[i] > 0)) return fail(GAPX_ERR_ARG, "gapx_ctx_md: masses must be positive");
   GAPX_HIP(hipSetDevice(c.device));
   flags = (flags | GAPX_GRAD) & ~GAPX_LAYOUT_F;
-  double *mass, *p, *coef, *dlog = nullptr;
+  const size_t n3 = 3 * (size_t)c.N;
+  double *mass, *p, *coef, *dlog = nullptr, *ck_x, *ck_p, *ck_f, *ck_e;
   int rc;
-  if ((rc = c.get("md_mass", c.N, &mass)) || (rc = c.get("md_p", 3 * (size_t)c.N, &p)) ||
-      (rc = c.get("md_coef", 4 * (size_t)c.nstruct, &coef)))
+  if ((rc = c.get("md_mass", c.N, &mass)) || (rc = c.get("md_p", n3, &p)) ||
+      (rc = c.get("md_coef", 4 * (size_t)c.nstruct, &coef)) || (rc = c.get("md_ck_x", n3, &ck_x)) ||
+      (rc = c.get("md_ck_p", n3, &ck_p)) || (rc = c.get("md_ck_f", n3, &ck_f)) ||
+      (rc = c.get("md_ck_e", c.nstruct, &ck_e)))
     return rc;
   if (log && (rc = c.get("md_log", 3 * (size_t)c.nstruct * (nsteps + 1), &dlog))) return rc;
   GAPX_HIP(hipMemcpyAsync(mass, masses, sizeof(double) * c.N, hipMemcpyHostToDevice, c.stream));
   if (momenta)
-    GAPX_HIP(hipMemcpyAsync(p, momenta, sizeof(double) * 3 * c.N, hipMemcpyHostToDevice, c.stream));
+    GAPX_HIP(hipMemcpyAsync(p, momenta, sizeof(double) * n3, hipMemcpyHostToDevice, c.stream));
   else
-    GAPX_HIP(launch_fill_double(c.stream, p, 3 * (size_t)c.N, 0.0));
+    GAPX_HIP(launch_fill_double(c.stream, p, n3, 0.0));
   const MdParams mp{dt, temperature, taut, kBoltzmannEv, fixcm ? 1 : 0, 0};
   const StructParams *d_sp = c.ptr<StructParams>("sp");
   const int *d_as = c.ptr<int>("astruct");
   if ((rc = ctx_run(c, flags)) || (rc = ctx_settle(c))) return rc;
+  {
+    // headroom for the trajectory (no rerun: the forces just computed fitted)
+    const int *h = c.host_stats;
+    const int cap = neighbor_cap(c, c.last_flags);
+    const int k = round_up(std::min(h[1] + std::max(8, h[1] / 8), std::max(cap, h[1])), 8);
+    const int t = round_up(h[5] + std::max(64, h[5] / 8), 64);
+    if (k > c.kcap || t > c.tcap) {
+      c.kcap = std::max(c.kcap, k);
+      c.tcap = std::max(c.tcap, t);
+      c.invalidate_graph();
+    }
+  }
   if (dlog) GAPX_HIP(launch_md_log(c.stream, c.nstruct, d_sp, mass, p, c.ptr<double>("force"), c.ptr<double>("ene"),
                                    mp.kB, dlog));
-  for (int step = 0; step < nsteps; ++step) {
-    double *x = c.ptr<double>("pos"), *f = c.ptr<double>("force");
-    GAPX_HIP(launch_md_prepare(c.stream, c.nstruct, d_sp, mass, p, f, mp, coef));
-    GAPX_HIP(launch_md_drift(c.stream, c.N, d_as, mass, f, coef, dt, p, x));
-    if ((rc = ctx_run(c, flags)) || (rc = ctx_settle(c))) return rc;
-    f = c.ptr<double>("force");
-    GAPX_HIP(launch_md_kick(c.stream, c.N, f, 0.5 * dt, p));
-    if (dlog)
-      GAPX_HIP(launch_md_log(c.stream, c.nstruct, d_sp, mass, p, f, c.ptr<double>("ene"), mp.kB,
-                             dlog + 3 * (size_t)c.nstruct * (step + 1)));
+  int hs[8];
+  for (int b0 = 0, redo = 0; b0 < nsteps;) {
+    const int nb = std::min(kMdCheckEvery, nsteps - b0);
+    // the block's starting state, and a cleared running maximum
+    GAPX_HIP(hipMemcpyAsync(ck_x, c.ptr<double>("pos"), sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    GAPX_HIP(hipMemcpyAsync(ck_p, p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    GAPX_HIP(hipMemcpyAsync(ck_f, c.ptr<double>("force"), sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    GAPX_HIP(hipMemcpyAsync(ck_e, c.ptr<double>("ene"), sizeof(double) * c.nstruct, hipMemcpyDeviceToDevice,
+                            c.stream));
+    int *sticky = c.ptr<int>("stats_acc") + 8;
+    GAPX_HIP(launch_fill_int(c.stream, sticky, 8, 0));
+    for (int step = b0; step < b0 + nb; ++step) {
+      double *x = c.ptr<double>("pos"), *f = c.ptr<double>("force");
+      GAPX_HIP(launch_md_prepare(c.stream, c.nstruct, d_sp, mass, p, f, mp, coef));
+      GAPX_HIP(launch_md_drift(c.stream, c.N, d_as, mass, f, coef, dt, p, x));
+      if ((rc = ctx_run(c, flags))) return rc;
+      f = c.ptr<double>("force");
+      GAPX_HIP(launch_md_kick(c.stream, c.N, f, 0.5 * dt, p));
+      if (dlog)
+        GAPX_HIP(launch_md_log(c.stream, c.nstruct, d_sp, mass, p, f, c.ptr<double>("ene"), mp.kB,
+                               dlog + 3 * (size_t)c.nstruct * (step + 1)));
+    }
+    sticky = c.ptr<int>("stats_acc") + 8;
+    GAPX_HIP(hipMemcpyAsync(hs, sticky, sizeof hs, hipMemcpyDeviceToHost, c.stream));
+    GAPX_HIP(hipStreamSynchronize(c.stream));
+    collect_timing(c);
+    bool grow;
+    if ((rc = capacity_verdict(c, hs, std::max(8, hs[1] / 8), std::max(64, hs[5] / 8), &grow))) return rc;
+    if (!grow) {
+      b0 += nb;
+      redo = 0;
+      continue;
+    }
+    if (++redo > 3) return fail(GAPX_ERR_NEIGHBORS, "neighbour capacity did not converge during MD");
+    // back to the block's start (buffers of the run's size class are not
+    // reallocated by a capacity change, so the saved state still applies)
+    GAPX_HIP(hipMemcpyAsync(c.ptr<double>("pos"), ck_x, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    GAPX_HIP(hipMemcpyAsync(p, ck_p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    GAPX_HIP(hipMemcpyAsync(c.ptr<double>("force"), ck_f, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    GAPX_HIP(hipMemcpyAsync(c.ptr<double>("ene"), ck_e, sizeof(double) * c.nstruct, hipMemcpyDeviceToDevice,
+                            c.stream));
   }
   if (log)
     GAPX_HIP(hipMemcpyAsync(log, dlog, sizeof(double) * 3 * c.nstruct * (nsteps + 1), hipMemcpyDeviceToHost,

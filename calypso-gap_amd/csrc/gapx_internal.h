@@ -265,6 +265,9 @@ struct ReduceBufs {
   double *part_sum;      // [nparts][8]
   int *ticket;           // [nstruct]
   int *stats_acc;        // [8], [7] the statistics ticket
+  // [8] or null: the elementwise maximum of every run's statistics since the
+  // caller last cleared it (a loop of runs checks its capacities once)
+  int *sticky;
 };
 
 // After the backward pass, one launch: per-atom forces (zeros without grad),

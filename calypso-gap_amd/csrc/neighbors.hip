@@ -755,6 +755,7 @@ __global__ __launch_bounds__(256) void k_forces_reduce(int N, int G, int nstruct
     if (t < 8 && t != 3 && t != 7) val = __hip_atomic_load(&rb.stats_acc[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (t == 3) val = nb.total[3];
     if (t < 8) stats_out[t] = val;
+    if (rb.sticky && t < 8) rb.sticky[t] = max(rb.sticky[t], val);
     __syncthreads();
     if (t < 8) rb.stats_acc[t] = 0;
     return;

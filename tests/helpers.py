@@ -53,7 +53,8 @@ _WL_CACHE = {}
 
 
 def workload_model(name):
-    """The benchmark workload's model, with descriptors from the CPU oracle (cached)."""
+    """The benchmark workload's model ("population" for the 512-structure
+    batch's), with descriptors from the CPU oracle (cached)."""
     if name in _WL_CACHE:
         return _WL_CACHE[name]
     from gapx import synth
@@ -64,7 +65,42 @@ def workload_model(name):
                           theta=np.ones(2 * len(t)), mm=np.zeros((1, 2 * len(t))), coeff=np.zeros(1),
                           sf_zeta=None if np.all(z == 1) else z)
         return oracle.descriptors(hm, species, lat, pos, rc, nthreads=16)
-    d = synth.workload_model(name, desc)
+    d = synth.population_model(desc) if name == "population" else synth.workload_model(name, desc)
     _WL_CACHE[name] = oracle.Model(z=d["z"], w=d["w"], sf_type=d["sf_type"], sf_alpha=d["sf_alpha"], sf_cut=d["sf_cut"],
                         theta=d["theta"], mm=d["mm"], coeff=d["coeff"], sf_zeta=d["sf_zeta"])
     return _WL_CACHE[name]
+
+
+def lbfgs_two_loop(S, Y, q, alpha):
+    """H q for the L-BFGS inverse-Hessian approximation of the pairs (S, Y),
+    oldest first, with H0 = 1 / alpha: the two-loop recursion as ASE 3.18.1
+    runs it (ase/optimize/lbfgs.py, LBFGS.step; Nocedal and Wright,
+    Numerical Optimization, algorithm 7.4)."""
+    R = [1.0 / np.dot(y, s) for s, y in zip(S, Y)]
+    L = len(S)
+    a = np.empty(L)
+    q = q.copy()
+    for i in range(L - 1, -1, -1):
+        a[i] = R[i] * np.dot(S[i], q)
+        q -= a[i] * Y[i]
+    z = q / alpha
+    for i in range(L):
+        b = R[i] * np.dot(Y[i], z)
+        z += S[i] * (a[i] - b)
+    return z
+
+
+def lbfgs_step_host(xs, fs, memory=100, alpha=70.0, maxstep=0.2):
+    """The position ASE 3.18.1's LBFGS (ase/optimize/lbfgs.py: LBFGS.step,
+    update, determine_step) moves one structure to after the positions
+    xs[0..k] and forces fs[0..k] of its first k + 1 steps: history pairs
+    s = x_j - x_{j-1}, y = f_{j-1} - f_j, the last `memory` of them; the step
+    -H (-f_k), scaled to maxstep by its longest per-atom displacement."""
+    k = len(xs) - 1
+    S = [(xs[j] - xs[j - 1]).ravel() for j in range(1, k + 1)][-memory:]
+    Y = [(fs[j - 1] - fs[j]).ravel() for j in range(1, k + 1)][-memory:]
+    dr = -lbfgs_two_loop(S, Y, -fs[k].ravel(), alpha).reshape(-1, 3)
+    longest = np.sqrt((dr ** 2).sum(1)).max()
+    if longest >= maxstep:
+        dr *= maxstep / longest
+    return xs[k] + dr

@@ -121,3 +121,48 @@ def test_velocity_verlet_energy_error_is_second_order(setup):
     ratio = e1 / e2
     print("energy excursion dt 0.5 fs %s, dt 0.25 fs %s, ratio %s" % (e1, e2, ratio))
     assert np.all(e1 > 0) and np.all((ratio > 2.5) & (ratio < 6.0))
+
+
+def converging_cluster(n, radius, box, seed, shrink=0.25, steps=48):
+    """n atoms at least 2.4 A apart inside a sphere at the centre of a cubic
+    box, with velocities -k (r - c) that would shrink every distance by the
+    factor (1 - shrink) in `steps` fs of free flight: the cluster contracts
+    and every atom's neighbour count grows step by step."""
+    rng = np.random.default_rng(seed)
+    c = np.full(3, box / 2)
+    pos = []
+    while len(pos) < n:
+        r = (rng.random(3) * 2 - 1) * radius
+        if np.linalg.norm(r) <= radius and all(np.linalg.norm(c + r - q) >= 2.4 for q in pos):
+            pos.append(c + r)
+    pos = np.array(pos)
+    species = np.array([(1, 8, 22)[k % 3] for k in range(n)], np.int32)
+    s = dict(species=species, lat=np.eye(3) * box, pos=pos)
+    masses = masses_of([s])
+    v = -(shrink / (steps * gapx.FS)) * (pos - c)
+    return s, masses, masses[:, None] * v
+
+
+def test_md_outgrowing_its_capacities_matches_host_restatement(setup):
+    """A contracting cluster: the neighbour lists outgrow the capacities (and
+    their headroom) set at the start, the device notices at a block's check,
+    goes back to the block's start and runs it with larger lists, and the
+    trajectory equals the host restatement's, which checks every step."""
+    m, _ = setup
+    s, masses, p0 = converging_cluster(120, 11.0, 40.0, 12)
+    dt = 1.0 * gapx.FS
+    ctx = gapx.Context(m)
+    ctx.load([s], 6.0)
+    ctx.run(gapx.GRAD)
+    ctx.results()
+    pairs0, _ = ctx.counts()
+    log = ctx.md(masses, 48, dt_fs=1.0, temperature=0.0, taut_fs=0.0, momenta=p0)
+    x_d, p_d = ctx.positions(), ctx.momenta()
+    pairs1, _ = ctx.counts()
+    ctx.close()
+    x_h, p_h = berendsen_host(m, [s], masses, p0, 48, dt, 0.0, 0.0)
+    dx, dp = np.abs(x_d - x_h).max(), np.abs(p_d - p_h).max()
+    print("contracting cluster: pairs %d -> %d; device vs host |dx| %.2e A, |dp| %.2e" % (pairs0, pairs1, dx, dp))
+    assert pairs1 > 1.6 * pairs0
+    assert dx <= 1e-9 and dp <= 1e-9
+    assert np.all(np.isfinite(log))
