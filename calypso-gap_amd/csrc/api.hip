@@ -392,6 +392,7 @@ struct Ctx {
   unsigned long long bins_clean = ~0ULL;  // allocation generations of bin_count/bin_ticket last cleared
   int red_parts = 0;                      // reduction parts over all structures (see ReduceBufs)
   unsigned long long red_clean = ~0ULL;   // allocation generations of red_ticket/stats_acc last cleared
+  unsigned long long pad_clean = ~0ULL;   // key of the contraction operands whose padding rows were cleared
   // timing
   bool timing = false;
   hipEvent_t ev[2 * kNumStages] = {};
@@ -863,8 +864,14 @@ static int enqueue_run(Ctx &c, int flags) {
       return rc;
     }
   }
-  // padding rows [N, Np) of the contraction operands stay zero
-  if (c.Np > c.N) {
+  // padding rows [N, Np) of the contraction operands are zero: cleared once
+  // per allocation and size (the descriptor kernels write rows below N only,
+  // and the other writers of these buffers reset the key)
+  const unsigned long long pad_key =
+      ((unsigned long long)c.mem["xn"].gen << 40) ^
+      ((unsigned long long)(c64 ? c.mem["xt64"].gen : c.mem["xt_hi"].gen + 7 * c.mem["xt_lo"].gen) << 20) ^
+      ((unsigned long long)c.N << 1) ^ (c64 ? 1ULL : 0ULL);
+  if (c.Np > c.N && pad_key != c.pad_clean) {
     if (c64) GAPX_HIP(launch_fill_double(c.stream, o.xt64 + (size_t)c.N * Dp, (size_t)(c.Np - c.N) * Dp, 0.0));
     GAPX_HIP(launch_fill_double(c.stream, o.xn + c.N, c.Np - c.N, 0.0));
     if (!c64) {
@@ -872,6 +879,7 @@ static int enqueue_run(Ctx &c, int flags) {
       GAPX_HIP(launch_fill_int(c.stream, (int *)(o.xt_lo + (size_t)c.N * Dp), (c.Np - c.N) * Dp / 2, 0));
     }
   }
+  c.pad_clean = pad_key;
   const int split_mode = c64 ? 0 : ((flags & GAPX_BF16) ? 2 : 1);
   const int chunk = c.chunk > 0 ? c.chunk : 16;
   const Tuning &tu = c.tune[fp64 ? 0 : 1];
@@ -1001,6 +1009,7 @@ static int enqueue_surrogate(Ctx &c, Mlp &mlp, int flags) {
   o.Dp = Dp;
   const size_t ND = (size_t)c.Np * Dp;
   if ((rc = c.get("xn", c.Np, &o.xn))) return rc;
+  c.pad_clean = ~0ULL;  // this path writes the operand buffers its own way
   if (fp64) {
     if ((rc = c.get("x64", ND, &o.x64)) || (rc = c.get("xt64", ND, &o.xt64))) return rc;
   } else {
@@ -1573,6 +1582,7 @@ static int ctx_car2acsf(Ctx &c, int na, int max_nb, int nf, const double *pos, c
       (rc = c.get("stats", 8, &stats)) || (rc = c.get("x64", (size_t)N * Dp, &o.x64)) ||
       (rc = c.get("xt64", (size_t)N * Dp, &o.xt64)) || (rc = c.get("xn", N, &o.xn)))
     return rc;
+  c.pad_clean = ~0ULL;  // rows of xt64 / xn are written here for this call's atoms
   nb.nb_w = nbw;
   nb.kcap = kcap;
   nb.total = stats;
