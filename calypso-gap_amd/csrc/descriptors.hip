@@ -388,10 +388,13 @@ __global__ __launch_bounds__(256) void k_desc_fwd(int N, const int *__restrict__
 // squaring chain per lambda, (lambda, zeta) products are unrolled, and only
 // the eta loop is a (uniform) runtime loop.
 // ---------------------------------------------------------------------------
-// P_k = pref_k b^zeta_k and dP_k/dcos = pref_k zeta_k lam b^(zeta_k - 1) for
-// b = 1 + lam cos, lam = +1 (p) and -1 (m).  zeta_k = 2^zlog[k] with zlog
-// ascending: one squaring chain, advanced by uniform trip counts (registers
-// only; an indexed table of powers would go to scratch memory).
+// With pref_k = 2^(1 - zeta_k) (the only prefactor the tables hold), the
+// angular factor is pref_k b^zeta_k = 2 u^zeta_k for u = b / 2 = (1 + lam cos)
+// / 2, lam = +1 (p) and -1 (m).  This returns the halves P_k = u^zeta_k and
+// dP_k/dcos = lam zeta_k u^(zeta_k - 1) / 2; the callers put the factor 2 on
+// their per-triplet products once.  zeta_k = 2^zlog[k] with zlog ascending:
+// one squaring chain, advanced by uniform trip counts (registers only; an
+// indexed table of powers would go to scratch memory).
 // Z16: the zeta list is {1, 2, 4, 16} (zlog {0, 1, 2, 4}, the usual grid),
 // known at compile time, so the chain is straight-line code with the same
 // multiplications in the same order.
@@ -399,8 +402,9 @@ template <typename T, int NZ, bool DERIV, bool Z16>
 __device__ __forceinline__ void grid_powers(T cth, const SFDev &sf, T (&Pp)[NZ], T (&Pm)[NZ], T (&dPp)[NZ],
                                             T (&dPm)[NZ]) {
   static_assert(!Z16 || NZ == 4, "the {1, 2, 4, 16} list has four zetas");
-  T p = T(1) + cth, m = T(1) - cth;  // b^(2^t)
-  T qp = T(1), qm = T(1);            // b^(2^t - 1)
+  const T hc = T(0.5) * cth;
+  T p = T(0.5) + hc, m = T(0.5) - hc;  // u^(2^t)
+  T qp = T(1), qm = T(1);              // u^(2^t - 1)
   int t = 0;
 #pragma unroll
   for (int k = 0; k < NZ; ++k) {
@@ -409,44 +413,101 @@ __device__ __forceinline__ void grid_powers(T cth, const SFDev &sf, T (&Pp)[NZ],
 #pragma unroll
       for (int u = 0; u < 4; ++u)  // at most 4 squarings per zeta; t and j are constants here
         if (t < j) {
-          qp *= p;
-          qm *= m;
+          if (DERIV) {
+            qp *= p;
+            qm *= m;
+          }
           p *= p;
           m *= m;
           ++t;
         }
     } else {
       for (; t < j; ++t) {
-        qp *= p;
-        qm *= m;
+        if (DERIV) {
+          qp *= p;
+          qm *= m;
+        }
         p *= p;
         m *= m;
       }
     }
-    const T pref = (T)sf.pref_k[k];
-    Pp[k] = pref * p;
-    Pm[k] = pref * m;
+    Pp[k] = p;
+    Pm[k] = m;
     if (DERIV) {
-      const T pz = pref * (T)sf.zeta_k[k];
-      dPp[k] = pz * qp;
-      dPm[k] = -pz * qm;
+      const T hz = T(0.5) * (T)sf.zeta_k[k];  // zeta_k / 2, exact
+      dPp[k] = hz * qp;
+      dPm[k] = -hz * qm;
     }
   }
 }
+
+// exp(-eta x) as the grid kernels evaluate it: in FP32 the hardware 2^x with
+// the coefficient -eta log2(e) formed once per eta (one multiplication per
+// term instead of two); in FP64 exp itself.
+template <typename T>
+__device__ __forceinline__ T eta_coef(T eta);
+template <>
+__device__ __forceinline__ float eta_coef<float>(float eta) { return -eta * 1.44269504088896341f; }
+template <>
+__device__ __forceinline__ double eta_coef<double>(double eta) { return -eta; }
+__device__ __forceinline__ float exp_coef(float c, float x) { return __builtin_amdgcn_exp2f(c * x); }
+__device__ __forceinline__ double exp_coef(double c, double x) { return exp(c * x); }
 
 // Sum NV per-lane values over the workgroup: s_out[j] = total of v[j].  Within
 // a wavefront a butterfly reduce-scatter: at offset m (32, 16, .., 1) every
 // lane keeps the half of its live values selected by its own bit m and adds
 // the partner's copy of that half, so lane l ends with the total of value l
-// (NV padded to 32 or 64; for 32 the two half-waves are folded first).  63
-// exchanges instead of NV full wave reductions.  The waves' totals are then
-// added in wave order (deterministic).
+// (NV padded to 32 or 64; for 32 the butterfly runs within each half-wave and
+// the halves' totals are added last).  NV - 1 exchanges instead of NV full
+// wave reductions; the offsets 32 and 16 use gfx950's lane swaps (below).
+// The waves' totals are then added in wave order (deterministic).
 // One butterfly level per instantiation, so every register index is a
 // compile-time constant (a loop over M left rolled by the compiler would
 // index the array at run time).
+// gfx950's lane swaps between the halves of a wave (permlane32) or between
+// the 16-lane rows of each half (permlane16), applied to a pair (a, b): a
+// becomes {a, b} of the lower halves (rows 0 and 2) and b becomes {a, b} of
+// the upper ones (rows 1 and 3), as the lanes see them.  Then a + b gives every
+// lower lane its own a plus its partner's a, and every upper lane its own b
+// plus its partner's b: one butterfly step without a shuffle and two selects.
+template <int M>
+__device__ __forceinline__ void swap_halves(unsigned &a, unsigned &b) {
+  static_assert(M == 32 || M == 16, "lane swaps exist for offsets 32 and 16");
+  if constexpr (M == 32) {
+    const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+    a = r[0];
+    b = r[1];
+  } else {
+    const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+    a = r[0];
+    b = r[1];
+  }
+}
+
+template <int M>
+__device__ __forceinline__ float swap_sum(float a, float b) {
+  unsigned u = __float_as_uint(a), v = __float_as_uint(b);
+  swap_halves<M>(u, v);
+  return __uint_as_float(u) + __uint_as_float(v);
+}
+
+template <int M>
+__device__ __forceinline__ double swap_sum(double a, double b) {
+  unsigned long long u = __double_as_longlong(a), v = __double_as_longlong(b);
+  unsigned ul = (unsigned)u, uh = (unsigned)(u >> 32), vl = (unsigned)v, vh = (unsigned)(v >> 32);
+  swap_halves<M>(ul, vl);
+  swap_halves<M>(uh, vh);
+  return __longlong_as_double((long long)(((unsigned long long)uh << 32) | ul)) +
+         __longlong_as_double((long long)(((unsigned long long)vh << 32) | vl));
+}
+
 template <typename T, int M, int NVP>
 __device__ __forceinline__ void butterfly(T (&x)[NVP], int lane) {
-  if constexpr (M >= 1) {
+  if constexpr (M == 32 || M == 16) {
+#pragma unroll
+    for (int j = 0; j < M; ++j) x[j] = swap_sum<M>(x[j], x[j + M]);
+    butterfly<T, M / 2, NVP>(x, lane);
+  } else if constexpr (M >= 1) {
     const bool upper = (lane & M) != 0;
 #pragma unroll
     for (int j = 0; j < M; ++j) {
@@ -465,11 +526,10 @@ __device__ __forceinline__ void group_sum(T (&v)[NV], int wv, int lane, int tid,
   T x[NVP];
 #pragma unroll
   for (int j = 0; j < NVP; ++j) x[j] = (j < NV) ? v[j] : T(0);
-  if (NVP == 32) {
-#pragma unroll
-    for (int j = 0; j < 32; ++j) x[j] += __shfl_xor(x[j], 32, 64);
-  }
   butterfly<T, NVP / 2, NVP>(x, lane);
+  // 32 values: the butterfly (offsets 16 .. 1) stays within each half of
+  // the wave; lanes l and l + 32 then add their halves' totals of value l
+  if (NVP == 32) x[0] += __shfl_xor(x[0], 32, 64);
   if (W == 1) {
     if (lane < NV) s_out[lane] = x[0];
     __syncthreads();
@@ -575,9 +635,10 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
     T acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) acc[j] = T(0);
-    T eta_h[NE];  // this pass's etas (uniform), loaded once
+    T ceta_h[NE];  // this pass's exp coefficients (uniform), formed once
 #pragma unroll
-    for (int hh = 0; hh < NE; ++hh) eta_h[hh] = (h0 + hh < sf.n_eta) ? (T)sf.ang_eta[(h0 + hh) * 2 * NZ] : T(0);
+    for (int hh = 0; hh < NE; ++hh)
+      ceta_h[hh] = eta_coef<T>((h0 + hh < sf.n_eta) ? (T)sf.ang_eta[(h0 + hh) * 2 * NZ] : T(0));
     constexpr int ST = 64 * W;
     uint32_t qc[4], qn[4];
     load_q4(tl, nt, 64 * wv + lane, ST, qc);
@@ -598,8 +659,9 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
         const T cth = (ra * ra + rb * rb - e2) * (T(0.5) * rwa.x * rwb.x);
         const T S2 = ra * ra + rb * rb + e2;
         const T wab = rwa.y * rwb.y;
-        const T fe = (e <= rc) ? T(0.5) * (Math<T>::cs(pirc * e) + T(1)) : T(0);
-        const T F3 = act ? s_fc[g * KP + a] * s_fc[g * KP + b] * fe : T(0);
+        // 2 fc(r_jk): the factor 2 of the halved powers (grid_powers) rides here
+        const T fe2 = (e <= rc) ? Math<T>::cs(pirc * e) + T(1) : T(0);
+        const T F3 = act ? s_fc[g * KP + a] * s_fc[g * KP + b] * fe2 : T(0);
         T Pp[NZ], Pm[NZ], dPp[NZ], dPm[NZ];
         grid_powers<T, NZ, false, Z16>(cth, sf, Pp, Pm, dPp, dPm);
 #pragma unroll
@@ -607,7 +669,7 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
           const int h = h0 + hh;
           if (h < sf.n_eta) {
             // plain and weighted sums as four FMAs per zeta (E w_ab formed once)
-            const T E = Math<T>::ex(-eta_h[hh] * S2) * F3, Ew = E * wab;
+            const T E = exp_coef(ceta_h[hh], S2) * F3, Ew = E * wab;
 #pragma unroll
             for (int k = 0; k < NZ; ++k) {
               acc[(hh * 2 + 0) * NZ + k] += E * Pp[k];
@@ -667,8 +729,8 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
 
 template <typename T, int NZ, bool Z16>
 __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDev &sf, const T *s_d, const T *s_rw,
-                                             const T *s_fd, const T *s_gA, const T *s_gW, const T *s_eta, int KP,
-                                             T rc, T pirc, double *s_T, double *s_F) {
+                                             const T *s_fd, const T *s_gA, const T *s_gW, const T *s_eta,
+                                             const T *s_ceta, int KP, T rc, T pirc, double *s_T, double *s_F) {
   if (!active) return;
   const int a = q & 0xffff, b = q >> 16;
   const T ax = s_d[4 * a], ay = s_d[4 * a + 1], az = s_d[4 * a + 2], ra = s_d[4 * a + 3];
@@ -690,14 +752,16 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
   const bool in = e <= rc;
   T sine, cosine;
   Math<T>::sc(pirc * e, sine, cosine);
-  const T fe = in ? T(0.5) * (cosine + T(1)) : T(0);
-  const T dfe = in ? T(-0.5) * sine * pirc : T(0);
+  // 2 fc(r_jk) and its derivative: every term below carries exactly one of
+  // them, which puts the factor 2 of the halved powers (grid_powers) on all
+  const T fe = in ? cosine + T(1) : T(0);
+  const T dfe = in ? -sine * pirc : T(0);
   T Pp[NZ], Pm[NZ], dPp[NZ], dPm[NZ];
   grid_powers<T, NZ, true, Z16>(cth, sf, Pp, Pm, dPp, dPm);
   T SA = 0, SB = 0, SC = 0;
   for (int h = 0; h < sf.n_eta; ++h) {
     const T eta = s_eta[h];
-    const T E = Math<T>::ex(-eta * S2);
+    const T E = exp_coef(s_ceta[h], S2);
     const int s0 = h * 2 * NZ;
     T sa = 0, sb = 0;
     T gA[2 * NZ], gW[2 * NZ];
@@ -848,7 +912,9 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   // region above has a multiple of 4 values) the grid form reads four lines'
   // g at once; n_ang <= Dp / 2, so both fit before s_eta
   T *s_gW = s_gA + ((sf.n_ang + 3) & ~3);
-  T *s_eta = s_g + 2 * Dp;                        // [Dp / 2] grid form: eta of each pass
+  // grid form: eta of each pass and its exp coefficient (at most D / 4 etas)
+  T *s_eta = s_g + 2 * Dp;     // [Dp / 4]
+  T *s_ceta = s_eta + Dp / 4;  // [Dp / 4]
 
   const int start = i * nb.kcap;
   const int K = nb.nb_count[i];
@@ -874,7 +940,10 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   }
   for (int k = tid; k < 4 * W * KP; k += 64 * W) s_T[k] = 0.0;  // s_T and s_F
   if (GNZ > 0)
-    for (int h = tid; h < sf.n_eta; h += 64 * W) s_eta[h] = (T)sf.ang_eta[h * 2 * GNZ];
+    for (int h = tid; h < sf.n_eta; h += 64 * W) {
+      s_eta[h] = (T)sf.ang_eta[h * 2 * GNZ];
+      s_ceta[h] = eta_coef<T>((T)sf.ang_eta[h * 2 * GNZ]);
+    }
   const size_t row = (size_t)i * Dp;
   if (gs.splits > 0) {
     // the contraction's finish for this atom (same order as k_gpr_finish)
@@ -938,8 +1007,8 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
         const bool act = t0 + u * ST + lane < nt;
         const uint32_t q = pop_q4(qc);
         if constexpr (GNZ > 0)
-          ang_bwd_grid<T, GNZ, Z16>(q, act, sf, s_d, s_rw, s_fd, s_gA, s_gW, s_eta, KP, rcA, pirc, s_T + wv * KP,
-                                    s_F + 3 * wv * KP);
+          ang_bwd_grid<T, GNZ, Z16>(q, act, sf, s_d, s_rw, s_fd, s_gA, s_gW, s_eta, s_ceta, KP, rcA, pirc,
+                                    s_T + wv * KP, s_F + 3 * wv * KP);
         else
           ang_bwd_term<T>(q, act, sf, s_d, s_rw, s_fd, s_gA, s_gW, KP, s_T + wv * KP, s_F + 3 * wv * KP);
       }
