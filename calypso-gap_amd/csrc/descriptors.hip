@@ -759,11 +759,12 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
   T Pp[NZ], Pm[NZ], dPp[NZ], dPm[NZ];
   grid_powers<T, NZ, true, Z16>(cth, sf, Pp, Pm, dPp, dPm);
   T SA = 0, SB = 0, SC = 0;
-  for (int h = 0; h < sf.n_eta; ++h) {
+  // one eta: the lines' g contracted with the powers (sb) and with their
+  // cos-derivatives (sa), each sum started from its first term
+  auto eta_terms = [&](int h) {
     const T eta = s_eta[h];
     const T E = exp_coef(s_ceta[h], S2);
     const int s0 = h * 2 * NZ;
-    T sa = 0, sb = 0;
     T gA[2 * NZ], gW[2 * NZ];
     if constexpr (sizeof(T) == 4 && NZ % 2 == 0) {
       // 16-byte reads: s0 is a multiple of 4 and both blocks are aligned
@@ -781,8 +782,11 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
         gW[j] = s_gW[s0 + j];
       }
     }
+    const T gp0 = gA[0] + wab * gW[0], gm0 = gA[NZ] + wab * gW[NZ];
+    T sa = gp0 * dPp[0] + gm0 * dPm[0];
+    T sb = gp0 * Pp[0] + gm0 * Pm[0];
 #pragma unroll
-    for (int k = 0; k < NZ; ++k) {
+    for (int k = 1; k < NZ; ++k) {
       const T gp = gA[k] + wab * gW[k];
       const T gm = gA[NZ + k] + wab * gW[NZ + k];
       sa += gp * dPp[k] + gm * dPm[k];
@@ -791,7 +795,8 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
     SA += sa * E;
     SB += sb * E;
     SC += eta * sb * E;
-  }
+  };
+  for (int h = 0; h < sf.n_eta; ++h) eta_terms(h);
   const T F = fa * fb * fe;
   const T Ta = F * SA * dca - T(2) * ra * F * SC + SB * dfa * fb * fe;
   const T Tb = F * SA * dcb - T(2) * rb * F * SC + SB * fa * dfb * fe;
