@@ -805,14 +805,14 @@ static int enqueue_forces(Ctx &c, Bufs &b, bool fp64, bool grad, const double *e
   double *part_sum;
   int *ticket, *stats_acc;
   if ((rc = c.get("red_sum", 8 * (size_t)c.red_parts, &part_sum)) || (rc = c.get("red_ticket", c.nstruct, &ticket)) ||
-      (rc = c.get("stats_acc", 16, &stats_acc)))
+      (rc = c.get("stats_acc", 17, &stats_acc)))
     return rc;
   {
     // the tickets and accumulators stay zero between runs; clear them once
     // after they are (re)allocated
     const unsigned long long g = ((unsigned long long)c.mem["red_ticket"].gen << 32) | c.mem["stats_acc"].gen;
     if (g != c.red_clean) {
-      GAPX_HIP(launch_zero_ints(c.stream, ticket, (int)(c.mem["red_ticket"].cap / sizeof(int)), stats_acc, 16,
+      GAPX_HIP(launch_zero_ints(c.stream, ticket, (int)(c.mem["red_ticket"].cap / sizeof(int)), stats_acc, 17,
                                 nullptr, 0));
       c.red_clean = g;
     }
@@ -823,6 +823,7 @@ static int enqueue_forces(Ctx &c, Bufs &b, bool fp64, bool grad, const double *e
   rb.ticket = ticket;
   rb.stats_acc = stats_acc;
   rb.sticky = stats_acc + 8;
+  rb.overflow_now = stats_acc + 16;
   if (grad) GAPX_HIP(hipStreamWaitEvent(c.stream, c.join_ev, 0));
   stage_begin(c, kForces);
   GAPX_HIP(launch_forces_reduce(c.stream, c.N, grad ? 1 : 0, nb, b.slot_f, b.force, c.nstruct, d_sp, eatom,
@@ -1265,6 +1266,72 @@ static int ctx_autotune(Ctx &c, int flags, bool force) {
 // step is one E+F evaluation (graph replay when enabled) and two small
 // kernels that update velocities and positions in place; the step waits for
 // the device once to check list capacities and convergence.
+// Relaxation loops (FIRE and LBFGS) without a host wait per step.  Each
+// iteration is the optimiser's step on the forces at hand (device kernels),
+// then an E+F evaluation of the new positions; the host reads the step flags,
+// the running maximum of the list statistics and the convergence states once
+// per block of kRelaxBlock iterations.  A step whose evaluation overflowed
+// its lists changes nothing on the device (StepCtl::skip) and is repeated
+// after the host has grown the capacities, so the steps are those a check
+// after every evaluation would give.  The capacities get the same headroom as
+// in MD.  After the last block the final evaluation is settled as any run.
+static constexpr int kRelaxBlock = 8;
+
+// Headroom for a trajectory: an eighth of the largest counts of the last
+// settled run (no rerun: that run fitted).
+static void trajectory_headroom(Ctx &c) {
+  const int *h = c.host_stats;
+  const int cap = neighbor_cap(c, c.last_flags);
+  const int k = round_up(std::min(h[1] + std::max(8, h[1] / 8), std::max(cap, h[1])), 8);
+  const int t = round_up(h[5] + std::max(64, h[5] / 8), 64);
+  if (k > c.kcap || t > c.tcap) {
+    c.kcap = std::max(c.kcap, k);
+    c.tcap = std::max(c.tcap, t);
+    c.invalidate_graph();
+  }
+}
+
+// The blocked loop shared by both optimisers.  step(ctl) enqueues one
+// optimiser step; conv() reads the number of converged structures from the
+// states the caller copied back.  Returns the steps that moved a structure.
+template <typename Step, typename Fetch>
+static int relax_blocks(Ctx &c, int flags, int max_steps, Step step, Fetch fetch_converged, int *steps_done,
+                        int *n_converged) {
+  int rc, done = 0, conv = 0;
+  int *flagbuf;
+  if ((rc = c.get("relax_flags", kRelaxBlock, &flagbuf))) return rc;
+  bool all_converged = false;
+  int hf[kRelaxBlock], hs[8];
+  while (done < max_steps && !all_converged) {
+    const int nb = std::min(kRelaxBlock, max_steps - done);
+    GAPX_HIP(launch_fill_int(c.stream, flagbuf, kRelaxBlock, 0));
+    GAPX_HIP(launch_fill_int(c.stream, c.ptr<int>("stats_acc") + 8, 8, 0));
+    for (int k = 0; k < nb; ++k) {
+      GAPX_HIP(step(StepCtl{c.ptr<int>("stats_acc") + 16, flagbuf + k}));
+      if ((rc = ctx_run(c, flags))) return rc;
+    }
+    GAPX_HIP(hipMemcpyAsync(hf, flagbuf, sizeof hf, hipMemcpyDeviceToHost, c.stream));
+    GAPX_HIP(hipMemcpyAsync(hs, c.ptr<int>("stats_acc") + 8, sizeof hs, hipMemcpyDeviceToHost, c.stream));
+    if ((rc = fetch_converged(&conv))) return rc;  // enqueues its copy and synchronises
+    collect_timing(c);
+    for (int k = 0; k < nb; ++k) {
+      if (hf[k] & 2) continue;  // skipped: its evaluation overflowed
+      if (hf[k] & 1) {
+        ++done;
+      } else {  // nothing moved: every structure had converged
+        all_converged = true;
+        break;
+      }
+    }
+    bool grow;
+    if ((rc = capacity_verdict(c, hs, std::max(8, hs[1] / 8), std::max(64, hs[5] / 8), &grow))) return rc;
+  }
+  if ((rc = ctx_settle(c))) return rc;  // the final evaluation
+  if (steps_done) *steps_done = done;
+  if (n_converged) *n_converged = conv;
+  return GAPX_OK;
+}
+
 static int ctx_relax(Ctx &c, int flags, int max_steps, double fmax, double maxstep, int *steps_done,
                      int *n_converged) {
   if (c.N <= 0) return fail(GAPX_ERR_ARG, "no structures loaded");
@@ -1284,33 +1351,23 @@ static int ctx_relax(Ctx &c, int flags, int max_steps, double fmax, double maxst
     S.a = fp.astart;
   }
   GAPX_HIP(hipMemcpyAsync(st, hs.data(), sizeof(FireState) * c.nstruct, hipMemcpyHostToDevice, c.stream));
-  int step = 0, conv = 0;
-  for (; step < max_steps; ++step) {
-    if ((rc = ctx_run(c, flags)) || (rc = ctx_settle(c))) return rc;
-    GAPX_HIP(launch_fire_step(c.stream, c.nstruct, c.N, c.ptr<StructParams>("sp"), c.ptr<int>("astruct"),
-                              c.ptr<double>("force"), vel, c.ptr<double>("pos"), st, fp));
+  if ((rc = ctx_run(c, flags)) || (rc = ctx_settle(c))) return rc;  // forces at the start
+  trajectory_headroom(c);
+  auto step = [&](StepCtl ctl) {
+    return launch_fire_step(c.stream, c.nstruct, c.N, c.ptr<StructParams>("sp"), c.ptr<int>("astruct"),
+                            c.ptr<double>("force"), vel, c.ptr<double>("pos"), st, fp, ctl);
+  };
+  auto fetch = [&](int *conv) {
     GAPX_HIP(hipMemcpyAsync(hs.data(), st, sizeof(FireState) * c.nstruct, hipMemcpyDeviceToHost, c.stream));
     GAPX_HIP(hipStreamSynchronize(c.stream));
-    conv = 0;
-    for (const FireState &S : hs) conv += S.converged;
-    if (conv == c.nstruct) {
-      ++step;
-      break;
-    }
-  }
-  // results at the final positions
-  if ((rc = ctx_run(c, flags)) || (rc = ctx_settle(c))) return rc;
-  if (steps_done) *steps_done = step;
-  if (n_converged) *n_converged = conv;
-  return GAPX_OK;
+    *conv = 0;
+    for (const FireState &S : hs) *conv += S.converged;
+    return GAPX_OK;
+  };
+  return relax_blocks(c, flags, max_steps, step, fetch, steps_done, n_converged);
 }
 
-// Relax every structure of the batch at fixed cell with LBFGS as ASE runs it
-// (relax.hip): each step is one E+F evaluation (graph replay when enabled)
-// and one launch that checks convergence, updates the history and moves the
-// structures that have not converged; the step waits for the device once
-// (list capacities and convergence).  memory is capped at max_steps (the
-// history never holds more entries than steps taken).
+// LBFGS relaxation of the loaded batch (relax.hip has the step).
 static int ctx_relax_lbfgs(Ctx &c, int flags, int max_steps, double fmax, double maxstep, int memory, double alpha,
                            int *steps_done, int *n_converged) {
   if (c.N <= 0) return fail(GAPX_ERR_ARG, "no structures loaded");
@@ -1331,29 +1388,29 @@ static int ctx_relax_lbfgs(Ctx &c, int flags, int max_steps, double fmax, double
   std::vector<LbfgsState> hs(c.nstruct, LbfgsState{0, 0, 0, 0, 0.0});
   GAPX_HIP(hipMemcpyAsync(st, hs.data(), sizeof(LbfgsState) * c.nstruct, hipMemcpyHostToDevice, c.stream));
   const LbfgsParams lp{maxstep, 1.0 / alpha, fmax, mem, 0};
-  int step = 0, conv = 0;
-  for (; step < max_steps; ++step) {
-    if ((rc = ctx_run(c, flags)) || (rc = ctx_settle(c))) return rc;
-    GAPX_HIP(launch_lbfgs_step(c.stream, c.nstruct, c.N, c.ptr<StructParams>("sp"), c.ptr<double>("force"),
-                               c.ptr<double>("pos"), r0, f0, S, Y, rho, q, st, lp));
+  if ((rc = ctx_run(c, flags)) || (rc = ctx_settle(c))) return rc;  // forces at the start
+  trajectory_headroom(c);
+  auto launch = [&](StepCtl ctl, const LbfgsParams &p) {
+    return launch_lbfgs_step(c.stream, c.nstruct, c.N, c.ptr<StructParams>("sp"), c.ptr<double>("force"),
+                             c.ptr<double>("pos"), r0, f0, S, Y, rho, q, st, p, ctl);
+  };
+  auto fetch = [&](int *conv) {
     GAPX_HIP(hipMemcpyAsync(hs.data(), st, sizeof(LbfgsState) * c.nstruct, hipMemcpyDeviceToHost, c.stream));
     GAPX_HIP(hipStreamSynchronize(c.stream));
-    conv = 0;
-    for (const LbfgsState &L : hs) conv += L.converged;
-    if (conv == c.nstruct) break;
+    *conv = 0;
+    for (const LbfgsState &L : hs) *conv += L.converged;
+    return GAPX_OK;
+  };
+  int done = 0, conv = 0;
+  if ((rc = relax_blocks(c, flags, max_steps, [&](StepCtl ctl) { return launch(ctl, lp); }, fetch, &done, &conv)))
+    return rc;
+  // after max_steps steps, ASE's last convergence test on the final positions
+  // (the kernel moves nothing in check_only mode)
+  if (done == max_steps) {
+    GAPX_HIP(launch(StepCtl{nullptr, nullptr}, LbfgsParams{maxstep, 0.0, fmax, mem, 1}));
+    if ((rc = fetch(&conv))) return rc;
   }
-  // results at the final positions; after max_steps steps, ASE's last
-  // convergence test on them (the kernel moves nothing in check_only mode)
-  if ((rc = ctx_run(c, flags)) || (rc = ctx_settle(c))) return rc;
-  if (step == max_steps) {
-    GAPX_HIP(launch_lbfgs_step(c.stream, c.nstruct, c.N, c.ptr<StructParams>("sp"), c.ptr<double>("force"),
-                               c.ptr<double>("pos"), r0, f0, S, Y, rho, q, st, LbfgsParams{maxstep, 0.0, fmax, mem, 1}));
-    GAPX_HIP(hipMemcpyAsync(hs.data(), st, sizeof(LbfgsState) * c.nstruct, hipMemcpyDeviceToHost, c.stream));
-    GAPX_HIP(hipStreamSynchronize(c.stream));
-    conv = 0;
-    for (const LbfgsState &L : hs) conv += L.converged;
-  }
-  if (steps_done) *steps_done = step;
+  if (steps_done) *steps_done = done;
   if (n_converged) *n_converged = conv;
   return GAPX_OK;
 }
@@ -1402,18 +1459,7 @@ static int ctx_md(Ctx &c, int flags, int nsteps, double dt, double temperature, 
   const StructParams *d_sp = c.ptr<StructParams>("sp");
   const int *d_as = c.ptr<int>("astruct");
   if ((rc = ctx_run(c, flags)) || (rc = ctx_settle(c))) return rc;
-  {
-    // headroom for the trajectory (no rerun: the forces just computed fitted)
-    const int *h = c.host_stats;
-    const int cap = neighbor_cap(c, c.last_flags);
-    const int k = round_up(std::min(h[1] + std::max(8, h[1] / 8), std::max(cap, h[1])), 8);
-    const int t = round_up(h[5] + std::max(64, h[5] / 8), 64);
-    if (k > c.kcap || t > c.tcap) {
-      c.kcap = std::max(c.kcap, k);
-      c.tcap = std::max(c.tcap, t);
-      c.invalidate_graph();
-    }
-  }
+  trajectory_headroom(c);
   if (dlog) GAPX_HIP(launch_md_log(c.stream, c.nstruct, d_sp, mass, p, c.ptr<double>("force"), c.ptr<double>("ene"),
                                    mp.kB, dlog));
   int hs[8];

@@ -268,6 +268,9 @@ struct ReduceBufs {
   // [8] or null: the elementwise maximum of every run's statistics since the
   // caller last cleared it (a loop of runs checks its capacities once)
   int *sticky;
+  // [1] or null: 1 when this run's lists overflowed (kcap, tcap or the
+  // descriptor kernels' layout), else 0; written by every run
+  int *overflow_now;
 };
 
 // After the backward pass, one launch: per-atom forces (zeros without grad),
@@ -330,8 +333,17 @@ struct FireParams {
   double dt0, dtmax, finc, fdec, astart, fa, maxstep, fmax;
   int nmin, pad;
 };
+// Device-side control of a relaxation step (both optimisers): when *skip is
+// set (the evaluation the step would use overflowed its lists) the step
+// changes nothing and sets bit 2 of *flag; a step that moves any structure
+// sets bit 1.  Either pointer may be null.
+struct StepCtl {
+  const int *skip;
+  int *flag;
+};
 hipError_t launch_fire_step(hipStream_t s, int nstruct, int N, const StructParams *sp, const int *atom_struct,
-                            const double *force, double *vel, double *pos, FireState *st, const FireParams &fp);
+                            const double *force, double *vel, double *pos, FireState *st, const FireParams &fp,
+                            StepCtl ctl);
 
 // Batched LBFGS (relax.hip): per-structure state and parameters.
 constexpr int kLbfgsMaxMemory = 1024;
@@ -350,7 +362,7 @@ struct LbfgsParams {
 // S, Y: [memory][3 N]; rho: [nstruct][memory]; r0, f0, q: [3 N].
 hipError_t launch_lbfgs_step(hipStream_t s, int nstruct, int N, const StructParams *sp, const double *force,
                              double *pos, double *r0, double *f0, double *S, double *Y, double *rho, double *q,
-                             LbfgsState *st, const LbfgsParams &lp);
+                             LbfgsState *st, const LbfgsParams &lp, StepCtl ctl);
 
 // Batched NVT MD (md.hip): ASE NVTBerendsen steps in ASE units.
 struct MdParams {

@@ -756,6 +756,10 @@ __global__ __launch_bounds__(256) void k_forces_reduce(int N, int G, int nstruct
     if (t == 3) val = nb.total[3];
     if (t < 8) stats_out[t] = val;
     if (rb.sticky && t < 8) rb.sticky[t] = max(rb.sticky[t], val);
+    if (rb.overflow_now) {  // lanes 2, 3 and 6 hold the overflow counts
+      const unsigned long long m = __ballot((t == 2 || t == 3 || t == 6) && val != 0);
+      if (t == 0) *rb.overflow_now = (m != 0ull) ? 1 : 0;
+    }
     __syncthreads();
     if (t < 8) rb.stats_acc[t] = 0;
     return;

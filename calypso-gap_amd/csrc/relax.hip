@@ -33,9 +33,13 @@ __device__ __forceinline__ double wave_max_d(double v) {
 
 __global__ __launch_bounds__(256) void k_fire_reduce(int nstruct, const StructParams *__restrict__ sp,
                                                      const double *__restrict__ force, const double *__restrict__ vel,
-                                                     FireState *__restrict__ st, FireParams fp) {
+                                                     FireState *__restrict__ st, FireParams fp, StepCtl ctl) {
   const int s = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
   if (s >= nstruct) return;
+  if (ctl.skip && *ctl.skip) {
+    if (s == 0 && t == 0 && ctl.flag) atomicOr(ctl.flag, 2);
+    return;
+  }
   const StructParams &P = sp[s];
   __shared__ double red[4][4];
   double vf = 0, ff = 0, vv = 0, fm = 0;
@@ -96,6 +100,7 @@ __global__ __launch_bounds__(256) void k_fire_reduce(int nstruct, const StructPa
     }
   }
   S.started = 1;
+  if (ctl.flag) atomicOr(ctl.flag, 1);
   // v' = cv v + (cf + dt) F;  dr = dt v', capped at maxstep in norm
   cf += S.dt;
   const double vn2 = cv * cv * vv + 2.0 * cv * cf * vf + cf * cf * ff;
@@ -107,9 +112,10 @@ __global__ __launch_bounds__(256) void k_fire_reduce(int nstruct, const StructPa
 }
 
 __global__ void k_fire_update(int N, const int *__restrict__ atom_struct, const double *__restrict__ force,
-                              double *__restrict__ vel, double *__restrict__ pos, const FireState *__restrict__ st) {
+                              double *__restrict__ vel, double *__restrict__ pos, const FireState *__restrict__ st,
+                              const int *__restrict__ skip) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+  if (i >= N || (skip && *skip)) return;
   const FireState &S = st[atom_struct[i]];
   if (S.converged) return;
   for (int c = 0; c < 3; ++c) {
@@ -156,11 +162,15 @@ __global__ __launch_bounds__(256) void k_lbfgs_step(int nstruct, const StructPar
                                                     double *__restrict__ r0, double *__restrict__ f0,
                                                     double *__restrict__ S, double *__restrict__ Y,
                                                     double *__restrict__ rho, double *__restrict__ q,
-                                                    LbfgsState *__restrict__ st, LbfgsParams lp) {
+                                                    LbfgsState *__restrict__ st, LbfgsParams lp, StepCtl ctl) {
   __shared__ double s_red[4];
   __shared__ double s_a[kLbfgsMaxMemory];
   const int s = blockIdx.x, t = threadIdx.x;
   if (s >= nstruct) return;
+  if (ctl.skip && *ctl.skip) {
+    if (s == 0 && t == 0 && ctl.flag) atomicOr(ctl.flag, 2);
+    return;
+  }
   const StructParams &P = sp[s];
   const int n3 = 3 * P.natoms;
   const size_t b0 = 3 * (size_t)P.atom_base;
@@ -242,6 +252,7 @@ __global__ __launch_bounds__(256) void k_lbfgs_step(int nstruct, const StructPar
     L.iter += 1;
     L.fmax = sqrt(fm);
     st[s] = L;
+    if (ctl.flag) atomicOr(ctl.flag, 1);
   }
 }
 
@@ -249,18 +260,19 @@ __global__ __launch_bounds__(256) void k_lbfgs_step(int nstruct, const StructPar
 
 hipError_t launch_lbfgs_step(hipStream_t s, int nstruct, int N, const StructParams *sp, const double *force,
                              double *pos, double *r0, double *f0, double *S, double *Y, double *rho, double *q,
-                             LbfgsState *st, const LbfgsParams &lp) {
+                             LbfgsState *st, const LbfgsParams &lp, StepCtl ctl) {
   if (lp.memory < 1 || lp.memory > kLbfgsMaxMemory) return hipErrorInvalidValue;
-  k_lbfgs_step<<<nstruct, 256, 0, s>>>(nstruct, sp, 3 * N, force, pos, r0, f0, S, Y, rho, q, st, lp);
+  k_lbfgs_step<<<nstruct, 256, 0, s>>>(nstruct, sp, 3 * N, force, pos, r0, f0, S, Y, rho, q, st, lp, ctl);
   return hipGetLastError();
 }
 
 hipError_t launch_fire_step(hipStream_t s, int nstruct, int N, const StructParams *sp, const int *atom_struct,
-                            const double *force, double *vel, double *pos, FireState *st, const FireParams &fp) {
-  k_fire_reduce<<<nstruct, 256, 0, s>>>(nstruct, sp, force, vel, st, fp);
+                            const double *force, double *vel, double *pos, FireState *st, const FireParams &fp,
+                            StepCtl ctl) {
+  k_fire_reduce<<<nstruct, 256, 0, s>>>(nstruct, sp, force, vel, st, fp, ctl);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_fire_update<<<(N + 255) / 256, 256, 0, s>>>(N, atom_struct, force, vel, pos, st);
+  k_fire_update<<<(N + 255) / 256, 256, 0, s>>>(N, atom_struct, force, vel, pos, st, ctl.skip);
   return hipGetLastError();
 }
 
