@@ -495,21 +495,19 @@ def probe(args):
     # times the benchmark's roofline uses, so the counters match them by grid
     ctx.timing(True)
     for flags, name in ((gapx.GRAD, "fp64"), (gapx.GRAD | gapx.MIXED, "mixed")):
-        # every launch shape the autotuner can choose (one knob at a time from
-        # the defaults, as it searches), `--probe` eager evaluations each; no
-        # graphs, whose replay the counter passes do not need
+        # every launch shape the autotuner can choose (the descriptor kernels'
+        # waves one at a time from the defaults, the contraction's row tile and
+        # workgroup target together, as it searches), `--probe` eager
+        # evaluations each; no graphs, whose replay the counter passes do not need
         base = dict(waves_fwd=1, waves_bwd=1, gpr_rows=64, gpr_blocks=512)
-        knobs = [("waves_fwd", (1, 2, 4)), ("waves_bwd", (1, 2, 4)), ("gpr_blocks", (256, 512, 1024, 2048))]
-        if flags & gapx.MIXED:
-            knobs.insert(2, ("gpr_rows", (64, 128)))
-        for k, vals in knobs:
-            for v in vals:
-                t = dict(base)
-                t[k] = v
-                ctx.set_tuning(flags, **t)
-                for _ in range(args.probe):
-                    ctx.run(flags)
-                ctx.results()
+        shapes = [dict(base, waves_fwd=v) for v in (1, 2, 4)] + [dict(base, waves_bwd=v) for v in (2, 4)]
+        for r in ((64, 128) if flags & gapx.MIXED else (64,)):
+            shapes += [dict(base, gpr_rows=r, gpr_blocks=b) for b in (256, 512, 768, 1024, 1536, 2048, 3072)]
+        for t in shapes:
+            ctx.set_tuning(flags, **t)
+            for _ in range(args.probe):
+                ctx.run(flags)
+            ctx.results()
         print("probe: %s launch shapes done" % name, flush=True)
     ctx.close()
     m.close()

@@ -357,6 +357,7 @@ struct Tuning {
   int waves_fwd = 4, waves_bwd = 4;  // wavefronts per atom in the descriptor kernels
   int gpr_rows = 64;                 // rows per workgroup of the contraction (64 or 128)
   int gpr_blocks = 512;              // target workgroups of the contraction (sets the splits)
+  int halves = 1;                    // the two-half schedule where the batch allows it (1) or one piece (0)
 };
 
 struct Ctx {
@@ -949,10 +950,13 @@ static int enqueue_run(Ctx &c, int flags) {
   // contraction (vector units and matrix cores busy together); B joins before
   // the force gather.  Smaller batches run in one piece: their descriptor
   // kernels are latency-bound at low occupancy, and a half-size launch takes
-  // about as long as a whole one (4096 atoms: 12 % slower split).
+  // about as long as a whole one (4096 atoms: 12 % slower split).  Whether it
+  // pays on a given batch is the autotuner's call (Tuning::halves): the
+  // contraction's workgroups fill their CUs' registers, so the overlap can
+  // also just share the CUs out.
   constexpr int kHalfMinAtoms = 8192;
   const int S = (c.N / 2) / g.rows * g.rows;
-  if (grad && !c.timing && S >= kHalfMinAtoms && c.N - S >= kHalfMinAtoms) {
+  if (grad && !c.timing && tu.halves && S >= kHalfMinAtoms && c.N - S >= kHalfMinAtoms) {
     GAPX_HIP(launch_descriptors_fwd(c.stream, fp64, S, d_species, m.d_wz, nb, b.tb, m.sfd, m.d_theta_inv, m.d_mu,
                                     split_mode, o, chunk, tu.waves_fwd, c.KP, 0));
     if ((rc = fork_reverse(c, b))) return rc;
@@ -1205,7 +1209,7 @@ static int ctx_autotune(Ctx &c, int flags, bool force) {
   // whole evaluations, back to back on the stream, replayed from a graph
   // captured for each candidate (so host launch costs, which do not depend
   // on the launch shapes, stay out of the comparison): the mean of `reps`
-  // runs between two events, best of two trials
+  // runs between two events, best of three trials
   const int reps = 10;
   c.use_graph = true;
   auto measure = [&](double &out) -> int {
@@ -1213,7 +1217,7 @@ static int ctx_autotune(Ctx &c, int flags, bool force) {
     int r = ctx_run(c, flags);  // capture, warm-up (buffers, caches)
     if (r) return r;
     out = 1e300;
-    for (int trial = 0; trial < 2; ++trial) {
+    for (int trial = 0; trial < 3; ++trial) {
       GAPX_HIP(hipEventRecord(e0, c.stream));
       for (int k = 0; k < reps; ++k)
         if ((r = ctx_run(c, flags))) return r;
@@ -1225,6 +1229,9 @@ static int ctx_autotune(Ctx &c, int flags, bool force) {
     }
     return GAPX_OK;
   };
+  // the descriptor kernels' waves one knob at a time; then the contraction's
+  // row tile and workgroup target together (they interact: the target sets
+  // the split of the sparse points, the tile the number of row tiles)
   struct Knob {
     int *field;
     std::vector<int> values;
@@ -1233,22 +1240,54 @@ static int ctx_autotune(Ctx &c, int flags, bool force) {
   std::vector<Knob> knobs;
   knobs.push_back({&t.waves_fwd, {1, 2, 4}});
   if (flags & GAPX_GRAD) knobs.push_back({&t.waves_bwd, {1, 2, 4}});
-  if (p == 1 && c.m->Dp <= 128) knobs.push_back({&t.gpr_rows, {64, 128}});
-  knobs.push_back({&t.gpr_blocks, {256, 512, 1024, 2048}});
-  for (Knob &k : knobs) {
-    if (rc) break;
-    int best = *k.field;
-    double best_ms = 1e300;
-    for (int v : k.values) {
-      *k.field = v;
-      double ms = 0;
-      if ((rc = measure(ms))) break;
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = v;
+  auto descend = [&]() {
+    for (Knob &k : knobs) {
+      if (rc) break;
+      int best = *k.field;
+      double best_ms = 1e300;
+      for (int v : k.values) {
+        *k.field = v;
+        double ms = 0;
+        if ((rc = measure(ms))) break;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = v;
+        }
       }
+      *k.field = best;
     }
-    *k.field = best;
+  };
+  descend();
+  if (!rc) {
+    const std::vector<int> rows = (p == 1 && c.m->Dp <= 128) ? std::vector<int>{64, 128} : std::vector<int>{t.gpr_rows};
+    const std::vector<int> blocks = {256, 512, 768, 1024, 1536, 2048, 3072};
+    int best_r = t.gpr_rows, best_b = t.gpr_blocks;
+    double best_ms = 1e300;
+    for (int r : rows) {
+      for (int b : blocks) {
+        t.gpr_rows = r;
+        t.gpr_blocks = b;
+        double ms = 0;
+        if ((rc = measure(ms))) break;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best_r = r;
+          best_b = b;
+        }
+      }
+      if (rc) break;
+    }
+    t.gpr_rows = best_r;
+    t.gpr_blocks = best_b;
+  }
+  descend();  // the waves again, now beside the chosen contraction shape
+  if (!rc && (flags & GAPX_GRAD)) {  // the two-half schedule, where the batch is large enough for it
+    double on_ms = 0, off_ms = 0;
+    t.halves = 1;
+    if (!(rc = measure(on_ms))) {
+      t.halves = 0;
+      if (!(rc = measure(off_ms))) t.halves = (on_ms <= off_ms) ? 1 : 0;
+    }
   }
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);
@@ -2509,27 +2548,32 @@ int gapx_ctx_autotune(gapx_ctx *c, int flags, int force) {
   return ctx_autotune(*c, flags & ~GAPX_LAYOUT_F, force != 0);
 }
 
-int gapx_ctx_get_tuning(gapx_ctx *c, int flags, int *waves_fwd, int *waves_bwd, int *gpr_rows, int *gpr_blocks) {
+int gapx_ctx_get_tuning(gapx_ctx *c, int flags, int *waves_fwd, int *waves_bwd, int *gpr_rows, int *gpr_blocks,
+                        int *halves) {
   if (!c) return fail(GAPX_ERR_ARG, "null context");
   const Tuning &t = c->tune[(run_flags(*c->m, flags) & GAPX_MIXED) ? 1 : 0];
   if (waves_fwd) *waves_fwd = t.waves_fwd;
   if (waves_bwd) *waves_bwd = t.waves_bwd;
   if (gpr_rows) *gpr_rows = t.gpr_rows;
   if (gpr_blocks) *gpr_blocks = t.gpr_blocks;
+  if (halves) *halves = t.halves;
   return GAPX_OK;
 }
 
-int gapx_ctx_set_tuning(gapx_ctx *c, int flags, int waves_fwd, int waves_bwd, int gpr_rows, int gpr_blocks) {
+int gapx_ctx_set_tuning(gapx_ctx *c, int flags, int waves_fwd, int waves_bwd, int gpr_rows, int gpr_blocks,
+                        int halves) {
   if (!c) return fail(GAPX_ERR_ARG, "null context");
   auto ok_w = [](int w) { return w == 1 || w == 2 || w == 4; };
   if (!ok_w(waves_fwd) || !ok_w(waves_bwd) || (gpr_rows != 64 && gpr_rows != 128) || gpr_blocks < 1 ||
-      gpr_blocks > (1 << 20))
-    return fail(GAPX_ERR_ARG, "gapx_ctx_set_tuning: waves must be 1, 2 or 4, rows 64 or 128, blocks in [1, 2^20]");
+      gpr_blocks > (1 << 20) || (halves != 0 && halves != 1))
+    return fail(GAPX_ERR_ARG,
+                "gapx_ctx_set_tuning: waves must be 1, 2 or 4, rows 64 or 128, blocks in [1, 2^20], halves 0 or 1");
   Tuning &t = c->tune[(run_flags(*c->m, flags) & GAPX_MIXED) ? 1 : 0];
   t.waves_fwd = waves_fwd;
   t.waves_bwd = waves_bwd;
   t.gpr_rows = gpr_rows;
   t.gpr_blocks = gpr_blocks;
+  t.halves = halves;
   c->invalidate_graph();
   return GAPX_OK;
 }

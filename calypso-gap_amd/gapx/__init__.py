@@ -118,8 +118,8 @@ def lib():
         L.gapx_model_read_cached.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp]
         L.gapx_ctx_md.argtypes = [vp, ci, ci, cd, cd, cd, ci, vp, vp, vp]
         L.gapx_ctx_momenta.argtypes = [vp, vp]
-        L.gapx_ctx_get_tuning.argtypes = [vp, ci, vp, vp, vp, vp]
-        L.gapx_ctx_set_tuning.argtypes = [vp, ci, ci, ci, ci, ci]
+        L.gapx_ctx_get_tuning.argtypes = [vp, ci, vp, vp, vp, vp, vp]
+        L.gapx_ctx_set_tuning.argtypes = [vp, ci, ci, ci, ci, ci, ci]
         L.gapx_ctx_stage_count.argtypes = [vp, vp]
         L.gapx_ctx_stage_info.argtypes = [vp, ci, vp, vp, vp]
         L.gapx_ctx_use_graph.argtypes = [vp, ci]
@@ -424,9 +424,9 @@ class Context:
         return self.tuning(flags)
 
     def tuning(self, flags=GRAD):
-        v = [ctypes.c_int() for _ in range(4)]
+        v = [ctypes.c_int() for _ in range(5)]
         _check(lib().gapx_ctx_get_tuning(self._h, int(flags), *[ctypes.byref(x) for x in v]))
-        return dict(zip(("waves_fwd", "waves_bwd", "gpr_rows", "gpr_blocks"), (x.value for x in v)))
+        return dict(zip(("waves_fwd", "waves_bwd", "gpr_rows", "gpr_blocks", "halves"), (x.value for x in v)))
 
     def relax(self, flags=GRAD, max_steps=500, fmax=0.05, maxstep=0.2, optimizer="fire", memory=100, alpha=70.0):
         """Relaxation of every loaded structure at fixed cell, on the device:
@@ -457,9 +457,9 @@ class Context:
         """Evaluate the loaded batch with the MLP in place of the GAP contraction."""
         _check(lib().gapx_ctx_run_surrogate(self._h, mlp._h, int(flags)))
 
-    def set_tuning(self, flags=GRAD, waves_fwd=1, waves_bwd=1, gpr_rows=64, gpr_blocks=512):
+    def set_tuning(self, flags=GRAD, waves_fwd=1, waves_bwd=1, gpr_rows=64, gpr_blocks=512, halves=1):
         _check(lib().gapx_ctx_set_tuning(self._h, int(flags), int(waves_fwd), int(waves_bwd), int(gpr_rows),
-                                         int(gpr_blocks)))
+                                         int(gpr_blocks), int(halves)))
 
     def counts(self):
         """(pairs, triplets) of the last run."""

@@ -269,12 +269,15 @@ int gapx_ctx_md(gapx_ctx *c, int flags, int nsteps, double dt, double temperatur
                 const double *masses, const double *momenta, double *log);
 int gapx_ctx_momenta(gapx_ctx *c, double *p);
 /* Pick launch shapes for runs with these flags (wavefronts per atom in the
- * descriptor kernels, contraction row tile and workgroup count) by timing the
- * loaded batch; results are cached per process for the shape class unless
- * force != 0.  No reference counterpart. */
+ * descriptor kernels, contraction row tile and workgroup count, and whether a
+ * large batch runs as two overlapped halves) by timing the loaded batch;
+ * results are cached per process for the shape class unless force != 0.
+ * No reference counterpart. */
 int gapx_ctx_autotune(gapx_ctx *c, int flags, int force);
-int gapx_ctx_get_tuning(gapx_ctx *c, int flags, int *waves_fwd, int *waves_bwd, int *gpr_rows, int *gpr_blocks);
-int gapx_ctx_set_tuning(gapx_ctx *c, int flags, int waves_fwd, int waves_bwd, int gpr_rows, int gpr_blocks);
+int gapx_ctx_get_tuning(gapx_ctx *c, int flags, int *waves_fwd, int *waves_bwd, int *gpr_rows, int *gpr_blocks,
+                        int *halves);
+int gapx_ctx_set_tuning(gapx_ctx *c, int flags, int waves_fwd, int waves_bwd, int gpr_rows, int gpr_blocks,
+                        int halves);
 /* Work counts of the last run: (centre, neighbour) pairs within rcut and
  * neighbour pairs j < k of a centre within the largest angular cutoff. */
 int gapx_ctx_counts(gapx_ctx *c, long long *pairs, long long *triplets);

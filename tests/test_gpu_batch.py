@@ -113,7 +113,7 @@ def test_autotune_keeps_results(pop):
     for flags in (gapx.GRAD, gapx.GRAD | gapx.MIXED):
         t = ctx.autotune(flags, force=True)
         assert t["waves_fwd"] in (1, 2, 4) and t["waves_bwd"] in (1, 2, 4)
-        assert t["gpr_rows"] in (64, 128) and t["gpr_blocks"] >= 1
+        assert t["gpr_rows"] in (64, 128) and t["gpr_blocks"] >= 1 and t["halves"] in (0, 1)
         print("tuned", "mixed" if flags & gapx.MIXED else "fp64", t)
     ctx.run(gapx.GRAD)
     r = ctx.results()
@@ -180,7 +180,12 @@ def test_two_half_schedule_equals_one_piece(pop, flags):
     for _ in range(2):
         ctx.run(flags)
     three = ctx.results()
-    for r in (two, three):
+    t = ctx.tuning(flags)
+    t["halves"] = 0  # the autotuner's other choice: one piece without per-stage timing
+    ctx.set_tuning(flags, **t)
+    ctx.run(flags)
+    four = ctx.results()
+    for r in (two, three, four):
         for k in ("energy", "forces", "stress", "eatom"):
             assert np.array_equal(one[k], r[k]), k
     ctx.close()
