@@ -80,13 +80,13 @@ def contraction_grid(n_atoms, des_len, nsparse, mixed, tune):
     of the sparse points over grid.y for the tuned workgroup target)."""
     dp = (des_len + 31) // 32 * 32
     mp = (nsparse + 63) // 64 * 64
-    rows = tune["gpr_rows"] if (mixed and dp <= 128) else 64
+    rows = (128 if dp > 128 else tune["gpr_rows"]) if mixed else 64
     tiles = (n_atoms + 127) // 128 * 128 // rows
     chunk = 64 if dp <= 128 else 32
     splits = min(max(1, -(-tune["gpr_blocks"] // tiles)), mp // chunk)
     per = -(-(-(-mp // splits)) // chunk) * chunk
     splits = -(-mp // per)
-    return tiles * splits * 256
+    return tiles * splits * (512 if (mixed and dp > 128) else 256)
 
 
 def stage_kernel(stage, mixed, tune, z16, des_len, n_atoms, nsparse, c64=False):
@@ -94,9 +94,9 @@ def stage_kernel(stage, mixed, tune, z16, des_len, n_atoms, nsparse, c64=False):
     as rocprofv3 names it, and its grid in threads for a one-piece launch over
     n_atoms (grid-form tables with 4 zetas; z16: the zeta list is {1, 2, 4,
     16}; c64: the mixed run contracts in FP64).  The contraction is
-    instantiated for the descriptor length padded to 32, with 64-point sparse
-    chunks up to 128 features and 32-point chunks above, and 128-row tiles
-    only up to 128 features."""
+    instantiated for the descriptor length padded to 32; the mixed one as
+    <bf16, D, row blocks per wave, waves>: four waves of one or two 16-row
+    blocks up to 128 features, eight waves of one block above."""
     t = "float" if mixed else "double"
     z = "true" if z16 else "false"
     if stage == "descriptors_fwd":
@@ -110,8 +110,9 @@ def stage_kernel(stage, mixed, tune, z16, des_len, n_atoms, nsparse, c64=False):
         grid = contraction_grid(n_atoms, des_len, nsparse, mixed and not c64, tune)
         if not mixed or c64:
             return "k_gpr_f64<%d>" % dp, grid
-        r = tune["gpr_rows"] // 64 if dp <= 128 else 1
-        return "k_gpr_mixed<false, %d, %d, %d>" % (dp, 64 if dp <= 128 else 32, r), grid
+        if dp > 128:
+            return "k_gpr_mixed<false, %d, 1, 8>" % dp, grid
+        return "k_gpr_mixed<false, %d, %d, 4>" % (dp, tune["gpr_rows"] // 64), grid
     return "k_gpr_finish", n_atoms * 128
 
 

@@ -738,11 +738,19 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b) {
 // forward was enqueued: it then runs beside the contraction, which leaves the
 // vector units mostly idle (running it beside the descriptor kernels slowed
 // them by as much as it saved).  enqueue_forces joins it before the gather.
+// With per-stage timing on, every stage runs alone on the work stream and
+// the map is built inside the neighbour stage instead (reverse_in_timing), so
+// that each stage's time is its own kernels' time.
 static int fork_reverse(Ctx &c, Bufs &b) {
+  if (c.timing) return GAPX_OK;
   GAPX_HIP(hipEventRecord(c.fork_ev, c.stream));
   GAPX_HIP(hipStreamWaitEvent(c.aux, c.fork_ev, 0));
   GAPX_HIP(launch_reverse_slots(c.aux, c.N, b.nb));
   GAPX_HIP(hipEventRecord(c.join_ev, c.aux));
+  return GAPX_OK;
+}
+static int reverse_in_timing(Ctx &c, Bufs &b, bool grad) {
+  if (c.timing && grad) GAPX_HIP(launch_reverse_slots(c.stream, c.N, b.nb));
   return GAPX_OK;
 }
 
@@ -755,7 +763,9 @@ static int gpr_layout(Ctx &c, GprArgs &g, bool f64_kernel, int tune) {
   g.Np = c.Np;
   g.Dp = m.Dp;
   g.Mp = m.Mp;
-  g.rows = (!f64_kernel && m.Dp <= 128 && t.gpr_rows == 128) ? 128 : 64;
+  // the mixed kernel above D = 128 runs 128-row tiles only (eight waves
+  // sharing one workgroup's two chunk buffers)
+  g.rows = f64_kernel ? 64 : m.Dp > 128 ? 128 : t.gpr_rows == 128 ? 128 : 64;
   // Split the sparse points over grid.y until there are about gpr_blocks
   // workgroups (resident waves; more splits only add partial-sum traffic).
   const int tiles = c.Np / g.rows;
@@ -825,7 +835,7 @@ static int enqueue_forces(Ctx &c, Bufs &b, bool fp64, bool grad, const double *e
   rb.stats_acc = stats_acc;
   rb.sticky = stats_acc + 8;
   rb.overflow_now = stats_acc + 16;
-  if (grad) GAPX_HIP(hipStreamWaitEvent(c.stream, c.join_ev, 0));
+  if (grad && !c.timing) GAPX_HIP(hipStreamWaitEvent(c.stream, c.join_ev, 0));
   stage_begin(c, kForces);
   GAPX_HIP(launch_forces_reduce(c.stream, c.N, grad ? 1 : 0, nb, b.slot_f, b.force, c.nstruct, d_sp, eatom,
                                 b.atom_vir, b.ene, b.stress, b.tb, rb, c.host_stats_dev));
@@ -844,7 +854,7 @@ static int enqueue_run(Ctx &c, int flags) {
   Bufs b;
   int rc;
   stage_begin(c, kNeighbors);
-  if ((rc = run_neighbors(c, flags, b))) return rc;
+  if ((rc = run_neighbors(c, flags, b)) || (rc = reverse_in_timing(c, b, grad))) return rc;
   NeighborBufs &nb = b.nb;
   const int *d_species = c.ptr<int>("species");
   stage_end(c, kNeighbors);
@@ -1007,7 +1017,7 @@ static int enqueue_surrogate(Ctx &c, Mlp &mlp, int flags) {
   Bufs b;
   int rc;
   stage_begin(c, kNeighbors);
-  if ((rc = run_neighbors(c, flags, b))) return rc;
+  if ((rc = run_neighbors(c, flags, b)) || (rc = reverse_in_timing(c, b, grad))) return rc;
   stage_end(c, kNeighbors);
   DescOut &o = b.out;
   o.N = c.N;
@@ -1777,11 +1787,11 @@ static int ctx_create(Model *m, Ctx **out) {
   std::memset(c->host_stats, 0, 8 * sizeof(int));
   // Grid-form tables run fastest with one wavefront per atom (measured on the
   // 4096-atom ternary cell, D = 128); the generic kernels with four.  The
-  // mixed contraction takes 128-row tiles where its registers allow.
+  // mixed contraction takes 128-row tiles (above D = 128 it has no other).
   for (int p = 0; p < 2; ++p) {
     Tuning &t = c->tune[p];
     if (c->m->sfd.grid) t.waves_fwd = t.waves_bwd = 1;
-    t.gpr_rows = (p == 1 && c->m->Dp <= 128) ? 128 : 64;
+    t.gpr_rows = p == 1 ? 128 : 64;
     if (const char *w = std::getenv("GAPX_DESC_WAVES_FWD")) t.waves_fwd = std::atoi(w);
     if (const char *w = std::getenv("GAPX_DESC_WAVES_BWD")) t.waves_bwd = std::atoi(w);
   }

@@ -223,6 +223,28 @@ __device__ __forceinline__ double split_sum(int n, F load) {
   return s;
 }
 
+// (row tile, split) of this workgroup of a contraction grid (tiles on x,
+// splits on y).  Workgroups go to the eight XCDs of the chip in turn by
+// linear id (x fastest), each XCD with its own L2; left as they are, every
+// XCD runs every split and reads the whole sparse set.  When the grid allows
+// an exact regrouping, XCD x takes only the splits with split % 8 == x (or,
+// with fewer than eight splits, a share of the tiles of split x % splits),
+// so each L2 holds an eighth (or 1/splits) of the sparse points.
+__device__ __forceinline__ void xcd_tile_split(int &tile, int &split) {
+  const int nx = gridDim.x, ny = gridDim.y;
+  const int id = blockIdx.x + nx * blockIdx.y, x = id & 7, loc = id >> 3;
+  if (ny % 8 == 0) {
+    split = x + 8 * (loc / nx);
+    tile = loc % nx;
+  } else if (8 % ny == 0 && (nx * ny) % 8 == 0) {
+    split = x % ny;
+    tile = loc * (8 / ny) + x / ny;
+  } else {
+    tile = blockIdx.x;
+    split = blockIdx.y;
+  }
+}
+
 struct GradSrc {
   const double *g64 = nullptr;
   const float *g32 = nullptr;

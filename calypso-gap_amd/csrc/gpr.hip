@@ -42,8 +42,9 @@ __global__ __launch_bounds__(256, DP <= 160 ? 2 : 1) void k_gpr_f64(GprArgs a) {
   __shared__ double s_mn[kPC], s_c[kPC];
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int l16 = lane & 15, lq = lane >> 4;
-  const int row0 = (a.tile0 + blockIdx.x) * 64 + wv * 16;
-  const int split = blockIdx.y;
+  int tile, split;
+  xcd_tile_split(tile, split);
+  const int row0 = (a.tile0 + tile) * 64 + wv * 16;
   const int pb = split * a.pts_per_split;
   const int pe = min(a.Mp, pb + a.pts_per_split);
 

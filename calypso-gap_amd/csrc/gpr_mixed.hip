@@ -25,7 +25,19 @@
 // feature-major copy of mt is stored in that order (api.hip), so W goes from
 // the exp straight into MFMA operands without touching LDS.  Each wave owns R
 // 16-row blocks, so every LDS fragment feeds 3 R MFMAs.
+//
+// Sparse points arrive in chunks of 32, double-buffered in LDS: while the
+// waves work on one chunk, the next one travels global -> LDS by direct
+// loads (global_load_lds_dwordx4: no registers, counted by vmcnt), so the
+// matrix cores do not wait for the chunk loads.  A chunk buffer holds the
+// point-major hi/lo tiles (32 rows of DP halves), the feature-major hi/lo
+// tiles (DP rows of 32 halves) and, per wave, the chunk's norms and alphas.
+// A direct load writes 64 consecutive 16-byte pieces per instruction, so the
+// tiles have no row padding; instead piece q of row r sits at q ^ swz(r),
+// which spreads the 16 rows a 16-lane group reads over all LDS banks.
 #include "gapx_internal.h"
+
+#include <type_traits>
 
 namespace gapx {
 
@@ -35,6 +47,7 @@ typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef __bf16 v8b __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));  // 16 bytes of halves
+typedef __attribute__((address_space(3))) void lds_void;
 
 template <bool BF>
 struct Half;
@@ -57,72 +70,98 @@ struct Half<true> {
   }
 };
 
-// One chunk of PC sparse points goes global -> registers -> LDS in two
-// batches, each with all its loads in flight together: the point-major tile
-// (PC rows of DP halves), then the feature-major tile (DP rows of PC halves,
-// the permuted K order), both split halves, 16 bytes per piece and NPT pieces
-// per thread (indices past the tile are clamped for the load and skipped for
-// the store).  Two batches rather than one keep the staging registers within
-// the budget of two waves per SIMD.
-template <int DP, int PC, int SM, int ST>
-__device__ __forceinline__ void load_chunk(const GprArgs &a, int p0, int tid, uint16_t *s_mh, uint16_t *s_ml,
-                                           uint16_t *s_th, uint16_t *s_tl) {
-  constexpr int NE = PC * DP / 8;
-  constexpr int NPT = (NE + 255) / 256;
-  v4u h[NPT], l[NPT];
-#pragma unroll
-  for (int u = 0; u < NPT; ++u) {
-    const int e = min(tid + 256 * u, NE - 1);
-    const int pc = e / (DP / 8), q = e - pc * (DP / 8);
-    const size_t g = (size_t)(p0 + pc) * DP + 8 * q;
-    h[u] = *(const v4u *)(a.mt_hi + g);
-    l[u] = *(const v4u *)(a.mt_lo + g);
+constexpr int kPC = 32;  // sparse points per chunk
+
+// Piece swizzle of a tile row of P 16-byte pieces: P >= 16 (a row covers
+// all banks): XOR with row % 16; P = 8 or 4 (two or four rows per 256 bytes
+// of banks): XOR with the row's index among the rows that share bank
+// positions.  Rows of other widths are left in order.
+template <int P>
+__device__ __forceinline__ int swz(int row) {
+  if constexpr (P >= 16 && (P & (P - 1)) == 0) return row & 15;
+  else if constexpr (P == 8) return (row >> 1) & 7;
+  else if constexpr (P == 4) return (row >> 2) & 3;
+  else return 0;
+}
+
+// One chunk buffer and how its operand pieces are spread over the waves:
+// piece s of [mh | ml | th | tl] goes with DMA instruction s / (64 NW) of
+// wave (s / 64) % NW, lane s % 64; instructions past the operands fill an
+// unused tail (they reload a valid piece).
+template <int DP, int NW>
+struct Chunk {
+  static constexpr int PM = DP / 8;         // pieces per point-major row
+  static constexpr int PF = kPC / 8;        // pieces per feature-major row
+  static constexpr int NE = kPC * PM;       // pieces per operand tile
+  static constexpr int NT = 4 * NE;
+  static constexpr int PER = 64 * NW;       // pieces per instruction over the workgroup
+  static constexpr int NI = (NT + PER - 1) / PER;
+  static constexpr int seg(int s) { return s < NT ? s / NE : 4; }
+  static constexpr bool one_seg(int u) { return seg(u * PER) == seg(u * PER + PER - 1); }
+  v4u op[NI * PER];
+  double ex[NW][128];  // per wave: norms of the chunk's points, their alphas, unused
+};
+
+// Offset (halves) of operand piece s from its tile's chunk origin, with the
+// tile index in bits 29..31 (4: the unused tail).
+template <int DP, int NW>
+__device__ __forceinline__ unsigned piece_src(int s, int Mp) {
+  typedef Chunk<DP, NW> C;
+  const int g = C::seg(s);
+  if (g == 4) return 4u << 29;
+  const int e = s - g * C::NE;
+  unsigned off;
+  if (g < 2) {
+    const int row = e / C::PM, q = (e % C::PM) ^ swz<C::PM>(row);
+    off = (unsigned)(row * DP + 8 * q);
+  } else {
+    const int d = e / C::PF, r = (e % C::PF) ^ swz<C::PF>(d);
+    off = (unsigned)(d * Mp + 8 * r);
   }
-#pragma unroll
-  for (int u = 0; u < NPT; ++u) {
-    const int e = tid + 256 * u;
-    if (NE % 256 == 0 || e < NE) {
-      const int pc = e / (DP / 8), q = e - pc * (DP / 8);
-      *(v4u *)(s_mh + pc * SM + 8 * q) = h[u];
-      *(v4u *)(s_ml + pc * SM + 8 * q) = l[u];
-    }
-  }
-#pragma unroll
-  for (int u = 0; u < NPT; ++u) {
-    const int e = min(tid + 256 * u, NE - 1);
-    const int d = e / (PC / 8), r = e - d * (PC / 8);
-    const size_t g = (size_t)d * a.Mp + p0 + 8 * r;
-    h[u] = *(const v4u *)(a.mtT_hi + g);
-    l[u] = *(const v4u *)(a.mtT_lo + g);
-  }
-#pragma unroll
-  for (int u = 0; u < NPT; ++u) {
-    const int e = tid + 256 * u;
-    if (NE % 256 == 0 || e < NE) {
-      const int d = e / (PC / 8), r = e - d * (PC / 8);
-      *(v4u *)(s_th + d * ST + 8 * r) = h[u];
-      *(v4u *)(s_tl + d * ST + 8 * r) = l[u];
-    }
+  return off | ((unsigned)g << 29);
+}
+
+template <int U, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (U < N) {
+    f(std::integral_constant<int, U>{});
+    static_for<U + 1, N>(f);
   }
 }
 
-template <bool BF, int DP, int PC, int R>
-__global__ __launch_bounds__(256, 2) void k_gpr_mixed(GprArgs a) {
+// vmcnt(v) with the other counters left alone (gfx9 encoding)
+constexpr int vm_wait(int v) { return (v & 15) | ((v >> 4) << 14) | (7 << 4) | (15 << 8); }
+
+// This wave's direct loads except the v most recent have landed; after the
+// barrier every wave's have.
+template <int V>
+__device__ __forceinline__ void chunk_landed() {
+  __builtin_amdgcn_s_waitcnt(vm_wait(V));
+  asm volatile("s_barrier" ::: "memory");
+}
+
+// Every wave has finished reading the buffer it just worked on.
+__device__ __forceinline__ void chunk_consumed() {
+  __builtin_amdgcn_s_waitcnt(vm_wait(63) & ~(15 << 8));  // lgkmcnt(0)
+  asm volatile("s_barrier" ::: "memory");
+}
+
+template <bool BF, int DP, int R, int NW>
+__global__ __launch_bounds__(64 * NW, 8 / NW) void k_gpr_mixed(GprArgs a) {
   typedef typename Half<BF>::vec vec;
   typedef typename Half<BF>::scalar hs;
-  constexpr int SM = DP + 8;  // row stride (halves) of the point-major tile
-  constexpr int ST = PC + 8;  // row stride (halves) of the feature-major tile
-  __shared__ __align__(16) uint16_t s_mh[PC * SM], s_ml[PC * SM];
-  __shared__ __align__(16) uint16_t s_th[DP * ST], s_tl[DP * ST];
-  __shared__ double s_mn[PC];
-  __shared__ float s_c[PC];
+  typedef Chunk<DP, NW> C;
+  __shared__ __align__(16) C t0;
+  __shared__ __align__(16) C t1;
 
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int l16 = lane & 15, lq = lane >> 4;
-  const int row0 = (a.tile0 + blockIdx.x) * (64 * R) + wv * (16 * R);
-  const int split = blockIdx.y;
+  int tile, split;
+  xcd_tile_split(tile, split);
+  const int row0 = (a.tile0 + tile) * (16 * R * NW) + wv * (16 * R);
   const int pb = split * a.pts_per_split;
   const int pe = min(a.Mp, pb + a.pts_per_split);
+  const int nch = (pe - pb) / kPC;
 
   // B operands of S^T: row l16 of each row block, 8 features per lane per
   // 32-wide step
@@ -140,6 +179,34 @@ __global__ __launch_bounds__(256, 2) void k_gpr_mixed(GprArgs a) {
     xnr[rb] = a.xn[row];
   }
 
+  // this lane's operand pieces, one per DMA instruction
+  unsigned src[C::NI];
+#pragma unroll
+  for (int u = 0; u < C::NI; ++u) src[u] = piece_src<DP, NW>((u * NW + wv) * 64 + lane, a.Mp);
+
+  // norms and alphas, two points per lane; lanes past them reload a norm
+  // into the unused half (the loads are issued under a full lane mask)
+  const double *exsrc = lane < kPC / 2 ? a.mn + 2 * lane : lane < kPC ? a.coeff + 2 * (lane - kPC / 2) : a.mn;
+
+  // chunk at p0 -> buffer B: NI + 1 direct loads per wave
+  auto issue = [&](C &B, int p0) {
+    const uint16_t *base[4] = {a.mt_hi + (size_t)p0 * DP, a.mt_lo + (size_t)p0 * DP, a.mtT_hi + p0, a.mtT_lo + p0};
+    static_for<0, C::NI>([&](auto uc) {
+      constexpr int u = decltype(uc)::value;
+      const uint16_t *p;
+      if constexpr (C::one_seg(u)) {
+        constexpr int g = C::seg(u * C::PER);
+        p = g == 4 ? base[0] : base[g] + (src[u] & ((1u << 29) - 1));
+      } else {
+        const unsigned g = src[u] >> 29;
+        p = (g == 4 ? base[0] : (g == 0 ? base[0] : g == 1 ? base[1] : g == 2 ? base[2] : base[3]) +
+                                    (src[u] & ((1u << 29) - 1)));
+      }
+      __builtin_amdgcn_global_load_lds((const void *)p, (lds_void *)&B.op[(u * NW + wv) * 64], 16, 0, 0);
+    });
+    __builtin_amdgcn_global_load_lds((const void *)(exsrc + p0), (lds_void *)&B.ex[wv][0], 16, 0, 0);
+  };
+
   v4f G[R][DP / 16];  // G^T: features 16 b + 4 lq + r of row l16
 #pragma unroll
   for (int rb = 0; rb < R; ++rb)
@@ -149,72 +216,84 @@ __global__ __launch_bounds__(256, 2) void k_gpr_mixed(GprArgs a) {
 #pragma unroll
   for (int rb = 0; rb < R; ++rb) er[rb] = 0.0;
 
-  for (int p0 = pb; p0 < pe; p0 += PC) {
-    __syncthreads();
-    load_chunk<DP, PC, SM, ST>(a, p0, tid, s_mh, s_ml, s_th, s_tl);
-    if (tid < PC) {
-      s_mn[tid] = a.mn[p0 + tid];
-      s_c[tid] = (float)a.coeff[p0 + tid];
-    }
-    __syncthreads();
-
+  auto compute = [&](const C &B) {
+    // per 16-point block: S^T for the R row blocks, then its exp; the next
+    // block's products do not depend on this block's exp, so the matrix
+    // cores can run them while the vector units work on the exp
+    vec wh[R], wl[R];  // W in registers, already in the B layout of the G step
 #pragma unroll
-    for (int sc = 0; sc < PC / 32; ++sc) {
-      // S^T for two 16-point blocks x R row blocks
-      v4f S[2][R];
+    for (int h = 0; h < 2; ++h) {
+      v4f S[R];
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+      for (int rb = 0; rb < R; ++rb) S[rb] = v4f{0.f, 0.f, 0.f, 0.f};
+      const int pt = 16 * h + l16;
 #pragma unroll
-        for (int rb = 0; rb < R; ++rb) S[h][rb] = v4f{0.f, 0.f, 0.f, 0.f};
+      for (int t = 0; t < DP / 32; ++t) {
+        const int s = pt * C::PM + ((4 * t + lq) ^ swz<C::PM>(pt));
+        const vec mh = *(const vec *)&B.op[s];
+        const vec ml = *(const vec *)&B.op[C::NE + s];
 #pragma unroll
-      for (int t = 0; t < DP / 32; ++t)
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int pt = 32 * sc + 16 * h + l16;
-          const vec mh = *(const vec *)(s_mh + pt * SM + 32 * t + 8 * lq);
-          const vec ml = *(const vec *)(s_ml + pt * SM + 32 * t + 8 * lq);
-#pragma unroll
-          for (int rb = 0; rb < R; ++rb) {
-            S[h][rb] = Half<BF>::mfma(mh, xh[rb][t], S[h][rb]);
-            S[h][rb] = Half<BF>::mfma(ml, xh[rb][t], S[h][rb]);
-            S[h][rb] = Half<BF>::mfma(mh, xl[rb][t], S[h][rb]);
-          }
+        for (int rb = 0; rb < R; ++rb) {
+          S[rb] = Half<BF>::mfma(mh, xh[rb][t], S[rb]);
+          S[rb] = Half<BF>::mfma(ml, xh[rb][t], S[rb]);
+          S[rb] = Half<BF>::mfma(mh, xl[rb][t], S[rb]);
         }
-      // W in registers, already in the B layout of the G step
-      vec wh[R], wl[R];
+      }
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+      for (int r = 0; r < 4; ++r) {
+        const int pl = 16 * h + 4 * lq + r;
+        const double mnp = B.ex[wv][pl];
+        const float cp = (float)B.ex[wv][kPC + pl];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int pl = 32 * sc + 16 * h + 4 * lq + r;
-          const double mnp = s_mn[pl];
-          const float cp = s_c[pl];
-#pragma unroll
-          for (int rb = 0; rb < R; ++rb) {
-            const float d2 = (float)(xnr[rb] + mnp - 2.0 * (double)S[h][rb][r]);
-            const float w = __expf(-0.5f * d2) * cp;
-            er[rb] += (double)w;
-            const float ws = w * a.wscale;
-            const hs hi = (hs)ws;
-            wh[rb][4 * h + r] = hi;
-            wl[rb][4 * h + r] = (hs)(ws - (float)hi);
-          }
-        }
-      if (a.grad) {
-#pragma unroll
-        for (int b = 0; b < DP / 16; ++b) {
-          const int d = 16 * b + l16;
-          const vec th = *(const vec *)(s_th + d * ST + 32 * sc + 8 * lq);
-          const vec tl = *(const vec *)(s_tl + d * ST + 32 * sc + 8 * lq);
-#pragma unroll
-          for (int rb = 0; rb < R; ++rb) {
-            G[rb][b] = Half<BF>::mfma(th, wh[rb], G[rb][b]);
-            G[rb][b] = Half<BF>::mfma(tl, wh[rb], G[rb][b]);
-            G[rb][b] = Half<BF>::mfma(th, wl[rb], G[rb][b]);
-          }
+        for (int rb = 0; rb < R; ++rb) {
+          const float d2 = (float)(xnr[rb] + mnp - 2.0 * (double)S[rb][r]);
+          const float w = __expf(-0.5f * d2) * cp;
+          er[rb] += (double)w;
+          const float ws = w * a.wscale;
+          const hs hi = (hs)ws;
+          wh[rb][4 * h + r] = hi;
+          wl[rb][4 * h + r] = (hs)(ws - (float)hi);
         }
       }
     }
+    if (a.grad) {
+#pragma unroll
+      for (int b = 0; b < DP / 16; ++b) {
+        const int d = 16 * b + l16;
+        const int s = 2 * C::NE + d * C::PF + (lq ^ swz<C::PF>(d));
+        const vec th = *(const vec *)&B.op[s];
+        const vec tl = *(const vec *)&B.op[C::NE + s];
+#pragma unroll
+        for (int rb = 0; rb < R; ++rb) {
+          G[rb][b] = Half<BF>::mfma(th, wh[rb], G[rb][b]);
+          G[rb][b] = Half<BF>::mfma(tl, wh[rb], G[rb][b]);
+          G[rb][b] = Half<BF>::mfma(th, wl[rb], G[rb][b]);
+        }
+      }
+    }
+  };
+
+  // Two chunks per trip, one per buffer, so that each buffer is named
+  // statically (the compiler then waits for the loads into the buffer being
+  // read, not for the prefetch into the other one).  Every step prefetches,
+  // so that the wait is always "all but the newest NI + 1 loads": past the
+  // last chunk it loads the last chunk again into the idle buffer.
+  if (nch > 0) {
+    issue(t0, pb);
+    for (int c = 0; c < nch; c += 2) {
+      const int p0 = pb + c * kPC;
+      issue(t1, p0 + kPC * min(1, nch - 1 - c));
+      chunk_landed<C::NI + 1>();
+      compute(t0);
+      chunk_consumed();
+      if (c + 1 < nch) {
+        issue(t0, p0 + kPC * min(2, nch - 1 - c));
+        chunk_landed<C::NI + 1>();
+        compute(t1);
+        chunk_consumed();
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(vm_wait(0));
   }
 
   const size_t base = (size_t)split * a.Np;
@@ -236,29 +315,34 @@ __global__ __launch_bounds__(256, 2) void k_gpr_mixed(GprArgs a) {
   }
 }
 
-template <bool BF, int DP, int PC>
+// rows per workgroup: 64 (four waves of one 16-row block) or 128 (four
+// waves of two blocks up to DP = 128; eight waves of one block above, where
+// two chunk buffers fill the LDS of a CU)
+template <bool BF, int DP>
 hipError_t launch_rows(hipStream_t s, const GprArgs &a) {
-  if constexpr (DP <= 128) {
-    if (a.rows == 128) {
-      k_gpr_mixed<BF, DP, PC, 2><<<dim3(a.ntiles > 0 ? a.ntiles : a.Np / 128, a.splits), 256, 0, s>>>(a);
-      return hipGetLastError();
-    }
+  const int tiles = a.ntiles > 0 ? a.ntiles : a.Np / a.rows;
+  if (a.rows == 128) {
+    if constexpr (DP <= 128)
+      k_gpr_mixed<BF, DP, 2, 4><<<dim3(tiles, a.splits), 256, 0, s>>>(a);
+    else
+      k_gpr_mixed<BF, DP, 1, 8><<<dim3(tiles, a.splits), 512, 0, s>>>(a);
+  } else {
+    k_gpr_mixed<BF, DP, 1, 4><<<dim3(tiles, a.splits), 256, 0, s>>>(a);
   }
-  k_gpr_mixed<BF, DP, PC, 1><<<dim3(a.ntiles > 0 ? a.ntiles : a.Np / 64, a.splits), 256, 0, s>>>(a);
   return hipGetLastError();
 }
 
 template <bool BF>
 hipError_t launch_typed(hipStream_t s, const GprArgs &a) {
   switch (a.Dp) {
-    case 32: return launch_rows<BF, 32, 64>(s, a);
-    case 64: return launch_rows<BF, 64, 64>(s, a);
-    case 96: return launch_rows<BF, 96, 64>(s, a);
-    case 128: return launch_rows<BF, 128, 64>(s, a);
-    case 160: return launch_rows<BF, 160, 32>(s, a);
-    case 192: return launch_rows<BF, 192, 32>(s, a);
-    case 224: return launch_rows<BF, 224, 32>(s, a);
-    case 256: return launch_rows<BF, 256, 32>(s, a);
+    case 32: return launch_rows<BF, 32>(s, a);
+    case 64: return launch_rows<BF, 64>(s, a);
+    case 96: return launch_rows<BF, 96>(s, a);
+    case 128: return launch_rows<BF, 128>(s, a);
+    case 160: return launch_rows<BF, 160>(s, a);
+    case 192: return launch_rows<BF, 192>(s, a);
+    case 224: return launch_rows<BF, 224>(s, a);
+    case 256: return launch_rows<BF, 256>(s, a);
     default: return hipErrorInvalidValue;
   }
 }
@@ -268,8 +352,7 @@ hipError_t launch_typed(hipStream_t s, const GprArgs &a) {
 int gpr_points_per_chunk(int Dp) { return Dp <= 128 ? 64 : 32; }
 
 hipError_t launch_gpr_mixed(hipStream_t s, const GprArgs &a) {
-  if (a.pts_per_split % gpr_points_per_chunk(a.Dp) != 0 || (a.rows != 64 && a.rows != 128) ||
-      a.Np % a.rows != 0)
+  if (a.pts_per_split % kPC != 0 || a.Mp % kPC != 0 || (a.rows != 64 && a.rows != 128) || a.Np % a.rows != 0)
     return hipErrorInvalidValue;
   return a.bf16 ? launch_typed<true>(s, a) : launch_typed<false>(s, a);
 }

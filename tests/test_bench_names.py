@@ -17,10 +17,13 @@ def name(*a, **k):
 
 def test_contraction_names_follow_the_padded_descriptor_length():
     t = {"gpr_rows": 128, "waves_fwd": 1, "waves_bwd": 2, "gpr_blocks": 512}
-    assert name("gpr_contraction", True, t, True, 128, 4096, 4000) == "k_gpr_mixed<false, 128, 64, 2>"
-    # above 128 features: 32-point chunks and 64-row tiles only
-    assert name("gpr_contraction", True, t, True, 256, 4096, 4000) == "k_gpr_mixed<false, 256, 32, 1>"
-    assert name("gpr_contraction", True, t, True, 66, 4096, 4000) == "k_gpr_mixed<false, 96, 64, 2>"
+    assert name("gpr_contraction", True, t, True, 128, 4096, 4000) == "k_gpr_mixed<false, 128, 2, 4>"
+    assert name("gpr_contraction", True, dict(t, gpr_rows=64), True, 128, 4096, 4000) == \
+        "k_gpr_mixed<false, 128, 1, 4>"
+    # above 128 features: eight waves of one 16-row block, whatever the tuning says
+    assert name("gpr_contraction", True, dict(t, gpr_rows=64), True, 256, 4096, 4000) == \
+        "k_gpr_mixed<false, 256, 1, 8>"
+    assert name("gpr_contraction", True, t, True, 66, 4096, 4000) == "k_gpr_mixed<false, 96, 2, 4>"
     assert name("gpr_contraction", False, t, True, 256, 4096, 4000) == "k_gpr_f64<256>"
     # a model whose mixed runs contract in FP64
     assert name("gpr_contraction", True, t, True, 66, 64, 129, c64=True) == "k_gpr_f64<96>"
@@ -41,3 +44,7 @@ def test_contraction_grid_follows_the_host_split():
     assert bench.contraction_grid(4096, 128, 4000, True, t) == 32 * 16 * 256
     # the FP64 kernel always takes 64-row tiles
     assert bench.contraction_grid(4096, 128, 4000, False, t) == 64 * 8 * 256
+    # 32768 atoms, D = 256, M = 8000: 128-row tiles of 512 threads, 32-point
+    # chunks, 1024 workgroups targeted: 256 tiles, 4 splits of 2016 points
+    t = {"gpr_rows": 64, "gpr_blocks": 1024}
+    assert bench.contraction_grid(32768, 256, 8000, True, t) == 256 * 4 * 512
