@@ -435,6 +435,7 @@ __global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ ato
       if (ok && pos < nb.kcap) {
         const size_t sl = base + pos;
         nb.nb_j[sl] = inf.x;
+        nb.nb_rev[sl] = -1;  // k_reverse fills it from whichever side leads the pair
         nb.nb_key[sl] = nb_key(inf.x, T0 - inf.y + si0, T1 - inf.z + si1, T2 - inf.w + si2);
         double2 *d2 = (double2 *)(nb.nb_d + 4 * sl);
         d2[0] = make_double2(dx, dy);
@@ -579,12 +580,16 @@ __global__ void k_set_u64(unsigned long long *p, unsigned long long v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
-// For slot (i -> j, n) find the slot (j -> i, -n); -1 if absent (a pair within
-// one rounding step of rcut can be seen from one side only).  One wavefront
-// per centre atom, one slot per lane; j's packed keys are read 16 at a time
-// (8 independent 16-byte loads) and the first match wins.  nb_key carries 16
-// keys of padding past the last atom's slots, so the batched reads stay in
-// bounds; entries at or beyond j's count are never compared.
+// For slot (i -> j, n) find the slot (j -> i, -n), and the other way round;
+// the slots start at -1 (k_build writes it), which stays where the partner is
+// absent (a pair within one rounding step of rcut can be seen from one side
+// only).  Each pair is searched once, from its leading side (i < j, or i == j
+// with n lexicographically positive), which writes both entries: no other
+// lane writes either.  One wavefront per centre atom, one slot per lane; j's
+// packed keys are read 16 at a time (8 independent 16-byte loads) and the
+// first match wins.  nb_key carries 16 keys of padding past the last atom's
+// slots, so the batched reads stay in bounds; entries at or beyond j's count
+// are never compared.
 __global__ __launch_bounds__(64) void k_reverse(int N, NeighborBufs nb) {
   const int i = blockIdx.x;
   if (i >= N) return;
@@ -594,6 +599,8 @@ __global__ __launch_bounds__(64) void k_reverse(int N, NeighborBufs nb) {
     const unsigned long long key = nb.nb_key[base + a];
     const int j = (int)(key >> 24);
     const int n0 = (int)((key >> 16) & 255) - 128, n1 = (int)((key >> 8) & 255) - 128, n2 = (int)(key & 255) - 128;
+    const bool lead = i < j || (i == j && (n0 > 0 || (n0 == 0 && (n1 > 0 || (n1 == 0 && n2 > 0)))));
+    if (!lead) continue;
     const unsigned long long want = nb_key(i, -n0, -n1, -n2);
     const size_t jb = (size_t)j * nb.kcap;
     const ulonglong2 *kj = (const ulonglong2 *)(nb.nb_key + jb);
@@ -611,7 +618,10 @@ __global__ __launch_bounds__(64) void k_reverse(int N, NeighborBufs nb) {
       for (int u = 15; u >= 0; --u)
         if (q0 + u < Kj && k[u] == want) found = q0 + u;
     }
-    nb.nb_rev[base + a] = (found >= 0) ? (int)(jb + found) : -1;
+    if (found >= 0) {
+      nb.nb_rev[base + a] = (int)(jb + found);
+      nb.nb_rev[jb + found] = (int)(base + a);
+    }
   }
 }
 
