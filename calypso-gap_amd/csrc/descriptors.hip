@@ -727,10 +727,17 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
   }
 }
 
+// Values per eta record of the grid-form backward (see k_desc_bwd): 4 NZ g
+// values, eta and its exp coefficient, rounded up to whole 16-byte groups.
+template <int NZ>
+struct EtaRec {
+  static constexpr int RS = NZ > 0 ? (4 * NZ + 2 + 3) & ~3 : 0;
+};
+
 template <typename T, int NZ, bool Z16>
 __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDev &sf, const T *s_d, const T *s_rw,
-                                             const T *s_fd, const T *s_gA, const T *s_gW, const T *s_eta,
-                                             const T *s_ceta, int KP, T rc, T pirc, double *s_T, double *s_F) {
+                                             const T *s_fd, const T *s_er, int KP, T rc, T pirc, double *s_T,
+                                             double *s_F) {
   if (!active) return;
   const int a = q & 0xffff, b = q >> 16;
   const T ax = s_d[4 * a], ay = s_d[4 * a + 1], az = s_d[4 * a + 2], ra = s_d[4 * a + 3];
@@ -762,13 +769,13 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
   // one eta: the lines' g contracted with the powers (sb) and with their
   // cos-derivatives (sa), each sum started from its first term
   auto eta_terms = [&](int h) {
-    const T eta = s_eta[h];
-    const T E = exp_coef(s_ceta[h], S2);
-    const int s0 = h * 2 * NZ;
+    const T *rec = s_er + h * EtaRec<NZ>::RS;
+    const T eta = rec[4 * NZ];
+    const T E = exp_coef(rec[4 * NZ + 1], S2);
     T gA[2 * NZ], gW[2 * NZ];
     if constexpr (sizeof(T) == 4 && NZ % 2 == 0) {
-      // 16-byte reads: s0 is a multiple of 4 and both blocks are aligned
-      const float4 *a4 = (const float4 *)(s_gA + s0), *w4 = (const float4 *)(s_gW + s0);
+      // 16-byte reads: records and both blocks are aligned
+      const float4 *a4 = (const float4 *)rec, *w4 = (const float4 *)(rec + 2 * NZ);
 #pragma unroll
       for (int v = 0; v < NZ / 2; ++v) {
         const float4 x = a4[v], y = w4[v];
@@ -778,19 +785,25 @@ __device__ __forceinline__ void ang_bwd_grid(uint32_t q, bool active, const SFDe
     } else {
 #pragma unroll
       for (int j = 0; j < 2 * NZ; ++j) {
-        gA[j] = s_gA[s0 + j];
-        gW[j] = s_gW[s0 + j];
+        gA[j] = rec[j];
+        gW[j] = rec[2 * NZ + j];
       }
     }
+    // one fused multiply-add per term (a sum of term pairs would cost a
+    // multiply, an fma and an add per pair)
     const T gp0 = gA[0] + wab * gW[0], gm0 = gA[NZ] + wab * gW[NZ];
-    T sa = gp0 * dPp[0] + gm0 * dPm[0];
-    T sb = gp0 * Pp[0] + gm0 * Pm[0];
+    T sa = gp0 * dPp[0];
+    T sb = gp0 * Pp[0];
+    sa = fma(gm0, dPm[0], sa);
+    sb = fma(gm0, Pm[0], sb);
 #pragma unroll
     for (int k = 1; k < NZ; ++k) {
       const T gp = gA[k] + wab * gW[k];
       const T gm = gA[NZ + k] + wab * gW[NZ + k];
-      sa += gp * dPp[k] + gm * dPm[k];
-      sb += gp * Pp[k] + gm * Pm[k];
+      sa = fma(gp, dPp[k], sa);
+      sb = fma(gp, Pp[k], sb);
+      sa = fma(gm, dPm[k], sa);
+      sb = fma(gm, Pm[k], sb);
     }
     SA += sa * E;
     SB += sb * E;
@@ -912,14 +925,16 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   // atom's row of the global buffer
   T *s_fd = FG ? (T *)nb.fc_scratch + (size_t)i * 2 * G * KP : s_rw + 2 * KP;
   T *s_g = FG ? s_rw + 2 * KP : s_fd + 2 * G * KP;
-  T *s_gA = s_g + Dp;  // per angular line: g of the plain and of the weighted feature
-  // s_gW starts on a 4-value boundary: with 16-byte aligned blocks (every
-  // region above has a multiple of 4 values) the grid form reads four lines'
-  // g at once; n_ang <= Dp / 2, so both fit before s_eta
+  // generic form: per angular line, g of the plain and of the weighted
+  // feature; s_gW starts on a 4-value boundary (n_ang <= Dp / 2)
+  T *s_gA = s_g + Dp;
   T *s_gW = s_gA + ((sf.n_ang + 3) & ~3);
-  // grid form: eta of each pass and its exp coefficient (at most D / 4 etas)
-  T *s_eta = s_g + 2 * Dp;     // [Dp / 4]
-  T *s_ceta = s_eta + Dp / 4;  // [Dp / 4]
+  // grid form: one record per eta, [g plain (2 NZ) | g weighted (2 NZ) | eta |
+  // exp coefficient | pad], so that a pass reads its eta's values from one
+  // base address; 16-byte aligned records (every region above has a multiple
+  // of 4 values), at most 2 Dp values (n_eta <= Dp / (4 NZ))
+  constexpr int RS = EtaRec<GNZ>::RS;
+  T *s_er = s_g + Dp;
 
   const int start = i * nb.kcap;
   const int K = nb.nb_count[i];
@@ -946,8 +961,8 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   for (int k = tid; k < 4 * W * KP; k += 64 * W) s_T[k] = 0.0;  // s_T and s_F
   if (GNZ > 0)
     for (int h = tid; h < sf.n_eta; h += 64 * W) {
-      s_eta[h] = (T)sf.ang_eta[h * 2 * GNZ];
-      s_ceta[h] = eta_coef<T>((T)sf.ang_eta[h * 2 * GNZ]);
+      s_er[h * RS + 4 * GNZ] = (T)sf.ang_eta[h * 2 * GNZ];
+      s_er[h * RS + 4 * GNZ + 1] = eta_coef<T>((T)sf.ang_eta[h * 2 * GNZ]);
     }
   const size_t row = (size_t)i * Dp;
   if (gs.splits > 0) {
@@ -976,8 +991,14 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
   __syncthreads();
   for (int s = tid; s < sf.n_ang; s += 64 * W) {
     const int f = sf.ang_feat[s];
-    s_gA[s] = s_g[f];
-    s_gW[s] = s_g[f + sf.nsf];
+    if constexpr (GNZ > 0) {
+      const int h = s / (2 * GNZ), k = s - h * 2 * GNZ;
+      s_er[h * RS + k] = s_g[f];
+      s_er[h * RS + 2 * GNZ + k] = s_g[f + sf.nsf];
+    } else {
+      s_gA[s] = s_g[f];
+      s_gW[s] = s_g[f + sf.nsf];
+    }
   }
   __syncthreads();
 
@@ -1012,8 +1033,8 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
         const bool act = t0 + u * ST + lane < nt;
         const uint32_t q = pop_q4(qc);
         if constexpr (GNZ > 0)
-          ang_bwd_grid<T, GNZ, Z16>(q, act, sf, s_d, s_rw, s_fd, s_gA, s_gW, s_eta, s_ceta, KP, rcA, pirc,
-                                    s_T + wv * KP, s_F + 3 * wv * KP);
+          ang_bwd_grid<T, GNZ, Z16>(q, act, sf, s_d, s_rw, s_fd, s_er, KP, rcA, pirc, s_T + wv * KP,
+                                    s_F + 3 * wv * KP);
         else
           ang_bwd_term<T>(q, act, sf, s_d, s_rw, s_fd, s_gA, s_gW, KP, s_T + wv * KP, s_F + 3 * wv * KP);
       }
@@ -1074,7 +1095,7 @@ size_t fwd_lds(const Lds &L) {
 template <typename T, int W, bool FG>
 size_t bwd_lds(const Lds &L) {
   return (6 * W + 4 * W * (size_t)L.KP) * sizeof(double) +
-         (size_t)(6 * L.KP + (FG ? 0 : 2 * L.G * L.KP) + 2 * L.Dp + L.Dp / 2) * sizeof(T);
+         (size_t)(6 * L.KP + (FG ? 0 : 2 * L.G * L.KP) + 3 * L.Dp) * sizeof(T);
 }
 
 template <typename T, int NZ, int W, bool FG>
