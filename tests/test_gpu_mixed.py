@@ -199,3 +199,67 @@ def test_row_tiles_agree():
         assert abs(out[rows]["energy"][0] - out[64]["energy"][0]) <= 1e-9 * max(1.0, abs(out[64]["energy"][0]))
         np.testing.assert_allclose(out[rows]["forces"], out[64]["forces"], rtol=0, atol=1e-9)
         np.testing.assert_allclose(out[rows]["stress"], out[64]["stress"], rtol=0, atol=1e-9)
+
+
+def table_model(rng, s, nrad, zeta=(1, 2, 4, 16), M=200):
+    """A model with nrad radial lines (widths log-spaced, two cutoffs) and
+    2 etas x 2 angular types x zeta lines, theta and the sparse points taken
+    from the structure's own descriptors (kernel values O(1))."""
+    types, alphas, cutl, zl = [], [], [], []
+    for k, a in enumerate(np.geomspace(0.01, 4.0, nrad)):
+        types.append(1 if k % 3 else 3)
+        alphas.append(float(a))
+        cutl.append(6.0 if k % 2 == 0 else 4.5)
+        zl.append(1.0)
+    for eta in (0.01, 0.1):
+        for t in (2, 4):
+            for zz in zeta:
+                types.append(t)
+                alphas.append(eta)
+                cutl.append(6.0)
+                zl.append(float(zz))
+    D = 2 * len(types)
+    om = oracle.Model(z=np.array((1, 8, 22), np.int32), w=np.array((1.0, -0.5, 2.0)),
+                      sf_type=np.array(types, np.int32), sf_alpha=np.array(alphas), sf_cut=np.array(cutl),
+                      theta=np.ones(D), mm=np.zeros((M, D)), coeff=rng.normal(0, 0.5, M), sf_zeta=np.array(zl))
+    x = oracle.descriptors(om, s["species"], s["lat"], s["pos"], 6.0)
+    # a feature (nearly) constant over the atoms gets a length scale of a
+    # hundredth of its size rather than its spread
+    om.theta = np.maximum(x.std(axis=0) * 1.5, 1e-2 * np.abs(x).max(axis=0)) + 1e-3
+    idx = rng.choice(len(x), size=M, replace=len(x) < M)
+    om.mm = x[idx] + rng.normal(0, 0.2, size=(M, D)) * om.theta
+    return om
+
+
+# descriptor lengths padded to 32, 96, 160, 192, 224 and 256: the mixed
+# contraction's chunk loads are laid out per length (tiles whose 16-byte
+# pieces fill whole load instructions at 64, 128 and 256; tiles that share an
+# instruction with the next tile at the others), and above 128 it runs eight
+# waves of one row block whatever the row-tile setting
+@pytest.mark.parametrize("nrad", [0, 30, 62, 78, 87, 110], ids=["Dp32", "Dp96", "Dp160", "Dp192", "Dp224", "Dp256"])
+def test_mixed_descriptor_lengths_vs_fp64(nrad):
+    rng = np.random.default_rng(700 + nrad)
+    s = ternary_structure(96, 12.0, 901 + nrad)
+    om = table_model(rng, s, nrad)
+    dp = (om.des_len + 31) // 32 * 32
+    assert dp == {0: 32, 30: 96, 62: 160, 78: 192, 87: 224, 110: 256}[nrad]
+    m = gapx.Model.from_model(om)
+    assert not m.precision()["mixed_runs_fp64"] and not m.precision()["mixed_contraction_fp64"]
+    r64 = m.calc(s["species"], s["lat"], s["pos"], 6.0)
+    flags = gapx.GRAD | gapx.MIXED
+    out = {}
+    for rows in (64, 128):
+        ctx = gapx.Context(m)
+        ctx.load([s], 6.0)
+        ctx.set_tuning(flags, waves_fwd=1, waves_bwd=1, gpr_rows=rows, gpr_blocks=64)
+        ctx.run(flags)
+        out[rows] = ctx.results()
+        ctx.close()
+    m.close()
+    for rows, r in out.items():
+        de, df, ds = errors((r["energy"][0], r["forces"], r["stress"][0]), r64, len(s["species"]))
+        print("Dp %d rows %d: dE %.3e dF %.3e dS %.3e" % (dp, rows, de, df, ds))
+        assert de <= TOL_E and df <= TOL_F and ds <= TOL_S
+    if dp > 128:  # one kernel shape above 128 columns: the row setting changes nothing
+        for k in ("energy", "forces", "stress"):
+            assert np.array_equal(out[64][k], out[128][k]), k
