@@ -1219,15 +1219,14 @@ static int ctx_autotune(Ctx &c, int flags, bool force) {
   // whole evaluations, back to back on the stream, replayed from a graph
   // captured for each candidate (so host launch costs, which do not depend
   // on the launch shapes, stay out of the comparison): the mean of `reps`
-  // runs between two events, best of three trials
-  const int reps = 10;
+  // runs between two events, best of `trials` trials
   c.use_graph = true;
-  auto measure = [&](double &out) -> int {
+  auto measure = [&](double &out, int reps, int trials) -> int {
     c.invalidate_graph();
     int r = ctx_run(c, flags);  // capture, warm-up (buffers, caches)
     if (r) return r;
     out = 1e300;
-    for (int trial = 0; trial < 3; ++trial) {
+    for (int trial = 0; trial < trials; ++trial) {
       GAPX_HIP(hipEventRecord(e0, c.stream));
       for (int k = 0; k < reps; ++k)
         if ((r = ctx_run(c, flags))) return r;
@@ -1239,27 +1238,33 @@ static int ctx_autotune(Ctx &c, int flags, bool force) {
     }
     return GAPX_OK;
   };
-  // the descriptor kernels' waves one knob at a time; then the contraction's
-  // row tile and workgroup target together (they interact: the target sets
-  // the split of the sparse points, the tile the number of row tiles)
+  // Run-to-run noise is about 1 %: a candidate replaces the current choice
+  // only when it is faster by more than kMargin, and each search starts
+  // from the current choice, measured first.
+  constexpr double kMargin = 0.01;
+  Tuning &t = c.tune[p];
+  const Tuning start = t;
   struct Knob {
     int *field;
     std::vector<int> values;
   };
-  Tuning &t = c.tune[p];
   std::vector<Knob> knobs;
   knobs.push_back({&t.waves_fwd, {1, 2, 4}});
   if (flags & GAPX_GRAD) knobs.push_back({&t.waves_bwd, {1, 2, 4}});
+  // the descriptor kernels' waves one knob at a time
   auto descend = [&]() {
     for (Knob &k : knobs) {
       if (rc) break;
-      int best = *k.field;
-      double best_ms = 1e300;
+      const int cur = *k.field;
+      double best_ms = 0;
+      if ((rc = measure(best_ms, 10, 3))) break;
+      int best = cur;
       for (int v : k.values) {
+        if (v == cur) continue;
         *k.field = v;
         double ms = 0;
-        if ((rc = measure(ms))) break;
-        if (ms < best_ms) {
+        if ((rc = measure(ms, 10, 3))) break;
+        if (ms < best_ms * (1.0 - kMargin)) {
           best_ms = ms;
           best = v;
         }
@@ -1267,36 +1272,51 @@ static int ctx_autotune(Ctx &c, int flags, bool force) {
       *k.field = best;
     }
   };
-  descend();
+  if (!rc) descend();
+  // then the contraction's row tile and workgroup target together (they
+  // interact: the target sets the split of the sparse points, the tile the
+  // number of row tiles)
   if (!rc) {
     const std::vector<int> rows = (p == 1 && c.m->Dp <= 128) ? std::vector<int>{64, 128} : std::vector<int>{t.gpr_rows};
     const std::vector<int> blocks = {256, 512, 768, 1024, 1536, 2048, 3072};
-    int best_r = t.gpr_rows, best_b = t.gpr_blocks;
-    double best_ms = 1e300;
+    const int cur_r = t.gpr_rows, cur_b = t.gpr_blocks;
+    double best_ms = 0;
+    rc = measure(best_ms, 10, 3);
+    int best_r = cur_r, best_b = cur_b;
     for (int r : rows) {
       for (int b : blocks) {
+        if (rc || (r == cur_r && b == cur_b)) continue;
         t.gpr_rows = r;
         t.gpr_blocks = b;
         double ms = 0;
-        if ((rc = measure(ms))) break;
-        if (ms < best_ms) {
+        if ((rc = measure(ms, 10, 3))) break;
+        if (ms < best_ms * (1.0 - kMargin)) {
           best_ms = ms;
           best_r = r;
           best_b = b;
         }
       }
-      if (rc) break;
     }
     t.gpr_rows = best_r;
     t.gpr_blocks = best_b;
   }
-  descend();  // the waves again, now beside the chosen contraction shape
+  if (!rc) descend();  // the waves again, now beside the chosen contraction shape
   if (!rc && (flags & GAPX_GRAD)) {  // the two-half schedule, where the batch is large enough for it
-    double on_ms = 0, off_ms = 0;
-    t.halves = 1;
-    if (!(rc = measure(on_ms))) {
-      t.halves = 0;
-      if (!(rc = measure(off_ms))) t.halves = (on_ms <= off_ms) ? 1 : 0;
+    const int cur = t.halves;
+    double cur_ms = 0, other_ms = 0;
+    if (!(rc = measure(cur_ms, 10, 3))) {
+      t.halves = 1 - cur;
+      if (!(rc = measure(other_ms, 10, 3))) t.halves = (other_ms < cur_ms * (1.0 - kMargin)) ? 1 - cur : cur;
+    }
+  }
+  // last, the result against the shapes the search started from, measured
+  // longer: the search keeps its result only if it is faster
+  if (!rc) {
+    const Tuning found = t;
+    double found_ms = 0, start_ms = 0;
+    if (!(rc = measure(found_ms, 20, 5))) {
+      t = start;
+      if (!(rc = measure(start_ms, 20, 5))) t = (found_ms < start_ms) ? found : start;
     }
   }
   if (e0) hipEventDestroy(e0);
