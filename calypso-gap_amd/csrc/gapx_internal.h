@@ -245,6 +245,14 @@ __device__ __forceinline__ void xcd_tile_split(int &tile, int &split) {
   }
 }
 
+// Work unit of block b of a 1-D range of nb blocks (nb a multiple of 8):
+// workgroups go to the eight XCDs in turn, so block b runs on XCD b % 8;
+// giving XCD x the consecutive units [x nb/8, (x + 1) nb/8) keeps the atoms
+// of one structure, whose lists point at each other, on one XCD and its L2.
+// Units at or past the real count are padding (the caller returns).
+__device__ __forceinline__ int xcd_unit(int b, int nb) { return (b & 7) * (nb >> 3) + (b >> 3); }
+constexpr int xcd_blocks(int n) { return (n + 7) & ~7; }
+
 struct GradSrc {
   const double *g64 = nullptr;
   const float *g32 = nullptr;

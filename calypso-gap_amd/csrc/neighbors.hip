@@ -642,8 +642,12 @@ __device__ __forceinline__ V wave_reduce_sum(V v) {
 //   blocks [P, P + T)      the list statistics of kReduceAtoms atoms each,
 //                          folded with integer atomics; the last block writes
 //                          them out and clears the accumulators;
-//   blocks P + T + g       forces, one wavefront per atom (four atoms per
-//                          block g < G): F_m = sum of own slots - sum of
+//   blocks from Q = P + T rounded up to a multiple of 8 (the blocks in
+//                          between are empty): forces, one wavefront per atom,
+//                          four atoms per group g < G, the groups laid out so
+//                          that each XCD takes a contiguous range of atoms
+//                          (xcd_unit: a structure's reverse slots are read
+//                          from one L2): F_m = sum of own slots - sum of
 //                          reverse slots in slot order (zeros without grad).
 // The sums go first so that they run beside the gather blocks.
 __global__ __launch_bounds__(256) void k_forces_reduce(int N, int G, int nstruct, NeighborBufs nb,
@@ -774,7 +778,9 @@ __global__ __launch_bounds__(256) void k_forces_reduce(int N, int G, int nstruct
     if (t < 8) rb.stats_acc[t] = 0;
     return;
   }
-  const int g = b - rb.nparts - T, m = 4 * g + wv;
+  const int Q = xcd_blocks(rb.nparts + T);  // the gather blocks start at an 8-aligned index
+  if (b < Q) return;
+  const int g = xcd_unit(b - Q, xcd_blocks(G)), m = 4 * g + wv;
   if (g >= G || m >= N) return;
   double f0 = 0, f1 = 0, f2 = 0;
   if (grad) {
@@ -904,8 +910,9 @@ hipError_t launch_forces_reduce(hipStream_t s, int N, int grad, const NeighborBu
                                 const ReduceBufs &rb, int *stats_out) {
   if (N <= 0 || nstruct <= 0 || rb.nparts < nstruct) return hipErrorInvalidValue;
   const int G = (N + 3) / 4, T = (N + kReduceAtoms - 1) / kReduceAtoms;
-  k_forces_reduce<<<rb.nparts + T + G, 256, 0, s>>>(N, G, nstruct, nb, slot_f, force, sp, eatom, atom_vir, grad, ene,
-                                                    stress6, tb, rb, stats_out);
+  k_forces_reduce<<<xcd_blocks(rb.nparts + T) + xcd_blocks(G), 256, 0, s>>>(N, G, nstruct, nb, slot_f, force, sp,
+                                                                            eatom, atom_vir, grad, ene, stress6,
+                                                                            tb, rb, stats_out);
   return hipGetLastError();
 }
 
