@@ -268,9 +268,11 @@ static int upload_model(Model &m) {
   // device layout
   Arena ar;
   double *rad_a, *rad_rs, *ang_eta, *ang_lam, *ang_pref, *ang_zr, *group_rc;
+  float *rad_a32, *rad_rs32;
   int *rad_group, *rad_feat, *ang_zeta, *ang_group, *ang_feat;
   const int nr = std::max(1, t.n_rad), na = std::max(1, t.n_ang), ng = std::max(1, t.n_groups);
   ar.add(&rad_a, nr); ar.add(&rad_rs, nr); ar.add(&rad_group, nr); ar.add(&rad_feat, nr);
+  ar.add(&rad_a32, nr); ar.add(&rad_rs32, nr);
   ar.add(&ang_eta, na); ar.add(&ang_lam, na); ar.add(&ang_pref, na); ar.add(&ang_zr, na);
   ar.add(&ang_zeta, na); ar.add(&ang_group, na); ar.add(&ang_feat, na); ar.add(&group_rc, ng);
   ar.add(&m.d_wz, kMaxSpeciesZ); ar.add(&m.d_hasz, kMaxSpeciesZ);
@@ -298,6 +300,13 @@ static int upload_model(Model &m) {
   GAPX_HIP(put(dst, (vec).empty() ? (const void *)zero.data() : (const void *)(vec).data(), \
                std::max<size_t>(1, (vec).size()) * sizeof(T)))
   PUT(rad_a, t.rad_a, double); PUT(rad_rs, t.rad_rs, double);
+  {
+    // the same tables rounded to FP32 for the FP32 kernels (no conversion per line)
+    std::vector<float> a32(t.rad_a.begin(), t.rad_a.end()), rs32(t.rad_rs.begin(), t.rad_rs.end());
+    std::vector<float> fzero(1, 0.f);
+    GAPX_HIP(put(rad_a32, a32.empty() ? fzero.data() : a32.data(), std::max<size_t>(1, a32.size()) * 4));
+    GAPX_HIP(put(rad_rs32, rs32.empty() ? fzero.data() : rs32.data(), std::max<size_t>(1, rs32.size()) * 4));
+  }
   if (!t.rad_group.empty()) { GAPX_HIP(put(rad_group, t.rad_group.data(), t.rad_group.size() * 4)); GAPX_HIP(put(rad_feat, t.rad_feat.data(), t.rad_feat.size() * 4)); }
   PUT(ang_eta, t.ang_eta, double); PUT(ang_lam, t.ang_lam, double); PUT(ang_pref, t.ang_pref, double);
   PUT(ang_zr, t.ang_zeta_real, double);
@@ -331,6 +340,7 @@ static int upload_model(Model &m) {
   sd.rc_max_rad = t.rc_max_rad;
   sd.rc_max_ang = t.rc_max_ang;
   sd.rad_a = rad_a; sd.rad_rs = rad_rs; sd.rad_group = rad_group; sd.rad_feat = rad_feat;
+  sd.rad_a32 = rad_a32; sd.rad_rs32 = rad_rs32;
   sd.ang_eta = ang_eta; sd.ang_lam = ang_lam; sd.ang_pref = ang_pref; sd.ang_zeta_real = ang_zr;
   sd.ang_zeta = ang_zeta; sd.ang_group = ang_group; sd.ang_feat = ang_feat; sd.group_rc = group_rc;
   detect_grid(t, sd);

@@ -83,6 +83,17 @@ struct Pair<double> {
   typedef double2 type;
 };
 
+// A radial-table entry in the kernel's precision: the FP32 copy for FP32
+// kernels (the same rounding as converting the FP64 entry, without a
+// conversion per line and lane).
+template <typename T>
+__device__ __forceinline__ T rad_table(const double *t64, const float *t32, int s) {
+  if constexpr (sizeof(T) == 4)
+    return t32[s];
+  else
+    return t64[s];
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
@@ -314,8 +325,8 @@ __global__ __launch_bounds__(256) void k_desc_fwd(int N, const int *__restrict__
       for (int l = 0; l < NCH; ++l) {
         const int s = c0 + l;
         if (s < sf.n_rad) {
-          const T dr = r - (T)sf.rad_rs[s];
-          const T t = Math<T>::ex(-(T)sf.rad_a[s] * dr * dr) * s_fc[sf.rad_group[s] * KP + a];
+          const T dr = r - rad_table<T>(sf.rad_rs, sf.rad_rs32, s);
+          const T t = Math<T>::ex(-rad_table<T>(sf.rad_a, sf.rad_a32, s) * dr * dr) * s_fc[sf.rad_group[s] * KP + a];
           acc[l] += t;
           accw[l] += t * w;
         }
@@ -610,8 +621,8 @@ __global__ __launch_bounds__(256) void k_desc_fwd_grid(int N, const int *__restr
       for (int l = 0; l < RCH; ++l) {
         const int s = c0 + l;
         if (s < sf.n_rad) {
-          const T dr = r - (T)sf.rad_rs[s];
-          const T t = Math<T>::ex(-(T)sf.rad_a[s] * dr * dr) * s_fc[sf.rad_group[s] * KP + a];
+          const T dr = r - rad_table<T>(sf.rad_rs, sf.rad_rs32, s);
+          const T t = Math<T>::ex(-rad_table<T>(sf.rad_a, sf.rad_a32, s) * dr * dr) * s_fc[sf.rad_group[s] * KP + a];
           acc[l] += t;
           acc[RCH + l] += t * w;
         }
@@ -1009,8 +1020,8 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
     for (int s = 0; s < sf.n_rad; ++s) {
       const int f = sf.rad_feat[s];
       const int g = sf.rad_group[s];
-      const T A = (T)sf.rad_a[s];
-      const T dr = r - (T)sf.rad_rs[s];
+      const T A = rad_table<T>(sf.rad_a, sf.rad_a32, s);
+      const T dr = r - rad_table<T>(sf.rad_rs, sf.rad_rs32, s);
       const T geff = s_g[f] + w * s_g[f + sf.nsf];
       acc += geff * Math<T>::ex(-A * dr * dr) * (T(-2) * A * dr * s_fd[2 * (g * KP + a)] + s_fd[2 * (g * KP + a) + 1]);
     }

@@ -45,6 +45,7 @@ struct SFDev {
   int zlog[5];             // log2 zeta_k
   double zeta_k[5], pref_k[5];
   const double *rad_a, *rad_rs;
+  const float *rad_a32, *rad_rs32;  // the same, rounded to FP32
   const int *rad_group, *rad_feat;
   const double *ang_eta, *ang_lam, *ang_pref, *ang_zeta_real;
   const int *ang_zeta, *ang_group, *ang_feat;
