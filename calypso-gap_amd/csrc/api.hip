@@ -278,6 +278,7 @@ static int upload_model(Model &m) {
   ar.add(&m.d_wz, kMaxSpeciesZ); ar.add(&m.d_hasz, kMaxSpeciesZ);
   ar.add(&m.d_theta_inv, Dp); ar.add(&m.d_mu, Dp); ar.add(&m.d_mt, (size_t)Mp * Dp); ar.add(&m.d_mn, Mp);
   ar.add(&m.d_coeff, Mp);
+  ar.add(&m.d_mnq, Mp); ar.add(&m.d_cq, Mp);
   ar.add(&m.d_mt_hi, (size_t)Mp * Dp); ar.add(&m.d_mt_lo, (size_t)Mp * Dp);
   ar.add(&m.d_mt_bhi, (size_t)Mp * Dp); ar.add(&m.d_mt_blo, (size_t)Mp * Dp);
   ar.add(&m.d_mtT_hi, (size_t)Mp * Dp); ar.add(&m.d_mtT_lo, (size_t)Mp * Dp);
@@ -322,6 +323,19 @@ static int upload_model(Model &m) {
   GAPX_HIP(put(m.d_theta_inv, tinv.data(), Dp * 8));
   GAPX_HIP(put(m.d_mu, mu.data(), Dp * 8));
   GAPX_HIP(put(m.d_mt, mt.data(), mt.size() * 8));
+  {
+    // the mixed contraction's per-point terms: -|mt|^2 log2(e) / 2, so that
+    // exp(-d2 / 2) = exp2(log2(e) S + xq + mnq) with the row's xq formed the
+    // same way, and the alpha times the W scale (a power of two) in FP32
+    std::vector<double> mnq(Mp);
+    std::vector<float> cq(Mp);
+    for (int k = 0; k < Mp; ++k) {
+      mnq[k] = -0.5 * kLog2e * mn[k];
+      cq[k] = (float)cf[k] * m.w_scale;
+    }
+    GAPX_HIP(put(m.d_mnq, mnq.data(), Mp * 8));
+    GAPX_HIP(put(m.d_cq, cq.data(), Mp * 4));
+  }
   GAPX_HIP(put(m.d_mn, mn.data(), Mp * 8));
   GAPX_HIP(put(m.d_coeff, cf.data(), Mp * 8));
   GAPX_HIP(put(m.d_mt_hi, hi.data(), hi.size() * 2));
@@ -917,6 +931,8 @@ static int enqueue_run(Ctx &c, int flags) {
   g.mt = m.d_mt;
   g.mn = m.d_mn;
   g.coeff = m.d_coeff;
+  g.mnq = m.d_mnq;
+  g.cq = m.d_cq;
   const bool bf = (flags & GAPX_BF16) != 0;
   g.mt_hi = bf ? m.d_mt_bhi : m.d_mt_hi;
   g.mt_lo = bf ? m.d_mt_blo : m.d_mt_lo;

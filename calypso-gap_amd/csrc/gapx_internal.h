@@ -12,6 +12,7 @@ struct gapx_mlp;
 namespace gapx {
 
 constexpr double kPi = 3.141592654;       // truncated pi of wacsf.f90:31
+constexpr double kLog2e = 1.4426950408889634;
 constexpr double kGPa = 6.24219e-3;       // eV/A^3 per GPa, gap_calc.f90:9
 constexpr int kRefNeighborCap = 1000;     // gap_calc.f90:68
 constexpr int kMaxSpeciesZ = 128;
@@ -72,6 +73,8 @@ struct Model {
   double *d_mt = nullptr;                // [Mp][Dp] centred scaled sparse set
   double *d_mn = nullptr;                // [Mp] |mt|^2
   double *d_coeff = nullptr;             // [Mp] (0 in padding)
+  double *d_mnq = nullptr;               // [Mp] -|mt|^2 log2(e) / 2 (mixed contraction)
+  float *d_cq = nullptr;                 // [Mp] coeff * w_scale in FP32 (mixed contraction)
   uint16_t *d_mt_hi = nullptr, *d_mt_lo = nullptr;     // FP16 split [Mp][Dp]
   uint16_t *d_mt_bhi = nullptr, *d_mt_blo = nullptr;   // BF16 split [Mp][Dp]
   uint16_t *d_mtT_hi = nullptr, *d_mtT_lo = nullptr;   // FP16 split, transposed [Dp][Mp]
@@ -324,6 +327,8 @@ struct GprArgs {
   const uint16_t *xt_hi, *xt_lo;
   const float *xt32;
   const double *mt, *mn, *coeff;
+  const double *mnq;              // mixed: -|mt|^2 log2(e) / 2
+  const float *cq;                // mixed: coeff * wscale
   const uint16_t *mt_hi, *mt_lo, *mtT_hi, *mtT_lo;
   const double *theta_inv;
   double *Gpart; double *Epart;   // [splits][Np][Dp], [splits][Np]

@@ -5,7 +5,9 @@
 // Same mathematics as k_gpr_f64 (gpr.hip): for a row tile against a range of
 // sparse points,
 //   S = xt mt^T,  d2 = |xt|^2 + |mt|^2 - 2 S  (norms in FP64),
-//   W = exp(-d2 / 2) c,  E_i += sum_k W_ik,  G += W mt.
+//   W = exp(-d2 / 2) c,  E_i += sum_k W_ik,  G += W mt,
+// with the exponential taken as exp2 of log2(e) S - (|xt|^2 + |mt|^2) log2(e) / 2
+// (the per-point term and c times the W scale tabulated at model load).
 // Every operand x is carried as hi = round(x), lo = round(x - hi) in the
 // 16-bit type and each product as hi*hi + hi*lo + lo*hi (the lo*lo term is
 // below FP32 rounding).  W is scaled by a power of two chosen from the
@@ -99,7 +101,7 @@ struct Chunk {
   static constexpr int seg(int s) { return s < NT ? s / NE : 4; }
   static constexpr bool one_seg(int u) { return seg(u * PER) == seg(u * PER + PER - 1); }
   v4u op[NI * PER];
-  double ex[NW][128];  // per wave: norms of the chunk's points, their alphas, unused
+  double ex[NW][128];  // per wave: the chunk's norm terms (kPC doubles), scaled alphas (kPC floats), unused
 };
 
 // Offset (halves) of operand piece s from its tile's chunk origin, with the
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_gpr_mixed(GprArgs a) {
   // B operands of S^T: row l16 of each row block, 8 features per lane per
   // 32-wide step
   vec xh[R][DP / 32], xl[R][DP / 32];
-  double xnr[R];
+  double xq[R];  // -|xt|^2 log2(e) / 2 of the row
 #pragma unroll
   for (int rb = 0; rb < R; ++rb) {
     const size_t row = (size_t)(row0 + 16 * rb + l16);
@@ -176,7 +178,7 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_gpr_mixed(GprArgs a) {
       xh[rb][t] = *(const vec *)(a.xt_hi + off);
       xl[rb][t] = *(const vec *)(a.xt_lo + off);
     }
-    xnr[rb] = a.xn[row];
+    xq[rb] = -0.5 * kLog2e * a.xn[row];
   }
 
   // this lane's operand pieces, one per DMA instruction
@@ -184,9 +186,13 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_gpr_mixed(GprArgs a) {
 #pragma unroll
   for (int u = 0; u < C::NI; ++u) src[u] = piece_src<DP, NW>((u * NW + wv) * 64 + lane, a.Mp);
 
-  // norms and alphas, two points per lane; lanes past them reload a norm
-  // into the unused half (the loads are issued under a full lane mask)
-  const double *exsrc = lane < kPC / 2 ? a.mn + 2 * lane : lane < kPC ? a.coeff + 2 * (lane - kPC / 2) : a.mn;
+  // the points' norm terms (FP64, two per lane) and scaled alphas (FP32, four
+  // per lane); lanes past them reload a norm term into the unused part (the
+  // loads are issued under a full lane mask)
+  const char *exsrc = lane < kPC / 2       ? (const char *)(a.mnq + 2 * lane)
+                      : lane < 3 * kPC / 4 ? (const char *)(a.cq + 4 * (lane - kPC / 2))
+                                           : (const char *)a.mnq;
+  const int exstride = (lane >= kPC / 2 && lane < 3 * kPC / 4) ? 4 : 8;  // bytes per point
 
   // chunk at p0 -> buffer B: NI + 1 direct loads per wave
   auto issue = [&](C &B, int p0) {
@@ -204,7 +210,8 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_gpr_mixed(GprArgs a) {
       }
       __builtin_amdgcn_global_load_lds((const void *)p, (lds_void *)&B.op[(u * NW + wv) * 64], 16, 0, 0);
     });
-    __builtin_amdgcn_global_load_lds((const void *)(exsrc + p0), (lds_void *)&B.ex[wv][0], 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void *)(exsrc + (size_t)p0 * exstride), (lds_void *)&B.ex[wv][0], 16, 0,
+                                     0);
   };
 
   v4f G[R][DP / 16];  // G^T: features 16 b + 4 lq + r of row l16
@@ -227,29 +234,38 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_gpr_mixed(GprArgs a) {
 #pragma unroll
       for (int rb = 0; rb < R; ++rb) S[rb] = v4f{0.f, 0.f, 0.f, 0.f};
       const int pt = 16 * h + l16;
+      // operands of step t + 1 read from LDS while step t's products run
+      auto op_at = [&](int t) { return pt * C::PM + ((4 * t + lq) ^ swz<C::PM>(pt)); };
+      vec mh = *(const vec *)&B.op[op_at(0)];
+      vec ml = *(const vec *)&B.op[C::NE + op_at(0)];
 #pragma unroll
       for (int t = 0; t < DP / 32; ++t) {
-        const int s = pt * C::PM + ((4 * t + lq) ^ swz<C::PM>(pt));
-        const vec mh = *(const vec *)&B.op[s];
-        const vec ml = *(const vec *)&B.op[C::NE + s];
+        vec mh_n = mh, ml_n = ml;
+        if (t + 1 < DP / 32) {
+          mh_n = *(const vec *)&B.op[op_at(t + 1)];
+          ml_n = *(const vec *)&B.op[C::NE + op_at(t + 1)];
+        }
 #pragma unroll
         for (int rb = 0; rb < R; ++rb) {
           S[rb] = Half<BF>::mfma(mh, xh[rb][t], S[rb]);
           S[rb] = Half<BF>::mfma(ml, xh[rb][t], S[rb]);
           S[rb] = Half<BF>::mfma(mh, xl[rb][t], S[rb]);
         }
+        mh = mh_n;
+        ml = ml_n;
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int pl = 16 * h + 4 * lq + r;
-        const double mnp = B.ex[wv][pl];
-        const float cp = (float)B.ex[wv][kPC + pl];
+        const double mq = B.ex[wv][pl];
+        const float cq = ((const float *)&B.ex[wv][kPC])[pl];
 #pragma unroll
         for (int rb = 0; rb < R; ++rb) {
-          const float d2 = (float)(xnr[rb] + mnp - 2.0 * (double)S[rb][r]);
-          const float w = __expf(-0.5f * d2) * cp;
-          er[rb] += (double)w;
-          const float ws = w * a.wscale;
+          // -d2 log2(e) / 2 in FP64 (norms minus twice the product), then
+          // ws = exp(-d2 / 2) c wscale
+          const float e2 = (float)fma((double)S[rb][r], kLog2e, xq[rb] + mq);
+          const float ws = __builtin_amdgcn_exp2f(e2) * cq;
+          er[rb] += (double)ws;
           const hs hi = (hs)ws;
           wh[rb][4 * h + r] = hi;
           wl[rb][4 * h + r] = (hs)(ws - (float)hi);
@@ -257,18 +273,28 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_gpr_mixed(GprArgs a) {
       }
     }
     if (a.grad) {
+      // operands of feature block b + 1 read while block b's products run
+      auto top_at = [&](int b) {
+        const int d = 16 * b + l16;
+        return 2 * C::NE + d * C::PF + (lq ^ swz<C::PF>(d));
+      };
+      vec th = *(const vec *)&B.op[top_at(0)];
+      vec tl = *(const vec *)&B.op[C::NE + top_at(0)];
 #pragma unroll
       for (int b = 0; b < DP / 16; ++b) {
-        const int d = 16 * b + l16;
-        const int s = 2 * C::NE + d * C::PF + (lq ^ swz<C::PF>(d));
-        const vec th = *(const vec *)&B.op[s];
-        const vec tl = *(const vec *)&B.op[C::NE + s];
+        vec th_n = th, tl_n = tl;
+        if (b + 1 < DP / 16) {
+          th_n = *(const vec *)&B.op[top_at(b + 1)];
+          tl_n = *(const vec *)&B.op[C::NE + top_at(b + 1)];
+        }
 #pragma unroll
         for (int rb = 0; rb < R; ++rb) {
           G[rb][b] = Half<BF>::mfma(th, wh[rb], G[rb][b]);
           G[rb][b] = Half<BF>::mfma(tl, wh[rb], G[rb][b]);
           G[rb][b] = Half<BF>::mfma(th, wl[rb], G[rb][b]);
         }
+        th = th_n;
+        tl = tl_n;
       }
     }
   };
@@ -311,7 +337,7 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void k_gpr_mixed(GprArgs a) {
     double v = er[rb];
     v += __shfl_xor(v, 16, 64);
     v += __shfl_xor(v, 32, 64);
-    if (lq == 0) a.Epart[base + row] = v;
+    if (lq == 0) a.Epart[base + row] = v * (1.0 / (double)a.wscale);  // a power of two: exact
   }
 }
 
