@@ -33,7 +33,8 @@
 // loads (global_load_lds_dwordx4: no registers, counted by vmcnt), so the
 // matrix cores do not wait for the chunk loads.  A chunk buffer holds the
 // point-major hi/lo tiles (32 rows of DP halves), the feature-major hi/lo
-// tiles (DP rows of 32 halves) and, per wave, the chunk's norms and alphas.
+// tiles (DP rows of 32 halves) and, per wave, the chunk's norm terms and
+// scaled alphas.
 // A direct load writes 64 consecutive 16-byte pieces per instruction, so the
 // tiles have no row padding; instead piece q of row r sits at q ^ swz(r),
 // which spreads the 16 rows a 16-lane group reads over all LDS banks.
