@@ -288,46 +288,72 @@ __device__ __forceinline__ unsigned long long nb_key(int j, int n0, int n1, int 
 // per-neighbour accumulators rarely collide within one instruction.  Entry c
 // of a step (c < K) always has neighbour c as one end (a = c in run d, b = c
 // in run K - d), so each lane keeps its own records in registers (K <= 256)
-// and reads only the partner's from LDS.  The cutoff test is a squared
-// comparison with a relative margin of 1e-6; a triplet admitted by the margin
-// contributes zero, since the descriptor kernels apply each line's cutoff to
-// every distance.
-__device__ __forceinline__ void triplets_from_lds(int i, int K, const float4 *s_p, double rcA, const TripletBufs &tb,
+// and reads only the partner's, record (c + d) mod K: the first K / 2
+// records are repeated after the K staged ones, so the read index is c + d.
+// The cutoff test is a squared comparison with a relative margin of 1e-6; a
+// triplet admitted by the margin contributes zero, since the descriptor
+// kernels apply each line's cutoff to every distance.  Records with r beyond
+// the margin are staged with a NaN x (trip_record), so every distance test
+// that involves them fails and no separate r test is needed.
+__device__ __forceinline__ float trip_rc(double rcA) { return (float)rcA * (1.0f + 1e-6f); }
+
+__device__ __forceinline__ float4 trip_record(double dx, double dy, double dz, double r, float rc) {
+  const float rf = (float)r;
+  return make_float4(rf <= rc ? (float)dx : __builtin_nanf(""), (float)dy, (float)dz, rf);
+}
+
+// LDS records the triplet builder needs for lists of up to kcap neighbours
+// (the repeated half only for lists of up to 256)
+__host__ __device__ constexpr size_t trip_lds_records(int kcap) { return (size_t)kcap + std::min(kcap, 256) / 2; }
+
+__device__ __forceinline__ void triplets_from_lds(int i, int K, float4 *s_p, double rcA, const TripletBufs &tb,
                                                   int lane) {
   constexpr int kOwn = 4;  // register-held record chunks: K <= 64 kOwn
-  const float rc = (float)rcA * (1.0f + 1e-6f);
+  K = __builtin_amdgcn_readfirstlane(K);
+  const float rc = trip_rc(rcA);
   const float rc2 = rc * rc;
-  const size_t out = (size_t)i * tb.tcap;
+  uint32_t *out = tb.list + (size_t)i * tb.tcap;
   int count = 0;
-  auto emit = [&](bool ok, int a, int b) {
+  // pairs of the step whose lanes pass `ok`, appended after `count` entries
+  auto emit = [&](bool ok, uint32_t pair) {
     const unsigned long long m = __ballot(ok);
-    const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
-    if (ok && pos < tb.tcap) tb.list[out + pos] = (uint32_t)a | ((uint32_t)b << 16);
+    const unsigned pre = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    if (ok && (int)pre < tb.tcap - count) out[count + pre] = pair;
     count += __popcll(m);
   };
   if (K <= 64 * kOwn) {
+    for (int t = lane; t < K / 2; t += 64) s_p[K + t] = s_p[t];
+    __syncthreads();
+    const int KX = K + K / 2;  // records readable at c + d
     float4 own[kOwn];
-    bool own_in[kOwn];
+    uint32_t self[kOwn];  // c | c << 16
 #pragma unroll
     for (int q = 0; q < kOwn; ++q) {
-      own[q] = s_p[min(64 * q + lane, max(K - 1, 0))];
-      own_in[q] = own[q].w <= rc;
+      const int c = 64 * q + lane;
+      own[q] = s_p[min(c, max(K - 1, 0))];
+      if (c >= K) own[q].x = __builtin_nanf("");  // idle lanes fail every test
+      self[q] = (uint32_t)c | ((uint32_t)c << 16);
     }
-    for (int d1 = 1; 2 * d1 <= K; ++d1) {
-      const int d2 = K - d1;
-      const int n1 = K - d1, n = (d2 != d1) ? K : n1;
+    // one step: run d1 (K - d1 pairs) and run K - d1 (d1 pairs), or, when the
+    // two are the same run (2 d1 = K), its n = K - d1 pairs only
+    auto step = [&](int d1, int n, auto whole) {
+      const int n1 = K - d1;
+      const uint32_t lo = (uint32_t)(d1 - K), hi = (uint32_t)d1 << 16;
 #pragma unroll
       for (int q = 0; q < kOwn; ++q) {
         if (64 * q >= n) break;
         const int c = 64 * q + lane;
-        const bool first = c < n1, live = c < n;
-        const int partner = first ? c + d1 : c - n1;  // b of run d1, or a of run d2
-        const float4 pp = s_p[live ? partner : 0];
+        const float4 pp = s_p[min(c + d1, KX - 1)];
         const float ex = pp.x - own[q].x, ey = pp.y - own[q].y, ez = pp.z - own[q].z;
-        const bool ok = live & own_in[q] & (pp.w <= rc) & (ex * ex + ey * ey + ez * ez <= rc2);
-        emit(ok, first ? c : partner, first ? partner : c);
+        bool ok = ex * ex + ey * ey + ez * ez <= rc2;
+        if (!decltype(whole)::value) ok = ok && c < n;
+        // run d1: (c, c + d1); run K - d1: (c + d1 - K, c)
+        emit(ok, self[q] + (c < n1 ? hi : lo));
       }
-    }
+    };
+    int d1 = 1;
+    for (; 2 * d1 < K; ++d1) step(d1, K, std::true_type{});
+    if (2 * d1 == K) step(d1, K - d1, std::false_type{});
   } else {
     for (int d1 = 1; 2 * d1 <= K; ++d1) {
       // run d1 (K - d1 pairs), then run d2 = K - d1 (d1 pairs) unless it is d1
@@ -342,7 +368,7 @@ __device__ __forceinline__ void triplets_from_lds(int i, int K, const float4 *s_
         const bool live = c < n;
         const float4 pa = s_p[live ? a : 0], pb = s_p[live ? b : 0];
         const float ex = pb.x - pa.x, ey = pb.y - pa.y, ez = pb.z - pa.z;
-        emit(live & (pa.w <= rc) & (pb.w <= rc) & (ex * ex + ey * ey + ez * ez <= rc2), a, b);
+        emit(live & (ex * ex + ey * ey + ez * ez <= rc2), (uint32_t)a | ((uint32_t)b << 16));
       }
     }
   }
@@ -366,7 +392,7 @@ __global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ ato
                                               const StructParams *__restrict__ sp, double rcut, NeighborBufs nb,
                                               double rcA, TripletBufs tb) {
   extern __shared__ __align__(16) unsigned char smem[];
-  float4 *s_p = (float4 *)smem;  // [kcap] with TRIP
+  float4 *s_p = (float4 *)smem;  // [trip_lds_records(kcap)] with TRIP
   const int i = blockIdx.x, lane = threadIdx.x;
   if (i >= N) return;
   const StructParams &P = sp[atom_struct[i]];
@@ -440,7 +466,7 @@ __global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ ato
         double2 *d2 = (double2 *)(nb.nb_d + 4 * sl);
         d2[0] = make_double2(dx, dy);
         d2[1] = make_double2(dz, r);
-        if (TRIP) s_p[pos] = make_float4((float)dx, (float)dy, (float)dz, (float)r);
+        if (TRIP) s_p[pos] = trip_record(dx, dy, dz, r, trip_rc(rcA));
       }
       count += __popcll(m);
     }
@@ -460,14 +486,14 @@ __global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ ato
 // builds the list as k_build<true> does.
 __global__ __launch_bounds__(64) void k_triplets_from_slots(int N, NeighborBufs nb, double rcA, TripletBufs tb) {
   extern __shared__ __align__(16) unsigned char smem[];
-  float4 *s_p = (float4 *)smem;  // [kcap]
+  float4 *s_p = (float4 *)smem;  // [trip_lds_records(kcap)]
   const int i = blockIdx.x, lane = threadIdx.x;
   if (i >= N) return;
   const int K = min(nb.nb_count[i], nb.kcap);
   const size_t base = (size_t)i * nb.kcap;
   for (int a = lane; a < K; a += 64) {
     const double *d = nb.nb_d + 4 * (base + a);
-    s_p[a] = make_float4((float)d[0], (float)d[1], (float)d[2], (float)d[3]);
+    s_p[a] = trip_record(d[0], d[1], d[2], d[3], trip_rc(rcA));
   }
   __syncthreads();
   triplets_from_lds(i, K, s_p, rcA, tb, lane);
@@ -875,7 +901,7 @@ hipError_t launch_bin_structures(hipStream_t s, int nstruct, const double *pos, 
 hipError_t launch_build_neighbors(hipStream_t s, int N, const int *atom_struct, const StructParams *sp,
                                   double rcut, NeighborBufs nb, double rcA, const TripletBufs *tb) {
   if (tb)
-    k_build<true><<<N, 64, 4 * sizeof(float) * (size_t)nb.kcap, s>>>(N, atom_struct, sp, rcut, nb, rcA, *tb);
+    k_build<true><<<N, 64, sizeof(float4) * trip_lds_records(nb.kcap), s>>>(N, atom_struct, sp, rcut, nb, rcA, *tb);
   else
     k_build<false><<<N, 64, 0, s>>>(N, atom_struct, sp, rcut, nb, 0.0, TripletBufs{});
   return hipGetLastError();
@@ -884,7 +910,7 @@ hipError_t launch_build_neighbors(hipStream_t s, int N, const int *atom_struct, 
 hipError_t launch_triplets_from_slots(hipStream_t s, int N, NeighborBufs nb, double rcA, TripletBufs tb) {
   if (N <= 0) return hipSuccess;
   if (nb.kcap < 1 || nb.kcap > kMaxNeighbors) return hipErrorInvalidValue;
-  k_triplets_from_slots<<<N, 64, 4 * sizeof(float) * (size_t)nb.kcap, s>>>(N, nb, rcA, tb);
+  k_triplets_from_slots<<<N, 64, sizeof(float4) * trip_lds_records(nb.kcap), s>>>(N, nb, rcA, tb);
   return hipGetLastError();
 }
 
