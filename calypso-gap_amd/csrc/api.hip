@@ -514,6 +514,19 @@ int gapx::cell_params(const double *L9, double rcut, int nb_max, StructParams &P
     // the list kernel's FP32 index arithmetic wants fewer than 2^20 images
     if (P.S[k] > 48) return fail(GAPX_ERR_ARG, "lattice plane spacing below rcut / 48: too many periodic images");
   }
+  // Cholesky factor of the Gram matrix of the bin edges E_k = L_k / nb_k
+  double G[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      G[a][b] = (L[3 * a] * L[3 * b] + L[3 * a + 1] * L[3 * b + 1] + L[3 * a + 2] * L[3 * b + 2]) / (P.nb[a] * P.nb[b]);
+  const double c00 = std::sqrt(G[0][0]), c10 = G[1][0] / c00, c20 = G[2][0] / c00;
+  const double q11 = G[1][1] - c10 * c10;
+  const double c11 = q11 > 0 ? std::sqrt(q11) : 0.0, c21 = c11 > 0 ? (G[2][1] - c20 * c10) / c11 : 0.0;
+  const double q22 = G[2][2] - c20 * c20 - c21 * c21;
+  const double c22 = q22 > 0 ? std::sqrt(q22) : 0.0;
+  const double Rv[6] = {c00, c10, c20, c11, c21, c22};
+  const bool ok = std::isfinite(c00) && c11 > 0 && c22 > 0;
+  for (int k = 0; k < 6; ++k) P.R[k] = ok ? (float)Rv[k] : 0.0f;
   P.bin_base = 0;
   P.atom_base = 0;
   P.natoms = 0;

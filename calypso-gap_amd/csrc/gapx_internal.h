@@ -100,12 +100,18 @@ struct StructParams {
   int atom_base;     // first atom
   int natoms;
   int pad;
+  // R00 R01 R02 R11 R12 R22 of the upper triangular R with R^T R = E E^T,
+  // E the bin edge vectors (rows L_k / nb_k): |sum_k f_k E_k| = |R f| for a
+  // displacement of f_k bins along axis k (zeros where the factor does not
+  // exist numerically: no bound)
+  float R[6];
 };
 
 // Cell-list parameters of one structure from its lattice rows L9: volume,
-// inverse, bins per axis (at least rcut / 2 wide, at most nb_max) and search
-// half-widths; GAPX_ERR_ARG for a singular lattice or a plane spacing below
-// rcut / 48.  bin_base, atom_base and natoms are left 0.
+// inverse, bins per axis (at least rcut / 2 wide, at most nb_max), search
+// half-widths and the bin metric factor R; GAPX_ERR_ARG for a singular
+// lattice or a plane spacing below rcut / 48.  bin_base, atom_base and natoms
+// are left 0.
 int cell_params(const double *L9, double rcut, int nb_max, StructParams &P);
 
 // Thread-local error message.

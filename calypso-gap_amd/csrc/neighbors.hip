@@ -25,13 +25,17 @@ namespace gapx {
 
 namespace {
 
-// floor(a / b) for b >= 1 and |a| < 2^22 through FP32, given invb = 1 / b:
-// (a + 1/2) / b is at least 1/(2b) from every integer, while the two
-// roundings (of invb and of the product) move it by at most |a + 1/2| 2^-23 / b,
-// which is smaller than that bound, so the floor is exact.
+// floor(a / b) for b >= 1 and |a| < 2^21 through FP32, given invb within
+// 2^-23 (relative) of 1 / b, as inv_small gives it: (a + 1/2) / b is at
+// least 1/(2b) from every integer, while the error of invb and the rounding
+// of the product move it by at most |a + 1/2| (2^-23 + 2^-24) / b < 3/(8b),
+// so the floor is exact.
 __device__ __forceinline__ int floordiv_small(int a, float invb) {
   return (int)floorf(((float)a + 0.5f) * invb);
 }
+
+// 1 / b by v_rcp_f32 (at most 1 ulp from the exact value)
+__device__ __forceinline__ float inv_small(int b) { return __builtin_amdgcn_rcpf((float)b); }
 
 __global__ void k_fill_int(int *p, int n, int v) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -402,11 +406,37 @@ __global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ ato
   const int si0 = nb.shift[3 * i], si1 = nb.shift[3 * i + 1], si2 = nb.shift[3 * i + 2];
   const int m1 = 2 * P.S[1] + 1, m2 = 2 * P.S[2] + 1, nimg = (2 * P.S[0] + 1) * m1 * m2;
   // image and bin indices stay below 2^20 (the host keeps S <= 48)
-  const float inv_m1 = 1.0f / m1, inv_m2 = 1.0f / m2;
-  const float inv_n0 = 1.0f / P.nb[0], inv_n1 = 1.0f / P.nb[1], inv_n2 = 1.0f / P.nb[2];
+  const float inv_m1 = inv_small(m1), inv_m2 = inv_small(m2);
+  const float inv_n0 = inv_small(P.nb[0]), inv_n1 = inv_small(P.nb[1]), inv_n2 = inv_small(P.nb[2]);
   const size_t base = (size_t)i * nb.kcap;
   const double2 *sp2 = (const double2 *)nb.spos;
   const int4 *sinfo = (const int4 *)nb.sinfo;
+  // Image bins that lie wholly beyond rcut + 0.01 A of the centre are skipped
+  // (no candidate there can pass, so the list is the same).  Image o of the
+  // centre's bin holds the points o_k + v_k - u_k bins away along axis k,
+  // v_k in [0, 1], u_k the centre's offset in its own bin; with the bin
+  // metric factor R (|displacement| = |R f|) each component of R f lies in an
+  // interval [c_a + m_a, c_a + M_a], c_a = sum_b R_ab (o_b - u_b), and the sum
+  // of the squared distances of those intervals from 0 bounds the squared
+  // distance from below.  FP32 suffices: its rounding stays far inside the
+  // 0.01 A margin.
+  const float R00 = P.R[0], R01 = P.R[1], R02 = P.R[2], R11 = P.R[3], R12 = P.R[4], R22 = P.R[5];
+  float cu0, cu1, cu2;  // -R u
+  {
+    float u[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double f = xi * P.Linv[k] + yi * P.Linv[3 + k] + zi * P.Linv[6 + k];
+      u[k] = (float)(f * P.nb[k] - (k == 0 ? bi0 : k == 1 ? bi1 : bi2));
+    }
+    cu0 = -(R00 * u[0] + R01 * u[1] + R02 * u[2]);
+    cu1 = -(R11 * u[1] + R12 * u[2]);
+    cu2 = -(R22 * u[2]);
+  }
+  const float m0 = fminf(R00, 0.f) + fminf(R01, 0.f) + fminf(R02, 0.f), m1r = fminf(R11, 0.f) + fminf(R12, 0.f);
+  const float M0 = fmaxf(R00, 0.f) + fmaxf(R01, 0.f) + fmaxf(R02, 0.f), M1r = fmaxf(R11, 0.f) + fmaxf(R12, 0.f);
+  const float m2r = fminf(R22, 0.f), M2r = fmaxf(R22, 0.f);
+  const float rp = (float)rcut + 0.01f, rp2 = rp * rp;
   int count = 0;
   for (int g0 = 0; g0 < nimg; g0 += 64) {
     const int img = g0 + lane;
@@ -419,9 +449,14 @@ __global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ ato
       t1 = floordiv_small(bb1, inv_n1);
       t2 = floordiv_small(bb2, inv_n2);
       const int w0 = bb0 - t0 * P.nb[0], w1 = bb1 - t1 * P.nb[1], w2 = bb2 - t2 * P.nb[2];
-      const int bin = P.bin_base + (w0 * P.nb[1] + w1) * P.nb[2] + w2;
-      lo = nb.bin_start[bin];
-      n = nb.bin_start[bin + 1] - lo;
+      const float c0 = cu0 + R00 * o0 + R01 * o1 + R02 * o2, c1 = cu1 + R11 * o1 + R12 * o2, c2 = cu2 + R22 * o2;
+      const float e0 = fmaxf(fmaxf(c0 + m0, -(c0 + M0)), 0.f), e1 = fmaxf(fmaxf(c1 + m1r, -(c1 + M1r)), 0.f),
+                  e2 = fmaxf(fmaxf(c2 + m2r, -(c2 + M2r)), 0.f);
+      if (e0 * e0 + e1 * e1 + e2 * e2 <= rp2) {
+        const int bin = P.bin_base + (w0 * P.nb[1] + w1) * P.nb[2] + w2;
+        lo = nb.bin_start[bin];
+        n = nb.bin_start[bin + 1] - lo;
+      }
     }
     int incl = n;
 #pragma unroll
@@ -538,8 +573,8 @@ __global__ __launch_bounds__(256) void k_min_bond(int N, const StructParams *__r
     const int si0 = nb.shift[3 * i], si1 = nb.shift[3 * i + 1], si2 = nb.shift[3 * i + 2];
     const double pi[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
     const int m1 = 2 * P.S[1] + 1, m2 = 2 * P.S[2] + 1, nimg = (2 * P.S[0] + 1) * m1 * m2;
-    const float inv_m1 = 1.0f / m1, inv_m2 = 1.0f / m2;
-    const float inv_n0 = 1.0f / P.nb[0], inv_n1 = 1.0f / P.nb[1], inv_n2 = 1.0f / P.nb[2];
+    const float inv_m1 = inv_small(m1), inv_m2 = inv_small(m2);
+    const float inv_n0 = inv_small(P.nb[0]), inv_n1 = inv_small(P.nb[1]), inv_n2 = inv_small(P.nb[2]);
     const double2 *sp2 = (const double2 *)nb.spos;
     const int4 *sinfo = (const int4 *)nb.sinfo;
     for (int g0 = 0; g0 < nimg; g0 += 64) {

@@ -18,6 +18,9 @@
   pinned by the reference's own vectors (tests/test_gpu_parity.py).
 * A cell too small for rcut / 48 is refused; lists above 256 slots (the
   triplet builder's other path) match the oracle.
+* Strongly sheared cells, where the list kernel's image-bin distance bound is
+  loosest: pair counts against a test of every image, descriptors and
+  results against the oracle.
 """
 import numpy as np
 import pytest
@@ -255,3 +258,44 @@ def test_more_than_256_neighbours_match_oracle(model):
     assert pairs / n > 256
     mix = m.calc(s["species"], s["lat"], s["pos"], 10.0, mixed=True)
     assert abs(mix[0] - e) / n <= 1e-3 and np.abs(mix[1] - f).max() <= 2e-2
+
+
+def brute_pairs(lat, pos, rcut):
+    """Ordered pairs (i, j, image) with 0 < r <= rcut, every image tested."""
+    h = 1.0 / np.linalg.norm(np.linalg.inv(lat), axis=0)  # plane spacings
+    rng3 = [np.arange(-k, k + 1) for k in np.ceil(rcut / h).astype(int) + 1]
+    shifts = np.stack(np.meshgrid(*rng3, indexing="ij"), -1).reshape(-1, 3) @ lat
+    d = pos[None, :, None, :] + shifts[None, None, :, :] - pos[:, None, None, :]
+    r = np.sqrt((d ** 2).sum(-1))
+    return int(((r <= rcut) & (r > 0)).sum())
+
+
+@pytest.mark.parametrize("cell", [
+    [[9.0, 0.0, 0.0], [7.5, 4.0, 0.0], [-6.0, 3.0, 5.0]],
+    [[14.0, 0.0, 0.0], [3.0, 13.0, 0.0], [5.0, -4.0, 3.2]],
+    [[7.0, 0.0, 0.0], [-5.5, 6.0, 0.0], [3.0, -2.5, 6.5]],
+    [[20.0, 0.0, 0.0], [12.0, 15.0, 0.0], [-8.0, 6.0, 14.0]],
+])
+def test_sheared_cells_match_oracle(model, cell):
+    """The list kernel skips image bins that a lower bound on their distance
+    (through the bin metric's Cholesky factor) puts beyond rcut; in strongly
+    sheared cells that bound is furthest from the true distance.  The pair
+    count against every image tested, the descriptors against the oracle's,
+    and energy, forces and stress."""
+    from gapx import synth
+    om, m = model
+    lat = np.array(cell)
+    rng = np.random.default_rng(int(abs(np.linalg.det(lat))))
+    n = max(8, int(0.065 * abs(np.linalg.det(lat))))
+    pos = synth.random_positions(lat, n, rng, min_dist=1.5)
+    species = rng.choice(np.array([1, 8, 22], np.int32), size=len(pos)).astype(np.int32)
+    ctx = gapx.Context(m)
+    ctx.load([dict(species=species, lat=lat, pos=pos)], 6.0)
+    ctx.run(gapx.GRAD)
+    pairs, _ = ctx.counts()
+    ctx.close()
+    assert pairs == brute_pairs(lat, pos, 6.0)
+    x = m.descriptors(species, lat, pos, 6.0)
+    xo = oracle.descriptors(om, species, lat, pos, 6.0)
+    np.testing.assert_allclose(x, xo, rtol=1e-11, atol=1e-13)
+    check(om, m, species, lat, pos)
