@@ -86,53 +86,77 @@ __global__ __launch_bounds__(256) void k_wrap_scan(int N, int nbins, const doubl
     atomicAdd(&nb.bin_count[bin], 1);
   }
   __shared__ int s_last;
-  __shared__ long long part[256];
   __threadfence();
   __syncthreads();
   if (t == 0) s_last = (atomicAdd(ticket, 1) == (int)gridDim.x - 1);
   __syncthreads();
   if (!s_last) return;
-  __threadfence();
-  // exclusive scan of bin_count, 256 segments; each thread reads its
-  // segment eight counts at a time, the loads of a batch in flight together
-  const int per = (nbins + 255) / 256, lo = t * per, hi = min(nbins, lo + per);
-  auto count_at = [&](int k) {
-    return __hip_atomic_load(&nb.bin_count[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __threadfence();  // every block's count atomics before this block's plain loads
+  // Exclusive scan of bin_count in tiles of 2048 bins, eight consecutive bins
+  // per thread read as two 16-byte words (the next tile's issued before this
+  // one is scanned): a wave prefix sum of the thread totals, the four wave
+  // totals through LDS, then bin_start written and bin_count and bin_cursor
+  // cleared with 16-byte stores.
+  __shared__ int s_wsum[4];
+  const int lane = t & 63, wv = t >> 6;
+  auto load8 = [&](int k0, int (&v)[8]) {
+    if (k0 + 8 <= nbins) {
+      const int4 x = *(const int4 *)(nb.bin_count + k0), y = *(const int4 *)(nb.bin_count + k0 + 4);
+      v[0] = x.x, v[1] = x.y, v[2] = x.z, v[3] = x.w, v[4] = y.x, v[5] = y.y, v[6] = y.z, v[7] = y.w;
+    } else {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = (k0 + u < nbins) ? nb.bin_count[k0 + u] : 0;
+    }
   };
-  long long sum = 0;
-  for (int k0 = lo; k0 < hi; k0 += 8) {
-    int v[8];
+  auto store8 = [&](int *p, int k0, const int (&v)[8]) {
+    if (k0 + 8 <= nbins) {
+      *(int4 *)(p + k0) = make_int4(v[0], v[1], v[2], v[3]);
+      *(int4 *)(p + k0 + 4) = make_int4(v[4], v[5], v[6], v[7]);
+    } else {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = count_at(min(k0 + u, hi - 1));
+      for (int u = 0; u < 8; ++u)
+        if (k0 + u < nbins) p[k0 + u] = v[u];
+    }
+  };
+  const int zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int carry = 0;
+  int cur[8], nxt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  load8(8 * t, cur);
+  for (int tile0 = 0; tile0 < nbins; tile0 += 2048) {
+    const int k0 = tile0 + 8 * t;
+    if (tile0 + 2048 < nbins) load8(k0 + 2048, nxt);
+    int tot = 0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (k0 + u < hi) sum += v[u];
-  }
-  part[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    const long long v = (t >= off) ? part[t - off] : 0;
+    for (int u = 0; u < 8; ++u) tot += cur[u];
+    int x = tot;  // inclusive wave prefix of the thread totals
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wsum[wv] = x;
     __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  long long run = part[t] - sum;
-  for (int k0 = lo; k0 < hi; k0 += 8) {
-    int v[8];
+    int run = carry + x - tot, tile_total = 0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = count_at(min(k0 + u, hi - 1));
+    for (int w = 0; w < 4; ++w) {
+      run += (w < wv) ? s_wsum[w] : 0;
+      tile_total += s_wsum[w];
+    }
+    __syncthreads();  // s_wsum is rewritten by the next tile
+    int st[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (k0 + u < hi) {
-        nb.bin_start[k0 + u] = (int)run;
-        run += v[u];
-      }
+    for (int u = 0; u < 8; ++u) {
+      st[u] = run;
+      run += cur[u];
+    }
+    store8(nb.bin_start, k0, st);
+    store8(nb.bin_count, k0, zero);
+    store8(nb.bin_cursor, k0, zero);
+    carry += tile_total;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cur[u] = nxt[u];
   }
-  for (int k = lo; k < hi; ++k) {
-    nb.bin_count[k] = 0;
-    nb.bin_cursor[k] = 0;
-  }
-  if (t == 255) nb.bin_start[nbins] = (int)part[255];
+  if (t == 0) nb.bin_start[nbins] = carry;
   if (t < 8) stats[t] = 0;
   if (t == 0) *ticket = 0;
 }
