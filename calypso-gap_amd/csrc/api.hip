@@ -719,7 +719,7 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b) {
       (rc = c.get("nb_count", c.N, &nb.nb_count)) || (rc = c.get("stats", 8, &stats)) ||
       (rc = c.get("nb_raw", c.N, &nb.nb_raw)) || (rc = c.get("spos", 4 * (size_t)c.N, &nb.spos)) ||
       (rc = c.get("sinfo", 4 * (size_t)c.N, &nb.sinfo)) || (rc = c.get("nb_j", slots, &nb.nb_j)) ||
-      (rc = c.get("nb_key", slots + 16, &nb.nb_key)) ||  // 16 keys of padding for batched reads
+      (rc = c.get("nb_key", slots + kRevBatch, &nb.nb_key)) ||  // padding for the batched reads
       (rc = c.get("nb_d", 4 * slots, &nb.nb_d)) || (rc = c.get("nb_rev", slots, &nb.nb_rev)))
     return rc;
   nb.total = stats;

@@ -151,6 +151,10 @@ struct NeighborBufs {
   const double *nb_w;
 };
 
+// Keys the reverse-slot search reads per batch; nb_key carries that many
+// keys of padding past the last slot.
+constexpr int kRevBatch = 16;
+
 // Wrap positions into their cells, bin and count the atoms, and scan the
 // counts into bin_start, in one launch; needs bin_count and *ticket zero on
 // entry and leaves them zero (with bin_cursor[0, nbins) and stats[0..7]).

@@ -671,10 +671,10 @@ __global__ void k_set_u64(unsigned long long *p, unsigned long long v) {
 // only).  Each pair is searched once, from its leading side (i < j, or i == j
 // with n lexicographically positive), which writes both entries: no other
 // lane writes either.  One wavefront per centre atom, one slot per lane; j's
-// packed keys are read 16 at a time (8 independent 16-byte loads) and the
-// first match wins.  nb_key carries 16 keys of padding past the last atom's
-// slots, so the batched reads stay in bounds; entries at or beyond j's count
-// are never compared.
+// packed keys are read kRevBatch at a time (independent 16-byte loads) and the
+// first match wins.  nb_key carries kRevBatch keys of padding past the last
+// atom's slots, so the batched reads stay in bounds; entries at or beyond j's
+// count are never compared.
 __global__ __launch_bounds__(64) void k_reverse(int N, NeighborBufs nb) {
   const int i = blockIdx.x;
   if (i >= N) return;
@@ -691,16 +691,16 @@ __global__ __launch_bounds__(64) void k_reverse(int N, NeighborBufs nb) {
     const ulonglong2 *kj = (const ulonglong2 *)(nb.nb_key + jb);
     const int Kj = nb.nb_count[j];
     int found = -1;
-    for (int q0 = 0; q0 < Kj && found < 0; q0 += 16) {
-      unsigned long long k[16];
+    for (int q0 = 0; q0 < Kj && found < 0; q0 += kRevBatch) {
+      unsigned long long k[kRevBatch];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < kRevBatch / 2; ++u) {
         const ulonglong2 v = kj[q0 / 2 + u];
         k[2 * u] = v.x;
         k[2 * u + 1] = v.y;
       }
 #pragma unroll
-      for (int u = 15; u >= 0; --u)
+      for (int u = kRevBatch - 1; u >= 0; --u)
         if (q0 + u < Kj && k[u] == want) found = q0 + u;
     }
     if (found >= 0) {
