@@ -330,9 +330,12 @@ __device__ __forceinline__ float4 trip_record(double dx, double dy, double dz, d
   return make_float4(rf <= rc ? (float)dx : __builtin_nanf(""), (float)dy, (float)dz, rf);
 }
 
-// LDS records the triplet builder needs for lists of up to kcap neighbours
-// (the repeated half only for lists of up to 256)
-__host__ __device__ constexpr size_t trip_lds_records(int kcap) { return (size_t)kcap + std::min(kcap, 256) / 2; }
+// LDS records the triplet builder needs for lists of up to kcap neighbours:
+// the K staged ones and, for K <= 256, the repeated first half after them
+// (at most 384 records in all), so never more than 64 KB for kcap <= 4096
+__host__ __device__ constexpr size_t trip_lds_records(int kcap) {
+  return (size_t)std::max(kcap, std::min(kcap, 256) + std::min(kcap, 256) / 2);
+}
 
 __device__ __forceinline__ void triplets_from_lds(int i, int K, float4 *s_p, double rcA, const TripletBufs &tb,
                                                   int lane) {
