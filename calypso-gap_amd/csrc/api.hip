@@ -433,10 +433,13 @@ struct Ctx {
   // bin count (grid of the rank kernel), which binning kernel runs and the
   // one-launch kernel's LDS sizes (largest structure and bin count)
   int graph_nbins = -1, graph_fused = -1, graph_fused_atoms = -1, graph_fused_bins_max = -1;
+  // the reverse-slot search's key width (32-bit keys: N <= 2^17 and every
+  // search half-width <= 15), set with the cell parameters
+  int key32 = 0, graph_key32 = -1;
   bool graph_matches(int flags) const {
     return gexec && graph_flags == flags && graph_kcap == kcap && graph_tcap == tcap && graph_N == N &&
            graph_nbins == nbins && graph_fused == (fused_bins ? 1 : 0) && graph_fused_atoms == fused_atoms &&
-           graph_fused_bins_max == fused_bins_max;
+           graph_fused_bins_max == fused_bins_max && graph_key32 == key32;
   }
 
   template <typename T>
@@ -559,6 +562,10 @@ static int struct_params(Ctx &c) {
   }
   c.fused_bins = std::getenv("GAPX_NO_FUSED_BINS") == nullptr && c.fused_atoms <= kFusedAtoms &&
                  fused_bin_lds_bytes(c.fused_atoms, c.fused_bins_max) <= kFusedLdsBytes;
+  int s_max = 0;
+  for (int s = 0; s < c.nstruct; ++s)
+    s_max = std::max({s_max, c.sp[s].S[0], c.sp[s].S[1], c.sp[s].S[2]});
+  c.key32 = (c.N <= (1 << 17) && s_max <= 15) ? 1 : 0;
   return GAPX_OK;
 }
 
@@ -725,6 +732,7 @@ static int run_neighbors(Ctx &c, int flags, Bufs &b) {
   nb.total = stats;
   nb.overflow = stats + 2;
   nb.kcap = c.kcap;
+  nb.key32 = c.key32;
   // cutoff tables in global memory for lists too long for the descriptor
   // kernels' LDS layouts
   nb.fc_scratch = nullptr;
@@ -1147,6 +1155,7 @@ static int ctx_run(Ctx &c, int flags, Mlp *mlp = nullptr) {
       c.graph_fused = c.fused_bins ? 1 : 0;
       c.graph_fused_atoms = c.fused_atoms;
       c.graph_fused_bins_max = c.fused_bins_max;
+      c.graph_key32 = c.key32;
     }
     GAPX_HIP(hipGraphLaunch(c.gexec, c.stream));
   } else {

@@ -135,6 +135,9 @@ struct NeighborBufs {
   int *nb_count;        // [N]
   int *nb_j;            // [N*kcap]
   unsigned long long *nb_key;  // [N*kcap] packed (j, image) for the reverse-slot search
+  // nonzero: nb_key holds 32-bit keys instead (j < 2^17 and image fields of
+  // 5 bits, valid while every search half-width S <= 15)
+  int key32;
   double *nb_d;         // [N*kcap*4] dx dy dz r (relative to the centre)
   int *nb_rev;          // [N*kcap] slot (j -> i, -n) of slot (i -> j, n), or -1
   int *nb_raw;          // [N] neighbour count before the kcap clamp

@@ -18,6 +18,7 @@
 // ascending index order so the lists (and every sum over them) are
 // deterministic.
 #include <algorithm>
+#include <type_traits>
 
 #include "gapx_internal.h"
 
@@ -306,6 +307,15 @@ __device__ __forceinline__ unsigned long long nb_key(int j, int n0, int n1, int 
          ((unsigned long long)((n1 + 128) & 255) << 8) | (unsigned long long)((n2 + 128) & 255);
 }
 
+// The compact key: j (below 2^17) and each image component modulo 32.  Two
+// images of one atom within a list differ by at most 2 S + 2 <= 32 cells per
+// axis when S <= 15, so the key stays unique within a list, and a self image
+// (j = i) has |n_k| <= S, so its components are recovered by sign extension.
+__device__ __forceinline__ uint32_t nb_key32(int j, int n0, int n1, int n2) {
+  return ((uint32_t)j << 15) | ((uint32_t)(n0 & 31) << 10) | ((uint32_t)(n1 & 31) << 5) | (uint32_t)(n2 & 31);
+}
+__device__ __forceinline__ int sext5(uint32_t v) { return (int)((v & 31u) ^ 16u) - 16; }
+
 // Triplet lists, from the neighbour records the building wave just staged in
 // LDS.  Pairs (a, b), a < b, with r_a, r_b, r_ab all within the largest
 // angular cutoff are appended (ballot + prefix count) to the atom's list at
@@ -524,7 +534,10 @@ __global__ __launch_bounds__(64) void k_build(int N, const int *__restrict__ ato
         const size_t sl = base + pos;
         nb.nb_j[sl] = inf.x;
         nb.nb_rev[sl] = -1;  // k_reverse fills it from whichever side leads the pair
-        nb.nb_key[sl] = nb_key(inf.x, T0 - inf.y + si0, T1 - inf.z + si1, T2 - inf.w + si2);
+        if (nb.key32)
+          ((uint32_t *)nb.nb_key)[sl] = nb_key32(inf.x, T0 - inf.y + si0, T1 - inf.z + si1, T2 - inf.w + si2);
+        else
+          nb.nb_key[sl] = nb_key(inf.x, T0 - inf.y + si0, T1 - inf.z + si1, T2 - inf.w + si2);
         double2 *d2 = (double2 *)(nb.nb_d + 4 * sl);
         d2[0] = make_double2(dx, dy);
         d2[1] = make_double2(dz, r);
@@ -674,33 +687,47 @@ __global__ void k_set_u64(unsigned long long *p, unsigned long long v) {
 // only).  Each pair is searched once, from its leading side (i < j, or i == j
 // with n lexicographically positive), which writes both entries: no other
 // lane writes either.  One wavefront per centre atom, one slot per lane; j's
-// packed keys are read kRevBatch at a time (independent 16-byte loads) and the
-// first match wins.  nb_key carries kRevBatch keys of padding past the last
-// atom's slots, so the batched reads stay in bounds; entries at or beyond j's
-// count are never compared.
+// keys (64-bit, or 32-bit with K32: half the bytes read) are read kRevBatch
+// at a time (independent 16-byte loads) and the first match wins.  nb_key
+// carries kRevBatch keys of padding past the last atom's slots, so the
+// batched reads stay in bounds; entries at or beyond j's count are never
+// compared.
+template <bool K32>
 __global__ __launch_bounds__(64) void k_reverse(int N, NeighborBufs nb) {
+  typedef typename std::conditional<K32, uint32_t, unsigned long long>::type Key;
+  constexpr int PER = 16 / (int)sizeof(Key);  // keys per 16-byte load
+  const Key *keys = (const Key *)nb.nb_key;
   const int i = blockIdx.x;
   if (i >= N) return;
   const int K = nb.nb_count[i];
   const size_t base = (size_t)i * nb.kcap;
   for (int a = (int)threadIdx.x; a < K; a += 64) {
-    const unsigned long long key = nb.nb_key[base + a];
-    const int j = (int)(key >> 24);
-    const int n0 = (int)((key >> 16) & 255) - 128, n1 = (int)((key >> 8) & 255) - 128, n2 = (int)(key & 255) - 128;
+    const Key key = keys[base + a];
+    int j, n0, n1, n2;
+    if constexpr (K32) {
+      j = (int)(key >> 15);
+      n0 = sext5(key >> 10), n1 = sext5(key >> 5), n2 = sext5(key);
+    } else {
+      j = (int)(key >> 24);
+      n0 = (int)((key >> 16) & 255) - 128, n1 = (int)((key >> 8) & 255) - 128, n2 = (int)(key & 255) - 128;
+    }
     const bool lead = i < j || (i == j && (n0 > 0 || (n0 == 0 && (n1 > 0 || (n1 == 0 && n2 > 0)))));
     if (!lead) continue;
-    const unsigned long long want = nb_key(i, -n0, -n1, -n2);
+    Key want;
+    if constexpr (K32)
+      want = nb_key32(i, -n0, -n1, -n2);
+    else
+      want = nb_key(i, -n0, -n1, -n2);
     const size_t jb = (size_t)j * nb.kcap;
-    const ulonglong2 *kj = (const ulonglong2 *)(nb.nb_key + jb);
+    const uint4 *kj = (const uint4 *)(keys + jb);  // kcap is a multiple of 8: 16-byte aligned rows
     const int Kj = nb.nb_count[j];
     int found = -1;
     for (int q0 = 0; q0 < Kj && found < 0; q0 += kRevBatch) {
-      unsigned long long k[kRevBatch];
+      Key k[kRevBatch];
 #pragma unroll
-      for (int u = 0; u < kRevBatch / 2; ++u) {
-        const ulonglong2 v = kj[q0 / 2 + u];
-        k[2 * u] = v.x;
-        k[2 * u + 1] = v.y;
+      for (int u = 0; u < kRevBatch / PER; ++u) {
+        const uint4 v = kj[q0 / PER + u];
+        __builtin_memcpy(k + PER * u, &v, 16);
       }
 #pragma unroll
       for (int u = kRevBatch - 1; u >= 0; --u)
@@ -988,7 +1015,11 @@ hipError_t launch_min_bond(hipStream_t s, int N, const StructParams *sp, const d
 }
 
 hipError_t launch_reverse_slots(hipStream_t s, int N, NeighborBufs nb) {
-  k_reverse<<<N, 64, 0, s>>>(N, nb);
+  if (nb.kcap % 8 != 0) return hipErrorInvalidValue;  // 16-byte aligned key rows
+  if (nb.key32)
+    k_reverse<true><<<N, 64, 0, s>>>(N, nb);
+  else
+    k_reverse<false><<<N, 64, 0, s>>>(N, nb);
   return hipGetLastError();
 }
 
