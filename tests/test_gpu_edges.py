@@ -21,6 +21,9 @@
 * Strongly sheared cells, where the list kernel's image-bin distance bound is
   loosest: pair counts against a test of every image, descriptors and
   results against the oracle.
+* A 0.4 A-thin slab whose image search reaches 15 bins (compact 32-bit
+  reverse-slot keys) and 18 bins (64-bit keys): pair counts and results
+  against the oracle.
 """
 import numpy as np
 import pytest
@@ -299,3 +302,30 @@ def test_sheared_cells_match_oracle(model, cell):
     xo = oracle.descriptors(om, species, lat, pos, 6.0)
     np.testing.assert_allclose(x, xo, rtol=1e-11, atol=1e-13)
     check(om, m, species, lat, pos)
+
+
+def test_wide_image_search_matches_oracle():
+    """A cell with one 0.4 A lattice axis needs a search half-width of 15
+    bins along it at rcut 6 for the 16-atom, 0.4 A-thin slab below (the list
+    kernel's compact 32-bit reverse-slot keys hold up to 15), and 18 at rcut 7
+    (64-bit keys).  Both against the oracle with a model fitted on the cell's
+    own descriptors (kernel values O(1), so a wrong reverse slot would show
+    in the forces), and the pair counts against a test of every image."""
+    rng = np.random.default_rng(31)
+    lat = np.array([[9.0, 0.0, 0.0], [2.0, 8.5, 0.0], [0.0, 0.0, 0.4]])
+    pos = np.concatenate([rng.random((16, 2)) * 8.0, rng.random((16, 1)) * 0.4], axis=1)
+    species = rng.choice(np.array([1, 8, 22], np.int32), size=16).astype(np.int32)
+    s = dict(species=species, lat=lat, pos=pos)
+    om = fitted_small_model(rng, s)
+    m = gapx.Model.from_model(om)
+    try:
+        for rcut in (6.0, 7.0):
+            ctx = gapx.Context(m)
+            ctx.load([s], rcut)
+            ctx.run(gapx.GRAD)
+            pairs, _ = ctx.counts()
+            ctx.close()
+            assert pairs == brute_pairs(lat, pos, rcut)
+            check(om, m, species, lat, pos, rcut=rcut)
+    finally:
+        m.close()
