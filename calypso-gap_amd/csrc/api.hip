@@ -882,7 +882,7 @@ static int enqueue_forces(Ctx &c, Bufs &b, bool fp64, bool grad, const double *e
   rb.overflow_now = stats_acc + 16;
   if (grad && !c.timing) GAPX_HIP(hipStreamWaitEvent(c.stream, c.join_ev, 0));
   stage_begin(c, kForces);
-  GAPX_HIP(launch_forces_reduce(c.stream, c.N, grad ? 1 : 0, nb, b.slot_f, b.force, c.nstruct, d_sp, eatom,
+  GAPX_HIP(launch_forces_reduce(c.stream, c.N, grad ? 1 : 0, nb, b.slot_f, !fp64, b.force, c.nstruct, d_sp, eatom,
                                 b.atom_vir, b.ene, b.stress, b.tb, rb, c.host_stats_dev));
   stage_end(c, kForces);
   return GAPX_OK;

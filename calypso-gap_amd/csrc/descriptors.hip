@@ -1072,10 +1072,19 @@ __global__ __launch_bounds__(256) void k_desc_bwd(int N, const int *__restrict__
     }
     const double sc = Ts / (double)r;
     const double dx = s_d[4 * a], dy = s_d[4 * a + 1], dz = s_d[4 * a + 2];
-    double *o = slot_f + 3 * (size_t)(start + a);
-    o[0] = sc * dx + Fx;
-    o[1] = sc * dy + Fy;
-    o[2] = sc * dz + Fz;
+    if constexpr (sizeof(T) == 4) {
+      // the FP32 kernels (mixed runs) store the slot forces in FP32: half the
+      // bytes written here and read by the force gather
+      float *o = (float *)slot_f + 3 * (size_t)(start + a);
+      o[0] = (float)(sc * dx + Fx);
+      o[1] = (float)(sc * dy + Fy);
+      o[2] = (float)(sc * dz + Fz);
+    } else {
+      double *o = slot_f + 3 * (size_t)(start + a);
+      o[0] = sc * dx + Fx;
+      o[1] = sc * dy + Fy;
+      o[2] = sc * dz + Fz;
+    }
     vir[0] += sc * dx * dx + Fx * dx;
     vir[1] += sc * dy * dy + Fy * dy;
     vir[2] += sc * dz * dz + Fz * dz;

@@ -283,7 +283,8 @@ struct GradSrc {
   double *g64_out = nullptr;
   float *g32_out = nullptr;
 };
-// Backward for atoms [atom0, atom0 + N).
+// Backward for atoms [atom0, atom0 + N).  Per-slot forces go to slot_f as
+// three values per slot: FP64 (fp64), or FP32 from the FP32 kernels.
 hipError_t launch_descriptors_bwd(hipStream_t s, bool fp64, int N, const int *species,
                                   const double *wz, const NeighborBufs &nb, const TripletBufs &tb,
                                   const SFDev &sf, int Dp, const GradSrc &gs,
@@ -324,11 +325,12 @@ struct ReduceBufs {
 // per-structure energy and stress, and the list statistics: out[0..7] = kept
 // pairs, largest raw pair count, atoms above kcap, descriptor guard
 // (nb.total[3]), kept triplets, largest raw triplet count, atoms above tcap, 0.
-// stats_out may be pinned host memory.
-hipError_t launch_forces_reduce(hipStream_t s, int N, int grad, const NeighborBufs &nb, const double *slot_f,
-                                double *force, int nstruct, const StructParams *sp, const double *eatom,
-                                const double *atom_vir, double *ene, double *stress6, const TripletBufs &tb,
-                                const ReduceBufs &rb, int *stats_out);
+// stats_out may be pinned host memory.  slot_f holds FP32 slot forces when
+// slot32 (the FP32 backward writes them so), else FP64; sums are FP64.
+hipError_t launch_forces_reduce(hipStream_t s, int N, int grad, const NeighborBufs &nb, const void *slot_f,
+                                bool slot32, double *force, int nstruct, const StructParams *sp,
+                                const double *eatom, const double *atom_vir, double *ene, double *stress6,
+                                const TripletBufs &tb, const ReduceBufs &rb, int *stats_out);
 
 // Contraction: E_i = sum_k exp(-|xt_i - mt_k|^2 / 2) c_k and
 // g_i = -(xt_i E_i - sum_k W_ik mt_k) * theta_inv.

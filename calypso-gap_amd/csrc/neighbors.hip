@@ -765,8 +765,9 @@ __device__ __forceinline__ V wave_reduce_sum(V v) {
 //                          from one L2): F_m = sum of own slots - sum of
 //                          reverse slots in slot order (zeros without grad).
 // The sums go first so that they run beside the gather blocks.
+template <typename SF>
 __global__ __launch_bounds__(256) void k_forces_reduce(int N, int G, int nstruct, NeighborBufs nb,
-                                                       const double *__restrict__ slot_f, double *__restrict__ force,
+                                                       const SF *__restrict__ slot_f, double *__restrict__ force,
                                                        const StructParams *__restrict__ sp,
                                                        const double *__restrict__ eatom,
                                                        const double *__restrict__ vir, int grad,
@@ -907,7 +908,7 @@ __global__ __launch_bounds__(256) void k_forces_reduce(int N, int G, int nstruct
     const size_t base = (size_t)m * nb.kcap;
     for (int a = lane; a < nb.kcap; a += 64) {
       const size_t sl = base + a;
-      const double o0 = slot_f[3 * sl], o1 = slot_f[3 * sl + 1], o2 = slot_f[3 * sl + 2];
+      const double o0 = (double)slot_f[3 * sl], o1 = (double)slot_f[3 * sl + 1], o2 = (double)slot_f[3 * sl + 2];
       const int rv = nb.nb_rev[sl];
       // an empty asm that takes the loaded values: the compiler must issue
       // the loads here, before the comparison with K, instead of moving
@@ -921,9 +922,9 @@ __global__ __launch_bounds__(256) void k_forces_reduce(int N, int G, int nstruct
         f2 += o2;
       }
       if (r >= 0) {
-        f0 -= slot_f[3 * (size_t)r];
-        f1 -= slot_f[3 * (size_t)r + 1];
-        f2 -= slot_f[3 * (size_t)r + 2];
+        f0 -= (double)slot_f[3 * (size_t)r];
+        f1 -= (double)slot_f[3 * (size_t)r + 1];
+        f2 -= (double)slot_f[3 * (size_t)r + 2];
       }
     }
     for (int o = 32; o > 0; o >>= 1) {
@@ -1023,15 +1024,19 @@ hipError_t launch_reverse_slots(hipStream_t s, int N, NeighborBufs nb) {
   return hipGetLastError();
 }
 
-hipError_t launch_forces_reduce(hipStream_t s, int N, int grad, const NeighborBufs &nb, const double *slot_f,
-                                double *force, int nstruct, const StructParams *sp, const double *eatom,
-                                const double *atom_vir, double *ene, double *stress6, const TripletBufs &tb,
-                                const ReduceBufs &rb, int *stats_out) {
+hipError_t launch_forces_reduce(hipStream_t s, int N, int grad, const NeighborBufs &nb, const void *slot_f,
+                                bool slot32, double *force, int nstruct, const StructParams *sp,
+                                const double *eatom, const double *atom_vir, double *ene, double *stress6,
+                                const TripletBufs &tb, const ReduceBufs &rb, int *stats_out) {
   if (N <= 0 || nstruct <= 0 || rb.nparts < nstruct) return hipErrorInvalidValue;
   const int G = (N + 3) / 4, T = (N + kReduceAtoms - 1) / kReduceAtoms;
-  k_forces_reduce<<<xcd_blocks(rb.nparts + T) + xcd_blocks(G), 256, 0, s>>>(N, G, nstruct, nb, slot_f, force, sp,
-                                                                            eatom, atom_vir, grad, ene, stress6,
-                                                                            tb, rb, stats_out);
+  const dim3 grid(xcd_blocks(rb.nparts + T) + xcd_blocks(G));
+  if (slot32)
+    k_forces_reduce<float><<<grid, 256, 0, s>>>(N, G, nstruct, nb, (const float *)slot_f, force, sp, eatom,
+                                                atom_vir, grad, ene, stress6, tb, rb, stats_out);
+  else
+    k_forces_reduce<double><<<grid, 256, 0, s>>>(N, G, nstruct, nb, (const double *)slot_f, force, sp, eatom,
+                                                 atom_vir, grad, ene, stress6, tb, rb, stats_out);
   return hipGetLastError();
 }
 
